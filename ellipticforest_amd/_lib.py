@@ -1,0 +1,246 @@
+"""ctypes binding of libefhps.so (declared in include/ef_hps.h).
+
+There is no CPU fallback: if the library or a GPU is missing, loading or creating a
+context raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libefhps.so")
+
+EF_OK = 0
+EF_ERR_ARG = 1
+EF_ERR_STATE = 2
+EF_ERR_DEVICE = 3
+EF_ERR_SINGULAR = 4
+EF_ERR_INTERNAL = 5
+
+EF_SOLVER_FIVE_POINT = 0
+EF_SOLVER_FISHPACK = 1
+EF_PRECISION_FP64 = 0
+EF_PRECISION_MIXED = 1
+
+# Every symbol include/ef_hps.h declares.
+EXPORTS = [
+    "ef_ctx_create", "ef_ctx_destroy", "ef_last_error", "ef_version", "ef_refine_by_function",
+    "ef_mesh_create", "ef_mesh_destroy", "ef_mesh_footprint", "ef_mesh_info", "ef_mesh_nodes",
+    "ef_mesh_set_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
+    "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+REFINE_FN = ctypes.CFUNCTYPE(ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_void_p)
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("nleaves", ctypes.c_int), ("level", _ip), ("ix", _ip), ("iy", _ip), ("N", ctypes.c_int),
+                ("xl", ctypes.c_double), ("xu", ctypes.c_double), ("yl", ctypes.c_double), ("yu", ctypes.c_double)]
+
+
+class Coeffs(ctypes.Structure):
+    _fields_ = [("solver_type", ctypes.c_int), ("alpha0", ctypes.c_double), ("beta0", ctypes.c_double),
+                ("lambda0", ctypes.c_double), ("cell", _dp)]
+
+
+class EfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[ef error {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ef_last_error.restype = ctypes.c_char_p
+        L.ef_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+        L.ef_ctx_destroy.argtypes = [_vp]
+        L.ef_refine_by_function.argtypes = [REFINE_FN, _vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip]
+        L.ef_mesh_create.argtypes = [_vp, ctypes.POINTER(MeshDesc), ctypes.POINTER(_vp)]
+        L.ef_mesh_destroy.argtypes = [_vp]
+        L.ef_mesh_footprint.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_size_t)]
+        L.ef_mesh_info.argtypes = [_vp, _ip]
+        L.ef_mesh_nodes.argtypes = [_vp, _ip, _dp]
+        L.ef_mesh_set_stream.argtypes = [_vp, _vp]
+        L.ef_mesh_set_options.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.ef_build.argtypes = [_vp, ctypes.POINTER(Coeffs), ctypes.c_int]
+        L.ef_upwards.argtypes = [_vp, _dp, ctypes.c_int]
+        L.ef_upwards_device.argtypes = [_vp, _vp, ctypes.c_int]
+        L.ef_solve.argtypes = [_vp, _dp, _dp, _dp, _dp, ctypes.c_int]
+        L.ef_solve_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+        L.ef_get_operator.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip]
+        L.ef_mesh_sync.argtypes = [_vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != EF_OK:
+        raise EfError(rc, lib().ef_last_error().decode())
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def refine_by_function(fn, nx, ny, bounds, min_level, max_level):
+    """Leaves (level, ix, iy) in z-order; ``fn(x, y)`` is vectorised over one quadrant's cell centres."""
+    L = lib()
+    err = []
+
+    def cb(px, py, npts, user):
+        try:
+            x = np.ctypeslib.as_array(px, shape=(npts,))
+            y = np.ctypeslib.as_array(py, shape=(npts,))
+            return 1 if bool(np.any(fn(x, y))) else 0
+        except Exception as e:  # propagate after the C call returns
+            err.append(e)
+            return 0
+
+    cfn = REFINE_FN(cb)
+    n = ctypes.c_int(0)
+    check(L.ef_refine_by_function(cfn, None, nx, ny, *bounds, min_level, max_level, 0, None, None, None,
+                                  ctypes.byref(n)))
+    if err:
+        raise err[0]
+    cap = n.value
+    lv, ix, iy = (np.zeros(cap, np.int32) for _ in range(3))
+    check(L.ef_refine_by_function(cfn, None, nx, ny, *bounds, min_level, max_level, cap, iptr(lv), iptr(ix),
+                                  iptr(iy), ctypes.byref(n)))
+    if err:
+        raise err[0]
+    return [(int(a), int(b), int(c)) for a, b, c in zip(lv, ix, iy)]
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _vp()
+        check(lib().ef_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ef_ctx_destroy(self.h)
+            self.h = None
+
+
+class DeviceMesh:
+    """One factorisable quadtree mesh on the GPU: the C-ABI handle plus host-side helpers."""
+
+    def __init__(self, ctx, leaves, N, bounds):
+        self.ctx = ctx
+        self.N = N
+        self.bounds = tuple(float(b) for b in bounds)
+        self.leaves = list(leaves)
+        self._lv = np.array([l[0] for l in leaves], np.int32)
+        self._ix = np.array([l[1] for l in leaves], np.int32)
+        self._iy = np.array([l[2] for l in leaves], np.int32)
+        d = MeshDesc(len(leaves), iptr(self._lv), iptr(self._ix), iptr(self._iy), N, *self.bounds)
+        h = _vp()
+        check(lib().ef_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        info = np.zeros(8, np.int32)
+        check(lib().ef_mesh_info(h, iptr(info)))
+        self.nnodes, self.nleaves, self.M, self.maxdepth = (int(v) for v in info[:4])
+        self.n_internal, self.max_n = int(info[5]), int(info[6])
+        ni = np.zeros(12 * self.nnodes, np.int32)
+        nb = np.zeros(4 * self.nnodes)
+        check(lib().ef_mesh_nodes(h, iptr(ni), dptr(nb)))
+        self.node_info = ni.reshape(-1, 12)
+        self.node_bounds = nb.reshape(-1, 4)
+        self.leaf_nodes = [i for i in range(self.nnodes) if self.node_info[i, 3]]
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ef_mesh_destroy(self.h)
+            self.h = None
+
+    def set_options(self, cache_operators=False, homogeneous_rhs=False):
+        check(lib().ef_mesh_set_options(self.h, int(cache_operators), int(homogeneous_rhs)))
+
+    def set_stream(self, stream_ptr):
+        check(lib().ef_mesh_set_stream(self.h, stream_ptr))
+
+    def build(self, solver_type=EF_SOLVER_FISHPACK, lambda0=0.0, alpha0=1.0, beta0=1.0, cell=None,
+              precision=EF_PRECISION_FP64):
+        keep = None
+        cp = ctypes.cast(None, _dp)
+        if cell is not None:
+            keep = f64(cell)
+            cp = dptr(keep)
+        c = Coeffs(solver_type, alpha0, beta0, lambda0, cp)
+        check(lib().ef_build(self.h, ctypes.byref(c), precision))
+        del keep
+
+    def upwards(self, f):
+        """f: [nrhs][nleaves][N*N] (or [nleaves][N*N] for one right-hand side)."""
+        f = f64(f)
+        k = 1 if f.ndim == 2 else f.shape[0]
+        check(lib().ef_upwards(self.h, dptr(f), k))
+
+    def solve(self, r, a, b=None):
+        """r: [nrhs][4M] (or [4M]); a, b: [4M].  Returns u [nrhs][nleaves][N*N] (or [nleaves][N*N])."""
+        r = f64(r)
+        single = r.ndim == 1
+        k = 1 if single else r.shape[0]
+        a = f64(np.broadcast_to(a, (4 * self.M,)))
+        bp = ctypes.cast(None, _dp)
+        if b is not None:
+            b = f64(np.broadcast_to(b, (4 * self.M,)))
+            bp = dptr(b)
+        u = np.zeros((k, self.nleaves, self.N * self.N))
+        check(lib().ef_solve(self.h, dptr(r), dptr(a), bp, dptr(u), k))
+        return u[0] if single else u
+
+    def upwards_device(self, f_ptr, k):
+        check(lib().ef_upwards_device(self.h, f_ptr, k))
+
+    def solve_device(self, f_ptr, r_ptr, a_ptr, b_ptr, u_ptr, k):
+        check(lib().ef_solve_device(self.h, f_ptr, r_ptr, a_ptr, b_ptr, u_ptr, k))
+
+    def sync(self):
+        check(lib().ef_mesh_sync(self.h))
+
+    CODES = dict(T=0, S=1, X=2, H=3, u=4, g=5, f=6, h=7, w=8)
+
+    def get(self, node, which, rhs=0):
+        code = self.CODES[which]
+        r, c = ctypes.c_int(), ctypes.c_int()
+        check(lib().ef_get_operator(self.h, node, code, rhs, ctypes.cast(None, _dp), ctypes.byref(r),
+                                    ctypes.byref(c)))
+        out = np.zeros((r.value, c.value))
+        check(lib().ef_get_operator(self.h, node, code, rhs, dptr(out), ctypes.byref(r), ctypes.byref(c)))
+        return out if which in ("T", "S", "X", "H") else out.ravel()
+
+    @staticmethod
+    def footprint(leaves, N, bounds, precision=EF_PRECISION_FP64, max_rhs=1):
+        lv = np.array([l[0] for l in leaves], np.int32)
+        ix = np.array([l[1] for l in leaves], np.int32)
+        iy = np.array([l[2] for l in leaves], np.int32)
+        d = MeshDesc(len(leaves), iptr(lv), iptr(ix), iptr(iy), N, *[float(b) for b in bounds])
+        out = ctypes.c_size_t()
+        check(lib().ef_mesh_footprint(ctypes.byref(d), precision, max_rhs, ctypes.byref(out)))
+        return int(out.value)

@@ -1,0 +1,1124 @@
+// ef_api.hip -- host side of libefhps.so: tree planner, device arenas, stage drivers
+// and the C ABI declared in include/ef_hps.h.
+//
+// The reference visits families one at a time through p4est callbacks
+// (Quadtree::merge/split, src/Quadtree.hpp:322-360) and keeps every operator in a
+// per-node std::vector.  Here the tree is planned once into level-ordered groups and
+// device arenas; each stage is a fixed sequence of batched launches per tree level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ef_hps.h"
+#include "ef_kernels.hpp"
+
+using namespace efk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct EfError : std::runtime_error {
+    int code;
+    EfError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define EF_HIP(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define EF_LAUNCH_CHECK_NAMED(name)                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = hipGetLastError();                                                           \
+        if (e_ != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string("launch of ") + name + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define EF_LAUNCH_CHECK() EF_LAUNCH_CHECK_NAMED(__func__ + std::string(":") + std::to_string(__LINE__))
+
+inline unsigned blocks_for(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count) EF_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void ensure(size_t count) {
+        if (count > n) alloc(count);
+    }
+    // Host -> device copy ordered on the stream the kernels that read it run on, and
+    // complete before the host vector can go away.
+    void upload(const std::vector<T>& v, hipStream_t s) {
+        alloc(v.size());
+        if (v.empty()) return;
+        EF_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        EF_HIP(hipStreamSynchronize(s));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Tree plan
+// ---------------------------------------------------------------------------
+struct NodeP {
+    int level = 0, ix = 0, iy = 0, parent = -1;
+    int child[4] = {-1, -1, -1, -1};
+    bool leaf = false;
+    int leaf_index = -1;
+    double xl = 0, xu = 0, yl = 0, yu = 0;
+    int s = 0;    // patch size after its own merge (grid nx): N for leaves
+    int tag = 0;  // n_coarsens set by the parent's merge
+    int n = 0;    // internal: child interface size
+    // operator arena offsets (elements)
+    long long T_off = -1;               // internal: own DtN (8n x 8n)
+    std::vector<long long> Tc_off;      // coarsened DtN after each coarsening step
+    long long S_off = -1, Xi_off = -1, H_off = -1;
+    // vector arena row offsets
+    long long h_off = -1, g_off = -1, gc_off = -1, gt_off = -1, hd_off = -1, w_off = -1, ui_off = -1;
+};
+
+struct Group {  // merges at one depth with one interface size
+    int depth = 0, n = 0;
+    std::vector<int> nodes;
+};
+
+struct Plan {
+    int N = 0;
+    double xl = 0, xu = 0, yl = 0, yu = 0;
+    std::vector<NodeP> nodes;
+    std::vector<int> leaves;                   // node ids, z-order
+    std::vector<std::vector<int>> internal_at; // internal node ids per depth
+    std::vector<std::vector<Group>> groups;    // per depth
+    int maxdepth = 0;
+    long long T_elems = 0, S_elems = 0, Xi_elems = 0, H_elems = 0;
+    long long vec_rows = 0;
+    long long work_X = 0, work_M = 0, work_U = 0;
+    int max_group = 0, max_n = 0;
+};
+
+int build_node(Plan& P, int parent, int level, int ix, int iy, double xl, double xu, double yl, double yu,
+               const ef_mesh_desc& d, int& cursor) {
+    if (level > 30) throw EfError(EF_ERR_ARG, "leaf list is not a pre-order quadtree leaf list (depth > 30)");
+    NodeP nd;
+    nd.level = level; nd.ix = ix; nd.iy = iy; nd.parent = parent;
+    nd.xl = xl; nd.xu = xu; nd.yl = yl; nd.yu = yu;
+    P.nodes.push_back(nd);
+    const int id = (int)P.nodes.size() - 1;
+    if (cursor >= d.nleaves) throw EfError(EF_ERR_ARG, "leaf list ended early");
+    if (d.level[cursor] == level) {
+        if (d.ix[cursor] != ix || d.iy[cursor] != iy)
+            throw EfError(EF_ERR_ARG, "leaf list is not a pre-order quadtree leaf list");
+        P.nodes[id].leaf = true;
+        P.nodes[id].leaf_index = (int)P.leaves.size();
+        P.leaves.push_back(id);
+        cursor++;
+        return id;
+    }
+    if (d.level[cursor] < level) throw EfError(EF_ERR_ARG, "leaf list is not a pre-order quadtree leaf list");
+    // child quadrants exactly as FiniteVolumeNodeFactory::createChildNode (:15-67)
+    const double xm = (xl + xu) / 2.0, ym = (yl + yu) / 2.0;
+    const double cb[4][4] = {{xl, xm, yl, ym}, {xm, xu, yl, ym}, {xl, xm, ym, yu}, {xm, xu, ym, yu}};
+    for (int c = 0; c < 4; c++) {
+        const int cid = build_node(P, id, level + 1, 2 * ix + (c & 1), 2 * iy + (c >> 1), cb[c][0], cb[c][1], cb[c][2],
+                                   cb[c][3], d, cursor);
+        P.nodes[id].child[c] = cid;
+    }
+    return id;
+}
+
+int ilog2(int v) {
+    int l = 0;
+    while ((1 << (l + 1)) <= v) l++;
+    return l;
+}
+
+// Sizes and coarsening tags (tagPatchesForCoarsening_, HPSAlgorithm.hpp:676-696;
+// mergePatch_ :1004-1009 makes the parent 2n cells wide), then arena offsets.
+void plan_tree(Plan& P, const ef_mesh_desc& d) {
+    if (d.N < 4 || (d.N & 1)) throw EfError(EF_ERR_ARG, "N must be even and >= 4");
+    if (d.nleaves < 1 || !d.level || !d.ix || !d.iy) throw EfError(EF_ERR_ARG, "empty leaf list");
+    if (!(d.xu > d.xl) || !(d.yu > d.yl)) throw EfError(EF_ERR_ARG, "empty domain");
+    P.N = d.N; P.xl = d.xl; P.xu = d.xu; P.yl = d.yl; P.yu = d.yu;
+    int cursor = 0;
+    build_node(P, -1, 0, 0, 0, d.xl, d.xu, d.yl, d.yu, d, cursor);
+    if (cursor != d.nleaves) throw EfError(EF_ERR_ARG, "leaf list has entries beyond a complete quadtree");
+    // post-order sizes
+    std::vector<int> order;  // reverse pre-order is a valid children-first order
+    for (int i = (int)P.nodes.size() - 1; i >= 0; i--) order.push_back(i);
+    P.maxdepth = 0;
+    for (int id : order) {
+        NodeP& nd = P.nodes[id];
+        P.maxdepth = std::max(P.maxdepth, nd.level);
+        if (nd.leaf) { nd.s = P.N; continue; }
+        int smin = 1 << 30;
+        for (int c = 0; c < 4; c++) smin = std::min(smin, P.nodes[nd.child[c]].s);
+        nd.n = smin;
+        nd.s = 2 * smin;
+        for (int c = 0; c < 4; c++) {
+            NodeP& ch = P.nodes[nd.child[c]];
+            ch.tag = ilog2(ch.s) - ilog2(smin);
+        }
+        P.max_n = std::max(P.max_n, nd.n);
+    }
+    P.internal_at.assign(P.maxdepth + 1, {});
+    for (int id = 0; id < (int)P.nodes.size(); id++)
+        if (!P.nodes[id].leaf) P.internal_at[P.nodes[id].level].push_back(id);
+    P.groups.assign(P.maxdepth + 1, {});
+    for (int dd = 0; dd <= P.maxdepth; dd++) {
+        std::map<int, std::vector<int>> byn;
+        for (int id : P.internal_at[dd]) byn[P.nodes[id].n].push_back(id);
+        for (auto& kv : byn) {
+            Group g;
+            g.depth = dd; g.n = kv.first; g.nodes = kv.second;
+            P.groups[dd].push_back(g);
+            const long long nm = (long long)g.nodes.size(), n = g.n;
+            P.work_X = std::max(P.work_X, nm * 16 * n * n);
+            P.work_M = std::max(P.work_M, nm * 4 * n * GJ_NB);
+            P.work_U = std::max(P.work_U, nm * GJ_NB * 16 * n);
+            P.max_group = std::max(P.max_group, (int)nm);
+        }
+    }
+    // operator arenas
+    long long T = 0, S = 0, Xi = 0, H = 0, rows = 0;
+    for (auto& nd : P.nodes) {
+        if (!nd.leaf) {
+            const long long n = nd.n;
+            nd.T_off = T; T += 64 * n * n;
+            nd.S_off = S; S += 32 * n * n;
+            nd.Xi_off = Xi; Xi += 16 * n * n;
+            nd.H_off = H; H += 16 * n * n;
+            int m = nd.s;
+            nd.Tc_off.clear();
+            for (int t = 0; t < nd.tag; t++) {
+                m /= 2;
+                nd.Tc_off.push_back(T);
+                T += 16LL * m * m;
+            }
+        }
+        // vectors
+        nd.h_off = rows; rows += 4LL * nd.s;
+        nd.g_off = rows; rows += 4LL * nd.s;
+        if (nd.tag > 0) { nd.gc_off = rows; rows += 4LL * (nd.s >> nd.tag); }
+        else nd.gc_off = nd.g_off;
+        if (nd.tag >= 2) { nd.gt_off = rows; rows += 4LL * nd.s; }
+        if (!nd.leaf) {
+            nd.hd_off = rows; rows += 4LL * nd.n;
+            nd.w_off = rows; rows += 4LL * nd.n;
+            nd.ui_off = rows; rows += 4LL * nd.n;
+        }
+    }
+    P.T_elems = T; P.S_elems = S; P.Xi_elems = Xi; P.H_elems = H; P.vec_rows = rows;
+}
+
+// ---------------------------------------------------------------------------
+// Context and mesh
+// ---------------------------------------------------------------------------
+struct SweepPlan {  // per right-hand-side count k
+    std::vector<std::unique_ptr<DevBuf<UpDesc>>> up_own;
+    std::vector<std::unique_ptr<DevBuf<GemmProb>>> up_w, up_h, down_u;
+    std::vector<std::unique_ptr<DevBuf<DownDesc>>> down;
+    std::vector<std::vector<std::unique_ptr<DevBuf<VecStepDesc>>>> unc;  // [depth][step]
+    std::vector<int> up_nmax, up_cnt, w_tiles, h_tiles, u_tiles, down_nmax, down_cnt;
+    std::vector<std::vector<int>> unc_cnt, unc_nf;
+};
+
+}  // namespace
+
+struct ef_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct ef_mesh_s {
+    ef_ctx ctx = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    Plan P;
+    bool cache_operators = false, homogeneous_rhs = false;
+    bool built = false;
+    int solver_type = EF_SOLVER_FISHPACK;
+    int precision = EF_PRECISION_FP64;
+    // leaf classes
+    std::vector<LeafClass> classes_h;
+    std::vector<int> leaf_class;  // per leaf
+    int tleaf_class0 = -1;
+    DevBuf<LeafClass> classes;
+    DevBuf<LeafDesc> leafdesc;
+    DevBuf<double> Q, Dinv, leafT;
+    // operator arenas
+    DevBuf<double> Tar, Sar, Xiar, Har;
+    // work
+    DevBuf<double> wX, wM, wU;
+    DevBuf<int> wpiv, winfo;
+    // build descriptors per group
+    std::vector<std::vector<std::unique_ptr<DevBuf<MergeDesc>>>> mdesc;
+    std::vector<std::vector<std::unique_ptr<DevBuf<GJDesc>>>> gjdesc;
+    std::vector<std::vector<std::unique_ptr<DevBuf<GemmProb>>>> tgemm;
+    std::vector<std::vector<std::vector<std::unique_ptr<DevBuf<CoarsenDesc>>>>> cdesc;  // [depth][group][step]
+    // vectors
+    int kcap = 0;
+    int k_up = 0;  // k of the last upwards call
+    DevBuf<double> vec;
+    DevBuf<double> fbuf, ubuf, rbuf, abuf, bbuf;  // host-call staging
+    const double* f_last = nullptr;
+    std::map<int, std::unique_ptr<SweepPlan>> sweeps;
+    // root Robin work
+    DevBuf<double> rootA;
+    DevBuf<double> rootM, rootU;
+    DevBuf<int> rootpiv, rootinfo;
+    DevBuf<GJDesc> rootgj;
+};
+
+namespace {
+
+int fail(const EfError& e) {
+    g_last_error = e.what();
+    return e.code;
+}
+
+#define EF_API_BEGIN try {
+#define EF_API_END                                              \
+    }                                                           \
+    catch (const EfError& e) { return fail(e); }                \
+    catch (const std::bad_alloc&) {                             \
+        g_last_error = "host out of memory";                    \
+        return EF_ERR_INTERNAL;                                 \
+    }                                                           \
+    catch (const std::exception& e) {                           \
+        g_last_error = e.what();                                \
+        return EF_ERR_INTERNAL;                                 \
+    }                                                           \
+    return EF_OK;
+
+const double* leaf_T_ptr(ef_mesh_s* M, int leaf) {
+    const int cls = M->cache_operators ? M->tleaf_class0 : M->leaf_class[leaf];
+    return M->leafT.p + (long long)cls * 16 * M->P.N * M->P.N;
+}
+
+// DtN the parent merge reads: coarsened if tagged (coarsen_ overwrites it, :736-737)
+const double* merge_T_ptr(ef_mesh_s* M, int id) {
+    const NodeP& nd = M->P.nodes[id];
+    if (nd.tag > 0) return M->Tar.p + nd.Tc_off.back();
+    if (nd.leaf) return leaf_T_ptr(M, nd.leaf_index);
+    return M->Tar.p + nd.T_off;
+}
+const double* full_T_ptr(ef_mesh_s* M, int id) {
+    const NodeP& nd = M->P.nodes[id];
+    if (nd.leaf) return leaf_T_ptr(M, nd.leaf_index);
+    return M->Tar.p + nd.T_off;
+}
+
+void launch_gemm(const DevBuf<GemmProb>& probs, int nprob, int max_tiles, hipStream_t s) {
+    if (nprob == 0 || max_tiles == 0) return;
+    hipLaunchKernelGGL(gemm_f64_kernel, dim3(max_tiles, nprob), dim3(256), 0, s, probs.p);
+    EF_LAUNCH_CHECK();
+}
+
+int tiles_of(int M_, int N_) { return ((M_ + 63) / 64) * ((N_ + 63) / 64); }
+
+// Gauss-Jordan over a batch of equally sized systems.
+void run_gj(const GJDesc* d_dev, int nsys, int m, int maxw, hipStream_t s) {
+    for (int k = 0; k < m; k += GJ_NB) {
+        const int nbb = std::min(GJ_NB, m - k);
+        if (m - k <= GJ_LDS_ROWS) {
+            const size_t lds = (size_t)(m - k) * (GJ_NB + 1) * sizeof(double);
+            hipLaunchKernelGGL(gj_panel_lds_kernel, dim3(nsys), dim3(256), lds, s, d_dev, k);
+        } else {
+            hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nsys), dim3(256), 0, s, d_dev, k);
+        }
+        EF_LAUNCH_CHECK();
+        const int wtot = (m - k - nbb) + maxw;
+        hipLaunchKernelGGL(gj_swap_kernel, dim3(blocks_for(wtot), nsys), dim3(256), 0, s, d_dev, k);
+        EF_LAUNCH_CHECK();
+        const int wmax = std::max(m - k - nbb, maxw);
+        hipLaunchKernelGGL(gj_update_kernel, dim3(tiles_of(m, wmax), nsys * 3), dim3(256), 0, s, d_dev, k);
+        EF_LAUNCH_CHECK();
+    }
+}
+
+// ----- build ------------------------------------------------------------------
+void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
+    Plan& P = M->P;
+    const int N = P.N;
+    const long long NN = (long long)N * N;
+    M->classes_h.clear();
+    M->leaf_class.assign(P.leaves.size(), -1);
+    std::map<std::vector<uint64_t>, int> seen;
+    auto bits = [](double v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
+    for (size_t li = 0; li < P.leaves.size(); li++) {
+        const NodeP& nd = P.nodes[P.leaves[li]];
+        const double dx = (nd.xu - nd.xl) / N, dy = (nd.yu - nd.yl) / N;
+        double cx, cy, lam;
+        if (c.solver_type == EF_SOLVER_FISHPACK) {
+            cx = 1.0 / (dx * dx); cy = 1.0 / (dy * dy); lam = c.lambda0;
+        } else if (c.cell == nullptr) {
+            cx = c.alpha0 * c.beta0 / (dx * dx); cy = c.alpha0 * c.beta0 / (dy * dy); lam = c.lambda0;
+        } else {
+            // per-cell coefficients: this build accepts them when they are constant over the mesh
+            const double* base = c.cell + li * 6 * NN;
+            const double a0 = base[0], b0 = base[NN], l0 = base[5 * NN];
+            for (long long e = 0; e < NN; e++) {
+                if (base[e] != a0 || base[NN + e] != b0 || base[2 * NN + e] != b0 || base[3 * NN + e] != b0 ||
+                    base[4 * NN + e] != b0 || base[5 * NN + e] != l0)
+                    throw EfError(EF_ERR_ARG, "variable per-cell coefficients are not supported by this build");
+            }
+            cx = a0 * b0 / (dx * dx); cy = a0 * b0 / (dy * dy); lam = l0;
+        }
+        std::vector<uint64_t> key = {bits(cx), bits(cy), bits(lam), bits(dx), bits(dy)};
+        auto it = seen.find(key);
+        int id;
+        if (it == seen.end()) {
+            id = (int)M->classes_h.size();
+            seen[key] = id;
+            LeafClass L;
+            L.cx = cx; L.cy = cy; L.lam = lam; L.dx = dx; L.dy = dy; L.T = nullptr;
+            M->classes_h.push_back(L);
+        } else id = it->second;
+        M->leaf_class[li] = id;
+    }
+    M->tleaf_class0 = M->leaf_class[0];
+    const int ncls = (int)M->classes_h.size();
+    M->leafT.alloc((size_t)ncls * 16 * NN);
+    for (int i = 0; i < ncls; i++) M->classes_h[i].T = M->leafT.p + (long long)i * 16 * NN;
+    M->classes.upload(M->classes_h, M->stream);
+    std::vector<LeafDesc> ld(P.leaves.size());
+    for (size_t li = 0; li < P.leaves.size(); li++) {
+        const NodeP& nd = P.nodes[P.leaves[li]];
+        LeafDesc& L = ld[li];
+        L.cls = M->leaf_class[li];
+        L.node = P.leaves[li];
+        L.h_off = nd.h_off;
+        L.g_off = nd.g_off;
+        L.coef = nullptr;
+        L.Sinv = nullptr;
+        L.dx = (nd.xu - nd.xl) / N;
+        L.dy = (nd.yu - nd.yl) / N;
+    }
+    M->leafdesc.upload(ld, M->stream);
+}
+
+void plan_build_descriptors(ef_mesh_s* M) {
+    Plan& P = M->P;
+    M->mdesc.clear();
+    M->mdesc.resize(P.maxdepth + 1);
+    M->gjdesc.clear();
+    M->gjdesc.resize(P.maxdepth + 1);
+    M->tgemm.clear();
+    M->tgemm.resize(P.maxdepth + 1);
+    M->cdesc.clear();
+    M->cdesc.resize(P.maxdepth + 1);
+    for (int dd = P.maxdepth; dd >= 0; dd--) {
+        for (auto& g : P.groups[dd]) {
+            const int n = g.n;
+            const int nm = (int)g.nodes.size();
+            // coarsening steps of tagged children
+            std::vector<std::vector<CoarsenDesc>> steps;
+            for (int id : g.nodes)
+                for (int c = 0; c < 4; c++) {
+                    const NodeP& ch = P.nodes[P.nodes[id].child[c]];
+                    for (int t = 0; t < ch.tag; t++) {
+                        if ((int)steps.size() <= t) steps.resize(t + 1);
+                        CoarsenDesc cd;
+                        const int m = ch.s >> t;
+                        cd.m = m;
+                        cd.src = (t == 0) ? full_T_ptr(M, P.nodes[id].child[c]) : M->Tar.p + ch.Tc_off[t - 1];
+                        cd.dst = M->Tar.p + ch.Tc_off[t];
+                        steps[t].push_back(cd);
+                    }
+                }
+            std::vector<std::unique_ptr<DevBuf<CoarsenDesc>>> cv;
+            for (auto& st : steps) {
+                auto b = std::make_unique<DevBuf<CoarsenDesc>>();
+                b->upload(st, M->stream);
+                cv.push_back(std::move(b));
+            }
+            M->cdesc[dd].push_back(std::move(cv));
+            // merges
+            std::vector<MergeDesc> md(nm);
+            std::vector<GJDesc> gj(nm);
+            std::vector<GemmProb> tg;
+            for (int i = 0; i < nm; i++) {
+                const NodeP& nd = P.nodes[g.nodes[i]];
+                MergeDesc& m = md[i];
+                m.n = n;
+                for (int c = 0; c < 4; c++) m.Tc[c] = merge_T_ptr(M, nd.child[c]);
+                m.X = M->wX.p + (long long)i * 16 * n * n;
+                m.S = M->Sar.p + nd.S_off;
+                m.Xi = M->Xiar.p + nd.Xi_off;
+                m.H = M->Har.p + nd.H_off;
+                m.T = M->Tar.p + nd.T_off;
+                GJDesc& G = gj[i];
+                G.m = 4 * n;
+                G.X = m.X; G.ldx = 4 * n;
+                G.R1 = m.S; G.w1 = 8 * n; G.ld1 = 8 * n;
+                G.R2 = m.Xi; G.w2 = 4 * n; G.ld2 = 4 * n;
+                G.M = M->wM.p + (long long)i * 4 * n * GJ_NB;
+                G.ldu = 16 * n;
+                G.U = M->wU.p + (long long)i * GJ_NB * 16 * n;
+                G.piv = M->wpiv.p + (long long)i * GJ_NB;
+                G.info = M->winfo.p + i;
+                // T = blkdiag + H S, one problem per parent boundary block q (mergeT_, :940-968)
+                static const int QC[8] = {0, 2, 1, 3, 0, 1, 2, 3};
+                static const int TAUQ[4][2] = {{0, 4}, {2, 5}, {1, 6}, {3, 7}};
+                static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
+                for (int q = 0; q < 8; q++) {
+                    const int c = QC[q];
+                    const int hrow = (TAUQ[c][0] == q) ? 0 : 1;
+                    GemmProb p{};
+                    const double* Hc = m.H + (long long)c * 4 * n * n + (long long)hrow * n * 2 * n;
+                    p.A = Hc; p.lda = 2 * n; p.K = n;
+                    p.B = m.S + (long long)CHIF[c][0] * n * 8 * n; p.ldb = 8 * n;
+                    p.A2 = Hc + n; p.lda2 = 2 * n; p.K2 = n;
+                    p.B2 = m.S + (long long)CHIF[c][1] * n * 8 * n; p.ldb2 = 8 * n;
+                    p.D = m.T + (long long)q * n * 8 * n; p.ldd = 8 * n;
+                    p.Cin = p.D; p.ldcin = 8 * n;
+                    p.M = n; p.N = 8 * n; p.alpha = 1.0; p.beta = 1.0;
+                    tg.push_back(p);
+                }
+            }
+            auto b1 = std::make_unique<DevBuf<MergeDesc>>();
+            b1->upload(md, M->stream);
+            M->mdesc[dd].push_back(std::move(b1));
+            auto b2 = std::make_unique<DevBuf<GJDesc>>();
+            b2->upload(gj, M->stream);
+            M->gjdesc[dd].push_back(std::move(b2));
+            auto b3 = std::make_unique<DevBuf<GemmProb>>();
+            b3->upload(tg, M->stream);
+            M->tgemm[dd].push_back(std::move(b3));
+        }
+    }
+}
+
+void do_build(ef_mesh_s* M, const ef_coeffs& c) {
+    Plan& P = M->P;
+    hipStream_t s = M->stream;
+    const int N = P.N;
+    M->solver_type = c.solver_type;
+    setup_leaf_classes(M, c);
+    // arenas
+    M->Tar.alloc(std::max<long long>(P.T_elems, 1));
+    M->Sar.alloc(std::max<long long>(P.S_elems, 1));
+    M->Xiar.alloc(std::max<long long>(P.Xi_elems, 1));
+    M->Har.alloc(std::max<long long>(P.H_elems, 1));
+    M->wX.alloc(std::max<long long>(P.work_X, 1));
+    M->wM.alloc(std::max<long long>(P.work_M, 1));
+    M->wU.alloc(std::max<long long>(P.work_U, 1));
+    M->wpiv.alloc(std::max<long long>((long long)P.max_group * GJ_NB, 1));
+    M->winfo.alloc(std::max(P.max_group, 1));
+    plan_build_descriptors(M);
+    // leaf DtN per class (buildD2N)
+    const int ncls = (int)M->classes_h.size();
+    M->Q.alloc((size_t)N * N);
+    M->Dinv.alloc((size_t)ncls * N * N);
+    hipLaunchKernelGGL(fd_basis_kernel, dim3(blocks_for((long long)N * N)), dim3(256), 0, s, M->Q.p, N);
+    EF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(fd_dinv_kernel, dim3(blocks_for((long long)N * N), ncls), dim3(256), 0, s, M->classes.p, M->Dinv.p, N);
+    EF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(class_dtn_kernel, dim3(4 * N, ncls), dim3(256), (size_t)3 * N * N * sizeof(double), s,
+                       M->classes.p, M->Q.p, M->Dinv.p, N);
+    EF_LAUNCH_CHECK();
+    // merges, deepest level first (post-order dependencies)
+    EF_HIP(hipMemsetAsync(M->winfo.p, 0, sizeof(int) * std::max(P.max_group, 1), s));
+    std::vector<int> info_h;
+    for (int dd = P.maxdepth; dd >= 0; dd--) {
+        for (size_t gi = 0; gi < P.groups[dd].size(); gi++) {
+            const Group& g = P.groups[dd][gi];
+            const int n = g.n, nm = (int)g.nodes.size();
+            auto& steps = M->cdesc[dd][gi];
+            for (size_t t = 0; t < steps.size(); t++) {
+                // largest coarsened block of this step
+                int maxm = 0;
+                for (int id : g.nodes)
+                    for (int c = 0; c < 4; c++) {
+                        const NodeP& ch = P.nodes[P.nodes[id].child[c]];
+                        if ((int)t < ch.tag) maxm = std::max(maxm, ch.s >> t);
+                    }
+                const long long out = (long long)(2 * maxm) * (2 * maxm);
+                hipLaunchKernelGGL(coarsen_kernel, dim3(blocks_for(out), (unsigned)steps[t]->n), dim3(256), 0, s,
+                                   steps[t]->p);
+                EF_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(merge_assemble_kernel, dim3(blocks_for(144LL * n * n), nm), dim3(256), 0, s,
+                               M->mdesc[dd][gi]->p);
+            EF_LAUNCH_CHECK();
+            run_gj(M->gjdesc[dd][gi]->p, nm, 4 * n, 12 * n, s);
+            launch_gemm(*M->tgemm[dd][gi], 8 * nm, tiles_of(n, 8 * n), s);
+            // zero-pivot report of this group
+            info_h.assign(nm, 0);
+            EF_HIP(hipMemcpyAsync(info_h.data(), M->winfo.p, sizeof(int) * nm, hipMemcpyDeviceToHost, s));
+            EF_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i < nm; i++)
+                if (info_h[i] != 0)
+                    throw EfError(EF_ERR_SINGULAR, "zero pivot in the interface system of a merge at depth " +
+                                                       std::to_string(dd) + " (column " + std::to_string(info_h[i]) + ")");
+        }
+    }
+    EF_HIP(hipStreamSynchronize(s));
+    M->built = true;
+}
+
+// ----- sweeps ----------------------------------------------------------------
+void ensure_vectors(ef_mesh_s* M, int k) {
+    if (k <= M->kcap) return;
+    M->vec.alloc((size_t)M->P.vec_rows * k);
+    EF_HIP(hipMemsetAsync(M->vec.p, 0, (size_t)M->P.vec_rows * k * sizeof(double), M->stream));
+    M->kcap = k;
+    M->sweeps.clear();  // offsets scale with k only, but keep plans tied to the arena
+}
+
+SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
+    auto it = M->sweeps.find(k);
+    if (it != M->sweeps.end()) return *it->second;
+    Plan& P = M->P;
+    auto sp = std::make_unique<SweepPlan>();
+    double* V = M->vec.p;
+    const int D = P.maxdepth + 1;
+    sp->up_own.resize(D);
+    sp->up_w.resize(D);
+    sp->up_h.resize(D);
+    sp->down.resize(D);
+    sp->down_u.resize(D);
+    sp->unc.resize(D);
+    sp->up_nmax.assign(D, 0); sp->up_cnt.assign(D, 0); sp->w_tiles.assign(D, 0); sp->h_tiles.assign(D, 0);
+    sp->u_tiles.assign(D, 0); sp->down_nmax.assign(D, 0); sp->down_cnt.assign(D, 0);
+    sp->unc_cnt.assign(D, {}); sp->unc_nf.assign(D, {});
+    static const int QC[8] = {0, 2, 1, 3, 0, 1, 2, 3};
+    static const int TAUQ[4][2] = {{0, 4}, {2, 5}, {1, 6}, {3, 7}};
+    static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
+    for (int dd = 0; dd < D; dd++) {
+        std::vector<UpDesc> ud;
+        std::vector<GemmProb> pw, ph, pu;
+        std::vector<DownDesc> dn;
+        std::vector<std::vector<VecStepDesc>> us;
+        for (int id : P.internal_at[dd]) {
+            const NodeP& nd = P.nodes[id];
+            const int n = nd.n;
+            UpDesc u{};
+            u.n = n;
+            for (int c = 0; c < 4; c++) {
+                const NodeP& ch = P.nodes[nd.child[c]];
+                u.tag[c] = ch.tag;
+                u.clen[c] = 4 * ch.s;
+                u.ch_off[c] = ch.h_off;
+            }
+            u.hd_off = nd.hd_off; u.w_off = nd.w_off; u.h_off = nd.h_off;
+            u.Xi = M->Xiar.p + nd.Xi_off;
+            u.H = M->Har.p + nd.H_off;
+            ud.push_back(u);
+            sp->up_nmax[dd] = std::max(sp->up_nmax[dd], n);
+            // w = Xi hd   (mergeW_ with a stored inverse instead of a re-factorisation)
+            GemmProb p{};
+            p.A = u.Xi; p.lda = 4 * n; p.K = 4 * n;
+            p.B = V + nd.hd_off * k; p.ldb = k;
+            p.D = V + nd.w_off * k; p.ldd = k;
+            p.M = 4 * n; p.N = k; p.alpha = 1.0; p.beta = 0.0; p.Cin = nullptr; p.ldcin = 0;
+            pw.push_back(p);
+            sp->w_tiles[dd] = std::max(sp->w_tiles[dd], tiles_of(4 * n, k));
+            // h[q] += H_c[rows of q] [w_if1; w_if2]   (mergeH_)
+            for (int q = 0; q < 8; q++) {
+                const int c = QC[q];
+                const int hrow = (TAUQ[c][0] == q) ? 0 : 1;
+                GemmProb h{};
+                const double* Hc = u.H + (long long)c * 4 * n * n + (long long)hrow * n * 2 * n;
+                h.A = Hc; h.lda = 2 * n; h.K = n;
+                h.B = V + (nd.w_off + (long long)CHIF[c][0] * n) * k; h.ldb = k;
+                h.A2 = Hc + n; h.lda2 = 2 * n; h.K2 = n;
+                h.B2 = V + (nd.w_off + (long long)CHIF[c][1] * n) * k; h.ldb2 = k;
+                h.D = V + (nd.h_off + (long long)q * n) * k; h.ldd = k;
+                h.Cin = h.D; h.ldcin = k;
+                h.M = n; h.N = k; h.alpha = 1.0; h.beta = 1.0;
+                ph.push_back(h);
+            }
+            sp->h_tiles[dd] = std::max(sp->h_tiles[dd], tiles_of(n, k));
+            // downward
+            DownDesc dsc{};
+            dsc.n = n; dsc.tag = nd.tag;
+            dsc.gc_off = nd.gc_off; dsc.g_off = nd.g_off; dsc.w_off = nd.w_off; dsc.ui_off = nd.ui_off;
+            for (int c = 0; c < 4; c++) dsc.cg_off[c] = P.nodes[nd.child[c]].gc_off;
+            dsc.S = M->Sar.p + nd.S_off;
+            dn.push_back(dsc);
+            sp->down_nmax[dd] = std::max(sp->down_nmax[dd], n);
+            GemmProb pd{};
+            pd.A = dsc.S; pd.lda = 8 * n; pd.K = 8 * n;
+            pd.B = V + nd.g_off * k; pd.ldb = k;
+            pd.D = V + nd.ui_off * k; pd.ldd = k;
+            pd.Cin = V + nd.w_off * k; pd.ldcin = k;
+            pd.M = 4 * n; pd.N = k; pd.alpha = 1.0; pd.beta = M->homogeneous_rhs ? 0.0 : 1.0;
+            pu.push_back(pd);
+            sp->u_tiles[dd] = std::max(sp->u_tiles[dd], tiles_of(4 * n, k));
+            // uncoarsen_ steps: coarse gc -> ... -> g, alternating through gt
+            for (int st = 1; st <= nd.tag; st++) {
+                if ((int)us.size() < st) us.resize(st);
+                VecStepDesc v;
+                const int nfine = nd.s >> (nd.tag - st);
+                v.nfine = nfine;
+                const bool dst_is_g = ((nd.tag - st) % 2) == 0;
+                v.dst_off = dst_is_g ? nd.g_off : nd.gt_off;
+                if (st == 1) v.src_off = nd.gc_off;
+                else v.src_off = (((nd.tag - (st - 1)) % 2) == 0) ? nd.g_off : nd.gt_off;
+                us[st - 1].push_back(v);
+            }
+        }
+        sp->up_cnt[dd] = (int)ud.size();
+        sp->down_cnt[dd] = (int)dn.size();
+        auto a = std::make_unique<DevBuf<UpDesc>>(); a->upload(ud, M->stream); sp->up_own[dd] = std::move(a);
+        auto b = std::make_unique<DevBuf<GemmProb>>(); b->upload(pw, M->stream); sp->up_w[dd] = std::move(b);
+        auto c2 = std::make_unique<DevBuf<GemmProb>>(); c2->upload(ph, M->stream); sp->up_h[dd] = std::move(c2);
+        auto e = std::make_unique<DevBuf<DownDesc>>(); e->upload(dn, M->stream); sp->down[dd] = std::move(e);
+        auto f = std::make_unique<DevBuf<GemmProb>>(); f->upload(pu, M->stream); sp->down_u[dd] = std::move(f);
+        for (auto& st : us) {
+            auto bb = std::make_unique<DevBuf<VecStepDesc>>();
+            bb->upload(st, M->stream);
+            int nfmax = 0;
+            for (auto& v : st) nfmax = std::max(nfmax, v.nfine);
+            sp->unc_cnt[dd].push_back((int)st.size());
+            sp->unc_nf[dd].push_back(nfmax);
+            sp->unc[dd].push_back(std::move(bb));
+        }
+    }
+    SweepPlan& ref = *sp;
+    M->sweeps[k] = std::move(sp);
+    return ref;
+}
+
+void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
+    if (!M->built) throw EfError(EF_ERR_STATE, "upwards stage called before the build stage");
+    if (k < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    Plan& P = M->P;
+    hipStream_t s = M->stream;
+    ensure_vectors(M, k);
+    SweepPlan& sp = sweep_plan(M, k);
+    const int N = P.N;
+    const int nl = (int)P.leaves.size();
+    // leaf particular Neumann data (particularNeumannData, FiniteVolumeSolver.cpp:457-462)
+    hipLaunchKernelGGL(leaf_fd_kernel, dim3(nl, k), dim3(256), (size_t)4 * N * N * sizeof(double), s, M->leafdesc.p,
+                       M->classes.p, M->Q.p, M->Dinv.p, N, nl, k, 0, f_dev, 0, (const double*)M->vec.p, M->vec.p,
+                       (double*)nullptr);
+    EF_LAUNCH_CHECK();
+    if (!M->homogeneous_rhs) {  // upwards4to1 is skipped entirely for homogeneous-rhs (:532)
+        for (int dd = P.maxdepth; dd >= 0; dd--) {
+            const int cnt = sp.up_cnt[dd];
+            if (cnt == 0) continue;
+            hipLaunchKernelGGL(up_gather_kernel, dim3(blocks_for(12LL * sp.up_nmax[dd] * k), cnt), dim3(256), 0, s,
+                               sp.up_own[dd]->p, M->vec.p, k);
+            EF_LAUNCH_CHECK();
+            launch_gemm(*sp.up_w[dd], cnt, sp.w_tiles[dd], s);
+            launch_gemm(*sp.up_h[dd], 8 * cnt, sp.h_tiles[dd], s);
+        }
+    }
+    M->k_up = k;
+    M->f_last = f_dev;
+}
+
+void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const double* a_dev, const double* b_dev,
+              bool robin, double* u_dev, int k) {
+    if (!M->built) throw EfError(EF_ERR_STATE, "solve stage called before the build stage");
+    if (k < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    if (!M->homogeneous_rhs && M->k_up != k)
+        throw EfError(EF_ERR_STATE, "solve stage needs a preceding upwards stage with the same number of right-hand sides");
+    Plan& P = M->P;
+    hipStream_t s = M->stream;
+    ensure_vectors(M, k);
+    SweepPlan& sp = sweep_plan(M, k);
+    const int N = P.N;
+    const int nl = (int)P.leaves.size();
+    const NodeP& root = P.nodes[0];
+    const int m = 4 * root.s;
+    double* V = M->vec.p;
+    if (!robin) {
+        hipLaunchKernelGGL(root_dirichlet_kernel, dim3(blocks_for((long long)m * k)), dim3(256), 0, s, r_dev, a_dev,
+                           V + root.g_off * k, m, k);
+        EF_LAUNCH_CHECK();
+    } else {
+        // g = (diag(a) + diag(b) T)^-1 (r - b h)
+        M->rootA.ensure((size_t)m * m);
+        M->rootM.ensure((size_t)m * GJ_NB);
+        M->rootU.ensure((size_t)GJ_NB * (m + k));
+        M->rootpiv.ensure(GJ_NB);
+        M->rootinfo.ensure(1);
+        const double* Troot = full_T_ptr(M, 0);
+        const double* hroot = (root.leaf || M->homogeneous_rhs) ? nullptr : V + root.h_off * k;
+        hipLaunchKernelGGL(root_robin_assemble_kernel, dim3(blocks_for((long long)m * m + (long long)m * k)), dim3(256), 0,
+                           s, Troot, a_dev, b_dev, r_dev, hroot, M->rootA.p, V + root.g_off * k, m, k);
+        EF_LAUNCH_CHECK();
+        GJDesc G{};
+        G.m = m; G.X = M->rootA.p; G.ldx = m;
+        G.R1 = V + root.g_off * k; G.w1 = k; G.ld1 = k;
+        G.R2 = nullptr; G.w2 = 0; G.ld2 = 0;
+        G.M = M->rootM.p; G.U = M->rootU.p; G.ldu = m + k;
+        G.piv = M->rootpiv.p; G.info = M->rootinfo.p;
+        std::vector<GJDesc> gv = {G};
+        M->rootgj.upload(gv, M->stream);
+        EF_HIP(hipMemsetAsync(M->rootinfo.p, 0, sizeof(int), s));
+        run_gj(M->rootgj.p, 1, m, k, s);
+        int info = 0;
+        EF_HIP(hipMemcpyAsync(&info, M->rootinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        EF_HIP(hipStreamSynchronize(s));
+        if (info) throw EfError(EF_ERR_SINGULAR, "zero pivot in the root boundary system");
+    }
+    for (int dd = 0; dd <= P.maxdepth; dd++) {
+        const int cnt = sp.down_cnt[dd];
+        if (cnt == 0) continue;
+        for (size_t st = 0; st < sp.unc[dd].size(); st++) {
+            hipLaunchKernelGGL(uncoarsen_kernel, dim3(blocks_for(4LL * sp.unc_nf[dd][st] * k), sp.unc_cnt[dd][st]),
+                               dim3(256), 0, s, sp.unc[dd][st]->p, V, k);
+            EF_LAUNCH_CHECK();
+        }
+        launch_gemm(*sp.down_u[dd], cnt, sp.u_tiles[dd], s);
+        hipLaunchKernelGGL(down_scatter_kernel, dim3(blocks_for(16LL * sp.down_nmax[dd] * k), cnt), dim3(256), 0, s,
+                           sp.down[dd]->p, V, k);
+        EF_LAUNCH_CHECK();
+    }
+    // leaf solves (leafSolve, HPSAlgorithm.hpp:584-596); homogeneous-rhs solves with f = 0
+    hipLaunchKernelGGL(leaf_fd_kernel, dim3(nl, k), dim3(256), (size_t)4 * N * N * sizeof(double), s, M->leafdesc.p,
+                       M->classes.p, M->Q.p, M->Dinv.p, N, nl, k, 1, f_dev ? f_dev : (const double*)V,
+                       (M->homogeneous_rhs || !f_dev) ? 1 : 0, (const double*)V, V, u_dev);
+    EF_LAUNCH_CHECK();
+}
+
+void set_lds_limits() {
+    static bool done = false;
+    if (done) return;
+    // gfx950: 160 KiB of LDS per workgroup, static plus dynamic
+    const int total = 160 * 1024;
+    const void* kernels[] = {(const void*)gj_panel_lds_kernel, (const void*)class_dtn_kernel,
+                             (const void*)leaf_fd_kernel};
+    for (const void* kfn : kernels) {
+        hipFuncAttributes attr;
+        EF_HIP(hipFuncGetAttributes(&attr, kfn));
+        const int dyn = total - (int)attr.sharedSizeBytes;
+        EF_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    }
+    (void)hipGetLastError();
+    done = true;
+}
+
+}  // namespace
+
+namespace efk {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace efk
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* ef_last_error(void) { return g_last_error.c_str(); }
+int ef_version(void) { return 1; }
+
+int ef_ctx_create(int device, ef_ctx* out) {
+    EF_API_BEGIN
+    if (!out) throw EfError(EF_ERR_ARG, "null output handle");
+    int ndev = 0;
+    EF_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw EfError(EF_ERR_ARG, "no such device");
+    EF_HIP(hipSetDevice(device));
+    set_lds_limits();
+    auto* c = new ef_ctx_s();
+    c->device = device;
+    *out = c;
+    EF_API_END
+}
+
+int ef_ctx_destroy(ef_ctx ctx) {
+    EF_API_BEGIN
+    delete ctx;
+    EF_API_END
+}
+
+int ef_mesh_create(ef_ctx ctx, const ef_mesh_desc* desc, ef_mesh* out) {
+    EF_API_BEGIN
+    if (!ctx || !desc || !out) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<ef_mesh_s> M(new ef_mesh_s());
+    M->ctx = ctx;
+    plan_tree(M->P, *desc);
+    EF_HIP(hipStreamCreate(&M->own_stream));
+    M->stream = M->own_stream;
+    *out = M.release();
+    EF_API_END
+}
+
+int ef_mesh_destroy(ef_mesh mesh) {
+    EF_API_BEGIN
+    if (!mesh) return EF_OK;
+    (void)hipSetDevice(mesh->ctx->device);
+    if (mesh->stream) (void)hipStreamSynchronize(mesh->stream);
+    hipStream_t s = mesh->own_stream;
+    delete mesh;
+    if (s) (void)hipStreamDestroy(s);
+    EF_API_END
+}
+
+int ef_mesh_footprint(const ef_mesh_desc* desc, int precision, int max_rhs, size_t* bytes) {
+    EF_API_BEGIN
+    if (!desc || !bytes) throw EfError(EF_ERR_ARG, "null argument");
+    Plan P;
+    plan_tree(P, *desc);
+    const long long N = P.N;
+    long long elems = P.T_elems + P.S_elems + P.Xi_elems + P.H_elems + P.work_X + P.work_M + P.work_U;
+    elems += (long long)P.leaves.size() * 16 * N * N;  // leaf DtN upper bound (one per leaf)
+    elems += P.vec_rows * std::max(max_rhs, 1);
+    elems += 2LL * std::max(max_rhs, 1) * (long long)P.leaves.size() * N * N;  // f and u staging
+    (void)precision;
+    *bytes = (size_t)elems * sizeof(double);
+    EF_API_END
+}
+
+int ef_mesh_info(ef_mesh mesh, int* info) {
+    EF_API_BEGIN
+    if (!mesh || !info) throw EfError(EF_ERR_ARG, "null argument");
+    const Plan& P = mesh->P;
+    int nint = 0;
+    for (auto& nd : P.nodes) nint += nd.leaf ? 0 : 1;
+    info[0] = (int)P.nodes.size();
+    info[1] = (int)P.leaves.size();
+    info[2] = P.nodes[0].s;
+    info[3] = P.maxdepth;
+    info[4] = P.N;
+    info[5] = nint;
+    info[6] = P.max_n;
+    info[7] = 0;
+    EF_API_END
+}
+
+int ef_mesh_nodes(ef_mesh mesh, int* info, double* bounds) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null argument");
+    const Plan& P = mesh->P;
+    for (size_t i = 0; i < P.nodes.size(); i++) {
+        const NodeP& nd = P.nodes[i];
+        if (info) {
+            int v[12] = {nd.level, nd.ix, nd.iy, nd.leaf ? 1 : 0, nd.leaf_index, nd.tag, nd.s,
+                         nd.child[0], nd.child[1], nd.child[2], nd.child[3], nd.parent};
+            std::memcpy(info + 12 * i, v, sizeof(v));
+        }
+        if (bounds) {
+            bounds[4 * i + 0] = nd.xl; bounds[4 * i + 1] = nd.xu;
+            bounds[4 * i + 2] = nd.yl; bounds[4 * i + 3] = nd.yu;
+        }
+    }
+    EF_API_END
+}
+
+int ef_mesh_set_stream(ef_mesh mesh, void* hip_stream) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    mesh->stream = hip_stream ? (hipStream_t)hip_stream : mesh->own_stream;
+    EF_API_END
+}
+
+int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    if ((cache_operators != 0) != mesh->cache_operators && mesh->built)
+        throw EfError(EF_ERR_STATE, "cache-operators must be set before the build stage");
+    mesh->cache_operators = cache_operators != 0;
+    if ((homogeneous_rhs != 0) != mesh->homogeneous_rhs) mesh->sweeps.clear();
+    mesh->homogeneous_rhs = homogeneous_rhs != 0;
+    EF_API_END
+}
+
+int ef_build(ef_mesh mesh, const ef_coeffs* coeffs, int precision) {
+    EF_API_BEGIN
+    if (!mesh || !coeffs) throw EfError(EF_ERR_ARG, "null argument");
+    if (coeffs->solver_type != EF_SOLVER_FISHPACK && coeffs->solver_type != EF_SOLVER_FIVE_POINT)
+        throw EfError(EF_ERR_ARG, "unknown solver type");
+    if (precision != EF_PRECISION_FP64) throw EfError(EF_ERR_ARG, "only FP64 precision is available in this build");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    mesh->built = false;
+    mesh->sweeps.clear();
+    mesh->k_up = 0;
+    mesh->precision = precision;
+    do_build(mesh, *coeffs);
+    EF_API_END
+}
+
+int ef_upwards_device(ef_mesh mesh, const double* f_dev, int nrhs) {
+    EF_API_BEGIN
+    if (!mesh || !f_dev) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    do_upwards(mesh, f_dev, nrhs);
+    EF_API_END
+}
+
+int ef_upwards(ef_mesh mesh, const double* f, int nrhs) {
+    EF_API_BEGIN
+    if (!mesh || !f) throw EfError(EF_ERR_ARG, "null argument");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    const size_t cnt = (size_t)nrhs * mesh->P.leaves.size() * mesh->P.N * mesh->P.N;
+    mesh->fbuf.ensure(cnt);
+    EF_HIP(hipMemcpyAsync(mesh->fbuf.p, f, cnt * sizeof(double), hipMemcpyHostToDevice, mesh->stream));
+    do_upwards(mesh, mesh->fbuf.p, nrhs);
+    EF_HIP(hipStreamSynchronize(mesh->stream));
+    EF_API_END
+}
+
+int ef_solve_device(ef_mesh mesh, const double* f_dev, const double* r_dev, const double* a_dev, const double* b_dev,
+                    double* u_dev, int nrhs) {
+    EF_API_BEGIN
+    if (!mesh || !r_dev || !a_dev || !u_dev) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    do_solve(mesh, f_dev, r_dev, a_dev, b_dev, b_dev != nullptr, u_dev, nrhs);
+    EF_API_END
+}
+
+int ef_solve(ef_mesh mesh, const double* r, const double* a, const double* b, double* u, int nrhs) {
+    EF_API_BEGIN
+    if (!mesh || !r || !a || !u) throw EfError(EF_ERR_ARG, "null argument");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    const Plan& P = mesh->P;
+    const int m = 4 * P.nodes[0].s;
+    bool robin = false;
+    if (b)
+        for (int i = 0; i < m; i++)
+            if (b[i] != 0.0) { robin = true; break; }
+    mesh->rbuf.ensure((size_t)m * nrhs);
+    mesh->abuf.ensure((size_t)m);
+    mesh->bbuf.ensure((size_t)m);
+    EF_HIP(hipMemcpyAsync(mesh->rbuf.p, r, sizeof(double) * m * nrhs, hipMemcpyHostToDevice, mesh->stream));
+    EF_HIP(hipMemcpyAsync(mesh->abuf.p, a, sizeof(double) * m, hipMemcpyHostToDevice, mesh->stream));
+    if (robin) EF_HIP(hipMemcpyAsync(mesh->bbuf.p, b, sizeof(double) * m, hipMemcpyHostToDevice, mesh->stream));
+    const size_t cnt = (size_t)nrhs * P.leaves.size() * P.N * P.N;
+    mesh->ubuf.ensure(cnt);
+    const double* f = mesh->homogeneous_rhs ? nullptr : mesh->f_last;
+    do_solve(mesh, f, mesh->rbuf.p, mesh->abuf.p, robin ? mesh->bbuf.p : nullptr, robin, mesh->ubuf.p, nrhs);
+    EF_HIP(hipMemcpyAsync(u, mesh->ubuf.p, cnt * sizeof(double), hipMemcpyDeviceToHost, mesh->stream));
+    EF_HIP(hipStreamSynchronize(mesh->stream));
+    EF_API_END
+}
+
+int ef_mesh_sync(ef_mesh mesh) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    EF_HIP(hipStreamSynchronize(mesh->stream));
+    EF_API_END
+}
+
+int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int* rows, int* cols) {
+    EF_API_BEGIN
+    if (!mesh || !rows || !cols) throw EfError(EF_ERR_ARG, "null argument");
+    ef_mesh_s* M = mesh;
+    Plan& P = M->P;
+    if (node < 0 || node >= (int)P.nodes.size()) throw EfError(EF_ERR_ARG, "node index out of range");
+    const NodeP& nd = P.nodes[node];
+    EF_HIP(hipSetDevice(M->ctx->device));
+    EF_HIP(hipStreamSynchronize(M->stream));
+    const int n = nd.n;
+    auto copy_mat = [&](const double* src, int r, int c) {
+        *rows = r; *cols = c;
+        if (out) {
+            EF_HIP(hipMemcpyAsync(out, src, sizeof(double) * r * c, hipMemcpyDeviceToHost, M->stream));
+            EF_HIP(hipStreamSynchronize(M->stream));
+        }
+    };
+    auto need_built = [&]() { if (!M->built) throw EfError(EF_ERR_STATE, "operators exist after the build stage"); };
+    auto need_internal = [&]() { if (nd.leaf) throw EfError(EF_ERR_ARG, "leaf nodes carry no S, X, H or w"); };
+    const int k = std::max(M->kcap, 1);
+    auto copy_vec = [&](long long off, int len) {
+        *rows = len; *cols = 1;
+        if (!out) return;
+        if (M->kcap == 0) throw EfError(EF_ERR_STATE, "no vectors yet");
+        if (rhs < 0 || rhs >= M->kcap) throw EfError(EF_ERR_ARG, "rhs out of range");
+        std::vector<double> tmp((size_t)len * k);
+        EF_HIP(hipMemcpyAsync(tmp.data(), M->vec.p + off * k, sizeof(double) * len * k, hipMemcpyDeviceToHost, M->stream));
+        EF_HIP(hipStreamSynchronize(M->stream));
+        for (int i = 0; i < len; i++) out[i] = tmp[(size_t)i * k + rhs];
+    };
+    switch (which) {
+        case 0: {  // T (coarsened if the node was tagged, as coarsen_ leaves it)
+            need_built();
+            const int m = 4 * (nd.s >> nd.tag);
+            copy_mat(merge_T_ptr(M, node), m, m);
+            break;
+        }
+        case 1: need_built(); need_internal(); copy_mat(M->Sar.p + nd.S_off, 4 * n, 8 * n); break;
+        case 2: {  // X: re-assembled from the children's stored DtN (mergeX_)
+            need_built(); need_internal();
+            DevBuf<double> tX, tS, tXi, tH, tT;
+            tX.alloc(16LL * n * n); tS.alloc(32LL * n * n); tXi.alloc(16LL * n * n); tH.alloc(16LL * n * n);
+            tT.alloc(64LL * n * n);
+            MergeDesc md{};
+            md.n = n;
+            for (int c = 0; c < 4; c++) md.Tc[c] = merge_T_ptr(M, nd.child[c]);
+            md.X = tX.p; md.S = tS.p; md.Xi = tXi.p; md.H = tH.p; md.T = tT.p;
+            DevBuf<MergeDesc> dmd;
+            dmd.upload(std::vector<MergeDesc>{md}, M->stream);
+            hipLaunchKernelGGL(merge_assemble_kernel, dim3(blocks_for(144LL * n * n), 1), dim3(256), 0, M->stream, dmd.p);
+            EF_LAUNCH_CHECK();
+            EF_HIP(hipStreamSynchronize(M->stream));
+            copy_mat(tX.p, 4 * n, 4 * n);
+            break;
+        }
+        case 3: {  // H: dense 8n x 4n of mergeT_ from the four 2n x 2n child blocks
+            need_built(); need_internal();
+            *rows = 8 * n; *cols = 4 * n;
+            if (out) {
+                std::vector<double> hc(16LL * n * n);
+                EF_HIP(hipMemcpyAsync(hc.data(), M->Har.p + nd.H_off, sizeof(double) * hc.size(), hipMemcpyDeviceToHost,
+                                      M->stream));
+                EF_HIP(hipStreamSynchronize(M->stream));
+                static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
+                std::fill(out, out + 32LL * n * n, 0.0);
+                for (int c = 0; c < 4; c++)
+                    for (int rr = 0; rr < 2 * n; rr++)
+                        for (int slot = 0; slot < 2; slot++)
+                            for (int j = 0; j < n; j++)
+                                out[(long long)(c * 2 * n + rr) * 4 * n + CHIF[c][slot] * n + j] =
+                                    hc[(long long)c * 4 * n * n + (long long)rr * 2 * n + slot * n + j];
+            }
+            break;
+        }
+        case 5: copy_vec(nd.g_off, 4 * nd.s); break;
+        case 7: {  // h, coarsened in place by coarsenUpwards_ for tagged nodes
+            const int len = 4 * nd.s;
+            if (!out) { *rows = len >> nd.tag; *cols = 1; break; }
+            std::vector<double> full(len);
+            copy_vec(nd.h_off, len);
+            std::copy(out, out + len, full.begin());
+            int cur = len;
+            for (int t = 0; t < nd.tag; t++) {
+                const int side = cur / 4;
+                std::vector<double> nx(cur / 2);
+                for (int sd = 0; sd < 4; sd++)
+                    for (int i = 0; i < side / 2; i++)
+                        nx[sd * side / 2 + i] = 0.5 * full[sd * side + 2 * i] + 0.5 * full[sd * side + 2 * i + 1];
+                full = nx;
+                cur /= 2;
+            }
+            std::copy(full.begin(), full.end(), out);
+            *rows = cur; *cols = 1;
+            break;
+        }
+        case 8: need_internal(); copy_vec(nd.w_off, 4 * n); break;
+        case 4: case 6: throw EfError(EF_ERR_ARG, "leaf u and f are returned by ef_solve / given to ef_upwards");
+        default: throw EfError(EF_ERR_ARG, "unknown operator code");
+    }
+    EF_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// ef_device.hpp -- descriptors shared by the host planner and the gfx950 kernels.
+//
+// Layout conventions (all row-major, FP64 unless stated):
+//  * a boundary vector / DtN block order is WESN, each side ordered by increasing
+//    coordinate (src/Patches/FiniteVolume/FiniteVolumeSolver.cpp:296-353);
+//  * a leaf cell vector is indexed j + i*N, x-major with y fastest
+//    (FiniteVolumeSolver.cpp:17-19);
+//  * an internal node tau with child interface size n stores
+//      S    4n x 8n   interior potential from WESN boundary data (columns already in
+//                     WESN order, i.e. after reorderOperators_, HPSAlgorithm.hpp:979)
+//      Xi   4n x 4n   inverse of the interface Schur complement X (mergeX_, :870)
+//      H    4 blocks of 2n x 2n, child c's tau-side rows x its two interface columns
+//      T    8n x 8n   DtN of tau in WESN order
+//  * multi-RHS vectors (h, g, w, u_int) live in per-batch arenas: node vector of
+//    length L starts at off*k and element (i, rhs) is at off*k + i*k + rhs;
+//  * leaf f / u for k right-hand sides: [rhs][leaf][cell].
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+namespace efk {
+
+enum Side { W = 0, E = 1, S = 2, N = 3 };
+
+// Batched GEMM problem: D = alpha * (A B + A2 B2) + beta * Cin, row-major.
+// The optional second segment lets one problem read its K range from two places.
+struct GemmProb {
+    const double* A;  const double* B;  int lda, ldb, K;
+    const double* A2; const double* B2; int lda2, ldb2, K2;
+    const double* Cin; int ldcin;
+    double* D; int ldd;
+    int M, N;
+    double alpha, beta;
+};
+
+// FP32 variant used by the mixed-precision merge products.
+struct GemmProbF {
+    const float* A;  const float* B;  int lda, ldb, K;
+    const float* A2; const float* B2; int lda2, ldb2, K2;
+    const float* Cin; int ldcin;
+    float* D; int ldd;
+    int M, N;
+    float alpha, beta;
+};
+
+// Blocked Gauss-Jordan elimination with partial pivoting of [X | R1 | R2]:
+// on exit X is reduced (not kept), R1 <- X^-1 R1 and R2 <- X^-1 R2.
+// The merge uses R1 = S_RHS (-> S) and R2 = I (-> X^-1); the root boundary solve
+// uses R1 = right-hand sides and no R2.
+struct GJDesc {
+    int m;                 // order of X
+    double* X;  int ldx;
+    double* R1; int w1, ld1;
+    double* R2; int w2, ld2;
+    double* M;             // m x nb multipliers (work)
+    double* U;             // nb x (m + w1 + w2) pivot rows (work), ld = ldu
+    int ldu;
+    int* piv;              // nb pivot rows of the current panel (work)
+    int* info;             // first zero pivot (1-based column) or 0
+};
+
+// One 4-to-1 merge (HPSAlgorithm::merge4to1, HPSAlgorithm.hpp:497-518).
+struct MergeDesc {
+    int n;                 // child interface size after coarsening
+    const double* Tc[4];   // children's DtN used by the merge, each 4n x 4n (coarsened)
+    double* X;             // 4n x 4n work: Schur complement, reduced in place
+    double* S;             // 4n x 8n
+    double* Xi;            // 4n x 4n
+    double* H;             // 4 x (2n x 2n)
+    double* T;             // 8n x 8n
+};
+
+// coarsen_ step (HPSAlgorithm.hpp:707-741): T (4m x 4m) -> L21 T L12 (2m x 2m) per side block.
+struct CoarsenDesc {
+    const double* src;  // 4m x 4m
+    double* dst;        // 2m x 2m ... i.e. (4 * m/2)^2
+    int m;              // fine side length
+};
+
+// Leaf class for constant coefficients: every leaf of a class has the same dx, dy and
+// stencil, hence the same DtN matrix and eigen-decomposition.
+struct LeafClass {
+    double cx, cy, lam;  // c_W = c_E = cx, c_S = c_N = cy, centre = -2cx - 2cy + lam
+    double dx, dy;
+    double* T;           // 4N x 4N
+};
+
+// Per-leaf view used by the leaf sweep kernels.
+struct LeafDesc {
+    int cls;        // constant-coefficient class (or -1 for the block-tridiagonal path)
+    int node;       // node id
+    long long h_off; // row offset of h in the vector arena (4N rows)
+    long long g_off; // row offset of g
+    const double* coef;  // variable coefficients: 6 N*N arrays (alpha, bW, bE, bS, bN, lambda) or null
+    double* Sinv;        // variable coefficients: N blocks of N x N (block-tridiagonal factors) or null
+    double dx, dy;
+};
+
+// Upward family step (upwards4to1, HPSAlgorithm.hpp:529-551) for one merge.
+struct UpDesc {
+    int n;
+    int tag[4];             // coarsenings of each child's h
+    int clen[4];            // child vector length before coarsening (4 * size)
+    long long ch_off[4];    // child h offsets (rows)
+    long long hd_off;       // hDiff work (4n rows)
+    long long w_off;        // w (4n rows)
+    long long h_off;        // tau h (8n rows)
+    const double* Xi;       // 4n x 4n
+    const double* H;        // 4 x (2n x 2n)
+};
+
+// Downward family step (split1to4, HPSAlgorithm.hpp:562-577) for one merge.
+struct DownDesc {
+    int n;
+    int tag;                // tau's own n_coarsens (uncoarsen_)
+    long long gc_off;       // tau g as delivered by its parent (coarse, 4*(s>>tag) rows)
+    long long g_off;        // tau g at full size (8n rows)
+    long long w_off;        // w (4n)
+    long long ui_off;       // interior potential work (4n)
+    long long cg_off[4];    // children's coarse-g offsets (each 4n)
+    const double* S;        // 4n x 8n
+};
+
+// One interpolation step on vectors in an arena (uncoarsen_: L12 of fine size nfine per side).
+struct VecStepDesc {
+    long long src_off, dst_off;
+    int nfine;
+};
+
+}  // namespace efk

@@ -1,0 +1,33 @@
+// ef_kernels.hpp -- kernel declarations and launch constants (host + device).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "ef_device.hpp"
+
+namespace efk {
+
+constexpr int GJ_NB = 32;          // Gauss-Jordan panel width
+constexpr int GJ_LDS_ROWS = 448;   // panel rows kept in LDS (448 x 33 x 8 B)
+
+__global__ void gemm_f64_kernel(const GemmProb* probs);
+__global__ void gj_panel_lds_kernel(const GJDesc* descs, int k);
+__global__ void gj_panel_global_kernel(const GJDesc* descs, int k);
+__global__ void gj_swap_kernel(const GJDesc* descs, int k);
+__global__ void gj_update_kernel(const GJDesc* descs, int k);
+__global__ void merge_assemble_kernel(const MergeDesc* descs);
+__global__ void coarsen_kernel(const CoarsenDesc* descs);
+__global__ void fd_basis_kernel(double* Q, int Nn);
+__global__ void fd_dinv_kernel(const LeafClass* cls, double* Dinv, int Nn);
+__global__ void class_dtn_kernel(const LeafClass* cls, const double* Q, const double* Dinv, int Nn);
+__global__ void leaf_fd_kernel(const LeafDesc* leaves, const LeafClass* cls, const double* Q, const double* Dinv,
+                               int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
+                               const double* garena, double* harena, double* u);
+__global__ void up_gather_kernel(const UpDesc* descs, double* harena, int k);
+__global__ void uncoarsen_kernel(const VecStepDesc* descs, double* garena, int k);
+__global__ void down_scatter_kernel(const DownDesc* descs, double* garena, int k);
+__global__ void root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k);
+__global__ void root_robin_assemble_kernel(const double* T, const double* a, const double* b, const double* r,
+                                           const double* h, double* A, double* rhs, int m, int k);
+__global__ void fill_kernel(double* p, double v, long long n);
+__global__ void copy_kernel(const double* src, double* dst, long long n);
+
+}  // namespace efk

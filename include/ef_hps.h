@@ -1,0 +1,139 @@
+/*
+ * ef_hps.h -- C ABI of the MI355X HPS factor-and-solve library (libefhps.so).
+ *
+ * The library replaces, for one GPU (one MPI rank), the numerical core under the
+ * reference's C++ API:
+ *   HPSAlgorithm::buildStage / upwardsStage / solveStage   (src/HPSAlgorithm.hpp:120-445)
+ *   FiniteVolumeSolver::solve / mapD2N / buildD2N / particularNeumannData
+ *                                   (src/Patches/FiniteVolume/FiniteVolumeSolver.cpp:25-462)
+ *   Matrix/Vector dgesv, dgemm, dgemv wrappers on that path (src/Matrix.hpp:789-953)
+ *   L21 / L12 interpolation operators                   (src/SpecialMatrices.hpp:93-173)
+ * The C++ drop-in headers (include/EllipticForest.hpp) and the Python host mirror
+ * (ellipticforest_amd/hps.py) call only these entry points.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a nonzero EF_ERR_* code; no exception
+ *    crosses the ABI.  ef_last_error() returns the message of the calling thread's
+ *    last failure.
+ *  - Host buffers are borrowed for the duration of a call.  Device arenas (operators,
+ *    vectors) are owned by the mesh.  Device-pointer variants (*_device) take memory
+ *    the caller allocated on the mesh's device and are ordered on the mesh's stream.
+ *  - A leaf cell vector is indexed j + i*N (x-major, y fastest).  Boundary vectors are
+ *    WESN blocks, each ordered by increasing coordinate.
+ *  - Multi-RHS arrays are right-hand-side major: f and u are [rhs][leaf][cell] with
+ *    leaves in z-order (pre-order), r is [rhs][4M] where M is the root side.
+ *  - Thread-safe across distinct meshes, not within one mesh.
+ */
+#ifndef EF_HPS_H_
+#define EF_HPS_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EF_OK 0
+#define EF_ERR_ARG 1        /* invalid argument (maps to std::invalid_argument)  */
+#define EF_ERR_STATE 2      /* call out of order, e.g. solve before build         */
+#define EF_ERR_DEVICE 3     /* HIP runtime failure or out of device memory        */
+#define EF_ERR_SINGULAR 4   /* zero pivot in an interface or root-boundary solve  */
+#define EF_ERR_INTERNAL 5
+
+/* Leaf patch solver (FiniteVolumeSolverType, FiniteVolumeSolver.hpp:17-20). */
+#define EF_SOLVER_FIVE_POINT 0   /* alpha*div(beta grad u) + lambda u = f, per-cell coefficients   */
+#define EF_SOLVER_FISHPACK 1     /* u_xx + u_yy + lambda0 u = f (alpha, beta ignored, :224-290)      */
+
+/* Arithmetic of the operators. */
+#define EF_PRECISION_FP64 0
+#define EF_PRECISION_MIXED 1     /* FP32 merge products, FP64 interface solve and refinement */
+
+typedef struct ef_ctx_s* ef_ctx;
+typedef struct ef_mesh_s* ef_mesh;
+
+/* Context: one per device. */
+int ef_ctx_create(int device, ef_ctx* out);
+int ef_ctx_destroy(ef_ctx ctx);
+const char* ef_last_error(void);
+int ef_version(void);
+
+/* Quadtree mesh given by its leaves in z-order (pre-order, children SW, SE, NW, NE),
+ * each leaf an N x N cell patch; child grids are parent quadrants with the same N
+ * (FiniteVolumeNodeFactory.cpp:15-67).  Replaces the p4est-backed
+ * Quadtree/Mesh storage of the path (src/Quadtree.hpp:118-196, src/Mesh.hpp:111-180). */
+typedef struct {
+    int nleaves;
+    const int* level;   /* leaf level (root = 0) */
+    const int* ix;      /* leaf x index at its level, 0 .. 2^level - 1 */
+    const int* iy;
+    int N;              /* cells per leaf side (>= 4, even) */
+    double xl, xu, yl, yu;
+} ef_mesh_desc;
+
+/* Mesh::refineByFunction (src/Mesh.hpp:111-180) without p4est: uniform quadrants at
+ * min_level, recursive refinement while level < max_level and fn marks any of the
+ * quadrant's nx*ny cell centres, then 2:1 face+corner balance (P4EST_CONNECT_CORNER).
+ * fn receives one quadrant's cell centres (index j + i*ny) and returns nonzero to refine.
+ * Leaves come back in z-order.  With capacity == 0 only *nleaves is set; with a
+ * capacity smaller than the leaf count EF_ERR_ARG is returned and *nleaves is set. */
+typedef int (*ef_refine_fn)(const double* x, const double* y, int npts, void* user);
+int ef_refine_by_function(ef_refine_fn fn, void* user, int nx, int ny, double xl, double xu, double yl, double yu,
+                          int min_level, int max_level, int capacity, int* level, int* ix, int* iy, int* nleaves);
+
+int ef_mesh_create(ef_ctx ctx, const ef_mesh_desc* desc, ef_mesh* out);
+int ef_mesh_destroy(ef_mesh mesh);
+/* Device bytes the mesh needs for operators plus vectors of max_rhs right-hand sides. */
+int ef_mesh_footprint(const ef_mesh_desc* desc, int precision, int max_rhs, size_t* bytes);
+/* info[0..7] = nnodes, nleaves, root side M, max depth, N, internal nodes, max interface n, 0 */
+int ef_mesh_info(ef_mesh mesh, int* info);
+/* Node table in pre-order: per node 12 ints = level, ix, iy, is_leaf, leaf_index,
+ * n_coarsens, grid nx, child0..3, parent; and 4 doubles of grid bounds. */
+int ef_mesh_nodes(ef_mesh mesh, int* info, double* bounds);
+/* Use a caller-owned hipStream_t (passed as void*); NULL restores the mesh's own stream. */
+int ef_mesh_set_stream(ef_mesh mesh, void* hip_stream);
+
+/* Stage options of EllipticForestApp (HPSAlgorithm.hpp:134, 532, 587, 1202). */
+int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs);
+
+/* Leaf coefficients for ef_build.
+ *  FISHPACK:   lambda0 = lambda(0,0) (FiniteVolumeSolver.cpp:262).
+ *  FIVE_POINT: cell != NULL -> per leaf 6 arrays of N*N doubles, in z-order:
+ *              alpha(x_i,y_j), beta(x_i -+ dx/2, y_j) (W, E), beta(x_i, y_j -+ dy/2) (S, N),
+ *              lambda(x_i,y_j), sampled as FiniteVolumeSolver.cpp:64-78 samples them;
+ *              cell == NULL -> constant alpha0, beta0, lambda0. */
+typedef struct {
+    int solver_type;
+    double alpha0, beta0, lambda0;
+    const double* cell;
+} ef_coeffs;
+
+/* buildStage: leaf DtN operators and all 4-to-1 merges, level by level. */
+int ef_build(ef_mesh mesh, const ef_coeffs* coeffs, int precision);
+
+/* upwardsStage with load data f ([rhs][leaf][N*N], host) for nrhs right-hand sides. */
+int ef_upwards(ef_mesh mesh, const double* f, int nrhs);
+int ef_upwards_device(ef_mesh mesh, const double* f_dev, int nrhs);
+
+/* solveStage with the general boundary condition a u + b du/dn = r at root face
+ * points (HPSAlgorithm.hpp:343-445).  a and b have 4M entries shared by the batch,
+ * r is [rhs][4M]; u receives [rhs][leaf][N*N].  b == NULL means Dirichlet (b = 0). */
+int ef_solve(ef_mesh mesh, const double* r, const double* a, const double* b, double* u, int nrhs);
+/* Device variant: f_dev is the load of the preceding upwards call (the leaf solves
+ * use it, as leafSolve uses vectorF), all pointers on the mesh's device. */
+int ef_solve_device(ef_mesh mesh, const double* f_dev, const double* r_dev, const double* a_dev,
+                    const double* b_dev, double* u_dev, int nrhs);
+
+/* Copy one stored operator or vector of a node to the host, as the reference patch
+ * holds it after the stages: which = 0 T, 1 S, 2 X, 3 H, 4 u, 5 g, 6 f, 7 h, 8 w
+ * (FiniteVolumePatch accessors, FiniteVolumePatch.cpp:30-68); rhs selects the column.
+ * With out == NULL only rows/cols are returned. */
+int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int* rows, int* cols);
+
+/* Wait for all work on the mesh's stream. */
+int ef_mesh_sync(ef_mesh mesh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EF_HPS_H_ */

@@ -1,0 +1,133 @@
+// hip_cpu_shim.cpp -- runtime of the CPU emulation (see hip_cpu_shim.h).
+#include "hip_cpu_shim.h"
+
+#include <map>
+#include <mutex>
+
+thread_local emu_idx threadIdx;
+emu_idx blockIdx;
+dim3 blockDim, gridDim;
+
+static const size_t PAGE = 4096;
+static std::map<void*, std::pair<void*, size_t>> g_maps;
+static std::mutex g_mu;
+
+// Each buffer sits inside its own reservation with 1 GiB of PROT_NONE on both sides, so
+// a stray access anywhere near it faults.  EF_EMU_GUARD=front aligns the buffer to the
+// front guard (catches underruns at once); the default aligns its end to the back guard.
+static const size_t GUARD = 1ULL << 30;
+static void* guarded_alloc(size_t bytes, void** base_out, size_t* len_out) {
+    const size_t body = ((bytes + PAGE - 1) / PAGE) * PAGE;
+    const size_t len = body + 2 * GUARD;
+    char* base = (char*)mmap(nullptr, len, PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE, -1, 0);
+    if (base == MAP_FAILED) return nullptr;
+    char* rw = base + GUARD;
+    if (mprotect(rw, body, PROT_READ | PROT_WRITE) != 0) return nullptr;
+    const char* mode = getenv("EF_EMU_GUARD");
+    const bool front = mode && std::strcmp(mode, "front") == 0;
+    *base_out = base;
+    *len_out = len;
+    std::memset(rw, 0xff, body);  // NaN pattern: reads of unwritten memory show up in results
+    if (front) return rw;
+    return rw + body - ((bytes + 7) / 8) * 8;  // end of the buffer touches the back guard
+}
+
+hipError_t hipMalloc(void** p, size_t bytes) {
+    void* base;
+    size_t len;
+    void* q = guarded_alloc(bytes ? bytes : 8, &base, &len);
+    if (!q) return 2;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_maps[q] = {base, len};
+    *p = q;
+    return 0;
+}
+hipError_t hipFree(void* p) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_maps.find(p);
+    if (it == g_maps.end()) { fprintf(stderr, "hipFree of unknown pointer\n"); abort(); }
+    munmap(it->second.first, it->second.second);
+    g_maps.erase(it);
+    return 0;
+}
+
+// ----- workgroup emulation -------------------------------------------------------
+static const int NT = 256;
+static std::barrier<>* g_block_bar = nullptr;
+static std::barrier<>* g_wave_bar[NT / 64];
+static double g_mfma_a[NT], g_mfma_b[NT];
+static void* g_dyn = nullptr;
+static void* g_dyn_base = nullptr;
+static size_t g_dyn_len = 0;
+
+void __syncthreads() { g_block_bar->arrive_and_wait(); }
+void emu_wave_sync() { g_wave_bar[threadIdx.x / 64]->arrive_and_wait(); }
+void* emu_dyn_smem() { return g_dyn; }
+
+emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int) {
+    const int t = threadIdx.x, w = t / 64 * 64, l = t % 64;
+    g_mfma_a[t] = a;
+    g_mfma_b[t] = b;
+    emu_wave_sync();
+    emu_d4 out = c;
+    for (int v = 0; v < 4; v++) {
+        const int row = 4 * v + l / 16, col = l % 16;
+        double s = 0.0;
+        for (int k = 0; k < 4; k++) s += g_mfma_a[w + row + 16 * k] * g_mfma_b[w + col + 16 * k];
+        out[v] += s;
+    }
+    emu_wave_sync();
+    return out;
+}
+
+// Persistent pool of 256 workers (one per lane of a block).  Per block the main thread
+// publishes the body, releases the workers through g_start and waits on g_end.  The
+// block and wave barriers are rebuilt per block because a finishing thread drops out
+// of them, as an exited wave stops counting at s_barrier.
+static std::barrier<>* g_start = nullptr;
+static std::barrier<>* g_end = nullptr;
+static const std::function<void()>* g_body = nullptr;
+static bool g_quit = false;
+static std::vector<std::thread>* g_pool = nullptr;
+
+static void pool_init() {
+    if (g_pool) return;
+    g_start = new std::barrier<>(NT + 1);
+    g_end = new std::barrier<>(NT + 1);
+    g_pool = new std::vector<std::thread>();
+    for (int t = 0; t < NT; t++)
+        g_pool->emplace_back([t]() {
+            threadIdx = {(unsigned)t, 0, 0};
+            for (;;) {
+                g_start->arrive_and_wait();
+                if (g_quit) return;
+                (*g_body)();
+                g_wave_bar[t / 64]->arrive_and_drop();
+                g_block_bar->arrive_and_drop();
+                g_end->arrive_and_wait();
+            }
+        });
+    for (auto& th : *g_pool) th.detach();
+}
+
+void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body) {
+    if (b.x != NT || b.y != 1 || b.z != 1) { fprintf(stderr, "emulator supports 256-thread blocks only\n"); abort(); }
+    pool_init();
+    gridDim = g;
+    blockDim = b;
+    if (shm) g_dyn = guarded_alloc(shm, &g_dyn_base, &g_dyn_len);
+    g_body = &body;
+    for (unsigned z = 0; z < g.z; z++)
+        for (unsigned y = 0; y < g.y; y++)
+            for (unsigned x = 0; x < g.x; x++) {
+                blockIdx = {x, y, z};
+                std::barrier<> bb(NT);
+                g_block_bar = &bb;
+                for (int i = 0; i < NT / 64; i++) g_wave_bar[i] = new std::barrier<>(64);
+                if (shm) std::memset(g_dyn, 0xff, shm);
+                g_start->arrive_and_wait();
+                g_end->arrive_and_wait();
+                for (int i = 0; i < NT / 64; i++) delete g_wave_bar[i];
+            }
+    if (shm) { munmap(g_dyn_base, g_dyn_len); g_dyn = nullptr; }
+}

@@ -1,0 +1,78 @@
+// hip_cpu_shim.h -- CPU emulation of the HIP subset used by ellipticforest_amd/csrc,
+// for checking kernel indexing on the host (test infrastructure only).
+//
+// * every device allocation is an mmap'd block whose end touches a PROT_NONE page,
+//   so any read or write past the end of a buffer faults at once on the CPU;
+// * a workgroup is 256 std::threads with a block barrier (__syncthreads) and a
+//   64-lane barrier per wave; the f64 16x16x4 MFMA is emulated with the lane layout
+//   of gfx950 (A[l%16][l/16], B[l/16][l%16], D slot v of lane l = D[4v + l/16][l%16]);
+// * blocks run one after another; dynamic shared memory is a guarded buffer too.
+#pragma once
+#include <sys/mman.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __constant__
+#define __launch_bounds__(x)
+using std::max;
+using std::min;
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx { unsigned x, y, z; };
+extern thread_local emu_idx threadIdx;
+extern emu_idx blockIdx;
+extern dim3 blockDim, gridDim;
+void __syncthreads();
+void emu_wave_sync();
+void* emu_dyn_smem();
+
+typedef int hipError_t;
+typedef void* hipStream_t;
+enum { hipSuccess = 0 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize };
+struct hipFuncAttributes { size_t sharedSizeBytes = 0; };
+inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
+hipError_t hipMalloc(void** p, size_t bytes);
+template <class T> hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc((void**)p, bytes); }
+hipError_t hipFree(void* p);
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memcpy(d, s, n); return 0; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return 0; }
+inline hipError_t hipGetLastError() { return 0; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return 0; }
+inline hipError_t hipSetDevice(int) { return 0; }
+inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (hipStream_t)0x1; return 0; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+inline hipError_t hipDeviceSynchronize() { return 0; }
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { a->sharedSizeBytes = 0; return 0; }
+inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
+
+typedef double emu_d4 __attribute__((ext_vector_type(4)));
+emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);
+#define __builtin_amdgcn_mfma_f64_16x16x4f64 emu_mfma_f64
+
+void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body);
+template <class F, class... A>
+void emu_launch(F* f, dim3 g, dim3 b, size_t shm, A... args) {
+    emu_run_grid(g, b, shm, [=]() { f(args...); });
+}
+#define hipLaunchKernelGGL(k, g, b, shm, s, ...) emu_launch(k, dim3(g), dim3(b), shm, ##__VA_ARGS__)

@@ -1,0 +1,189 @@
+"""GPU parity: the HIP library, called through the C ABI, against the oracle restatement
+and the independent global-solve fixtures.
+
+Tolerances (FP64 path): relative max-norm 1e-10 against the oracle on the solution and
+1e-11 on stored operators; the discretisation error must equal the oracle's to 1e-10
+relative.  Integer/index work (mesh, node tables) is compared exactly.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mesh as omesh
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+U = lambda x, y: np.sin(x) + np.sin(y)
+F = lambda x, y: -(np.sin(x) + np.sin(y))
+TOL_U = 1e-10
+TOL_OP = 1e-11
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def solve_both(ctx, leaves, N, bounds, lam=0.0, check_ops=False, nodes=None):
+    import ellipticforest_amd as ef
+    O = oracle.Forest(leaves, N, bounds, solver=oracle.FISHPACK, lambda0=lam)
+    O.build()
+    f = O.sample_rhs(F)
+    O.upwards(f)
+    r, a, b = O.boundary_data(lambda s, x, y: (U(x, y), 1.0, 0.0))
+    uo = O.solve(r, a, b)
+    D = ef.DeviceMesh(ctx, leaves, N, bounds)
+    D.build(lambda0=lam)
+    D.upwards(f)
+    ug = D.solve(r, a)
+    if check_ops:
+        for node in (nodes if nodes is not None else range(D.nnodes)):
+            assert rel(D.get(node, "T"), O.get(node, "T")) < TOL_OP, ("T", node)
+            if not D.node_info[node, 3]:
+                for w in ("S", "X", "H"):
+                    assert rel(D.get(node, w), O.get(node, w)) < TOL_OP, (w, node)
+                assert rel(D.get(node, "w"), O.get(node, "w")) < TOL_OP, ("w", node)
+            assert rel(D.get(node, "g"), O.get(node, "g")) < TOL_U, ("g", node)
+    return O, D, uo, ug
+
+
+def test_uniform_L1_N8(ctx):
+    say("uniform level 1, N=8, [-1,1]^2, FISHPACK")
+    O, D, uo, ug = solve_both(ctx, omesh.uniform(1), 8, (-1, 1, -1, 1), check_ops=True)
+    assert rel(ug, uo) < TOL_U
+
+
+@pytest.mark.parametrize("name", ["global_5pt_L2N8_pm1", "global_5pt_L2N8_pm10", "global_5pt_L2N8_helmholtz",
+                                  "global_5pt_L3N16_pm10"])
+def test_uniform_equals_global_solve(ctx, name):
+    import ellipticforest_amd as ef
+    say(f"golden {name}")
+    d = np.load(os.path.join(GOLD, name + ".npz"))
+    level, N, bounds, lam = int(d["level"]), int(d["N"]), tuple(d["bounds"]), float(d["lam"])
+    O = oracle.Forest(omesh.uniform(level), N, bounds, solver=oracle.FISHPACK, lambda0=lam)
+    f = O.sample_rhs(F)
+    D = ef.DeviceMesh(ctx, omesh.uniform(level), N, bounds)
+    side, px, py = oracle.root_face_points(D.M, bounds)
+    D.build(lambda0=lam)
+    D.upwards(f)
+    u = D.solve(U(px, py), np.ones_like(px))
+    assert rel(u, d["u"]) < 1e-11
+
+
+def test_cfg1_operators_and_error(ctx):
+    """[-10,10]^2, uniform level 3, 16x16 cells per patch, FISHPACK Poisson."""
+    say("cfg1: uniform L3 N16 [-10,10]^2, operators of every node")
+    O, D, uo, ug = solve_both(ctx, omesh.uniform(3), 16, (-10, 10, -10, 10), check_ops=True)
+    assert rel(ug, uo) < TOL_U
+    eo = eg = 0.0
+    for k, bd in enumerate(O.leaf_bounds):
+        X, Y, _, _ = oracle.cell_centres(16, bd)
+        eo = max(eo, np.abs(uo[k] - U(X, Y)).max())
+        eg = max(eg, np.abs(ug[k] - U(X, Y)).max())
+    assert abs(eg - eo) <= 1e-10 * eo
+
+
+def test_adaptive_fixture(ctx):
+    import ellipticforest_amd as ef
+    say("adaptive fixture: |f|>1.2, levels 0..3, N=8")
+    d = np.load(os.path.join(GOLD, "oracle_fixtures.npz"))
+    leaves = [tuple(int(v) for v in l) for l in d["adaptive_leaves"]]
+    O = oracle.Forest(leaves, 8, (-10, 10, -10, 10), solver=oracle.FISHPACK)
+    f = O.sample_rhs(F)
+    D = ef.DeviceMesh(ctx, leaves, 8, (-10, 10, -10, 10))
+    side, px, py = oracle.root_face_points(D.M, (-10, 10, -10, 10))
+    D.build()
+    D.upwards(f)
+    assert rel(D.solve(U(px, py), np.ones_like(px)), d["adaptive_u"]) < TOL_U
+
+
+@pytest.mark.parametrize("maxl,N,lam", [(4, 8, 0.0), (5, 16, -1.0), (6, 8, 0.0)])
+def test_adaptive_vs_oracle(ctx, maxl, N, lam):
+    say(f"adaptive |f|>1.2 levels 0..{maxl}, N={N}, lambda={lam}")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, N, N, -10, 10, -10, 10, 0, maxl)
+    O, D, uo, ug = solve_both(ctx, leaves, N, (-10, 10, -10, 10), lam=lam, check_ops=(maxl == 4))
+    assert rel(ug, uo) < TOL_U
+
+
+def test_adaptive_linear_exactness(ctx):
+    import ellipticforest_amd as ef
+    say("adaptive levels 0..5 N=8: linear solution reproduced")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 8, 8, -10, 10, -10, 10, 0, 5)
+    D = ef.DeviceMesh(ctx, leaves, 8, (-10, 10, -10, 10))
+    D.build()
+    D.upwards(np.zeros((len(leaves), 64)))
+    lin = lambda x, y: 0.3 * x - 0.7 * y + 2.0
+    side, px, py = oracle.root_face_points(D.M, (-10, 10, -10, 10))
+    u = D.solve(lin(px, py), np.ones_like(px))
+    for k, node in enumerate(D.leaf_nodes):
+        X, Y, _, _ = oracle.cell_centres(8, tuple(D.node_bounds[node]))
+        assert np.abs(u[k] - lin(X, Y)).max() < 1e-10
+
+
+def test_multi_rhs_batch_equals_single(ctx):
+    import ellipticforest_amd as ef
+    say("multi-RHS: batch of 5 vs one at a time, adaptive levels 0..4 N=8")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 8, 8, -10, 10, -10, 10, 0, 4)
+    D = ef.DeviceMesh(ctx, leaves, 8, (-10, 10, -10, 10))
+    D.build()
+    rng = np.random.default_rng(0)
+    k = 5
+    f = rng.standard_normal((k, len(leaves), 64))
+    r = rng.standard_normal((k, 4 * D.M))
+    a = np.ones(4 * D.M)
+    D.upwards(f)
+    ub = D.solve(r, a)
+    for j in range(k):
+        D.upwards(f[j])
+        us = D.solve(r[j], a)
+        assert rel(ub[j], us) < 1e-13
+
+
+def test_homogeneous_rhs_option(ctx):
+    import ellipticforest_amd as ef
+    say("homogeneous-rhs option: leaves solve with f = 0, w skipped")
+    leaves = omesh.uniform(2)
+    O = oracle.Forest(leaves, 8, (0, 1, 0, 1), solver=oracle.FISHPACK, homogeneous_rhs=True)
+    O.build()
+    f = O.sample_rhs(F)
+    O.upwards(f)
+    r, a, b = O.boundary_data(lambda s, x, y: (U(x, y), 1.0, 0.0))
+    uo = O.solve(r, a, b)
+    D = ef.DeviceMesh(ctx, leaves, 8, (0, 1, 0, 1))
+    D.set_options(homogeneous_rhs=True)
+    D.build()
+    D.upwards(f)
+    assert rel(D.solve(r, a), uo) < TOL_U
+
+
+def test_robin_root_condition(ctx):
+    import ellipticforest_amd as ef
+    say("root condition a u + b du/dn = r with b != 0, uniform level 2 N=8")
+    leaves = omesh.uniform(2)
+    O = oracle.Forest(leaves, 8, (0, 1, 0, 1), solver=oracle.FISHPACK)
+    O.build()
+    f = O.sample_rhs(F)
+    O.upwards(f)
+    r, a, b = O.boundary_data(lambda s, x, y: (U(x, y), 1.0 + 0 * x, 0.25 + 0.1 * np.cos(x + y)))
+    uo = O.solve(r, a, b)
+    D = ef.DeviceMesh(ctx, leaves, 8, (0, 1, 0, 1))
+    D.build()
+    D.upwards(f)
+    assert rel(D.solve(r, a, b), uo) < TOL_U
+
+
+def test_cfg2_full_size(ctx):
+    """cfg2 at full size: 7,750 leaves on levels 5..7, 16x16 cells, against the oracle."""
+    import ellipticforest_amd as ef
+    say("cfg2 full size: |f|>1.2 levels 0..7, N=16, 7750 leaves")
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    leaves = ef.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 16, 16, (-10, 10, -10, 10), 0, 7)
+    assert len(leaves) == 7750
+    O, D, uo, ug = solve_both(ctx, leaves, 16, (-10, 10, -10, 10), check_ops=True, nodes=[0, 1, 2, 5, 21])
+    assert rel(ug, uo) < TOL_U
