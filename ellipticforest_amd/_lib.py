@@ -59,39 +59,45 @@ class EfError(RuntimeError):
 _lib = None
 
 
+def load_library(path):
+    """Open one build of the C ABI and declare its signatures."""
+    L = ctypes.CDLL(path)
+    L.ef_last_error.restype = ctypes.c_char_p
+    L.ef_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.ef_ctx_destroy.argtypes = [_vp]
+    L.ef_refine_by_function.argtypes = [REFINE_FN, _vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip]
+    L.ef_mesh_create.argtypes = [_vp, ctypes.POINTER(MeshDesc), ctypes.POINTER(_vp)]
+    L.ef_mesh_destroy.argtypes = [_vp]
+    L.ef_mesh_footprint.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    L.ef_mesh_info.argtypes = [_vp, _ip]
+    L.ef_mesh_nodes.argtypes = [_vp, _ip, _dp]
+    L.ef_mesh_set_stream.argtypes = [_vp, _vp]
+    L.ef_mesh_set_options.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.ef_build.argtypes = [_vp, ctypes.POINTER(Coeffs), ctypes.c_int]
+    L.ef_upwards.argtypes = [_vp, _dp, ctypes.c_int]
+    L.ef_upwards_device.argtypes = [_vp, _vp, ctypes.c_int]
+    L.ef_solve.argtypes = [_vp, _dp, _dp, _dp, _dp, ctypes.c_int]
+    L.ef_solve_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+    L.ef_get_operator.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip]
+    L.ef_mesh_sync.argtypes = [_vp]
+    return L
+
+
 def lib():
+    """The product library, libefhps.so next to this file."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (no CPU fallback exists)")
-        L = ctypes.CDLL(LIB_PATH)
-        L.ef_last_error.restype = ctypes.c_char_p
-        L.ef_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
-        L.ef_ctx_destroy.argtypes = [_vp]
-        L.ef_refine_by_function.argtypes = [REFINE_FN, _vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [
-            ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip]
-        L.ef_mesh_create.argtypes = [_vp, ctypes.POINTER(MeshDesc), ctypes.POINTER(_vp)]
-        L.ef_mesh_destroy.argtypes = [_vp]
-        L.ef_mesh_footprint.argtypes = [ctypes.POINTER(MeshDesc), ctypes.c_int, ctypes.c_int,
-                                        ctypes.POINTER(ctypes.c_size_t)]
-        L.ef_mesh_info.argtypes = [_vp, _ip]
-        L.ef_mesh_nodes.argtypes = [_vp, _ip, _dp]
-        L.ef_mesh_set_stream.argtypes = [_vp, _vp]
-        L.ef_mesh_set_options.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
-        L.ef_build.argtypes = [_vp, ctypes.POINTER(Coeffs), ctypes.c_int]
-        L.ef_upwards.argtypes = [_vp, _dp, ctypes.c_int]
-        L.ef_upwards_device.argtypes = [_vp, _vp, ctypes.c_int]
-        L.ef_solve.argtypes = [_vp, _dp, _dp, _dp, _dp, ctypes.c_int]
-        L.ef_solve_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
-        L.ef_get_operator.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip]
-        L.ef_mesh_sync.argtypes = [_vp]
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
 
 
-def check(rc):
+def check(rc, L=None):
     if rc != EF_OK:
-        raise EfError(rc, lib().ef_last_error().decode())
+        raise EfError(rc, (L or lib()).ef_last_error().decode())
 
 
 def dptr(a):
@@ -136,14 +142,18 @@ def refine_by_function(fn, nx, ny, bounds, min_level, max_level):
 
 
 class Context:
-    def __init__(self, device=0):
+    """A device context.  ``library`` selects a build of the C ABI (default: the product
+    library); the test suite passes its CPU-emulation build here."""
+
+    def __init__(self, device=0, library=None):
+        self.L = library if library is not None else lib()
         h = _vp()
-        check(lib().ef_ctx_create(device, ctypes.byref(h)))
+        check(self.L.ef_ctx_create(device, ctypes.byref(h)), self.L)
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib is not None:
-            _lib.ef_ctx_destroy(self.h)
+        if getattr(self, "h", None) is not None:
+            self.L.ef_ctx_destroy(self.h)
             self.h = None
 
 
@@ -158,31 +168,35 @@ class DeviceMesh:
         self._lv = np.array([l[0] for l in leaves], np.int32)
         self._ix = np.array([l[1] for l in leaves], np.int32)
         self._iy = np.array([l[2] for l in leaves], np.int32)
+        self.L = ctx.L
         d = MeshDesc(len(leaves), iptr(self._lv), iptr(self._ix), iptr(self._iy), N, *self.bounds)
         h = _vp()
-        check(lib().ef_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        self._check(self.L.ef_mesh_create(ctx.h, ctypes.byref(d), ctypes.byref(h)))
         self.h = h
         info = np.zeros(8, np.int32)
-        check(lib().ef_mesh_info(h, iptr(info)))
+        self._check(self.L.ef_mesh_info(h, iptr(info)))
         self.nnodes, self.nleaves, self.M, self.maxdepth = (int(v) for v in info[:4])
         self.n_internal, self.max_n = int(info[5]), int(info[6])
         ni = np.zeros(12 * self.nnodes, np.int32)
         nb = np.zeros(4 * self.nnodes)
-        check(lib().ef_mesh_nodes(h, iptr(ni), dptr(nb)))
+        self._check(self.L.ef_mesh_nodes(h, iptr(ni), dptr(nb)))
         self.node_info = ni.reshape(-1, 12)
         self.node_bounds = nb.reshape(-1, 4)
         self.leaf_nodes = [i for i in range(self.nnodes) if self.node_info[i, 3]]
 
+    def _check(self, rc):
+        check(rc, self.L)
+
     def __del__(self):
-        if getattr(self, "h", None) is not None and _lib is not None:
-            _lib.ef_mesh_destroy(self.h)
+        if getattr(self, "h", None) is not None:
+            self.L.ef_mesh_destroy(self.h)
             self.h = None
 
     def set_options(self, cache_operators=False, homogeneous_rhs=False):
-        check(lib().ef_mesh_set_options(self.h, int(cache_operators), int(homogeneous_rhs)))
+        self._check(self.L.ef_mesh_set_options(self.h, int(cache_operators), int(homogeneous_rhs)))
 
     def set_stream(self, stream_ptr):
-        check(lib().ef_mesh_set_stream(self.h, stream_ptr))
+        self._check(self.L.ef_mesh_set_stream(self.h, stream_ptr))
 
     def build(self, solver_type=EF_SOLVER_FISHPACK, lambda0=0.0, alpha0=1.0, beta0=1.0, cell=None,
               precision=EF_PRECISION_FP64):
@@ -192,14 +206,14 @@ class DeviceMesh:
             keep = f64(cell)
             cp = dptr(keep)
         c = Coeffs(solver_type, alpha0, beta0, lambda0, cp)
-        check(lib().ef_build(self.h, ctypes.byref(c), precision))
+        self._check(self.L.ef_build(self.h, ctypes.byref(c), precision))
         del keep
 
     def upwards(self, f):
         """f: [nrhs][nleaves][N*N] (or [nleaves][N*N] for one right-hand side)."""
         f = f64(f)
         k = 1 if f.ndim == 2 else f.shape[0]
-        check(lib().ef_upwards(self.h, dptr(f), k))
+        self._check(self.L.ef_upwards(self.h, dptr(f), k))
 
     def solve(self, r, a, b=None):
         """r: [nrhs][4M] (or [4M]); a, b: [4M].  Returns u [nrhs][nleaves][N*N] (or [nleaves][N*N])."""
@@ -212,27 +226,27 @@ class DeviceMesh:
             b = f64(np.broadcast_to(b, (4 * self.M,)))
             bp = dptr(b)
         u = np.zeros((k, self.nleaves, self.N * self.N))
-        check(lib().ef_solve(self.h, dptr(r), dptr(a), bp, dptr(u), k))
+        self._check(self.L.ef_solve(self.h, dptr(r), dptr(a), bp, dptr(u), k))
         return u[0] if single else u
 
     def upwards_device(self, f_ptr, k):
-        check(lib().ef_upwards_device(self.h, f_ptr, k))
+        self._check(self.L.ef_upwards_device(self.h, f_ptr, k))
 
     def solve_device(self, f_ptr, r_ptr, a_ptr, b_ptr, u_ptr, k):
-        check(lib().ef_solve_device(self.h, f_ptr, r_ptr, a_ptr, b_ptr, u_ptr, k))
+        self._check(self.L.ef_solve_device(self.h, f_ptr, r_ptr, a_ptr, b_ptr, u_ptr, k))
 
     def sync(self):
-        check(lib().ef_mesh_sync(self.h))
+        self._check(self.L.ef_mesh_sync(self.h))
 
     CODES = dict(T=0, S=1, X=2, H=3, u=4, g=5, f=6, h=7, w=8)
 
     def get(self, node, which, rhs=0):
         code = self.CODES[which]
         r, c = ctypes.c_int(), ctypes.c_int()
-        check(lib().ef_get_operator(self.h, node, code, rhs, ctypes.cast(None, _dp), ctypes.byref(r),
+        self._check(self.L.ef_get_operator(self.h, node, code, rhs, ctypes.cast(None, _dp), ctypes.byref(r),
                                     ctypes.byref(c)))
         out = np.zeros((r.value, c.value))
-        check(lib().ef_get_operator(self.h, node, code, rhs, dptr(out), ctypes.byref(r), ctypes.byref(c)))
+        self._check(self.L.ef_get_operator(self.h, node, code, rhs, dptr(out), ctypes.byref(r), ctypes.byref(c)))
         return out if which in ("T", "S", "X", "H") else out.ravel()
 
     @staticmethod

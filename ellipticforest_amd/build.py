@@ -31,9 +31,16 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# The device build path stays disabled until the cause of the GPU memory fault it raised is
+# shown; EF_DEVICE_BUILD=1 builds it enabled (for a run that tests a fix).
+DEVICE_BUILD_ENABLED = os.environ.get("EF_DEVICE_BUILD") == "1"
+
+
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs + ["-o", LIB + ".tmp"]
+    defs = [] if DEVICE_BUILD_ENABLED else ["-DEF_DEVICE_BUILD_DISABLED"]
+    cmd = [hipcc()] + HIPCC_FLAGS + defs + ["-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs + [
+        "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

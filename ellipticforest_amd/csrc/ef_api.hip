@@ -944,6 +944,12 @@ int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs) 
 int ef_build(ef_mesh mesh, const ef_coeffs* coeffs, int precision) {
     EF_API_BEGIN
     if (!mesh || !coeffs) throw EfError(EF_ERR_ARG, "null argument");
+#ifdef EF_DEVICE_BUILD_DISABLED
+    // The device build faulted an MI355X twice with no cause found in the indexing (the same
+    // sources pass bounds-checked CPU emulation).  Until the cause is shown it stays off.
+    throw EfError(EF_ERR_STATE, "the device build path is disabled in this library build "
+                                "(GPU memory fault under investigation); build with EF_DEVICE_BUILD=1 to enable");
+#endif
     if (coeffs->solver_type != EF_SOLVER_FISHPACK && coeffs->solver_type != EF_SOLVER_FIVE_POINT)
         throw EfError(EF_ERR_ARG, "unknown solver type");
     if (precision != EF_PRECISION_FP64) throw EfError(EF_ERR_ARG, "only FP64 precision is available in this build");

@@ -22,18 +22,39 @@ def gpu_present():
 
 EMULATOR = os.environ.get("EF_TEST_EMULATOR") == "1"
 
+# The device build path of libefhps.so is switched off in this build: on an MI355X it raised
+# a GPU memory fault twice, and the cause has not been shown yet (the same kernels pass the
+# bounds-checked CPU emulation in tests/cpu_emu).  GPU parity tests are skipped meanwhile.
+DEVICE_BUILD_DISABLED_REASON = (
+    "device build path disabled: it faulted the GPU twice and the cause is not shown yet; "
+    "the same kernels are checked on the CPU emulator (tests/test_emulated_kernels.py)")
+
+
+def emulator_library():
+    """The CPU-emulation build of the HIP sources (built on first use)."""
+    from cpu_emu import build_emu
+    from ellipticforest_amd import _lib
+    if not os.path.exists(build_emu.LIB) or build_emu.stale():
+        build_emu.build()
+    return _lib.load_library(build_emu.LIB)
+
+
+@pytest.fixture(scope="session")
+def emu_ctx():
+    import ellipticforest_amd as ef
+    return ef.Context(0, library=emulator_library())
+
 
 @pytest.fixture(scope="session")
 def ctx():
     import ellipticforest_amd as ef
-    from ellipticforest_amd import _lib
+    from ellipticforest_amd import _lib, build as efbuild
     if EMULATOR:
-        # CPU emulation of the same HIP sources (tests/cpu_emu), for checking kernel
-        # indexing on the host; never used outside the test suite
-        from cpu_emu import build_emu
-        _lib.LIB_PATH = build_emu.LIB
-        assert _lib._lib is None or _lib._lib._name == build_emu.LIB, "real library already loaded"
-    elif not gpu_present():
+        print("[gpu] context on the CPU emulator", flush=True)
+        return ef.Context(0, library=emulator_library())
+    if not efbuild.DEVICE_BUILD_ENABLED:
+        pytest.skip(DEVICE_BUILD_DISABLED_REASON)
+    if not gpu_present():
         pytest.fail("GPU test selected but no GPU device is visible")
     print(f"[gpu] creating context on device 0 ({_lib.LIB_PATH})", flush=True)
     return ef.Context(0)

@@ -36,6 +36,16 @@ def clangxx():
     raise RuntimeError("clang++ not found")
 
 
+def stale():
+    """True when a product source is newer than the emulator library."""
+    if not os.path.exists(LIB):
+        return True
+    t = os.path.getmtime(LIB)
+    srcs = [os.path.join(CSRC, n) for n in SOURCES + HEADERS] + [os.path.join(HERE, "hip_cpu_shim.cpp"),
+                                                                 os.path.join(HERE, "hip_cpu_shim.h")]
+    return any(os.path.getmtime(s) > t for s in srcs)
+
+
 def build():
     os.makedirs(OUT, exist_ok=True)
     for name in SOURCES + HEADERS:
