@@ -1,0 +1,167 @@
+"""Benchmark of the HPS factor-and-solve path on one MI355X.
+
+Workload (default): adaptive Poisson on [-10,10]^2, f = -(sin x + sin y), Dirichlet
+u = sin x + sin y, refineByFunction(|f| > 1.2, min level 0, max level 7), 16x16 cells per
+patch (7,750 leaves, root side 512), FISHPACK semantics, FP64.
+
+A step is one upwards + solve pass of a batch of --nrhs right-hand sides whose loads and
+boundary data are already resident in HBM; value = solves per second over the K timed
+steps.  buildStage seconds and per-request latency percentiles are reported beside it.
+
+The CPU baseline is the oracle restatement of the reference algorithm (oracle/), timed
+on this host: one buildStage with all host threads, then single right-hand-side
+upwards + solve passes.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+U = lambda x, y: np.sin(x) + np.sin(y)
+F = lambda x, y: -(np.sin(x) + np.sin(y))
+BOUNDS = (-10.0, 10.0, -10.0, 10.0)
+METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(max_level, N):
+    import ellipticforest_amd as ef
+    leaves = ef.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, N, N, BOUNDS, 0, max_level)
+    return leaves
+
+
+def cpu_baseline(leaves, N, nsolve, threads):
+    """Oracle (reference algorithm restated in C++) on this host."""
+    import oracle
+    oracle.set_threads(threads)
+    O = oracle.Forest(leaves, N, BOUNDS, solver=oracle.FISHPACK)
+    t0 = time.perf_counter()
+    O.build()
+    t_build = time.perf_counter() - t0
+    f = O.sample_rhs(F)
+    r, a, b = O.boundary_data(lambda s, x, y: (U(x, y), 1.0, 0.0))
+    oracle.set_threads(1)
+    times = []
+    u = None
+    for _ in range(nsolve):
+        t0 = time.perf_counter()
+        O.upwards(f)
+        u = O.solve(r, a, b)
+        times.append(time.perf_counter() - t0)
+    err = 0.0
+    for k, bd in enumerate(O.leaf_bounds):
+        X, Y, _, _ = oracle.cell_centres(N, bd)
+        err = max(err, float(np.abs(u[k] - U(X, Y)).max()))
+    return dict(build_s=t_build, solve_s=float(np.mean(times)), solves_per_s=1.0 / float(np.mean(times)),
+                build_threads=threads, solve_threads=1, disc_err=err, u=u, f=f, r=r, a=a)
+
+
+def gpu_run(leaves, N, k, steps, warmup, ref):
+    """Device path; returns a dict or raises EfError."""
+    import torch
+    import ellipticforest_amd as ef
+    ctx = ef.Context(0)
+    D = ef.DeviceMesh(ctx, leaves, N, BOUNDS)
+    stream = torch.cuda.current_stream()
+    D.set_stream(stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    D.build()
+    D.sync()
+    t_build = time.perf_counter() - t0
+    nl = len(leaves)
+    dev = torch.device("cuda:0")
+    f = torch.tensor(np.broadcast_to(ref["f"], (k,) + ref["f"].shape).copy(), device=dev, dtype=torch.float64)
+    r = torch.tensor(np.broadcast_to(ref["r"], (k,) + ref["r"].shape).copy(), device=dev, dtype=torch.float64)
+    a = torch.tensor(ref["a"], device=dev, dtype=torch.float64)
+    u = torch.empty((k, nl, N * N), device=dev, dtype=torch.float64)
+
+    def step(kk, uu):
+        D.upwards_device(f.data_ptr(), kk)
+        D.solve_device(f.data_ptr(), r.data_ptr(), a.data_ptr(), None, uu.data_ptr(), kk)
+
+    for _ in range(warmup):
+        step(k, u)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        step(k, u)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / steps
+    rel = float((u[0].cpu().numpy() - ref["u"]).__abs__().max() / np.abs(ref["u"]).max())
+    # per-request latency: one right-hand side at a time
+    u1 = torch.empty((1, nl, N * N), device=dev, dtype=torch.float64)
+    lat = []
+    for _ in range(max(20, steps)):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step(1, u1)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    return dict(build_s=t_build, ms_per_step=ms_step, wall_s=wall, value=k * steps / (ms_step * steps / 1e3),
+                rel_err=rel, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--max-level", type=int, default=7)
+    ap.add_argument("--N", type=int, default=16)
+    ap.add_argument("--cpu-solves", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the HPS path of this build runs on a single MI355X; "
+              f"--gpus {args.gpus} is not supported (multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    leaves = workload(args.max_level, args.N)
+    log(f"workload: {len(leaves)} leaves, N={args.N}")
+    threads = min(os.cpu_count() or 1, 64)
+    log(f"cpu baseline: oracle build with {threads} threads, {args.cpu_solves} single-thread solves")
+    ref = cpu_baseline(leaves, args.N, args.cpu_solves, threads)
+    log(f"cpu baseline: build {ref['build_s']:.2f}s, solve {ref['solve_s']:.2f}s, disc err {ref['disc_err']:.3e}")
+    out = dict(metric=METRIC, value=None, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic (analytic f and Dirichlet data sampled at cell centres / root face points)",
+               config=dict(workload="adaptive Poisson [-10,10]^2, |f|>1.2 refinement, levels 0..%d, %dx%d cells per "
+                           "patch, FISHPACK semantics" % (args.max_level, args.N, args.N),
+                           leaves=len(leaves), nrhs_per_step=args.nrhs, parallelism="single GPU"),
+               build_s=None, p50_ms=None, p99_ms=None, rel_err_vs_fp64_oracle=None,
+               roofline=None,
+               cpu_baseline=dict(value=ref["solves_per_s"], unit="solves/s", cores=1, kind="port",
+                                 sample="oracle (reference algorithm restated in C++): upwards+solve of 1 RHS on the "
+                                        "full workload, mean of %d passes at 1 thread; its buildStage took %.2f s on "
+                                        "%d threads" % (args.cpu_solves, ref["build_s"], threads),
+                                 build_s=ref["build_s"], build_threads=threads,
+                                 discretization_error=ref["disc_err"]))
+    try:
+        g = gpu_run(leaves, args.N, args.nrhs, args.steps, args.warmup, ref)
+        out.update(value=g["value"], ms_per_step=g["ms_per_step"], build_s=g["build_s"], p50_ms=g["p50_ms"],
+                   p99_ms=g["p99_ms"], rel_err_vs_fp64_oracle=g["rel_err"])
+    except Exception as e:  # the device path reports why it did not run
+        out["error"] = f"{type(e).__name__}: {e}"
+        log(out["error"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -51,5 +51,29 @@ def build(verbose=False):
     return LIB
 
 
+EXAMPLES = ["elliptic_single"]
+
+
+def build_examples(lib_dir=None, out_dir=None, verbose=False):
+    """Drivers written against the drop-in C++ headers, linked to libefhps.so."""
+    lib_dir = lib_dir or HERE
+    out_dir = out_dir or os.path.join(HERE, "examples")
+    outs = []
+    for name in EXAMPLES:
+        src = os.path.join(HERE, "examples", name + ".cpp")
+        exe = os.path.join(out_dir, name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), src, "-o", exe, "-L", lib_dir,
+               "-lefhps", "-Wl,-rpath," + lib_dir]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"building example {name} failed")
+        outs.append(exe)
+    return outs
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_examples(verbose=True))
