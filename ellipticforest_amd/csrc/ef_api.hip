@@ -266,6 +266,9 @@ struct ef_mesh_s {
     DevBuf<LeafClass> classes;
     DevBuf<LeafDesc> leafdesc;
     DevBuf<double> Q, Dinv, leafT;
+    // variable-coefficient leaves (block-tridiagonal path)
+    bool variable = false;
+    DevBuf<double> vcoef, vstencil, vSinv, vwork;
     // operator arenas
     DevBuf<double> Tar, Sar, Xiar, Har;
     // work
@@ -312,8 +315,10 @@ int fail(const EfError& e) {
     return EF_OK;
 
 const double* leaf_T_ptr(ef_mesh_s* M, int leaf) {
+    const long long per = 16LL * M->P.N * M->P.N;
+    if (M->variable) return M->leafT.p + (M->cache_operators ? 0 : (long long)leaf) * per;
     const int cls = M->cache_operators ? M->tleaf_class0 : M->leaf_class[leaf];
-    return M->leafT.p + (long long)cls * 16 * M->P.N * M->P.N;
+    return M->leafT.p + (long long)cls * per;
 }
 
 // DtN the parent merge reads: coarsened if tagged (coarsen_ overwrites it, :736-737)
@@ -358,15 +363,57 @@ void run_gj(const GJDesc* d_dev, int nsys, int m, int maxw, hipStream_t s) {
 }
 
 // ----- build ------------------------------------------------------------------
+// True when per-cell coefficients differ between cells or leaves.
+bool coefficients_vary(const Plan& P, const ef_coeffs& c) {
+    if (c.solver_type != EF_SOLVER_FIVE_POINT || c.cell == nullptr) return false;
+    const long long NN = (long long)P.N * P.N;
+    const double a0 = c.cell[0], b0 = c.cell[NN], l0 = c.cell[5 * NN];
+    for (size_t li = 0; li < P.leaves.size(); li++) {
+        const double* base = c.cell + li * 6 * NN;
+        for (long long e = 0; e < NN; e++)
+            if (base[e] != a0 || base[NN + e] != b0 || base[2 * NN + e] != b0 || base[3 * NN + e] != b0 ||
+                base[4 * NN + e] != b0 || base[5 * NN + e] != l0)
+                return true;
+    }
+    return false;
+}
+
 void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     Plan& P = M->P;
     const int N = P.N;
     const long long NN = (long long)N * N;
+    const size_t nl = P.leaves.size();
     M->classes_h.clear();
-    M->leaf_class.assign(P.leaves.size(), -1);
+    M->leaf_class.assign(nl, -1);
+    M->variable = coefficients_vary(P, c);
+    if (M->variable) {
+        if (N > BT_MAX_N) throw EfError(EF_ERR_ARG, "variable coefficients need N <= " + std::to_string(BT_MAX_N));
+        M->vcoef.alloc(nl * 6 * NN);
+        EF_HIP(hipMemcpyAsync(M->vcoef.p, c.cell, sizeof(double) * nl * 6 * NN, hipMemcpyHostToDevice, M->stream));
+        M->vstencil.alloc(nl * 5 * NN);
+        M->vSinv.alloc(nl * (size_t)N * NN);
+        M->leafT.alloc(nl * 16 * NN);
+        std::vector<LeafDesc> ld(nl);
+        for (size_t li = 0; li < nl; li++) {
+            const NodeP& nd = P.nodes[P.leaves[li]];
+            LeafDesc& L = ld[li];
+            L.cls = -1;
+            L.node = P.leaves[li];
+            L.h_off = nd.h_off;
+            L.g_off = nd.g_off;
+            L.coef = M->vcoef.p + li * 6 * NN;
+            L.stencil = M->vstencil.p + li * 5 * NN;
+            L.Sinv = M->vSinv.p + li * (size_t)N * NN;
+            L.T = M->leafT.p + li * 16 * NN;
+            L.dx = (nd.xu - nd.xl) / N;
+            L.dy = (nd.yu - nd.yl) / N;
+        }
+        M->leafdesc.upload(ld, M->stream);
+        return;
+    }
     std::map<std::vector<uint64_t>, int> seen;
     auto bits = [](double v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
-    for (size_t li = 0; li < P.leaves.size(); li++) {
+    for (size_t li = 0; li < nl; li++) {
         const NodeP& nd = P.nodes[P.leaves[li]];
         const double dx = (nd.xu - nd.xl) / N, dy = (nd.yu - nd.yl) / N;
         double cx, cy, lam;
@@ -375,14 +422,8 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
         } else if (c.cell == nullptr) {
             cx = c.alpha0 * c.beta0 / (dx * dx); cy = c.alpha0 * c.beta0 / (dy * dy); lam = c.lambda0;
         } else {
-            // per-cell coefficients: this build accepts them when they are constant over the mesh
-            const double* base = c.cell + li * 6 * NN;
-            const double a0 = base[0], b0 = base[NN], l0 = base[5 * NN];
-            for (long long e = 0; e < NN; e++) {
-                if (base[e] != a0 || base[NN + e] != b0 || base[2 * NN + e] != b0 || base[3 * NN + e] != b0 ||
-                    base[4 * NN + e] != b0 || base[5 * NN + e] != l0)
-                    throw EfError(EF_ERR_ARG, "variable per-cell coefficients are not supported by this build");
-            }
+            // constant per-cell samples: the fast-diagonalisation class path
+            const double a0 = c.cell[0], b0 = c.cell[NN], l0 = c.cell[5 * NN];
             cx = a0 * b0 / (dx * dx); cy = a0 * b0 / (dy * dy); lam = l0;
         }
         std::vector<uint64_t> key = {bits(cx), bits(cy), bits(lam), bits(dx), bits(dy)};
@@ -402,8 +443,8 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     M->leafT.alloc((size_t)ncls * 16 * NN);
     for (int i = 0; i < ncls; i++) M->classes_h[i].T = M->leafT.p + (long long)i * 16 * NN;
     M->classes.upload(M->classes_h, M->stream);
-    std::vector<LeafDesc> ld(P.leaves.size());
-    for (size_t li = 0; li < P.leaves.size(); li++) {
+    std::vector<LeafDesc> ld(nl);
+    for (size_t li = 0; li < nl; li++) {
         const NodeP& nd = P.nodes[P.leaves[li]];
         LeafDesc& L = ld[li];
         L.cls = M->leaf_class[li];
@@ -412,6 +453,8 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
         L.g_off = nd.g_off;
         L.coef = nullptr;
         L.Sinv = nullptr;
+        L.stencil = nullptr;
+        L.T = M->classes_h[L.cls].T;
         L.dx = (nd.xu - nd.xl) / N;
         L.dy = (nd.yu - nd.yl) / N;
     }
@@ -527,17 +570,33 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     M->wpiv.alloc(std::max<long long>((long long)P.max_group * GJ_NB, 1));
     M->winfo.alloc(std::max(P.max_group, 1));
     plan_build_descriptors(M);
-    // leaf DtN per class (buildD2N)
-    const int ncls = (int)M->classes_h.size();
-    M->Q.alloc((size_t)N * N);
-    M->Dinv.alloc((size_t)ncls * N * N);
-    hipLaunchKernelGGL(fd_basis_kernel, dim3(blocks_for((long long)N * N)), dim3(256), 0, s, M->Q.p, N);
-    EF_LAUNCH_CHECK();
-    hipLaunchKernelGGL(fd_dinv_kernel, dim3(blocks_for((long long)N * N), ncls), dim3(256), 0, s, M->classes.p, M->Dinv.p, N);
-    EF_LAUNCH_CHECK();
-    hipLaunchKernelGGL(class_dtn_kernel, dim3(4 * N, ncls), dim3(256), (size_t)3 * N * N * sizeof(double), s,
-                       M->classes.p, M->Q.p, M->Dinv.p, N);
-    EF_LAUNCH_CHECK();
+    if (M->variable) {
+        // block-tridiagonal factors, then the DtN of every leaf (buildD2N) in chunks of leaves
+        const int nl = (int)P.leaves.size();
+        hipLaunchKernelGGL(bt_factor_kernel, dim3(nl), dim3(256), 0, s, M->leafdesc.p, N);
+        EF_LAUNCH_CHECK();
+        const long long per = (long long)N * N * 4 * N;
+        const int chunk = (int)std::max<long long>(1, std::min<long long>(nl, (512LL << 20) / (per * 8)));
+        M->vwork.ensure((size_t)chunk * per);
+        for (int first = 0; first < nl; first += chunk) {
+            const int cnt = std::min(chunk, nl - first);
+            hipLaunchKernelGGL(bt_dtn_kernel, dim3(cnt), dim3(256), 0, s, M->leafdesc.p, N, first, M->vwork.p);
+            EF_LAUNCH_CHECK();
+        }
+    } else {
+        // leaf DtN per class (buildD2N)
+        const int ncls = (int)M->classes_h.size();
+        M->Q.alloc((size_t)N * N);
+        M->Dinv.alloc((size_t)ncls * N * N);
+        hipLaunchKernelGGL(fd_basis_kernel, dim3(blocks_for((long long)N * N)), dim3(256), 0, s, M->Q.p, N);
+        EF_LAUNCH_CHECK();
+        hipLaunchKernelGGL(fd_dinv_kernel, dim3(blocks_for((long long)N * N), ncls), dim3(256), 0, s, M->classes.p,
+                           M->Dinv.p, N);
+        EF_LAUNCH_CHECK();
+        hipLaunchKernelGGL(class_dtn_kernel, dim3(4 * N, ncls), dim3(256), (size_t)3 * N * N * sizeof(double), s,
+                           M->classes.p, M->Q.p, M->Dinv.p, N);
+        EF_LAUNCH_CHECK();
+    }
     // merges, deepest level first (post-order dependencies)
     EF_HIP(hipMemsetAsync(M->winfo.p, 0, sizeof(int) * std::max(P.max_group, 1), s));
     std::vector<int> info_h;
@@ -702,6 +761,21 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
     return ref;
 }
 
+// Leaf solves of every (leaf, rhs): mode 0 particular Neumann data, mode 1 interior solution.
+void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u, int k) {
+    const int N = M->P.N;
+    const int nl = (int)M->P.leaves.size();
+    if (M->variable) {
+        hipLaunchKernelGGL(bt_solve_kernel, dim3(nl, k), dim3(256), 0, M->stream, M->leafdesc.p, N, nl, k, mode, f,
+                           f_zero, (const double*)M->vec.p, M->vec.p, u);
+    } else {
+        hipLaunchKernelGGL(leaf_fd_kernel, dim3(nl, k), dim3(256), (size_t)4 * N * N * sizeof(double), M->stream,
+                           M->leafdesc.p, M->classes.p, M->Q.p, M->Dinv.p, N, nl, k, mode, f, f_zero,
+                           (const double*)M->vec.p, M->vec.p, u);
+    }
+    EF_LAUNCH_CHECK();
+}
+
 void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     if (!M->built) throw EfError(EF_ERR_STATE, "upwards stage called before the build stage");
     if (k < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
@@ -712,10 +786,7 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     const int N = P.N;
     const int nl = (int)P.leaves.size();
     // leaf particular Neumann data (particularNeumannData, FiniteVolumeSolver.cpp:457-462)
-    hipLaunchKernelGGL(leaf_fd_kernel, dim3(nl, k), dim3(256), (size_t)4 * N * N * sizeof(double), s, M->leafdesc.p,
-                       M->classes.p, M->Q.p, M->Dinv.p, N, nl, k, 0, f_dev, 0, (const double*)M->vec.p, M->vec.p,
-                       (double*)nullptr);
-    EF_LAUNCH_CHECK();
+    launch_leaf(M, 0, f_dev, 0, nullptr, k);
     if (!M->homogeneous_rhs) {  // upwards4to1 is skipped entirely for homogeneous-rhs (:532)
         for (int dd = P.maxdepth; dd >= 0; dd--) {
             const int cnt = sp.up_cnt[dd];
@@ -791,10 +862,7 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         EF_LAUNCH_CHECK();
     }
     // leaf solves (leafSolve, HPSAlgorithm.hpp:584-596); homogeneous-rhs solves with f = 0
-    hipLaunchKernelGGL(leaf_fd_kernel, dim3(nl, k), dim3(256), (size_t)4 * N * N * sizeof(double), s, M->leafdesc.p,
-                       M->classes.p, M->Q.p, M->Dinv.p, N, nl, k, 1, f_dev ? f_dev : (const double*)V,
-                       (M->homogeneous_rhs || !f_dev) ? 1 : 0, (const double*)V, V, u_dev);
-    EF_LAUNCH_CHECK();
+    launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, (M->homogeneous_rhs || !f_dev) ? 1 : 0, u_dev, k);
 }
 
 void set_lds_limits() {

@@ -92,7 +92,9 @@ struct LeafDesc {
     long long h_off; // row offset of h in the vector arena (4N rows)
     long long g_off; // row offset of g
     const double* coef;  // variable coefficients: 6 N*N arrays (alpha, bW, bE, bS, bN, lambda) or null
-    double* Sinv;        // variable coefficients: N blocks of N x N (block-tridiagonal factors) or null
+    double* Sinv;        // variable coefficients: N blocks of N x N, inverses of the block-LU pivots
+    double* stencil;     // variable coefficients: 5 N*N arrays cW, cE, cS, cN, cC
+    double* T;           // leaf DtN (4N x 4N)
     double dx, dy;
 };
 

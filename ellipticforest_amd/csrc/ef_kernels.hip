@@ -732,3 +732,250 @@ __global__ void copy_kernel(const double* src, double* dst, long long n) {
 }
 
 }  // namespace efk
+
+namespace efk {
+
+// ---------------------------------------------------------------------------
+// Variable-coefficient leaves: batched block-tridiagonal LU of the 5-point matrix.
+// With cell index j + i*N, block row i (an x-line) couples to block i-1 through
+// diag(cW(i,:)) and to block i+1 through diag(cE(i,:)); the diagonal block D_i is
+// tridiagonal in j with cS/cN off the diagonal and the centre coefficient (minus the
+// neighbour coefficient on a physical side, ghost = 2g - u, FiniteVolumeSolver.cpp:100-184).
+// Block LU: S_0 = D_0, S_i = D_i - diag(cW_i) S_{i-1}^-1 diag(cE_{i-1}); the kernels keep
+// Sinv_i = S_i^-1.  Solve: z_0 = Sinv_0 b_0, z_i = Sinv_i (b_i - cW_i .* z_{i-1}),
+// x_{N-1} = z_{N-1}, x_i = z_i - Sinv_i (cE_i .* x_{i+1}).
+// ---------------------------------------------------------------------------
+constexpr int BT_MAXN = 32;
+
+// stencil of one cell (FiniteVolumeSolver.cpp:64-85)
+__device__ inline void bt_stencil(const double* coef, int NN, int I, double dx2, double dy2, double& cW, double& cE,
+                                  double& cS, double& cN, double& cC) {
+    const double a = coef[I], bW = coef[NN + I], bE = coef[2 * NN + I], bS = coef[3 * NN + I], bN = coef[4 * NN + I];
+    const double lam = coef[5 * NN + I];
+    cW = a * bW / dx2; cE = a * bE / dx2; cS = a * bS / dy2; cN = a * bN / dy2;
+    cC = -a * ((bE + bW) / dx2 + (bN + bS) / dy2) + lam;
+}
+
+// One workgroup per leaf.  Gauss-Jordan with partial pivoting of [S | I] in LDS per block.
+__global__ void __launch_bounds__(256) bt_factor_kernel(const LeafDesc* leaves, int Nn) {
+    __shared__ double aug[BT_MAXN][2 * BT_MAXN + 1];
+    __shared__ double prev[BT_MAXN][BT_MAXN + 1];
+    __shared__ double cw[BT_MAXN], ce_prev[BT_MAXN];
+    __shared__ int piv_row;
+    const LeafDesc L = leaves[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int NN = Nn * Nn;
+    const double dx2 = L.dx * L.dx, dy2 = L.dy * L.dy;
+    // stencil of every cell
+    for (int I = tid; I < NN; I += 256) {
+        double cW, cE, cS, cN, cC;
+        bt_stencil(L.coef, NN, I, dx2, dy2, cW, cE, cS, cN, cC);
+        L.stencil[I] = cW; L.stencil[NN + I] = cE; L.stencil[2 * NN + I] = cS; L.stencil[3 * NN + I] = cN;
+        L.stencil[4 * NN + I] = cC;
+    }
+    __syncthreads();
+    for (int i = 0; i < Nn; i++) {
+        // S_i = D_i - diag(cW_i) Sinv_{i-1} diag(cE_{i-1}), augmented with I
+        for (int j = tid; j < Nn; j += 256) {
+            cw[j] = L.stencil[i * Nn + j];
+            ce_prev[j] = (i > 0) ? L.stencil[NN + (i - 1) * Nn + j] : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < Nn * 2 * Nn; e += 256) {
+            const int r = e / (2 * Nn), c = e % (2 * Nn);
+            double v = 0.0;
+            if (c < Nn) {
+                const int I = r + i * Nn;
+                if (c == r) {
+                    v = L.stencil[4 * NN + I];
+                    if (i == 0) v -= L.stencil[I];
+                    if (i == Nn - 1) v -= L.stencil[NN + I];
+                    if (r == 0) v -= L.stencil[2 * NN + I];
+                    if (r == Nn - 1) v -= L.stencil[3 * NN + I];
+                } else if (c == r - 1) v = L.stencil[2 * NN + I];
+                else if (c == r + 1) v = L.stencil[3 * NN + I];
+                if (i > 0) v -= cw[r] * prev[r][c] * ce_prev[c];
+            } else {
+                v = (c - Nn == r) ? 1.0 : 0.0;
+            }
+            aug[r][c] = v;
+        }
+        __syncthreads();
+        for (int k = 0; k < Nn; k++) {
+            if (tid == 0) {
+                int p = k;
+                double best = fabs(aug[k][k]);
+                for (int r = k + 1; r < Nn; r++) {
+                    const double v = fabs(aug[r][k]);
+                    if (v > best) { best = v; p = r; }
+                }
+                piv_row = p;
+            }
+            __syncthreads();
+            const int p = piv_row;
+            if (p != k)
+                for (int c = tid; c < 2 * Nn; c += 256) {
+                    const double t = aug[k][c];
+                    aug[k][c] = aug[p][c];
+                    aug[p][c] = t;
+                }
+            __syncthreads();
+            const double pk = aug[k][k];
+            const double inv = (pk != 0.0) ? 1.0 / pk : 0.0;
+            // eliminate column k from every other row, then normalise row k
+            for (int e = tid; e < Nn * 2 * Nn; e += 256) {
+                const int r = e / (2 * Nn), c = e % (2 * Nn);
+                if (r != k && c != k) aug[r][c] -= aug[r][k] * inv * aug[k][c];
+            }
+            __syncthreads();
+            for (int e = tid; e < Nn * 2 * Nn; e += 256) {
+                const int r = e / (2 * Nn), c = e % (2 * Nn);
+                if (r != k && c == k) aug[r][c] = 0.0;
+            }
+            for (int c = tid; c < 2 * Nn; c += 256) aug[k][c] *= inv;
+            __syncthreads();
+        }
+        double* Si = L.Sinv + (long long)i * NN;
+        for (int e = tid; e < NN; e += 256) {
+            const int r = e / Nn, c = e % Nn;
+            const double v = aug[r][Nn + c];
+            Si[e] = v;
+            prev[r][c] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Block substitution of one right-hand side b (N x N, in LDS) in place -> solution.
+__device__ inline void bt_substitute(const LeafDesc& L, int Nn, double* b, double* tmp) {
+    const int tid = threadIdx.x;
+    const int NN = Nn * Nn;
+    // forward: z_i = Sinv_i (b_i - cW_i .* z_{i-1})
+    for (int i = 0; i < Nn; i++) {
+        for (int j = tid; j < Nn; j += 256) {
+            double v = b[i * Nn + j];
+            if (i > 0) v -= L.stencil[i * Nn + j] * b[(i - 1) * Nn + j];
+            tmp[j] = v;
+        }
+        __syncthreads();
+        const double* Si = L.Sinv + (long long)i * NN;
+        for (int r = tid; r < Nn; r += 256) {
+            double s = 0.0;
+            for (int c = 0; c < Nn; c++) s += Si[r * Nn + c] * tmp[c];
+            b[i * Nn + r] = s;
+        }
+        __syncthreads();
+    }
+    // backward: x_i = z_i - Sinv_i (cE_i .* x_{i+1})
+    for (int i = Nn - 2; i >= 0; i--) {
+        for (int j = tid; j < Nn; j += 256) tmp[j] = L.stencil[NN + i * Nn + j] * b[(i + 1) * Nn + j];
+        __syncthreads();
+        const double* Si = L.Sinv + (long long)i * NN;
+        for (int r = tid; r < Nn; r += 256) {
+            double s = 0.0;
+            for (int c = 0; c < Nn; c++) s += Si[r * Nn + c] * tmp[c];
+            b[i * Nn + r] -= s;
+        }
+        __syncthreads();
+    }
+}
+
+// Leaf solve for every (leaf, rhs) with variable coefficients; modes as leaf_fd_kernel.
+__global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode,
+                                                       const double* f, int f_zero, const double* garena,
+                                                       double* harena, double* u) {
+    __shared__ double b[BT_MAXN * BT_MAXN];
+    __shared__ double tmp[BT_MAXN];
+    const int leaf = blockIdx.x, rhs = blockIdx.y;
+    const LeafDesc L = leaves[leaf];
+    const int tid = threadIdx.x;
+    const int NN = Nn * Nn;
+    const double* fl = f + ((long long)rhs * nleaves + leaf) * NN;
+    for (int e = tid; e < NN; e += 256) {
+        const int i = e / Nn, j = e % Nn;
+        double v = f_zero ? 0.0 : fl[e];
+        if (mode == 1 && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
+            const double* g = garena + L.g_off * k + rhs;
+            double add = 0.0;
+            if (i == 0) add += L.stencil[e] * g[(long long)(0 * Nn + j) * k];
+            if (i == Nn - 1) add += L.stencil[NN + e] * g[(long long)(1 * Nn + j) * k];
+            if (j == 0) add += L.stencil[2 * NN + e] * g[(long long)(2 * Nn + i) * k];
+            if (j == Nn - 1) add += L.stencil[3 * NN + e] * g[(long long)(3 * Nn + i) * k];
+            v += -2.0 * add;
+        }
+        b[e] = v;
+    }
+    __syncthreads();
+    bt_substitute(L, Nn, b, tmp);
+    if (mode == 1) {
+        double* ul = u + ((long long)rhs * nleaves + leaf) * NN;
+        for (int e = tid; e < NN; e += 256) ul[e] = b[e];
+    } else {
+        double* h = harena + L.h_off * k + rhs;
+        for (int t = tid; t < 4 * Nn; t += 256) {
+            int i, j;
+            bcell(t, Nn, i, j);
+            const int sd = t / Nn;
+            const double dd = (sd < 2) ? L.dx : L.dy;
+            const double sg = (sd == W || sd == S) ? 2.0 / dd : -2.0 / dd;
+            h[(long long)t * k] = sg * b[i * Nn + j];
+        }
+    }
+}
+
+// Leaf DtN (buildD2N, FiniteVolumeSolver.cpp:355-455) with variable coefficients: column b is
+// driven by -2 c_b at the boundary cell of b; Z (N blocks of N x 4N per leaf) is global work.
+__global__ void __launch_bounds__(256) bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work) {
+    const LeafDesc L = leaves[first + blockIdx.x];
+    const int tid = threadIdx.x;
+    const int NN = Nn * Nn, NB = 4 * Nn;
+    double* Z = work + (long long)blockIdx.x * Nn * Nn * NB;  // block i: Z + i*N*4N, row r, col b
+    // right-hand side block rows, then forward substitution, all 4N columns at once
+    for (int i = 0; i < Nn; i++) {
+        double* Zi = Z + (long long)i * Nn * NB;
+        const double* Si = L.Sinv + (long long)i * NN;
+        for (int e = tid; e < Nn * NB; e += 256) {
+            const int r = e / NB, bcol = e % NB;
+            double acc = 0.0;
+            for (int c = 0; c < Nn; c++) {
+                int ib, jb;
+                bcell(bcol, Nn, ib, jb);
+                double v = 0.0;
+                if (ib == i && jb == c) {
+                    const int sb = bcol / Nn;
+                    v = -2.0 * L.stencil[sb * NN + ib * Nn + jb];
+                }
+                if (i > 0) v -= L.stencil[i * Nn + c] * Z[(long long)(i - 1) * Nn * NB + c * NB + bcol];
+                acc += Si[r * Nn + c] * v;
+            }
+            Zi[r * NB + bcol] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = Nn - 2; i >= 0; i--) {
+        double* Zi = Z + (long long)i * Nn * NB;
+        const double* Si = L.Sinv + (long long)i * NN;
+        // x_i = z_i - Sinv_i (cE_i .* x_{i+1}); each thread owns one (r, b) of block i
+        for (int e = tid; e < Nn * NB; e += 256) {
+            const int r = e / NB, bcol = e % NB;
+            double acc = 0.0;
+            for (int c = 0; c < Nn; c++)
+                acc += Si[r * Nn + c] * L.stencil[NN + i * Nn + c] * Z[(long long)(i + 1) * Nn * NB + c * NB + bcol];
+            Zi[r * NB + bcol] -= acc;
+        }
+        __syncthreads();
+    }
+    // T[a][b] = sgn_a (2/d_a) (u_cell(a) - delta_ab)
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int a = e / NB, bcol = e % NB;
+        int ia, ja;
+        bcell(a, Nn, ia, ja);
+        const double ua = Z[(long long)ia * Nn * NB + ja * NB + bcol];
+        const int sa = a / Nn;
+        const double da = (sa < 2) ? L.dx : L.dy;
+        const double sg = (sa == W || sa == S) ? 2.0 / da : -2.0 / da;
+        L.T[e] = sg * (ua - (a == bcol ? 1.0 : 0.0));
+    }
+}
+
+}  // namespace efk

@@ -27,6 +27,11 @@ __global__ void down_scatter_kernel(const DownDesc* descs, double* garena, int k
 __global__ void root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k);
 __global__ void root_robin_assemble_kernel(const double* T, const double* a, const double* b, const double* r,
                                            const double* h, double* A, double* rhs, int m, int k);
+__global__ void bt_factor_kernel(const LeafDesc* leaves, int Nn);
+__global__ void bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f,
+                                int f_zero, const double* garena, double* harena, double* u);
+__global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work);
+constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)
 __global__ void fill_kernel(double* p, double v, long long n);
 __global__ void copy_kernel(const double* src, double* dst, long long n);
 

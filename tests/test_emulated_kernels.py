@@ -43,3 +43,9 @@ def test_emu_robin_root(emu_ctx):
 
 def test_emu_homogeneous_rhs(emu_ctx):
     gp.test_homogeneous_rhs_option(emu_ctx)
+
+
+def test_emu_variable_coefficients(emu_ctx):
+    """Block-tridiagonal leaves (bt_factor / bt_dtn / bt_solve) with cfg3's beta and lambda."""
+    uo, ug, err = gp.variable_both(emu_ctx, omesh.uniform(1), 8, check_ops=True)
+    assert gp.rel(ug, uo) < gp.TOL_U

@@ -187,3 +187,57 @@ def test_cfg2_full_size(ctx):
     assert len(leaves) == 7750
     O, D, uo, ug = solve_both(ctx, leaves, 16, (-10, 10, -10, 10), check_ops=True, nodes=[0, 1, 2, 5, 21])
     assert rel(ug, uo) < TOL_U
+
+
+# ----- variable coefficients (cfg3's operator at reduced sizes) -----------------------
+ALPHA = lambda x, y: np.ones_like(x)
+BETA = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+LAMBDA = lambda x, y: -np.ones_like(x)
+U3 = lambda x, y: np.sin(3 * x) * np.cosh(y)
+
+
+def F3(x, y):
+    """f = div(beta grad u) - u for u = sin(3x) cosh(y) (alpha = 1, lambda = -1)."""
+    lap = -8.0 * np.sin(3 * x) * np.cosh(y)
+    ux, uy = 3 * np.cos(3 * x) * np.cosh(y), np.sin(3 * x) * np.sinh(y)
+    bx = np.pi * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    by = -np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)
+    return BETA(x, y) * lap + bx * ux + by * uy - U3(x, y)
+
+
+def variable_both(ctx, leaves, N, bounds=(-1.0, 1.0, -1.0, 1.0), check_ops=False):
+    import ellipticforest_amd as ef
+    O = oracle.Forest(leaves, N, bounds, solver=oracle.FIVE_POINT, coeff_fns=(ALPHA, BETA, LAMBDA))
+    O.build()
+    f = O.sample_rhs(F3)
+    O.upwards(f)
+    r, a, b = O.boundary_data(lambda s, x, y: (U3(x, y), 1.0, 0.0))
+    uo = O.solve(r, a, b)
+    D = ef.DeviceMesh(ctx, leaves, N, bounds)
+    cell = np.concatenate([oracle.sample_coeffs(N, tuple(D.node_bounds[n]), ALPHA, BETA, LAMBDA).ravel()
+                           for n in D.leaf_nodes])
+    D.build(solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    D.upwards(f)
+    ug = D.solve(r, a)
+    if check_ops:
+        for node in range(D.nnodes):
+            assert rel(D.get(node, "T"), O.get(node, "T")) < TOL_OP, ("T", node)
+            if not D.node_info[node, 3]:
+                assert rel(D.get(node, "S"), O.get(node, "S")) < TOL_OP, ("S", node)
+    err = max(np.abs(ug[k] - U3(*oracle.cell_centres(N, tuple(D.node_bounds[n]))[:2])).max()
+              for k, n in enumerate(D.leaf_nodes))
+    return uo, ug, err
+
+
+def test_variable_coefficients_uniform(ctx):
+    say("variable beta, lambda=-1, uniform level 2, N=16, [-1,1]^2")
+    uo, ug, err = variable_both(ctx, omesh.uniform(2), 16, check_ops=True)
+    assert rel(ug, uo) < TOL_U
+
+
+def test_variable_coefficients_cfg3_shape(ctx):
+    """cfg3 operator (N = 32 patches) at level 3: parity and second-order error."""
+    say("variable beta cfg3 shape: uniform level 3, N=32")
+    uo, ug, err = variable_both(ctx, omesh.uniform(3), 32)
+    assert rel(ug, uo) < TOL_U
+    assert err < 2e-4

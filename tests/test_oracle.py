@@ -185,3 +185,21 @@ def test_mesh_cfg2_leaf_counts():
             for dy in (-1, 0, 1):
                 c = mesh.covering_leaf(S, l, x + dx, y + dy) if 0 <= x + dx < 2**l and 0 <= y + dy < 2**l else None
                 assert c is None or c[0] >= l - 1
+
+
+def test_variable_coefficient_second_order():
+    """cfg3's operator (beta = 1 + 0.5 sin 2pi x cos 2pi y, lambda = -1) with the manufactured
+    u = sin(3x) cosh(y): the five-point HPS solution converges at second order."""
+    import test_gpu_parity as gp
+    errs = []
+    for level in (1, 2, 3):
+        Fo = oracle.Forest(mesh.uniform(level), 8, (-1.0, 1.0, -1.0, 1.0), solver=oracle.FIVE_POINT,
+                           coeff_fns=(gp.ALPHA, gp.BETA, gp.LAMBDA))
+        Fo.build()
+        Fo.upwards(Fo.sample_rhs(gp.F3))
+        r, a, b = Fo.boundary_data(lambda s, x, y: (gp.U3(x, y), 1.0, 0.0))
+        u = Fo.solve(r, a, b)
+        errs.append(max(np.abs(u[k] - gp.U3(*oracle.cell_centres(8, bd)[:2])).max()
+                        for k, bd in enumerate(Fo.leaf_bounds)))
+    orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert orders.min() > 1.8, (errs, orders)
