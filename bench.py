@@ -70,53 +70,54 @@ def cpu_baseline(leaves, N, nsolve, threads):
 
 
 def gpu_run(leaves, N, k, steps, warmup, ref):
-    """Device path; returns a dict or raises EfError."""
-    import torch
+    """Device path through the C ABI only (device memory, streams and HIP-event timers of
+    libefhps.so; no second HIP runtime in the process).  Returns a dict or raises EfError."""
     import ellipticforest_amd as ef
     ctx = ef.Context(0)
     D = ef.DeviceMesh(ctx, leaves, N, BOUNDS)
-    stream = torch.cuda.current_stream()
-    D.set_stream(stream.cuda_stream)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     D.build()
     D.sync()
     t_build = time.perf_counter() - t0
     nl = len(leaves)
-    dev = torch.device("cuda:0")
-    f = torch.tensor(np.broadcast_to(ref["f"], (k,) + ref["f"].shape).copy(), device=dev, dtype=torch.float64)
-    r = torch.tensor(np.broadcast_to(ref["r"], (k,) + ref["r"].shape).copy(), device=dev, dtype=torch.float64)
-    a = torch.tensor(ref["a"], device=dev, dtype=torch.float64)
-    u = torch.empty((k, nl, N * N), device=dev, dtype=torch.float64)
+    f = ef.DeviceArray(ctx, (k, nl, N * N))
+    r = ef.DeviceArray(ctx, (k, 4 * D.M))
+    a = ef.DeviceArray(ctx, (4 * D.M,))
+    u = ef.DeviceArray(ctx, (k, nl, N * N))
+    f.upload(np.broadcast_to(ref["f"], (k,) + ref["f"].shape))
+    r.upload(np.broadcast_to(ref["r"], (k,) + ref["r"].shape))
+    a.upload(ref["a"])
 
     def step(kk, uu):
-        D.upwards_device(f.data_ptr(), kk)
-        D.solve_device(f.data_ptr(), r.data_ptr(), a.data_ptr(), None, uu.data_ptr(), kk)
+        D.upwards_device(f.ptr, kk)
+        D.solve_device(f.ptr, r.ptr, a.ptr, None, uu.ptr, kk)
 
     for _ in range(warmup):
         step(k, u)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    D.sync()
+    up_ms, solve_ms = 0.0, 0.0
     t0 = time.perf_counter()
-    ev0.record(stream)
     for _ in range(steps):
         step(k, u)
-    ev1.record(stream)
-    torch.cuda.synchronize()
+        st = D.stage_ms()  # event pair per stage on the mesh stream (waits for the step)
+        up_ms += st["upwards-stage"]
+        solve_ms += st["solve-stage"]
     wall = time.perf_counter() - t0
-    ms_step = ev0.elapsed_time(ev1) / steps
-    rel = float((u[0].cpu().numpy() - ref["u"]).__abs__().max() / np.abs(ref["u"]).max())
-    # per-request latency: one right-hand side at a time
-    u1 = torch.empty((1, nl, N * N), device=dev, dtype=torch.float64)
+    ms_step = (up_ms + solve_ms) / steps
+    u0 = u.download(nl * N * N).reshape(nl, N * N)
+    rel = float(np.abs(u0 - ref["u"]).max() / np.abs(ref["u"]).max())
+    # per-request latency: one right-hand side at a time, host clock around a synchronised step
+    u1 = ef.DeviceArray(ctx, (1, nl, N * N))
     lat = []
     for _ in range(max(20, steps)):
-        torch.cuda.synchronize()
+        D.sync()
         t1 = time.perf_counter()
         step(1, u1)
-        torch.cuda.synchronize()
+        D.sync()
         lat.append((time.perf_counter() - t1) * 1e3)
-    return dict(build_s=t_build, ms_per_step=ms_step, wall_s=wall, value=k * steps / (ms_step * steps / 1e3),
-                rel_err=rel, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
+    return dict(build_s=t_build, build_ms_events=D.stage_ms()["build-stage"], ms_per_step=ms_step, wall_s=wall,
+                value=k / (ms_step / 1e3), rel_err=rel, upwards_ms=up_ms / steps, solve_ms=solve_ms / steps,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
 
 
 def main():

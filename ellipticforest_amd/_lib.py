@@ -30,7 +30,8 @@ EXPORTS = [
     "ef_ctx_create", "ef_ctx_destroy", "ef_last_error", "ef_version", "ef_refine_by_function",
     "ef_mesh_create", "ef_mesh_destroy", "ef_mesh_footprint", "ef_mesh_info", "ef_mesh_nodes",
     "ef_mesh_set_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
-    "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync",
+    "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
+    "ef_memcpy", "ef_mesh_stage_ms",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -82,6 +83,10 @@ def load_library(path):
     L.ef_solve_device.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
     L.ef_get_operator.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip]
     L.ef_mesh_sync.argtypes = [_vp]
+    L.ef_device_alloc.argtypes = [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
+    L.ef_device_free.argtypes = [_vp, _vp]
+    L.ef_memcpy.argtypes = [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int]
+    L.ef_mesh_stage_ms.argtypes = [_vp, _dp]
     return L
 
 
@@ -155,6 +160,35 @@ class Context:
         if getattr(self, "h", None) is not None:
             self.L.ef_ctx_destroy(self.h)
             self.h = None
+
+
+class DeviceArray:
+    """A float64 array in the context's device memory (ef_device_alloc)."""
+
+    def __init__(self, ctx, shape):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.size = int(np.prod(self.shape))
+        p = _vp()
+        check(ctx.L.ef_device_alloc(ctx.h, self.size * 8, ctypes.byref(p)), ctx.L)
+        self.ptr = p.value
+
+    def upload(self, host, offset=0):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        if offset + host.size > self.size:
+            raise ValueError("upload past the end of the device array")
+        check(self.ctx.L.ef_memcpy(self.ctx.h, self.ptr + 8 * offset, host.ctypes.data, host.size * 8, 0), self.ctx.L)
+
+    def download(self, count=None):
+        count = self.size if count is None else count
+        out = np.empty(count)
+        check(self.ctx.L.ef_memcpy(self.ctx.h, out.ctypes.data, self.ptr, count * 8, 1), self.ctx.L)
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.ctx.L.ef_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
 
 
 class DeviceMesh:
@@ -237,6 +271,12 @@ class DeviceMesh:
 
     def sync(self):
         self._check(self.L.ef_mesh_sync(self.h))
+
+    def stage_ms(self):
+        """HIP-event times of the last build, upwards and solve calls (ms; -1 if not run)."""
+        out = np.zeros(3)
+        self._check(self.L.ef_mesh_stage_ms(self.h, dptr(out)))
+        return dict(zip(("build-stage", "upwards-stage", "solve-stage"), out.tolist()))
 
     CODES = dict(T=0, S=1, X=2, H=3, u=4, g=5, f=6, h=7, w=8)
 

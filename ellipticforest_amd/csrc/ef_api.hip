@@ -291,6 +291,9 @@ struct ef_mesh_s {
     DevBuf<double> rootM, rootU;
     DevBuf<int> rootpiv, rootinfo;
     DevBuf<GJDesc> rootgj;
+    // stage timers: event pairs for build, upwards, solve
+    hipEvent_t ev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    bool ev_recorded[3] = {false, false, false};
 };
 
 namespace {
@@ -551,6 +554,18 @@ void plan_build_descriptors(ef_mesh_s* M) {
             M->tgemm[dd].push_back(std::move(b3));
         }
     }
+}
+
+void stage_begin(ef_mesh_s* M, int st) {
+    if (!M->ev[st][0]) {
+        EF_HIP(hipEventCreate(&M->ev[st][0]));
+        EF_HIP(hipEventCreate(&M->ev[st][1]));
+    }
+    EF_HIP(hipEventRecord(M->ev[st][0], M->stream));
+}
+void stage_end(ef_mesh_s* M, int st) {
+    EF_HIP(hipEventRecord(M->ev[st][1], M->stream));
+    M->ev_recorded[st] = true;
 }
 
 void do_build(ef_mesh_s* M, const ef_coeffs& c) {
@@ -934,6 +949,9 @@ int ef_mesh_destroy(ef_mesh mesh) {
     if (!mesh) return EF_OK;
     (void)hipSetDevice(mesh->ctx->device);
     if (mesh->stream) (void)hipStreamSynchronize(mesh->stream);
+    for (auto& pr : mesh->ev)
+        for (auto& e : pr)
+            if (e) (void)hipEventDestroy(e);
     hipStream_t s = mesh->own_stream;
     delete mesh;
     if (s) (void)hipStreamDestroy(s);
@@ -1026,7 +1044,9 @@ int ef_build(ef_mesh mesh, const ef_coeffs* coeffs, int precision) {
     mesh->sweeps.clear();
     mesh->k_up = 0;
     mesh->precision = precision;
+    stage_begin(mesh, 0);
     do_build(mesh, *coeffs);
+    stage_end(mesh, 0);
     EF_API_END
 }
 
@@ -1034,7 +1054,9 @@ int ef_upwards_device(ef_mesh mesh, const double* f_dev, int nrhs) {
     EF_API_BEGIN
     if (!mesh || !f_dev) throw EfError(EF_ERR_ARG, "null argument");
     EF_HIP(hipSetDevice(mesh->ctx->device));
+    stage_begin(mesh, 1);
     do_upwards(mesh, f_dev, nrhs);
+    stage_end(mesh, 1);
     EF_API_END
 }
 
@@ -1046,7 +1068,9 @@ int ef_upwards(ef_mesh mesh, const double* f, int nrhs) {
     const size_t cnt = (size_t)nrhs * mesh->P.leaves.size() * mesh->P.N * mesh->P.N;
     mesh->fbuf.ensure(cnt);
     EF_HIP(hipMemcpyAsync(mesh->fbuf.p, f, cnt * sizeof(double), hipMemcpyHostToDevice, mesh->stream));
+    stage_begin(mesh, 1);
     do_upwards(mesh, mesh->fbuf.p, nrhs);
+    stage_end(mesh, 1);
     EF_HIP(hipStreamSynchronize(mesh->stream));
     EF_API_END
 }
@@ -1056,7 +1080,9 @@ int ef_solve_device(ef_mesh mesh, const double* f_dev, const double* r_dev, cons
     EF_API_BEGIN
     if (!mesh || !r_dev || !a_dev || !u_dev) throw EfError(EF_ERR_ARG, "null argument");
     EF_HIP(hipSetDevice(mesh->ctx->device));
+    stage_begin(mesh, 2);
     do_solve(mesh, f_dev, r_dev, a_dev, b_dev, b_dev != nullptr, u_dev, nrhs);
+    stage_end(mesh, 2);
     EF_API_END
 }
 
@@ -1080,7 +1106,9 @@ int ef_solve(ef_mesh mesh, const double* r, const double* a, const double* b, do
     const size_t cnt = (size_t)nrhs * P.leaves.size() * P.N * P.N;
     mesh->ubuf.ensure(cnt);
     const double* f = mesh->homogeneous_rhs ? nullptr : mesh->f_last;
+    stage_begin(mesh, 2);
     do_solve(mesh, f, mesh->rbuf.p, mesh->abuf.p, robin ? mesh->bbuf.p : nullptr, robin, mesh->ubuf.p, nrhs);
+    stage_end(mesh, 2);
     EF_HIP(hipMemcpyAsync(u, mesh->ubuf.p, cnt * sizeof(double), hipMemcpyDeviceToHost, mesh->stream));
     EF_HIP(hipStreamSynchronize(mesh->stream));
     EF_API_END
@@ -1090,6 +1118,47 @@ int ef_mesh_sync(ef_mesh mesh) {
     EF_API_BEGIN
     if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
     EF_HIP(hipStreamSynchronize(mesh->stream));
+    EF_API_END
+}
+
+int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr) {
+    EF_API_BEGIN
+    if (!ctx || !ptr) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(ctx->device));
+    EF_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    EF_API_END
+}
+
+int ef_device_free(ef_ctx ctx, void* ptr) {
+    EF_API_BEGIN
+    if (!ctx) throw EfError(EF_ERR_ARG, "null context");
+    EF_HIP(hipSetDevice(ctx->device));
+    if (ptr) EF_HIP(hipFree(ptr));
+    EF_API_END
+}
+
+int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind) {
+    EF_API_BEGIN
+    if (!ctx || (bytes && (!dst || !src))) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(ctx->device));
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    if (kind < 0 || kind > 2) throw EfError(EF_ERR_ARG, "copy kind must be 0, 1 or 2");
+    if (bytes) EF_HIP(hipMemcpy(dst, src, bytes, k));
+    EF_API_END
+}
+
+int ef_mesh_stage_ms(ef_mesh mesh, double* ms) {
+    EF_API_BEGIN
+    if (!mesh || !ms) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipStreamSynchronize(mesh->stream));
+    for (int st = 0; st < 3; st++) {
+        ms[st] = -1.0;
+        if (!mesh->ev_recorded[st]) continue;
+        float t = 0.0f;
+        EF_HIP(hipEventElapsedTime(&t, mesh->ev[st][0], mesh->ev[st][1]));
+        ms[st] = t;
+    }
     EF_API_END
 }
 

@@ -129,4 +129,34 @@ struct VecStepDesc {
     int nfine;
 };
 
+
+// The planner fills these on the host and the kernels read them on the device: both
+// compiler passes must agree on every size and offset (x86-64 and amdgcn are both LP64).
+static_assert(sizeof(GemmProb) == 120, "GemmProb");
+static_assert(offsetof(GemmProb, A2) == 32, "GemmProb.A2");
+static_assert(offsetof(GemmProb, Cin) == 64, "GemmProb.Cin");
+static_assert(offsetof(GemmProb, D) == 80, "GemmProb.D");
+static_assert(offsetof(GemmProb, alpha) == 104, "GemmProb.alpha");
+static_assert(sizeof(GemmProbF) == 112, "GemmProbF");
+static_assert(sizeof(GJDesc) == 96, "GJDesc");
+static_assert(offsetof(GJDesc, X) == 8, "GJDesc.X");
+static_assert(offsetof(GJDesc, R1) == 24, "GJDesc.R1");
+static_assert(offsetof(GJDesc, R2) == 40, "GJDesc.R2");
+static_assert(offsetof(GJDesc, M) == 56, "GJDesc.M");
+static_assert(offsetof(GJDesc, piv) == 80, "GJDesc.piv");
+static_assert(sizeof(MergeDesc) == 80, "MergeDesc");
+static_assert(offsetof(MergeDesc, Tc) == 8, "MergeDesc.Tc");
+static_assert(sizeof(CoarsenDesc) == 24, "CoarsenDesc");
+static_assert(sizeof(LeafClass) == 48, "LeafClass");
+static_assert(sizeof(LeafDesc) == 72, "LeafDesc");
+static_assert(offsetof(LeafDesc, h_off) == 8, "LeafDesc.h_off");
+static_assert(offsetof(LeafDesc, coef) == 24, "LeafDesc.coef");
+static_assert(sizeof(UpDesc) == 112, "UpDesc");
+static_assert(offsetof(UpDesc, ch_off) == 40, "UpDesc.ch_off");
+static_assert(offsetof(UpDesc, Xi) == 96, "UpDesc.Xi");
+static_assert(sizeof(DownDesc) == 80, "DownDesc");
+static_assert(offsetof(DownDesc, gc_off) == 8, "DownDesc.gc_off");
+static_assert(offsetof(DownDesc, S) == 72, "DownDesc.S");
+static_assert(sizeof(VecStepDesc) == 24, "VecStepDesc");
+
 }  // namespace efk

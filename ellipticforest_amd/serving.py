@@ -13,8 +13,9 @@
   so a request's result is the same alone and in a batch.
 * ``stats()`` reports solves/s and p50/p99 latency (arrival to result on the host).
 
-Device memory comes from a buffer provider: ``TorchBuffers`` on a GPU (torch is used
-only to allocate HBM and move bytes).
+Device memory comes from a buffer provider: ``DeviceBuffers`` allocates HBM and moves
+bytes through the library's own entry points (``ef_device_alloc``, ``ef_memcpy``), so the
+process holds a single HIP runtime.
 """
 from __future__ import annotations
 
@@ -28,27 +29,23 @@ import numpy as np
 from . import _lib
 
 
-class TorchBuffers:
-    """HBM buffers and copies through torch on cuda:0."""
+class DeviceBuffers:
+    """HBM buffers of the context's GPU (``_lib.DeviceArray``)."""
 
-    def __init__(self, device="cuda:0"):
-        import torch
-        self.torch = torch
-        self.device = torch.device(device)
+    def __init__(self, ctx):
+        self.ctx = ctx
 
     def empty(self, shape):
-        return self.torch.empty(shape, dtype=self.torch.float64, device=self.device)
+        return _lib.DeviceArray(self.ctx, shape)
 
     def ptr(self, buf):
-        return buf.data_ptr()
+        return buf.ptr
 
     def upload(self, buf, host, offset=0):
-        flat = buf.view(-1)
-        flat[offset:offset + host.size].copy_(self.torch.from_numpy(np.ascontiguousarray(host).ravel()),
-                                              non_blocking=False)
+        buf.upload(np.ascontiguousarray(host).ravel(), offset)
 
     def download(self, buf, count):
-        return buf.view(-1)[:count].cpu().numpy()
+        return buf.download(count)
 
 
 @dataclass

@@ -132,6 +132,17 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
 /* Wait for all work on the mesh's stream. */
 int ef_mesh_sync(ef_mesh mesh);
 
+/* Device memory of the context's GPU for the *_device entry points (callers that do not
+ * bring their own allocator).  kind: 0 host->device, 1 device->host, 2 device->device. */
+int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr);
+int ef_device_free(ef_ctx ctx, void* ptr);
+int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* Stage timers measured with HIP events on the mesh's stream, in milliseconds, of the
+ * last build, upwards and solve calls: ms[0] build-stage, ms[1] upwards-stage,
+ * ms[2] solve-stage (the timer names of HPSAlgorithm.hpp:125, 183, 295). */
+int ef_mesh_stage_ms(ef_mesh mesh, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

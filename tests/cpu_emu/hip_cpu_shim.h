@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <atomic>
 #include <barrier>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -65,6 +66,16 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
 inline hipError_t hipDeviceSynchronize() { return 0; }
 inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { a->sharedSizeBytes = 0; return 0; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
+
+// events: host wall clock (launches run synchronously in the emulation)
+typedef double* hipEvent_t;
+inline double emu_now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new double(0.0); return 0; }
+inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return 0; }
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { *e = emu_now_ms(); return 0; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(*b - *a); return 0; }
 
 typedef double emu_d4 __attribute__((ext_vector_type(4)));
 emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);

@@ -49,3 +49,34 @@ def test_emu_variable_coefficients(emu_ctx):
     """Block-tridiagonal leaves (bt_factor / bt_dtn / bt_solve) with cfg3's beta and lambda."""
     uo, ug, err = gp.variable_both(emu_ctx, omesh.uniform(1), 8, check_ops=True)
     assert gp.rel(ug, uo) < gp.TOL_U
+
+
+def test_emu_device_memory_and_stage_timers(emu_ctx):
+    """ef_device_alloc / ef_memcpy round trip, the device-pointer stages against the host
+    ones (bitwise: same kernels, same order), and the stage timers of ef_mesh_stage_ms."""
+    from ellipticforest_amd import _lib
+    leaves, N, bounds = omesh.uniform(1), 8, (-1, 1, -1, 1)
+    D = _lib.DeviceMesh(emu_ctx, leaves, N, bounds)
+    assert all(v == -1.0 for v in D.stage_ms().values())  # nothing run yet
+    D.build()
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((2, len(leaves), N * N))
+    r = rng.standard_normal((2, 4 * D.M))
+    a = np.ones(4 * D.M)
+    buf = _lib.DeviceArray(emu_ctx, (7,))
+    buf.upload(np.arange(3.0), offset=2)
+    assert np.array_equal(buf.download()[2:5], np.arange(3.0))
+    with pytest.raises(ValueError):
+        buf.upload(np.zeros(6), offset=2)
+    D.upwards(f)
+    u_host = D.solve(r, a)
+    fd, rd, ad = _lib.DeviceArray(emu_ctx, f.shape), _lib.DeviceArray(emu_ctx, r.shape), _lib.DeviceArray(emu_ctx, a.shape)
+    ud = _lib.DeviceArray(emu_ctx, f.shape)
+    fd.upload(f); rd.upload(r); ad.upload(a)
+    D.upwards_device(fd.ptr, 2)
+    D.solve_device(fd.ptr, rd.ptr, ad.ptr, None, ud.ptr, 2)
+    D.sync()
+    assert np.array_equal(ud.download().reshape(u_host.shape), u_host)
+    st = D.stage_ms()
+    assert set(st) == {"build-stage", "upwards-stage", "solve-stage"}
+    assert all(v >= 0.0 for v in st.values())

@@ -1,6 +1,6 @@
 """Serving loop (ellipticforest_amd/serving.py) driven through the CPU-emulation build of
-the library: host arrays stand in for HBM there, so the device-pointer entry points run
-unchanged.  Checks: a request's result is the same alone and in a batch (bitwise: every
+the library: buffers come from the library's own ef_device_alloc / ef_memcpy (host memory
+in the emulation), so the device-pointer entry points run unchanged.  Checks: a request's result is the same alone and in a batch (bitwise: every
 right-hand side is reduced in a batch-independent order), results match the oracle, and
 meshes are admitted / evicted by footprint."""
 import numpy as np
@@ -9,23 +9,6 @@ import pytest
 import oracle
 from oracle import mesh as omesh
 from ellipticforest_amd import serving
-
-
-class HostBuffers:
-    """Buffer provider for the emulated library: numpy arrays are its device memory."""
-
-    def empty(self, shape):
-        return np.zeros(shape)
-
-    def ptr(self, buf):
-        return buf.ctypes.data
-
-    def upload(self, buf, host, offset=0):
-        flat = buf.reshape(-1)
-        flat[offset:offset + host.size] = np.ascontiguousarray(host).ravel()
-
-    def download(self, buf, count):
-        return buf.reshape(-1)[:count].copy()
 
 
 def reference(leaves, N, bounds, f, r):
@@ -37,7 +20,7 @@ def reference(leaves, N, bounds, f, r):
 
 def test_batch_invariance_and_parity(emu_ctx):
     leaves, N, bounds = omesh.uniform(1), 8, (0.0, 1.0, 0.0, 1.0)
-    srv = serving.Server(emu_ctx, HostBuffers(), max_batch=4)
+    srv = serving.Server(emu_ctx, serving.DeviceBuffers(emu_ctx), max_batch=4)
     dm = srv.add_mesh("m", leaves, N, bounds)
     rng = np.random.default_rng(7)
     reqs = [(rng.standard_normal((len(leaves), N * N)), rng.standard_normal(4 * dm.M)) for _ in range(6)]
@@ -56,7 +39,7 @@ def test_batch_invariance_and_parity(emu_ctx):
 
 def test_admission_by_footprint(emu_ctx):
     leaves, N, bounds = omesh.uniform(1), 8, (0.0, 1.0, 0.0, 1.0)
-    srv = serving.Server(emu_ctx, HostBuffers(), max_batch=2)
+    srv = serving.Server(emu_ctx, serving.DeviceBuffers(emu_ctx), max_batch=2)
     srv.add_mesh("a", leaves, N, bounds)
     one = srv.used_bytes()
     srv.budget = int(one * 1.5)  # room for one mesh only
