@@ -120,6 +120,81 @@ def gpu_run(leaves, N, k, steps, warmup, ref):
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
 
 
+def leaf_bounds(leaves, N):
+    out = []
+    for lv, ix, iy in leaves:
+        w = (BOUNDS[1] - BOUNDS[0]) / (1 << lv)
+        h = (BOUNDS[3] - BOUNDS[2]) / (1 << lv)
+        out.append((BOUNDS[0] + ix * w, BOUNDS[0] + (ix + 1) * w, BOUNDS[2] + iy * h, BOUNDS[2] + (iy + 1) * h))
+    return np.array(out)
+
+
+def patch_inputs(P, N, k, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((k, P, 4 * N)), rng.standard_normal((k, P, N * N))
+
+
+def patch_check_items(P, k, count=24, seed=1):
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(k)), int(p)) for p in rng.choice(P, count, replace=False)]
+
+
+def cpu_patch_baseline(bounds, N, k, sample, seconds=10.0):
+    """Oracle leaf solves (the reference's FiniteVolumeSolver::solve restated), one thread,
+    on the first patches of the workload; also the oracle's answers for the items the GPU
+    result is checked on."""
+    import oracle
+    oracle.set_threads(1)
+    g, f = patch_inputs(len(bounds), N, k)
+    n, t0 = 0, time.perf_counter()
+    while n < sample and time.perf_counter() - t0 < seconds:
+        p = n % len(bounds)
+        oracle.leaf_solve(N, bounds[p], g[0, p], f[0, p])
+        n += 1
+    dt = time.perf_counter() - t0
+    check = {(c, p): oracle.leaf_solve(N, bounds[p], g[c, p], f[c, p]) for c, p in patch_check_items(len(bounds), k)}
+    return dict(value=n / dt, patches=n, seconds=dt, check=check)
+
+
+def patch_bytes_flops(P, N, k):
+    """Algorithmic HBM bytes and flops of one batched leaf-solve launch: read f (N^2) and
+    g (4N), write u (N^2) per patch and right-hand side; four N x N x N products."""
+    NB = 1 if N <= 16 else 2
+    Np = 16 * NB
+    return P * k * (2 * N * N + 4 * N) * 8, P * k * 4 * 2 * Np ** 3
+
+
+def gpu_patch_run(bounds, N, k, steps, warmup, check=None):
+    """Leaf solves of the solve stage as a batch (ef_patches_apply_device, SOLVE): every
+    leaf patch of the workload with k right-hand sides resident in HBM."""
+    import ellipticforest_amd as ef
+    ctx = ef.Context(0)
+    P = len(bounds)
+    pt = ef.Patches(ctx, bounds, N)
+    g_h, f_h = patch_inputs(P, N, k)
+    g, f, u = ef.DeviceArray(ctx, g_h.shape), ef.DeviceArray(ctx, f_h.shape), ef.DeviceArray(ctx, f_h.shape)
+    g.upload(g_h)
+    f.upload(f_h)
+    for _ in range(warmup):
+        pt.apply_device(ef._lib.EF_PATCH_SOLVE, g.ptr, f.ptr, u.ptr, k)
+    pt.sync()
+    kern_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pt.apply_device(ef._lib.EF_PATCH_SOLVE, g.ptr, f.ptr, u.ptr, k)
+        kern_ms.append(pt.last_ms())  # HIP events around the launch on the batch's stream
+    wall = time.perf_counter() - t0
+    err = None
+    if check:
+        uu = u.download().reshape(k, P, N * N)
+        err = max(float(np.abs(uu[c, p] - uo).max() / np.abs(uo).max()) for (c, p), uo in check.items())
+    ms = float(np.mean(kern_ms))
+    nbytes, nflops = patch_bytes_flops(P, N, k)
+    return dict(ms=ms, ms_min=float(np.min(kern_ms)), wall_ms=wall * 1e3 / steps, value=P * k / (ms / 1e3),
+                rel_err=err, bytes=nbytes, flops=nflops, gbs=nbytes / (ms / 1e3) / 1e9,
+                tflops=nflops / (ms / 1e3) / 1e12)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +204,8 @@ def main():
     ap.add_argument("--max-level", type=int, default=7)
     ap.add_argument("--N", type=int, default=16)
     ap.add_argument("--cpu-solves", type=int, default=3)
+    ap.add_argument("--patch-nrhs", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the HPS path of this build runs on a single MI355X; "
@@ -137,9 +214,15 @@ def main():
     leaves = workload(args.max_level, args.N)
     log(f"workload: {len(leaves)} leaves, N={args.N}")
     threads = min(os.cpu_count() or 1, 64)
-    log(f"cpu baseline: oracle build with {threads} threads, {args.cpu_solves} single-thread solves")
-    ref = cpu_baseline(leaves, args.N, args.cpu_solves, threads)
-    log(f"cpu baseline: build {ref['build_s']:.2f}s, solve {ref['solve_s']:.2f}s, disc err {ref['disc_err']:.3e}")
+    if args.no_cpu_baseline:
+        ref = dict(solves_per_s=None, build_s=None, disc_err=None)
+        cpu_patch = dict(value=None, patches=0, seconds=0.0, check=None)
+    else:
+        log(f"cpu baseline: oracle build with {threads} threads, {args.cpu_solves} single-thread solves")
+        ref = cpu_baseline(leaves, args.N, args.cpu_solves, threads)
+        log(f"cpu baseline: build {ref['build_s']:.2f}s, solve {ref['solve_s']:.2f}s, disc err {ref['disc_err']:.3e}")
+        cpu_patch = cpu_patch_baseline(leaf_bounds(leaves, args.N), args.N, args.patch_nrhs, 20000)
+        log(f"cpu baseline: {cpu_patch['value']:.0f} leaf solves/s (oracle, 1 thread)")
     out = dict(metric=METRIC, value=None, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
                data="synthetic (analytic f and Dirichlet data sampled at cell centres / root face points)",
@@ -148,12 +231,34 @@ def main():
                            leaves=len(leaves), nrhs_per_step=args.nrhs, parallelism="single GPU"),
                build_s=None, p50_ms=None, p99_ms=None, rel_err_vs_fp64_oracle=None,
                roofline=None,
-               cpu_baseline=dict(value=ref["solves_per_s"], unit="solves/s", cores=1, kind="port",
-                                 sample="oracle (reference algorithm restated in C++): upwards+solve of 1 RHS on the "
-                                        "full workload, mean of %d passes at 1 thread; its buildStage took %.2f s on "
-                                        "%d threads" % (args.cpu_solves, ref["build_s"], threads),
-                                 build_s=ref["build_s"], build_threads=threads,
-                                 discretization_error=ref["disc_err"]))
+               cpu_baseline=None if args.no_cpu_baseline else dict(
+                   value=ref["solves_per_s"], unit="solves/s", cores=1, kind="port",
+                   sample="oracle (reference algorithm restated in C++): upwards+solve of 1 RHS on the full "
+                          "workload, mean of %d passes at 1 thread; its buildStage took %.2f s on %d threads"
+                          % (args.cpu_solves, ref["build_s"], threads),
+                   build_s=ref["build_s"], build_threads=threads, discretization_error=ref["disc_err"]))
+    # leaf-solve stage on its own: the whole leaf set of the workload as one batch
+    try:
+        bounds = leaf_bounds(leaves, args.N)
+        k = args.patch_nrhs
+        pr = gpu_patch_run(bounds, args.N, k, args.steps, args.warmup, cpu_patch["check"])
+        log(f"leaf solves: {pr['ms']:.4f} ms per launch of {len(bounds)} x {k}, {pr['gbs']:.0f} GB/s, "
+            f"{pr['tflops']:.2f} TFLOP/s, rel err {pr['rel_err']}")
+        out["components"] = [dict(
+            name="leaf solve stage (FiniteVolumeSolver::solve on every leaf, ef_patches_apply_device)",
+            kernel="patch_fd_kernel<1>", value=pr["value"], unit="leaf solves/s", ms_per_launch=pr["ms"],
+            ms_min=pr["ms_min"], wall_ms_per_step=pr["wall_ms"], patches=len(bounds), nrhs=k,
+            rel_err_vs_fp64_oracle=pr["rel_err"], tflops=pr["tflops"],
+            cpu_baseline=dict(value=cpu_patch["value"], unit="leaf solves/s", cores=1, kind="port",
+                              sample="oracle leaf solve (banded LU of the 5-point system), %d patches of the "
+                                     "workload in %.1f s at 1 thread" % (cpu_patch["patches"], cpu_patch["seconds"])))]
+        out["roofline"] = dict(bound="hbm", achieved=pr["gbs"], peak=8000.0, unit="GB/s", frac=pr["gbs"] / 8000.0,
+                               traffic=None, kernel="patch_fd_kernel<1>",
+                               algorithmic_bytes_per_launch=pr["bytes"],
+                               scope="leaf-solve stage only; the build stage is switched off in this library build")
+    except Exception as e:
+        out["components_error"] = f"{type(e).__name__}: {e}"
+        log(out["components_error"])
     try:
         g = gpu_run(leaves, args.N, args.nrhs, args.steps, args.warmup, ref)
         out.update(value=g["value"], ms_per_step=g["ms_per_step"], build_s=g["build_s"], p50_ms=g["p50_ms"],

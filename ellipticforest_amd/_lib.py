@@ -31,8 +31,13 @@ EXPORTS = [
     "ef_mesh_create", "ef_mesh_destroy", "ef_mesh_footprint", "ef_mesh_info", "ef_mesh_nodes",
     "ef_mesh_set_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
-    "ef_memcpy", "ef_mesh_stage_ms",
+    "ef_memcpy", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
+    "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_dtn", "ef_patches_sync",
+    "ef_patches_last_ms",
 ]
+
+EF_PATCH_SOLVE = 0
+EF_PATCH_MAP_D2N = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -49,6 +54,10 @@ class MeshDesc(ctypes.Structure):
 class Coeffs(ctypes.Structure):
     _fields_ = [("solver_type", ctypes.c_int), ("alpha0", ctypes.c_double), ("beta0", ctypes.c_double),
                 ("lambda0", ctypes.c_double), ("cell", _dp)]
+
+
+class PatchBatch(ctypes.Structure):
+    _fields_ = [("npatches", ctypes.c_int), ("N", ctypes.c_int), ("bounds", _dp)]
 
 
 class EfError(RuntimeError):
@@ -87,6 +96,15 @@ def load_library(path):
     L.ef_device_free.argtypes = [_vp, _vp]
     L.ef_memcpy.argtypes = [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int]
     L.ef_mesh_stage_ms.argtypes = [_vp, _dp]
+    L.ef_patches_create.argtypes = [_vp, ctypes.POINTER(PatchBatch), ctypes.POINTER(Coeffs), ctypes.POINTER(_vp)]
+    L.ef_patches_destroy.argtypes = [_vp]
+    L.ef_patches_set_stream.argtypes = [_vp, _vp]
+    L.ef_patches_info.argtypes = [_vp, _ip]
+    L.ef_patches_apply.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+    L.ef_patches_apply_device.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int]
+    L.ef_patches_dtn.argtypes = [_vp, _dp]
+    L.ef_patches_sync.argtypes = [_vp]
+    L.ef_patches_last_ms.argtypes = [_vp, _dp]
     return L
 
 
@@ -298,3 +316,66 @@ class DeviceMesh:
         out = ctypes.c_size_t()
         check(lib().ef_mesh_footprint(ctypes.byref(d), precision, max_rhs, ctypes.byref(out)))
         return int(out.value)
+
+
+class Patches:
+    """A batch of independent N x N leaf patches (FiniteVolumeSolver on one grid each,
+    FiniteVolumeSolver.hpp:122-149) on the device.
+
+    ``bounds`` is [npatches][4] (xl, xu, yl, yu).  ``cell`` (five-point only) holds per-cell
+    samples [npatches][6][N*N] (alpha, beta W/E/S/N, lambda); without it alpha0, beta0
+    and lambda0 apply to every cell."""
+
+    def __init__(self, ctx, bounds, N, solver_type=EF_SOLVER_FISHPACK, lambda0=0.0, alpha0=1.0, beta0=1.0,
+                 cell=None):
+        self.ctx, self.L = ctx, ctx.L
+        self.bounds = f64(np.atleast_2d(bounds))
+        self.P, self.N = self.bounds.shape[0], int(N)
+        self._cell = None if cell is None else f64(cell)
+        b = PatchBatch(self.P, self.N, dptr(self.bounds))
+        c = Coeffs(solver_type, alpha0, beta0, lambda0, dptr(self._cell) if self._cell is not None else None)
+        h = _vp()
+        check(self.L.ef_patches_create(ctx.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(h)), self.L)
+        self.h = h
+        info = np.zeros(3, np.int32)
+        check(self.L.ef_patches_info(self.h, iptr(info)), self.L)
+        self.variable = bool(info[2])
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            self.L.ef_patches_destroy(self.h)
+            self.h = None
+
+    def _apply(self, op, g, f, nrhs):
+        P, N = self.P, self.N
+        g = None if g is None else f64(g).reshape(nrhs, P, 4 * N)
+        f = None if f is None else f64(f).reshape(nrhs, P, N * N)
+        out = np.empty((nrhs, P, N * N if op == EF_PATCH_SOLVE else 4 * N))
+        check(self.L.ef_patches_apply(self.h, op, None if g is None else dptr(g), None if f is None else dptr(f),
+                                      dptr(out), nrhs), self.L)
+        return out
+
+    def solve(self, g, f, nrhs=1):
+        """u [nrhs][npatches][N*N] from Dirichlet data g and load f (either may be None: zero)."""
+        return self._apply(EF_PATCH_SOLVE, g, f, nrhs)
+
+    def map_d2n(self, g, f, nrhs=1):
+        """Neumann data h [nrhs][npatches][4N]; g = None gives particularNeumannData."""
+        return self._apply(EF_PATCH_MAP_D2N, g, f, nrhs)
+
+    def dtn(self):
+        """DtN operator of every patch, [npatches][4N][4N]."""
+        T = np.empty((self.P, 4 * self.N, 4 * self.N))
+        check(self.L.ef_patches_dtn(self.h, dptr(T)), self.L)
+        return T
+
+    def apply_device(self, op, g_ptr, f_ptr, out_ptr, nrhs):
+        check(self.L.ef_patches_apply_device(self.h, op, g_ptr, f_ptr, out_ptr, nrhs), self.L)
+
+    def sync(self):
+        check(self.L.ef_patches_sync(self.h), self.L)
+
+    def last_ms(self):
+        ms = ctypes.c_double(0.0)
+        check(self.L.ef_patches_last_ms(self.h, ctypes.byref(ms)), self.L)
+        return ms.value

@@ -13,70 +13,20 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "ef_hps.h"
 #include "ef_kernels.hpp"
+#include "ef_host.hpp"
 
 using namespace efk;
 
 namespace {
 
 thread_local std::string g_last_error;
-
-struct EfError : std::runtime_error {
-    int code;
-    EfError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define EF_HIP(x)                                                                                    \
-    do {                                                                                             \
-        hipError_t e_ = (x);                                                                         \
-        if (e_ != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-#define EF_LAUNCH_CHECK_NAMED(name)                                                                  \
-    do {                                                                                             \
-        hipError_t e_ = hipGetLastError();                                                           \
-        if (e_ != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string("launch of ") + name + ": " + hipGetErrorString(e_)); \
-    } while (0)
-#define EF_LAUNCH_CHECK() EF_LAUNCH_CHECK_NAMED(__func__ + std::string(":") + std::to_string(__LINE__))
-
-inline unsigned blocks_for(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
-
-// Owning device buffer.
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        release();
-        if (count) EF_HIP(hipMalloc(&p, count * sizeof(T)));
-        n = count;
-    }
-    void ensure(size_t count) {
-        if (count > n) alloc(count);
-    }
-    // Host -> device copy ordered on the stream the kernels that read it run on, and
-    // complete before the host vector can go away.
-    void upload(const std::vector<T>& v, hipStream_t s) {
-        alloc(v.size());
-        if (v.empty()) return;
-        EF_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-        EF_HIP(hipStreamSynchronize(s));
-    }
-};
 
 // ---------------------------------------------------------------------------
 // Tree plan
@@ -245,11 +195,6 @@ struct SweepPlan {  // per right-hand-side count k
 
 }  // namespace
 
-struct ef_ctx_s {
-    int device = 0;
-    hipStream_t stream = nullptr;
-};
-
 struct ef_mesh_s {
     ef_ctx ctx = nullptr;
     hipStream_t own_stream = nullptr;
@@ -298,25 +243,6 @@ struct ef_mesh_s {
 
 namespace {
 
-int fail(const EfError& e) {
-    g_last_error = e.what();
-    return e.code;
-}
-
-#define EF_API_BEGIN try {
-#define EF_API_END                                              \
-    }                                                           \
-    catch (const EfError& e) { return fail(e); }                \
-    catch (const std::bad_alloc&) {                             \
-        g_last_error = "host out of memory";                    \
-        return EF_ERR_INTERNAL;                                 \
-    }                                                           \
-    catch (const std::exception& e) {                           \
-        g_last_error = e.what();                                \
-        return EF_ERR_INTERNAL;                                 \
-    }                                                           \
-    return EF_OK;
-
 const double* leaf_T_ptr(ef_mesh_s* M, int leaf) {
     const long long per = 16LL * M->P.N * M->P.N;
     if (M->variable) return M->leafT.p + (M->cache_operators ? 0 : (long long)leaf) * per;
@@ -345,12 +271,26 @@ void launch_gemm(const DevBuf<GemmProb>& probs, int nprob, int max_tiles, hipStr
 
 int tiles_of(int M_, int N_) { return ((M_ + 63) / 64) * ((N_ + 63) / 64); }
 
+// gj_panel_lds_kernel keeps up to GJ_LDS_ROWS x (GJ_NB + 1) doubles of the panel in dynamic
+// LDS (118 KiB), above the 64 KiB a launch gets without raising the kernel's limit; gfx950
+// has 160 KiB per workgroup, static plus dynamic.  Raised once, on first need.
+void raise_panel_lds_limit(size_t dyn_bytes) {
+    static bool done = false;
+    if (done || dyn_bytes <= 64 * 1024) return;
+    hipFuncAttributes attr;
+    EF_HIP(hipFuncGetAttributes(&attr, (const void*)gj_panel_lds_kernel));
+    const int dyn = 160 * 1024 - (int)attr.sharedSizeBytes;
+    EF_HIP(hipFuncSetAttribute((const void*)gj_panel_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    done = true;
+}
+
 // Gauss-Jordan over a batch of equally sized systems.
 void run_gj(const GJDesc* d_dev, int nsys, int m, int maxw, hipStream_t s) {
     for (int k = 0; k < m; k += GJ_NB) {
         const int nbb = std::min(GJ_NB, m - k);
         if (m - k <= GJ_LDS_ROWS) {
             const size_t lds = (size_t)(m - k) * (GJ_NB + 1) * sizeof(double);
+            raise_panel_lds_limit(lds);
             hipLaunchKernelGGL(gj_panel_lds_kernel, dim3(nsys), dim3(256), lds, s, d_dev, k);
         } else {
             hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nsys), dim3(256), 0, s, d_dev, k);
@@ -392,7 +332,7 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     if (M->variable) {
         if (N > BT_MAX_N) throw EfError(EF_ERR_ARG, "variable coefficients need N <= " + std::to_string(BT_MAX_N));
         M->vcoef.alloc(nl * 6 * NN);
-        EF_HIP(hipMemcpyAsync(M->vcoef.p, c.cell, sizeof(double) * nl * 6 * NN, hipMemcpyHostToDevice, M->stream));
+        efk::copy_h2d(M->vcoef.p, c.cell, sizeof(double) * nl * 6 * NN, M->stream);
         M->vstencil.alloc(nl * 5 * NN);
         M->vSinv.alloc(nl * (size_t)N * NN);
         M->leafT.alloc(nl * 16 * NN);
@@ -640,7 +580,7 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
             launch_gemm(*M->tgemm[dd][gi], 8 * nm, tiles_of(n, 8 * n), s);
             // zero-pivot report of this group
             info_h.assign(nm, 0);
-            EF_HIP(hipMemcpyAsync(info_h.data(), M->winfo.p, sizeof(int) * nm, hipMemcpyDeviceToHost, s));
+            efk::copy_d2h(info_h.data(), M->winfo.p, sizeof(int) * nm, s);
             EF_HIP(hipStreamSynchronize(s));
             for (int i = 0; i < nm; i++)
                 if (info_h[i] != 0)
@@ -859,7 +799,7 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         EF_HIP(hipMemsetAsync(M->rootinfo.p, 0, sizeof(int), s));
         run_gj(M->rootgj.p, 1, m, k, s);
         int info = 0;
-        EF_HIP(hipMemcpyAsync(&info, M->rootinfo.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        efk::copy_d2h(&info, M->rootinfo.p, sizeof(int), s);
         EF_HIP(hipStreamSynchronize(s));
         if (info) throw EfError(EF_ERR_SINGULAR, "zero pivot in the root boundary system");
     }
@@ -880,27 +820,46 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
     launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, (M->homogeneous_rhs || !f_dev) ? 1 : 0, u_dev, k);
 }
 
-void set_lds_limits() {
-    static bool done = false;
-    if (done) return;
-    // gfx950: 160 KiB of LDS per workgroup, static plus dynamic
-    const int total = 160 * 1024;
-    const void* kernels[] = {(const void*)gj_panel_lds_kernel, (const void*)class_dtn_kernel,
-                             (const void*)leaf_fd_kernel};
-    for (const void* kfn : kernels) {
-        hipFuncAttributes attr;
-        EF_HIP(hipFuncGetAttributes(&attr, kfn));
-        const int dyn = total - (int)attr.sharedSizeBytes;
-        EF_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-    }
-    (void)hipGetLastError();
-    done = true;
-}
 
 }  // namespace
 
 namespace efk {
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+std::mutex g_stage_mu;
+void* g_stage = nullptr;  // kept until the process ends
+constexpr size_t STAGE_BYTES = size_t(16) << 20;
+
+void* stage_buffer() {
+    if (!g_stage) EF_HIP(hipHostMalloc(&g_stage, STAGE_BYTES, hipHostMallocDefault));
+    return g_stage;
+}
+}  // namespace
+
+void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    void* st = stage_buffer();
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        std::memcpy(st, (const char*)src + off, n);
+        EF_HIP(hipMemcpyAsync((char*)dst + off, st, n, hipMemcpyHostToDevice, s));
+        EF_HIP(hipStreamSynchronize(s));
+    }
+}
+
+void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    void* st = stage_buffer();
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES) {
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        EF_HIP(hipMemcpyAsync(st, (const char*)src + off, n, hipMemcpyDeviceToHost, s));
+        EF_HIP(hipStreamSynchronize(s));
+        std::memcpy((char*)dst + off, st, n);
+    }
+}
 }  // namespace efk
 
 // ===========================================================================
@@ -918,7 +877,6 @@ int ef_ctx_create(int device, ef_ctx* out) {
     EF_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw EfError(EF_ERR_ARG, "no such device");
     EF_HIP(hipSetDevice(device));
-    set_lds_limits();
     auto* c = new ef_ctx_s();
     c->device = device;
     *out = c;
@@ -1067,7 +1025,7 @@ int ef_upwards(ef_mesh mesh, const double* f, int nrhs) {
     EF_HIP(hipSetDevice(mesh->ctx->device));
     const size_t cnt = (size_t)nrhs * mesh->P.leaves.size() * mesh->P.N * mesh->P.N;
     mesh->fbuf.ensure(cnt);
-    EF_HIP(hipMemcpyAsync(mesh->fbuf.p, f, cnt * sizeof(double), hipMemcpyHostToDevice, mesh->stream));
+    efk::copy_h2d(mesh->fbuf.p, f, cnt * sizeof(double), mesh->stream);
     stage_begin(mesh, 1);
     do_upwards(mesh, mesh->fbuf.p, nrhs);
     stage_end(mesh, 1);
@@ -1100,16 +1058,16 @@ int ef_solve(ef_mesh mesh, const double* r, const double* a, const double* b, do
     mesh->rbuf.ensure((size_t)m * nrhs);
     mesh->abuf.ensure((size_t)m);
     mesh->bbuf.ensure((size_t)m);
-    EF_HIP(hipMemcpyAsync(mesh->rbuf.p, r, sizeof(double) * m * nrhs, hipMemcpyHostToDevice, mesh->stream));
-    EF_HIP(hipMemcpyAsync(mesh->abuf.p, a, sizeof(double) * m, hipMemcpyHostToDevice, mesh->stream));
-    if (robin) EF_HIP(hipMemcpyAsync(mesh->bbuf.p, b, sizeof(double) * m, hipMemcpyHostToDevice, mesh->stream));
+    efk::copy_h2d(mesh->rbuf.p, r, sizeof(double) * m * nrhs, mesh->stream);
+    efk::copy_h2d(mesh->abuf.p, a, sizeof(double) * m, mesh->stream);
+    if (robin) efk::copy_h2d(mesh->bbuf.p, b, sizeof(double) * m, mesh->stream);
     const size_t cnt = (size_t)nrhs * P.leaves.size() * P.N * P.N;
     mesh->ubuf.ensure(cnt);
     const double* f = mesh->homogeneous_rhs ? nullptr : mesh->f_last;
     stage_begin(mesh, 2);
     do_solve(mesh, f, mesh->rbuf.p, mesh->abuf.p, robin ? mesh->bbuf.p : nullptr, robin, mesh->ubuf.p, nrhs);
     stage_end(mesh, 2);
-    EF_HIP(hipMemcpyAsync(u, mesh->ubuf.p, cnt * sizeof(double), hipMemcpyDeviceToHost, mesh->stream));
+    efk::copy_d2h(u, mesh->ubuf.p, cnt * sizeof(double), mesh->stream);
     EF_HIP(hipStreamSynchronize(mesh->stream));
     EF_API_END
 }
@@ -1144,7 +1102,9 @@ int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind) {
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
                                                                           : hipMemcpyDeviceToDevice;
     if (kind < 0 || kind > 2) throw EfError(EF_ERR_ARG, "copy kind must be 0, 1 or 2");
-    if (bytes) EF_HIP(hipMemcpy(dst, src, bytes, k));
+    if (kind == 0) efk::copy_h2d(dst, src, bytes, nullptr);
+    else if (kind == 1) efk::copy_d2h(dst, src, bytes, nullptr);
+    else if (bytes) EF_HIP(hipMemcpy(dst, src, bytes, k));
     EF_API_END
 }
 
@@ -1175,7 +1135,7 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
     auto copy_mat = [&](const double* src, int r, int c) {
         *rows = r; *cols = c;
         if (out) {
-            EF_HIP(hipMemcpyAsync(out, src, sizeof(double) * r * c, hipMemcpyDeviceToHost, M->stream));
+            efk::copy_d2h(out, src, sizeof(double) * r * c, M->stream);
             EF_HIP(hipStreamSynchronize(M->stream));
         }
     };
@@ -1188,7 +1148,7 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
         if (M->kcap == 0) throw EfError(EF_ERR_STATE, "no vectors yet");
         if (rhs < 0 || rhs >= M->kcap) throw EfError(EF_ERR_ARG, "rhs out of range");
         std::vector<double> tmp((size_t)len * k);
-        EF_HIP(hipMemcpyAsync(tmp.data(), M->vec.p + off * k, sizeof(double) * len * k, hipMemcpyDeviceToHost, M->stream));
+        efk::copy_d2h(tmp.data(), M->vec.p + off * k, sizeof(double) * len * k, M->stream);
         EF_HIP(hipStreamSynchronize(M->stream));
         for (int i = 0; i < len; i++) out[i] = tmp[(size_t)i * k + rhs];
     };
@@ -1222,8 +1182,7 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
             *rows = 8 * n; *cols = 4 * n;
             if (out) {
                 std::vector<double> hc(16LL * n * n);
-                EF_HIP(hipMemcpyAsync(hc.data(), M->Har.p + nd.H_off, sizeof(double) * hc.size(), hipMemcpyDeviceToHost,
-                                      M->stream));
+                efk::copy_d2h(hc.data(), M->Har.p + nd.H_off, sizeof(double) * hc.size(), M->stream);
                 EF_HIP(hipStreamSynchronize(M->stream));
                 static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
                 std::fill(out, out + 32LL * n * n, 0.0);

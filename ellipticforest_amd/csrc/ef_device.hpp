@@ -123,6 +123,31 @@ struct DownDesc {
     const double* S;        // 4n x 8n
 };
 
+// Batched patch solves by fast diagonalisation (constant coefficients per patch).
+enum PatchOp { PATCH_OP_SOLVE = 0, PATCH_OP_MAP_D2N = 1, PATCH_OP_DTN = 2 };
+
+struct PatchCoef {
+    double cx, cy, lam;  // stencil: c_W = c_E = cx, c_S = c_N = cy, centre -2cx - 2cy + lam
+    double dx, dy;
+};
+
+struct PatchFDArgs {
+    int N, npatches, nrhs, op;
+    const PatchCoef* coef;
+    const int* cls;         // patch -> coefficient entry, or null (one entry per patch)
+    const double* QA;       // basis in the MFMA A layout, lane-swizzled: [NB*NB*4][64]
+    const double* QB;       // basis in the MFMA B layout, lane-swizzled
+    const double* mu;       // eigenvalues of K, padded to 16*NB entries
+    const double* f;        // loads [nrhs][npatches][N*N], or null (zero)
+    const double* g;        // boundary data, or null (zero); see g_off
+    double* out;            // u [nrhs][npatches][N*N] | h (see h_off) | T [npatches][4N][4N]
+    // Vector addressing.  Null offsets: g and h are [nrhs][npatches][4N].  Otherwise the
+    // multi-RHS arena layout: element t of patch p, rhs c at (off[p] + t) * vk + c.
+    const long long* g_off;
+    const long long* h_off;
+    int vk;
+};
+
 // One interpolation step on vectors in an arena (uncoarsen_: L12 of fine size nfine per side).
 struct VecStepDesc {
     long long src_off, dst_off;
@@ -158,5 +183,9 @@ static_assert(sizeof(DownDesc) == 80, "DownDesc");
 static_assert(offsetof(DownDesc, gc_off) == 8, "DownDesc.gc_off");
 static_assert(offsetof(DownDesc, S) == 72, "DownDesc.S");
 static_assert(sizeof(VecStepDesc) == 24, "VecStepDesc");
+static_assert(sizeof(PatchCoef) == 40, "PatchCoef");
+static_assert(sizeof(PatchFDArgs) == 104, "PatchFDArgs");
+static_assert(offsetof(PatchFDArgs, coef) == 16, "PatchFDArgs.coef");
+static_assert(offsetof(PatchFDArgs, vk) == 96, "PatchFDArgs.vk");
 
 }  // namespace efk

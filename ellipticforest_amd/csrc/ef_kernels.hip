@@ -880,7 +880,8 @@ __device__ inline void bt_substitute(const LeafDesc& L, int Nn, double* b, doubl
     }
 }
 
-// Leaf solve for every (leaf, rhs) with variable coefficients; modes as leaf_fd_kernel.
+// Leaf solve for every (leaf, rhs) with variable coefficients.  Modes: 0 particular
+// Neumann data (g = 0, h out), 1 solve with g (u out), 2 mapD2N with g (h out).
 __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode,
                                                        const double* f, int f_zero, const double* garena,
                                                        double* harena, double* u) {
@@ -890,12 +891,12 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
     const LeafDesc L = leaves[leaf];
     const int tid = threadIdx.x;
     const int NN = Nn * Nn;
-    const double* fl = f + ((long long)rhs * nleaves + leaf) * NN;
+    const double* fl = f_zero ? nullptr : f + ((long long)rhs * nleaves + leaf) * NN;
+    const double* g = (mode != 0) ? garena + L.g_off * k + rhs : nullptr;
     for (int e = tid; e < NN; e += 256) {
         const int i = e / Nn, j = e % Nn;
-        double v = f_zero ? 0.0 : fl[e];
-        if (mode == 1 && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
-            const double* g = garena + L.g_off * k + rhs;
+        double v = fl ? fl[e] : 0.0;
+        if (g && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
             double add = 0.0;
             if (i == 0) add += L.stencil[e] * g[(long long)(0 * Nn + j) * k];
             if (i == Nn - 1) add += L.stencil[NN + e] * g[(long long)(1 * Nn + j) * k];
@@ -918,7 +919,7 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
             const int sd = t / Nn;
             const double dd = (sd < 2) ? L.dx : L.dy;
             const double sg = (sd == W || sd == S) ? 2.0 / dd : -2.0 / dd;
-            h[(long long)t * k] = sg * b[i * Nn + j];
+            h[(long long)t * k] = sg * (b[i * Nn + j] - (g ? g[(long long)t * k] : 0.0));
         }
     }
 }

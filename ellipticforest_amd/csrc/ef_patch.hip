@@ -1,0 +1,358 @@
+// ef_patch.hip -- C ABI of batched leaf-patch operations: FiniteVolumeSolver::solve,
+// mapD2N, particularNeumannData and buildD2N (src/Patches/FiniteVolume/
+// FiniteVolumeSolver.cpp:25-462) over many independent patches at once.
+//
+// Constant coefficients within every patch (FISHPACK semantics, or the five-point
+// stencil with constant alpha, beta, lambda) run on patch_fd_kernel (fast
+// diagonalisation on the f64 MFMA, one wave per patch and right-hand side).  Patches
+// whose per-cell coefficients vary run on the block-tridiagonal kernels (factorised
+// once at creation).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ef_hps.h"
+#include "ef_kernels.hpp"
+#include "ef_host.hpp"
+
+using namespace efk;
+
+struct ef_patches_s {
+    ef_ctx ctx = nullptr;
+    int N = 0, P = 0, NB = 1;
+    long long max_blocks = 2048;
+    bool variable = false;
+    hipStream_t stream = nullptr, own_stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ev_recorded = false;
+    std::vector<PatchCoef> coef_h;
+    // fast-diagonalisation tables
+    DevBuf<PatchCoef> coef;
+    DevBuf<double> QA, QB, mu;
+    // block-tridiagonal path
+    std::vector<LeafDesc> ld_h;
+    DevBuf<LeafDesc> leafdesc;
+    DevBuf<double> vcoef, vstencil, vSinv, vwork;
+    // staging of the host-buffer entry points
+    DevBuf<double> gbuf, fbuf, obuf;
+
+    ~ef_patches_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+};
+
+namespace {
+
+// Workgroups of a patch launch: enough to keep every CU busy (256 CUs x 8); each wave
+// then loops over its items.  EF_PATCH_MAX_BLOCKS, read when a batch is created,
+// overrides it (tuning sweeps, tests).
+long long patch_max_blocks() {
+    const char* e = std::getenv("EF_PATCH_MAX_BLOCKS");
+    const long long x = e ? std::atoll(e) : 0;
+    return x > 0 ? x : 2048LL;
+}
+
+// Basis of K = tridiag(1, -2, 1) with -3 ends (ghost 2g - u on both sides), padded to
+// 16*NB, and the MFMA lane tables: QA[x][y][s] = Q[16x + r][16y + 4s + q],
+// QB[x][y][s] = Q[16x + 4s + q][16y + r], lane = 16q + r.
+void fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, std::vector<double>& mu) {
+    const int Np = 16 * NB;
+    std::vector<double> Q((size_t)Np * Np, 0.0);
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < N; k++) {
+            const double nrm = (k == N - 1) ? std::sqrt(1.0 / N) : std::sqrt(2.0 / N);
+            Q[(size_t)i * Np + k] = nrm * std::sin(M_PI * (double)(k + 1) * ((double)i + 0.5) / (double)N);
+        }
+    mu.assign(Np, 0.0);
+    for (int k = 0; k < N; k++) {
+        const double s = std::sin(M_PI * (double)(k + 1) / (2.0 * N));
+        mu[k] = -4.0 * s * s;
+    }
+    QA.assign((size_t)NB * NB * 4 * 64, 0.0);
+    QB.assign((size_t)NB * NB * 4 * 64, 0.0);
+    for (int x = 0; x < NB; x++)
+        for (int y = 0; y < NB; y++)
+            for (int s = 0; s < 4; s++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int q = lane / 16, r = lane % 16;
+                    const size_t at = ((size_t)(x * NB + y) * 4 + s) * 64 + lane;
+                    QA[at] = Q[(size_t)(16 * x + r) * Np + 16 * y + 4 * s + q];
+                    QB[at] = Q[(size_t)(16 * x + 4 * s + q) * Np + 16 * y + r];
+                }
+}
+
+// True when every per-cell array of the patch is constant and the four face betas agree.
+bool patch_constant(const double* c, long long NN) {
+    const double a0 = c[0], b0 = c[NN], l0 = c[5 * NN];
+    for (long long e = 0; e < NN; e++)
+        if (c[e] != a0 || c[NN + e] != b0 || c[2 * NN + e] != b0 || c[3 * NN + e] != b0 || c[4 * NN + e] != b0 ||
+            c[5 * NN + e] != l0)
+            return false;
+    return true;
+}
+
+void record_begin(ef_patches_s* p) {
+    if (!p->ev[0]) {
+        EF_HIP(hipEventCreate(&p->ev[0]));
+        EF_HIP(hipEventCreate(&p->ev[1]));
+    }
+    EF_HIP(hipEventRecord(p->ev[0], p->stream));
+}
+void record_end(ef_patches_s* p) {
+    EF_HIP(hipEventRecord(p->ev[1], p->stream));
+    p->ev_recorded = true;
+}
+
+void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
+    PatchFDArgs a{};
+    a.N = p->N;
+    a.npatches = p->P;
+    a.nrhs = nrhs;
+    a.op = op;
+    a.coef = p->coef.p;
+    a.cls = nullptr;
+    a.QA = p->QA.p;
+    a.QB = p->QB.p;
+    a.mu = p->mu.p;
+    a.f = f;
+    a.g = g;
+    a.out = out;
+    a.g_off = nullptr;
+    a.h_off = nullptr;
+    a.vk = 1;
+    const long long waves = (op == PATCH_OP_DTN) ? (long long)p->P * 4 * p->N : (long long)p->P * nrhs;
+    // enough workgroups to keep every CU busy; each wave then loops over its items
+    const unsigned blocks = (unsigned)std::min<long long>((waves + 3) / 4, p->max_blocks);
+    if (p->NB == 1)
+        hipLaunchKernelGGL(patch_fd_kernel<1>, dim3(blocks), dim3(256), 0, p->stream, a);
+    else
+        hipLaunchKernelGGL(patch_fd_kernel<2>, dim3(blocks), dim3(256), 0, p->stream, a);
+    EF_LAUNCH_CHECK();
+}
+
+// Block-tridiagonal path, one right-hand side per launch ([patch][...] layouts with k = 1).
+void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
+    const long long NN = (long long)p->N * p->N, B4 = 4LL * p->N;
+    if (op == PATCH_OP_MAP_D2N && g == nullptr) {
+        for (int c = 0; c < nrhs; c++) {
+            hipLaunchKernelGGL(bt_solve_kernel, dim3(p->P, 1), dim3(256), 0, p->stream, p->leafdesc.p, p->N, p->P, 1,
+                               0, f ? f + c * p->P * NN : nullptr, f ? 0 : 1, (const double*)nullptr,
+                               out + c * p->P * B4, (double*)nullptr);
+            EF_LAUNCH_CHECK();
+        }
+        return;
+    }
+    if (g == nullptr) throw EfError(EF_ERR_ARG, "boundary data g is required");
+    const int mode = (op == PATCH_OP_SOLVE) ? 1 : 2;
+    for (int c = 0; c < nrhs; c++) {
+        hipLaunchKernelGGL(bt_solve_kernel, dim3(p->P, 1), dim3(256), 0, p->stream, p->leafdesc.p, p->N, p->P, 1, mode,
+                           f ? f + c * p->P * NN : nullptr, f ? 0 : 1, g + c * p->P * B4,
+                           mode == 2 ? out + c * p->P * B4 : (double*)nullptr,
+                           mode == 1 ? out + c * p->P * NN : (double*)nullptr);
+        EF_LAUNCH_CHECK();
+    }
+}
+
+void apply(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
+    if (op != PATCH_OP_SOLVE && op != PATCH_OP_MAP_D2N) throw EfError(EF_ERR_ARG, "op must be EF_PATCH_SOLVE or EF_PATCH_MAP_D2N");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    if (!out) throw EfError(EF_ERR_ARG, "null output");
+    if (p->variable) launch_bt(p, op, g, f, out, nrhs);
+    else launch_fd(p, op, g, f, out, nrhs);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, ef_patches* out) {
+    EF_API_BEGIN
+    if (!ctx || !b || !c || !out) throw EfError(EF_ERR_ARG, "null argument");
+    if (b->npatches < 1) throw EfError(EF_ERR_ARG, "npatches must be >= 1");
+    if (b->N < 4 || b->N > 32) throw EfError(EF_ERR_ARG, "patch side N must be in [4, 32]");
+    if (!b->bounds) throw EfError(EF_ERR_ARG, "null patch bounds");
+    if (c->solver_type != EF_SOLVER_FISHPACK && c->solver_type != EF_SOLVER_FIVE_POINT)
+        throw EfError(EF_ERR_ARG, "unknown solver type");
+    EF_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<ef_patches_s> p(new ef_patches_s());
+    p->ctx = ctx;
+    p->N = b->N;
+    p->P = b->npatches;
+    p->NB = (b->N <= 16) ? 1 : 2;
+    p->max_blocks = patch_max_blocks();
+    EF_HIP(hipStreamCreate(&p->own_stream));
+    p->stream = p->own_stream;
+    const int N = p->N;
+    const long long NN = (long long)N * N;
+    p->coef_h.resize(p->P);
+    for (int i = 0; i < p->P; i++) {
+        const double* bd = b->bounds + 4LL * i;
+        const double dx = (bd[1] - bd[0]) / N, dy = (bd[3] - bd[2]) / N;
+        if (!(dx > 0.0) || !(dy > 0.0)) throw EfError(EF_ERR_ARG, "patch " + std::to_string(i) + " has empty bounds");
+        PatchCoef& k = p->coef_h[i];
+        k.dx = dx;
+        k.dy = dy;
+        if (c->solver_type == EF_SOLVER_FISHPACK) {
+            k.cx = 1.0 / (dx * dx); k.cy = 1.0 / (dy * dy); k.lam = c->lambda0;
+        } else if (c->cell == nullptr) {
+            k.cx = c->alpha0 * c->beta0 / (dx * dx); k.cy = c->alpha0 * c->beta0 / (dy * dy); k.lam = c->lambda0;
+        } else {
+            const double* cc = c->cell + (long long)i * 6 * NN;
+            if (!patch_constant(cc, NN)) p->variable = true;
+            k.cx = cc[0] * cc[NN] / (dx * dx); k.cy = cc[0] * cc[NN] / (dy * dy); k.lam = cc[5 * NN];
+        }
+    }
+    if (p->variable) {
+        const size_t P = p->P;
+        p->vcoef.alloc(P * 6 * NN);
+        efk::copy_h2d(p->vcoef.p, c->cell, sizeof(double) * P * 6 * NN, p->stream);
+        p->vstencil.alloc(P * 5 * NN);
+        p->vSinv.alloc(P * N * NN);
+        p->ld_h.assign(P, LeafDesc{});
+        for (size_t i = 0; i < P; i++) {
+            LeafDesc& L = p->ld_h[i];
+            L.cls = -1;
+            L.node = (int)i;
+            L.h_off = (long long)i * 4 * N;
+            L.g_off = (long long)i * 4 * N;
+            L.coef = p->vcoef.p + i * 6 * NN;
+            L.stencil = p->vstencil.p + i * 5 * NN;
+            L.Sinv = p->vSinv.p + i * N * NN;
+            L.T = nullptr;
+            L.dx = p->coef_h[i].dx;
+            L.dy = p->coef_h[i].dy;
+        }
+        p->leafdesc.upload(p->ld_h, p->stream);
+        hipLaunchKernelGGL(bt_factor_kernel, dim3(p->P), dim3(256), 0, p->stream, p->leafdesc.p, N);
+        EF_LAUNCH_CHECK();
+    } else {
+        std::vector<double> QA, QB, mu;
+        fd_tables(N, p->NB, QA, QB, mu);
+        p->QA.upload(QA, p->stream);
+        p->QB.upload(QB, p->stream);
+        p->mu.upload(mu, p->stream);
+        p->coef.upload(p->coef_h, p->stream);
+    }
+    EF_HIP(hipStreamSynchronize(p->stream));
+    *out = p.release();
+    EF_API_END
+}
+
+int ef_patches_destroy(ef_patches p) {
+    EF_API_BEGIN
+    if (!p) return EF_OK;
+    (void)hipSetDevice(p->ctx->device);
+    delete p;
+    EF_API_END
+}
+
+int ef_patches_set_stream(ef_patches p, void* stream) {
+    EF_API_BEGIN
+    if (!p) throw EfError(EF_ERR_ARG, "null handle");
+    EF_HIP(hipStreamSynchronize(p->stream));
+    p->stream = stream ? (hipStream_t)stream : p->own_stream;
+    EF_API_END
+}
+
+int ef_patches_info(ef_patches p, int* info) {
+    EF_API_BEGIN
+    if (!p || !info) throw EfError(EF_ERR_ARG, "null argument");
+    info[0] = p->P;
+    info[1] = p->N;
+    info[2] = p->variable ? 1 : 0;
+    EF_API_END
+}
+
+int ef_patches_apply_device(ef_patches p, int op, const double* g_dev, const double* f_dev, double* out_dev, int nrhs) {
+    EF_API_BEGIN
+    if (!p) throw EfError(EF_ERR_ARG, "null handle");
+    EF_HIP(hipSetDevice(p->ctx->device));
+    record_begin(p);
+    apply(p, op, g_dev, f_dev, out_dev, nrhs);
+    record_end(p);
+    EF_API_END
+}
+
+int ef_patches_apply(ef_patches p, int op, const double* g, const double* f, double* out, int nrhs) {
+    EF_API_BEGIN
+    if (!p || !out) throw EfError(EF_ERR_ARG, "null argument");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    EF_HIP(hipSetDevice(p->ctx->device));
+    const size_t NN = (size_t)p->N * p->N, B4 = 4 * (size_t)p->N, P = p->P;
+    const size_t gcnt = (size_t)nrhs * P * B4, fcnt = (size_t)nrhs * P * NN;
+    const size_t ocnt = (op == PATCH_OP_SOLVE) ? fcnt : gcnt;
+    if (g) {
+        p->gbuf.ensure(gcnt);
+        efk::copy_h2d(p->gbuf.p, g, gcnt * sizeof(double), p->stream);
+    }
+    if (f) {
+        p->fbuf.ensure(fcnt);
+        efk::copy_h2d(p->fbuf.p, f, fcnt * sizeof(double), p->stream);
+    }
+    p->obuf.ensure(ocnt);
+    record_begin(p);
+    apply(p, op, g ? p->gbuf.p : nullptr, f ? p->fbuf.p : nullptr, p->obuf.p, nrhs);
+    record_end(p);
+    efk::copy_d2h(out, p->obuf.p, ocnt * sizeof(double), p->stream);
+    EF_HIP(hipStreamSynchronize(p->stream));
+    EF_API_END
+}
+
+int ef_patches_dtn(ef_patches p, double* T) {
+    EF_API_BEGIN
+    if (!p || !T) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(p->ctx->device));
+    const size_t per = 16 * (size_t)p->N * p->N;
+    p->obuf.ensure(per * p->P);
+    record_begin(p);
+    if (p->variable) {
+        // bt_dtn_kernel writes each leaf's T through its descriptor
+        for (int i = 0; i < p->P; i++) p->ld_h[i].T = p->obuf.p + (size_t)i * per;
+        p->leafdesc.upload(p->ld_h, p->stream);
+        const long long zper = (long long)p->N * p->N * 4 * p->N;
+        const int chunk = (int)std::max<long long>(1, std::min<long long>(p->P, (256LL << 20) / (zper * 8)));
+        p->vwork.ensure((size_t)chunk * zper);
+        for (int first = 0; first < p->P; first += chunk) {
+            const int cnt = std::min(chunk, p->P - first);
+            hipLaunchKernelGGL(bt_dtn_kernel, dim3(cnt), dim3(256), 0, p->stream, p->leafdesc.p, p->N, first, p->vwork.p);
+            EF_LAUNCH_CHECK();
+        }
+    } else {
+        launch_fd(p, PATCH_OP_DTN, nullptr, nullptr, p->obuf.p, 1);
+    }
+    record_end(p);
+    efk::copy_d2h(T, p->obuf.p, per * p->P * sizeof(double), p->stream);
+    EF_HIP(hipStreamSynchronize(p->stream));
+    EF_API_END
+}
+
+int ef_patches_sync(ef_patches p) {
+    EF_API_BEGIN
+    if (!p) throw EfError(EF_ERR_ARG, "null handle");
+    EF_HIP(hipStreamSynchronize(p->stream));
+    EF_API_END
+}
+
+int ef_patches_last_ms(ef_patches p, double* ms) {
+    EF_API_BEGIN
+    if (!p || !ms) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipStreamSynchronize(p->stream));
+    *ms = -1.0;
+    if (p->ev_recorded) {
+        float t = 0.0f;
+        EF_HIP(hipEventElapsedTime(&t, p->ev[0], p->ev[1]));
+        *ms = t;
+    }
+    EF_API_END
+}
+
+}  // extern "C"

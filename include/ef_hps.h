@@ -9,7 +9,7 @@
  *   Matrix/Vector dgesv, dgemm, dgemv wrappers on that path (src/Matrix.hpp:789-953)
  *   L21 / L12 interpolation operators                   (src/SpecialMatrices.hpp:93-173)
  * The C++ drop-in headers (include/EllipticForest.hpp) and the Python host mirror
- * (ellipticforest_amd/hps.py) call only these entry points.
+ * (ellipticforest_amd/_lib.py) call only these entry points.
  *
  * Conventions
  *  - Every function returns 0 on success or a nonzero EF_ERR_* code; no exception
@@ -142,6 +142,43 @@ int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
  * last build, upwards and solve calls: ms[0] build-stage, ms[1] upwards-stage,
  * ms[2] solve-stage (the timer names of HPSAlgorithm.hpp:125, 183, 295). */
 int ef_mesh_stage_ms(ef_mesh mesh, double* ms);
+
+/* ------------------------------------------------------------------------------------
+ * Batched leaf patches: the FiniteVolumeSolver operations on one grid
+ * (FiniteVolumeSolver.hpp:122-149, FiniteVolumeSolver.cpp:25-462) applied to many
+ * independent N x N patches at once.  Patches with constant coefficients (FISHPACK
+ * semantics, or five-point with alpha, beta, lambda constant within the patch) use fast
+ * diagonalisation; otherwise the five-point matrix is factorised once at creation.
+ * Layouts: g and h are [rhs][patch][4N] (WESN), f and u are [rhs][patch][N*N].
+ * ------------------------------------------------------------------------------------ */
+typedef struct ef_patches_s* ef_patches;
+
+typedef struct ef_patch_batch {
+    int npatches;
+    int N;                  /* cells per side, 4 <= N <= 32                  */
+    const double* bounds;   /* [npatches][4]: xl, xu, yl, yu of each patch   */
+} ef_patch_batch;
+
+#define EF_PATCH_SOLVE 0     /* FiniteVolumeSolver::solve (:25-294): u from g, f            */
+#define EF_PATCH_MAP_D2N 1   /* FiniteVolumeSolver::mapD2N (:296-353): h from g, f;
+                                with g == NULL, particularNeumannData (:457-462)            */
+
+/* Set up a batch.  Coefficients as for ef_build; per-cell samples (coeffs->cell) are
+ * [npatches][6][N*N] (alpha, beta W/E/S/N faces, lambda). */
+int ef_patches_create(ef_ctx ctx, const ef_patch_batch* batch, const ef_coeffs* coeffs, ef_patches* out);
+int ef_patches_destroy(ef_patches p);
+int ef_patches_set_stream(ef_patches p, void* stream);
+/* info[0] npatches, info[1] N, info[2] 1 if the factorised (variable-coefficient) path is used */
+int ef_patches_info(ef_patches p, int* info);
+/* op = EF_PATCH_SOLVE (out = u) or EF_PATCH_MAP_D2N (out = h); g or f may be NULL (zero). */
+int ef_patches_apply(ef_patches p, int op, const double* g, const double* f, double* out, int nrhs);
+int ef_patches_apply_device(ef_patches p, int op, const double* g_dev, const double* f_dev, double* out_dev,
+                            int nrhs);
+/* FiniteVolumeSolver::buildD2N (:355-455) of every patch: T is [npatches][4N][4N]. */
+int ef_patches_dtn(ef_patches p, double* T);
+int ef_patches_sync(ef_patches p);
+/* HIP-event time (ms) of the kernels of the last apply or dtn call on the batch's stream. */
+int ef_patches_last_ms(ef_patches p, double* ms);
 
 #ifdef __cplusplus
 }

@@ -58,3 +58,18 @@ def ctx():
         pytest.fail("GPU test selected but no GPU device is visible")
     print(f"[gpu] creating context on device 0 ({_lib.LIB_PATH})", flush=True)
     return ef.Context(0)
+
+
+@pytest.fixture(scope="session")
+def patch_ctx():
+    """Context for the batched patch entry points (ef_patches_*).  They share no kernel
+    with the build stage, so they run while that path is switched off."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    if EMULATOR:
+        print("[gpu] patch context on the CPU emulator", flush=True)
+        return ef.Context(0, library=emulator_library())
+    if not gpu_present():
+        pytest.fail("GPU test selected but no GPU device is visible")
+    print(f"[gpu] creating context on device 0 ({_lib.LIB_PATH})", flush=True)
+    return ef.Context(0)

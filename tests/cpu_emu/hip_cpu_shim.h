@@ -63,6 +63,9 @@ inline hipError_t hipSetDevice(int) { return 0; }
 inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (hipStream_t)0x1; return 0; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+#define hipHostMallocDefault 0u
+inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { *p = std::malloc(n ? n : 1); return *p ? 0 : 2; }
+inline hipError_t hipHostFree(void* p) { std::free(p); return 0; }
 inline hipError_t hipDeviceSynchronize() { return 0; }
 inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { a->sharedSizeBytes = 0; return 0; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
