@@ -32,7 +32,7 @@ __global__ void bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int
                                 int f_zero, const double* garena, double* harena, double* u);
 __global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work);
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)
-template <int NB>
+template <int NB, bool FULL, int OP>
 __global__ void patch_fd_kernel(PatchFDArgs p);
 __global__ void fill_kernel(double* p, double v, long long n);
 __global__ void copy_kernel(const double* src, double* dst, long long n);

@@ -132,10 +132,22 @@ void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double
     const long long waves = (op == PATCH_OP_DTN) ? (long long)p->P * 4 * p->N : (long long)p->P * nrhs;
     // enough workgroups to keep every CU busy; each wave then loops over its items
     const unsigned blocks = (unsigned)std::min<long long>((waves + 3) / 4, p->max_blocks);
-    if (p->NB == 1)
-        hipLaunchKernelGGL(patch_fd_kernel<1>, dim3(blocks), dim3(256), 0, p->stream, a);
-    else
-        hipLaunchKernelGGL(patch_fd_kernel<2>, dim3(blocks), dim3(256), 0, p->stream, a);
+    const bool full = (p->N == 16 * p->NB);
+    const dim3 grid(blocks), block(256);
+#define EF_PATCH_LAUNCH(NB_, FULL_)                                                                         \
+    do {                                                                                                     \
+        if (op == PATCH_OP_SOLVE) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE>), grid, block, 0, p->stream, a); \
+        else if (op == PATCH_OP_MAP_D2N) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N>), grid, block, 0, p->stream, a); \
+        else hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN>), grid, block, 0, p->stream, a); \
+    } while (0)
+    if (p->NB == 1) {
+        if (full) EF_PATCH_LAUNCH(1, true);
+        else EF_PATCH_LAUNCH(1, false);
+    } else {
+        if (full) EF_PATCH_LAUNCH(2, true);
+        else EF_PATCH_LAUNCH(2, false);
+    }
+#undef EF_PATCH_LAUNCH
     EF_LAUNCH_CHECK();
 }
 

@@ -9,10 +9,11 @@
 //     U = Q ((Q^T B Q) ./ D) Q^T,   D[k][l] = cx mu_k + cy mu_l + lam,
 // where B is the load with the boundary data folded in (-2 c g at the adjacent cell).
 //
-// One wave per (patch, right-hand side) at a time, four independent waves per workgroup
-// looping over the items with a grid stride (the basis tables stay in registers), no LDS.
-// Every product is done on v_mfma_f64_16x16x4_f64 in registers.  With q = lane/16 and
-// r = lane%16, the gfx950 operand layouts for a 16x16 block are
+// Work: one wave per (patch, right-hand side) item, four independent waves per
+// workgroup, a grid-stride loop over the items (the basis tables stay in registers) with
+// the next item's load and boundary data fetched while the current one is computed.
+// No LDS.  Every product is done on v_mfma_f64_16x16x4_f64 in registers.  With
+// q = lane/16 and r = lane%16, the gfx950 operand layouts for a 16x16 block are
 //     A operand, step s: A[r][4s + q]        B operand, step s: B[4s + q][r]
 //     accumulator slot v:  C[4v + q][r]
 // so an accumulator block IS the B operand of the next product, and the accumulator of
@@ -22,7 +23,10 @@
 //     Z3   = Z2 ./ D        (in the accumulator)
 //     Z4^T = Z3^T Q^T       (A = Z3^T from the accumulator of Z3)
 //     U    = Z4 Q^T         (A = Z4 from the accumulator of Z4^T)
-// Patches with N <= 16 use one padded 16x16 block, N <= 32 a 2x2 block grid (NB).
+// Patches with N <= 16 use one padded 16x16 block, N <= 32 a 2x2 block grid (NB);
+// FULL marks N = 16 NB (no padding, no bounds tests).  The boundary vector of an item
+// (4N values) is read with one coalesced load per 64 entries and handed to the lanes
+// that need it by cross-lane shuffles.
 #include <hip/hip_runtime.h>
 #include "ef_device.hpp"
 #include "ef_kernels.hpp"
@@ -47,18 +51,80 @@ __device__ inline void patch_mm(FA a, FB b, pd4 (&C)[NB][NB]) {
         }
 }
 
+// Inputs of one item held in registers: the load in the A layout of B^T and the
+// boundary vector spread over the lanes (entry t in lane t % 64 of register t / 64).
 template <int NB>
+struct PatchItemIn {
+    double F[NB][NB][4];
+    double G[NB];
+};
+
+template <int NB, bool FULL, int OP>
+__device__ inline void patch_fetch(const PatchFDArgs& p, int patch, int rhs, int lane, PatchItemIn<NB>& in) {
+    const int Nn = p.N, q = lane >> 4, r = lane & 15;
+    const long long slot = (long long)rhs * p.npatches + patch;
+    const double* f = p.f ? p.f + slot * Nn * Nn : nullptr;
+#pragma unroll
+    for (int bi = 0; bi < NB; bi++)
+#pragma unroll
+        for (int bk = 0; bk < NB; bk++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int i = 16 * bk + 4 * s + q, j = 16 * bi + r;
+                double v = 0.0;
+                if (f) {
+                    if (FULL) {
+                        v = f[i * Nn + j];
+                    } else {
+                        // clamped address, then select: no divergent branch around the load
+                        const double x = f[min(i, Nn - 1) * Nn + min(j, Nn - 1)];
+                        v = (i < Nn && j < Nn) ? x : 0.0;
+                    }
+                }
+                in.F[bi][bk][s] = v;
+            }
+    if (OP == PATCH_OP_DTN) return;
+    const double* g = nullptr;
+    long long gs = 1;
+    if (p.g) {
+        if (p.g_off) { g = p.g + p.g_off[patch] * p.vk + rhs; gs = p.vk; }
+        else g = p.g + slot * 4 * Nn;
+    }
+#pragma unroll
+    for (int m = 0; m < NB; m++) {
+        const int t = 64 * m + lane;
+        in.G[m] = (g && t < 4 * Nn) ? g[(long long)t * gs] : 0.0;
+    }
+}
+
+// Boundary value at WESN position t of the current item (t may differ per lane).
+template <int NB, int OP>
+__device__ inline double patch_g(const PatchItemIn<NB>& in, int t, int col) {
+    if (OP == PATCH_OP_DTN) return (t == col) ? 1.0 : 0.0;
+    if (NB == 1) return __shfl(in.G[0], t & 63);
+    const double a = __shfl(in.G[0], t & 63);
+    const double b = __shfl(in.G[NB - 1], t & 63);
+    return (t < 64) ? a : b;
+}
+
+template <int NB, bool FULL, int OP>
 __global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
     const int Nn = p.N;
-    const long long total = (p.op == PATCH_OP_DTN) ? (long long)p.npatches * 4 * Nn : (long long)p.npatches * p.nrhs;
+    // items: (rhs, patch) rhs-major, or (patch, DtN column) patch-major
+    const int inner = (OP == PATCH_OP_DTN) ? 4 * Nn : p.npatches;
+    const int outer = (OP == PATCH_OP_DTN) ? p.npatches : p.nrhs;
+    const long long first = (long long)blockIdx.x * 4 + wave;
+    if (first >= (long long)inner * outer) return;  // uniform per wave
+    const int stride = (int)gridDim.x * 4;
+    const int step_o = stride / inner, step_i = stride % inner;
 
-    // Q in both operand layouts, loaded once per wave: QA[x][y][s] = Q[16x + r][16y + 4s + q],
-    // QB[x][y][s] = Q[16x + 4s + q][16y + r]
-    double QA[NB][NB][4], QB[NB][NB][4];
+    // basis tables, loaded once per wave: QA[x][y][s] = Q[16x + r][16y + 4s + q],
+    // QB[x][y][s] = Q[16x + 4s + q][16y + r]; eigenvalues of this lane's modes
+    double QA[NB][NB][4], QB[NB][NB][4], muk[NB][4], mul[NB];
 #pragma unroll
-    for (int x = 0; x < NB; x++)
+    for (int x = 0; x < NB; x++) {
 #pragma unroll
         for (int y = 0; y < NB; y++)
 #pragma unroll
@@ -66,57 +132,56 @@ __global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
                 QA[x][y][s] = p.QA[(((x * NB) + y) * 4 + s) * 64 + lane];
                 QB[x][y][s] = p.QB[(((x * NB) + y) * 4 + s) * 64 + lane];
             }
+#pragma unroll
+        for (int v = 0; v < 4; v++) muk[x][v] = p.mu[16 * x + 4 * v + q];
+        mul[x] = p.mu[16 * x + r];
+    }
 
-    // grid-stride loop over (patch, rhs) items; the trip count is uniform per wave
-    for (long long item = (long long)blockIdx.x * 4 + wave; item < total; item += (long long)gridDim.x * 4) {
-        int patch, rhs, col = -1;
-        if (p.op == PATCH_OP_DTN) {
-            patch = (int)(item / (4 * Nn));
-            col = (int)(item % (4 * Nn));
-            rhs = 0;
-        } else {
-            rhs = (int)(item / p.npatches);
-            patch = (int)(item % p.npatches);
+    int o = (int)(first / inner), in_i = (int)(first % inner);
+    PatchItemIn<NB> cur{}, nxt{};
+    {
+        const int patch = (OP == PATCH_OP_DTN) ? o : in_i, rhs = (OP == PATCH_OP_DTN) ? 0 : o;
+        patch_fetch<NB, FULL, OP>(p, patch, rhs, lane, cur);
+    }
+    while (o < outer) {
+        const int patch = (OP == PATCH_OP_DTN) ? o : in_i;
+        const int rhs = (OP == PATCH_OP_DTN) ? 0 : o;
+        const int col = (OP == PATCH_OP_DTN) ? in_i : -1;
+        // next item, fetched while this one is computed
+        int o2 = o + step_o, i2 = in_i + step_i;
+        if (i2 >= inner) { i2 -= inner; o2++; }
+        if (o2 < outer) {
+            const int patch2 = (OP == PATCH_OP_DTN) ? o2 : i2, rhs2 = (OP == PATCH_OP_DTN) ? 0 : o2;
+            patch_fetch<NB, FULL, OP>(p, patch2, rhs2, lane, nxt);
         }
         const PatchCoef cf = p.coef[p.cls ? p.cls[patch] : patch];
-        const long long slot = (long long)rhs * p.npatches + patch;
-        const double* f = p.f ? p.f + slot * Nn * Nn : nullptr;
-        const double* g = nullptr;
-        long long gs = 1;
-        if (p.g && col < 0) {
-            if (p.g_off) { g = p.g + p.g_off[patch] * p.vk + rhs; gs = p.vk; }
-            else g = p.g + slot * 4 * Nn;
-        }
-        // boundary value at WESN position t (side*N + i)
-        auto gval = [&](int t) -> double {
-            if (col >= 0) return (t == col) ? 1.0 : 0.0;
-            return g ? g[t * gs] : 0.0;
-        };
-        const bool use_g = (col >= 0) || (g != nullptr);
+        const bool has_g = (OP == PATCH_OP_DTN) || p.g != nullptr;
 
-        // B^T in the A layout: element (bi, bk, s) is B[i][j] with i = 16bk + 4s + q, j = 16bi + r
+        // fold the boundary data into the load: -2 (c_W g_W + c_E g_E) on the x-edge
+        // cells, -2 (c_S g_S + c_N g_N) on the y-edge cells
         double BT[NB][NB][4];
 #pragma unroll
-        for (int bi = 0; bi < NB; bi++)
+        for (int bi = 0; bi < NB; bi++) {
+            const int j = 16 * bi + r;
+            const double gw = has_g ? patch_g<NB, OP>(cur, W * Nn + min(j, Nn - 1), col) : 0.0;
+            const double ge = has_g ? patch_g<NB, OP>(cur, E * Nn + min(j, Nn - 1), col) : 0.0;
 #pragma unroll
             for (int bk = 0; bk < NB; bk++)
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    const int i = 16 * bk + 4 * s + q, j = 16 * bi + r;
-                    double v = 0.0;
-                    if (i < Nn && j < Nn) {
-                        if (f) v = f[i * Nn + j];
-                        if (use_g && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
-                            double add = 0.0;
-                            if (i == 0) add += cf.cx * gval(W * Nn + j);
-                            if (i == Nn - 1) add += cf.cx * gval(E * Nn + j);
-                            if (j == 0) add += cf.cy * gval(S * Nn + i);
-                            if (j == Nn - 1) add += cf.cy * gval(N * Nn + i);
-                            v += -2.0 * add;
-                        }
+                    const int i = 16 * bk + 4 * s + q;
+                    double v = cur.F[bi][bk][s];
+                    if (has_g) {
+                        const double gsv = patch_g<NB, OP>(cur, S * Nn + min(i, Nn - 1), col);
+                        const double gnv = patch_g<NB, OP>(cur, N * Nn + min(i, Nn - 1), col);
+                        const bool inb = FULL || (i < Nn && j < Nn);
+                        const double ax = (i == 0 ? gw : 0.0) + (i == Nn - 1 ? ge : 0.0);
+                        const double ay = (j == 0 ? gsv : 0.0) + (j == Nn - 1 ? gnv : 0.0);
+                        v += inb ? -2.0 * (cf.cx * ax + cf.cy * ay) : 0.0;
                     }
                     BT[bi][bk][s] = v;
                 }
+        }
 
         pd4 Z[NB][NB], Y[NB][NB];
         // Z1^T = B^T Q
@@ -133,9 +198,9 @@ __global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int kk = 16 * bi + 4 * v + q, ll = 16 * bj + r;
-                    double z = 0.0;
-                    if (kk < Nn && ll < Nn) z = Y[bi][bj][v] / (cf.cx * p.mu[kk] + cf.cy * p.mu[ll] + cf.lam);
-                    Y[bi][bj][v] = z;
+                    const double d = cf.cx * muk[bi][v] + cf.cy * mul[bj] + cf.lam;
+                    const double z = Y[bi][bj][v] / d;
+                    Y[bi][bj][v] = (FULL || (kk < Nn && ll < Nn)) ? z : 0.0;
                 }
         // Z4^T = Z3^T Q^T: A operand of Z3^T = accumulator of Z3 swapped; B operand of Q^T = QA swapped
         patch_mm<NB>([&](int bi, int bk, int s) { return Y[bk][bi][s]; },
@@ -144,8 +209,8 @@ __global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
         patch_mm<NB>([&](int bi, int bk, int s) { return Z[bk][bi][s]; },
                      [&](int bk, int bj, int s) { return QA[bj][bk][s]; }, Y);
 
-        if (p.op == PATCH_OP_SOLVE) {
-            double* u = p.out + slot * Nn * Nn;
+        if (OP == PATCH_OP_SOLVE) {
+            double* u = p.out + ((long long)rhs * p.npatches + patch) * Nn * Nn;
 #pragma unroll
             for (int bi = 0; bi < NB; bi++)
 #pragma unroll
@@ -153,44 +218,59 @@ __global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         const int i = 16 * bi + 4 * v + q, j = 16 * bj + r;
-                        if (i < Nn && j < Nn) u[i * Nn + j] = Y[bi][bj][v];
+                        if (FULL || (i < Nn && j < Nn)) u[i * Nn + j] = Y[bi][bj][v];
                     }
-            continue;
-        }
-        // Neumann data (mapD2N, FiniteVolumeSolver.cpp:296-353):
-        //   h_W = (2/dx)(u_W - g_W), h_E = -(2/dx)(u_E - g_E), h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N)
-        // written to out[t * ostride] of this (patch, rhs) or of DtN column col.
-        double* h;
-        long long ostride;
-        if (col >= 0) {
-            h = p.out + (long long)patch * 16 * Nn * Nn + col;
-            ostride = 4 * Nn;
-        } else if (p.h_off) {
-            h = p.out + p.h_off[patch] * p.vk + rhs;
-            ostride = p.vk;
         } else {
-            h = p.out + slot * 4 * Nn;
-            ostride = 1;
+            // Neumann data (mapD2N, FiniteVolumeSolver.cpp:296-353):
+            //   h_W = (2/dx)(u_W - g_W), h_E = -(2/dx)(u_E - g_E),
+            //   h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N)
+            // into h of this (patch, rhs), or column col of the patch's DtN.
+            double* h;
+            long long ostride;
+            if (OP == PATCH_OP_DTN) {
+                h = p.out + (long long)patch * 16 * Nn * Nn + col;
+                ostride = 4 * Nn;
+            } else if (p.h_off) {
+                h = p.out + p.h_off[patch] * p.vk + rhs;
+                ostride = p.vk;
+            } else {
+                h = p.out + ((long long)rhs * p.npatches + patch) * 4 * Nn;
+                ostride = 1;
+            }
+            const double sx = 2.0 / cf.dx, sy = 2.0 / cf.dy;
+#pragma unroll
+            for (int bi = 0; bi < NB; bi++)
+#pragma unroll
+                for (int bj = 0; bj < NB; bj++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int i = 16 * bi + 4 * v + q, j = 16 * bj + r;
+                        const int ic = min(i, Nn - 1), jc = min(j, Nn - 1);
+                        const double uu = Y[bi][bj][v];
+                        const double gw = has_g ? patch_g<NB, OP>(cur, W * Nn + jc, col) : 0.0;
+                        const double ge = has_g ? patch_g<NB, OP>(cur, E * Nn + jc, col) : 0.0;
+                        const double gsv = has_g ? patch_g<NB, OP>(cur, S * Nn + ic, col) : 0.0;
+                        const double gnv = has_g ? patch_g<NB, OP>(cur, N * Nn + ic, col) : 0.0;
+                        if (i >= Nn || j >= Nn) continue;
+                        if (i == 0) h[(long long)(W * Nn + j) * ostride] = sx * (uu - gw);
+                        if (i == Nn - 1) h[(long long)(E * Nn + j) * ostride] = -sx * (uu - ge);
+                        if (j == 0) h[(long long)(S * Nn + i) * ostride] = sy * (uu - gsv);
+                        if (j == Nn - 1) h[(long long)(N * Nn + i) * ostride] = -sy * (uu - gnv);
+                    }
         }
-        const double sx = 2.0 / cf.dx, sy = 2.0 / cf.dy;
-#pragma unroll
-        for (int bi = 0; bi < NB; bi++)
-#pragma unroll
-            for (int bj = 0; bj < NB; bj++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int i = 16 * bi + 4 * v + q, j = 16 * bj + r;
-                    if (i >= Nn || j >= Nn) continue;
-                    const double uu = Y[bi][bj][v];
-                    if (i == 0) h[(long long)(W * Nn + j) * ostride] = sx * (uu - gval(W * Nn + j));
-                    if (i == Nn - 1) h[(long long)(E * Nn + j) * ostride] = -sx * (uu - gval(E * Nn + j));
-                    if (j == 0) h[(long long)(S * Nn + i) * ostride] = sy * (uu - gval(S * Nn + i));
-                    if (j == Nn - 1) h[(long long)(N * Nn + i) * ostride] = -sy * (uu - gval(N * Nn + i));
-                }
+        o = o2;
+        in_i = i2;
+        cur = nxt;
     }
 }
 
-template __global__ void patch_fd_kernel<1>(PatchFDArgs p);
-template __global__ void patch_fd_kernel<2>(PatchFDArgs p);
+#define EF_PATCH_INSTANCES(NB, FULL)                                              \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_SOLVE>(PatchFDArgs); \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_MAP_D2N>(PatchFDArgs); \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_DTN>(PatchFDArgs);
+EF_PATCH_INSTANCES(1, true)
+EF_PATCH_INSTANCES(1, false)
+EF_PATCH_INSTANCES(2, true)
+EF_PATCH_INSTANCES(2, false)
 
 }  // namespace efk

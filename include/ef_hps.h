@@ -133,7 +133,9 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
 int ef_mesh_sync(ef_mesh mesh);
 
 /* Device memory of the context's GPU for the *_device entry points (callers that do not
- * bring their own allocator).  kind: 0 host->device, 1 device->host, 2 device->device. */
+ * bring their own allocator).  kind: 0 host->device, 1 device->host, 2 device->device
+ * (whole, aligned doubles run on the library's copy kernel).  All three return when the
+ * copy is complete. */
 int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_device_free(ef_ctx ctx, void* ptr);
 int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);

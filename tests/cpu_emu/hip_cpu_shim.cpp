@@ -80,6 +80,16 @@ emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int) {
     return out;
 }
 
+static double g_shfl[NT];
+double __shfl(double v, int src, int) {
+    const int t = threadIdx.x, w = t / 64 * 64;
+    g_shfl[t] = v;
+    emu_wave_sync();
+    const double out = g_shfl[w + (src & 63)];
+    emu_wave_sync();
+    return out;
+}
+
 // Persistent pool of 256 workers (one per lane of a block).  Per block the main thread
 // publishes the body, releases the workers through g_start and waits on g_end.  The
 // block and wave barriers are rebuilt per block because a finishing thread drops out

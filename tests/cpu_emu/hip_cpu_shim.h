@@ -82,6 +82,8 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { *
 
 typedef double emu_d4 __attribute__((ext_vector_type(4)));
 emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);
+// cross-lane read within a wave (ds_bpermute semantics: lane reads src % 64 of its wave)
+double __shfl(double v, int src, int width = 64);
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 emu_mfma_f64
 
 void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body);

@@ -68,6 +68,9 @@ def test_emu_device_memory_and_stage_timers(emu_ctx):
     assert np.array_equal(buf.download()[2:5], np.arange(3.0))
     with pytest.raises(ValueError):
         buf.upload(np.zeros(6), offset=2)
+    other = _lib.DeviceArray(emu_ctx, (7,))
+    _lib.check(emu_ctx.L.ef_memcpy(emu_ctx.h, other.ptr, buf.ptr, 7 * 8, 2), emu_ctx.L)  # copy kernel
+    assert np.array_equal(other.download()[2:5], np.arange(3.0))
     D.upwards(f)
     u_host = D.solve(r, a)
     fd, rd, ad = _lib.DeviceArray(emu_ctx, f.shape), _lib.DeviceArray(emu_ctx, r.shape), _lib.DeviceArray(emu_ctx, a.shape)
