@@ -195,6 +195,27 @@ def gpu_patch_run(bounds, N, k, steps, warmup, check=None):
                 tflops=nflops / (ms / 1e3) / 1e12)
 
 
+def traffic_probe(bounds, N, k, copy_doubles=32 << 20):
+    """Launches for the byte counters: the leaf-solve kernel, then a plain copy of known size."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    ctx = ef.Context(0)
+    P = len(bounds)
+    g_h, f_h = patch_inputs(P, N, k)
+    g, f, u = ef.DeviceArray(ctx, g_h.shape), ef.DeviceArray(ctx, f_h.shape), ef.DeviceArray(ctx, f_h.shape)
+    g.upload(g_h)
+    f.upload(f_h)
+    pt = ef.Patches(ctx, bounds, N)
+    for _ in range(5):
+        pt.apply_device(_lib.EF_PATCH_SOLVE, g.ptr, f.ptr, u.ptr, k)
+    pt.sync()
+    a, b = ef.DeviceArray(ctx, (copy_doubles,)), ef.DeviceArray(ctx, (copy_doubles,))
+    for _ in range(3):
+        _lib.check(ctx.L.ef_memcpy(ctx.h, b.ptr, a.ptr, copy_doubles * 8, 2), ctx.L)
+    log(f"traffic probe: leaf solve {patch_bytes_flops(P, N, k)[0]} algorithmic bytes per launch; "
+        f"copy reads and writes {copy_doubles * 8} bytes each")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +227,9 @@ def main():
     ap.add_argument("--cpu-solves", type=int, default=3)
     ap.add_argument("--patch-nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only 5 leaf-solve launches and 3 device-to-device copies of 256 MiB on the library's "
+                         "copy kernel (the calibration of HBM byte counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the HPS path of this build runs on a single MI355X; "
@@ -213,6 +237,9 @@ def main():
         sys.exit(2)
     leaves = workload(args.max_level, args.N)
     log(f"workload: {len(leaves)} leaves, N={args.N}")
+    if args.traffic_probe:
+        traffic_probe(leaf_bounds(leaves, args.N), args.N, args.patch_nrhs)
+        return
     threads = min(os.cpu_count() or 1, 64)
     if args.no_cpu_baseline:
         ref = dict(solves_per_s=None, build_s=None, disc_err=None)

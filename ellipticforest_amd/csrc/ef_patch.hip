@@ -26,7 +26,7 @@ using namespace efk;
 struct ef_patches_s {
     ef_ctx ctx = nullptr;
     int N = 0, P = 0, NB = 1;
-    long long max_blocks = 2048;
+    long long max_blocks = 4096;
     bool variable = false;
     hipStream_t stream = nullptr, own_stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -52,13 +52,14 @@ struct ef_patches_s {
 
 namespace {
 
-// Workgroups of a patch launch: enough to keep every CU busy (256 CUs x 8); each wave
+// Workgroups of a patch launch: several rounds of every CU's resident waves (256 CUs x
+// 4 SIMDs x 5 waves / 4 waves per workgroup = 1,280 at a time); each wave
 // then loops over its items.  EF_PATCH_MAX_BLOCKS, read when a batch is created,
 // overrides it (tuning sweeps, tests).
 long long patch_max_blocks() {
     const char* e = std::getenv("EF_PATCH_MAX_BLOCKS");
     const long long x = e ? std::atoll(e) : 0;
-    return x > 0 ? x : 2048LL;
+    return x > 0 ? x : 4096LL;
 }
 
 // Basis of K = tridiag(1, -2, 1) with -3 ends (ghost 2g - u on both sides), padded to

@@ -107,9 +107,15 @@ __device__ inline double patch_g(const PatchItemIn<NB>& in, int t, int col) {
     return (t < 64) ? a : b;
 }
 
+// The NB = 1 solve fits five waves per SIMD (at most 102 registers), which overlaps one
+// wave's loads with another's MFMA chain; the Neumann-data variants would spill there and
+// NB = 2 needs about 330 registers, so they keep the compiler's choice.
 template <int NB, bool FULL, int OP>
-__global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 && OP == PATCH_OP_SOLVE ? 5 : 1, 8)))
+patch_fd_kernel(PatchFDArgs p) {
+    // the wave index is uniform: keep it (and every item index and base address derived
+    // from it) in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
     const int Nn = p.N;
     // items: (rhs, patch) rhs-major, or (patch, DtN column) patch-major
@@ -199,7 +205,11 @@ __global__ void __launch_bounds__(256) patch_fd_kernel(PatchFDArgs p) {
                 for (int v = 0; v < 4; v++) {
                     const int kk = 16 * bi + 4 * v + q, ll = 16 * bj + r;
                     const double d = cf.cx * muk[bi][v] + cf.cy * mul[bj] + cf.lam;
-                    const double z = Y[bi][bj][v] / d;
+                    // 1/d: hardware estimate and two Newton steps (relative error ~ 1 ulp)
+                    double rc = __builtin_amdgcn_rcp(d);
+                    rc = fma(rc, fma(-d, rc, 1.0), rc);
+                    rc = fma(rc, fma(-d, rc, 1.0), rc);
+                    const double z = Y[bi][bj][v] * rc;
                     Y[bi][bj][v] = (FULL || (kk < Nn && ll < Nn)) ? z : 0.0;
                 }
         // Z4^T = Z3^T Q^T: A operand of Z3^T = accumulator of Z3 swapped; B operand of Q^T = QA swapped

@@ -26,6 +26,8 @@ def transform(text):
     text = text.replace("#include <hip/hip_runtime.h>", '#include "hip_cpu_shim.h"')
     text = re.sub(r"extern __shared__ (\w+) (\w+)\[\];", r"\1* \2 = (\1*)emu_dyn_smem();", text)
     text = text.replace("__shared__", "static")
+    # occupancy hints are for the GPU compiler only
+    text = re.sub(r"__attribute__\(\(amdgpu_waves_per_eu\([^;{]*?\)\)\)\s*", "", text)
     return text
 
 

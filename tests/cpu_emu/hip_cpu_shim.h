@@ -85,6 +85,9 @@ emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);
 // cross-lane read within a wave (ds_bpermute semantics: lane reads src % 64 of its wave)
 double __shfl(double v, int src, int width = 64);
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 emu_mfma_f64
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+// only applied to values that are uniform across the wave by construction
+#define __builtin_amdgcn_readfirstlane(x) (x)
 
 void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body);
 template <class F, class... A>
