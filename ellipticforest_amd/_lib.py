@@ -33,7 +33,8 @@ EXPORTS = [
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
     "ef_memcpy", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
     "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_dtn", "ef_patches_sync",
-    "ef_patches_last_ms",
+    "ef_patches_last_ms", "ef_patches_stream", "ef_timer_create", "ef_timer_destroy", "ef_timer_record",
+    "ef_timer_elapsed_ms",
 ]
 
 EF_PATCH_SOLVE = 0
@@ -105,6 +106,11 @@ def load_library(path):
     L.ef_patches_dtn.argtypes = [_vp, _dp]
     L.ef_patches_sync.argtypes = [_vp]
     L.ef_patches_last_ms.argtypes = [_vp, _dp]
+    L.ef_patches_stream.argtypes = [_vp, ctypes.POINTER(_vp)]
+    L.ef_timer_create.argtypes = [_vp, ctypes.POINTER(_vp)]
+    L.ef_timer_destroy.argtypes = [_vp]
+    L.ef_timer_record.argtypes = [_vp, _vp, ctypes.c_int]
+    L.ef_timer_elapsed_ms.argtypes = [_vp, _dp]
     return L
 
 
@@ -378,4 +384,36 @@ class Patches:
     def last_ms(self):
         ms = ctypes.c_double(0.0)
         check(self.L.ef_patches_last_ms(self.h, ctypes.byref(ms)), self.L)
+        return ms.value
+
+    @property
+    def stream(self):
+        s = _vp()
+        check(self.L.ef_patches_stream(self.h, ctypes.byref(s)), self.L)
+        return s.value
+
+
+class Timer:
+    """Two HIP events on a stream: GPU time of everything enqueued between start and stop."""
+
+    def __init__(self, ctx):
+        self.ctx, self.L = ctx, ctx.L
+        h = _vp()
+        check(self.L.ef_timer_create(ctx.h, ctypes.byref(h)), self.L)
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            self.L.ef_timer_destroy(self.h)
+            self.h = None
+
+    def start(self, stream=None):
+        check(self.L.ef_timer_record(self.h, stream, 0), self.L)
+
+    def stop(self, stream=None):
+        check(self.L.ef_timer_record(self.h, stream, 1), self.L)
+
+    def elapsed_ms(self):
+        ms = ctypes.c_double(0.0)
+        check(self.L.ef_timer_elapsed_ms(self.h, ctypes.byref(ms)), self.L)
         return ms.value

@@ -355,6 +355,13 @@ int ef_patches_sync(ef_patches p) {
     EF_API_END
 }
 
+int ef_patches_stream(ef_patches p, void** stream) {
+    EF_API_BEGIN
+    if (!p || !stream) throw EfError(EF_ERR_ARG, "null argument");
+    *stream = (void*)p->stream;
+    EF_API_END
+}
+
 int ef_patches_last_ms(ef_patches p, double* ms) {
     EF_API_BEGIN
     if (!p || !ms) throw EfError(EF_ERR_ARG, "null argument");
@@ -365,6 +372,59 @@ int ef_patches_last_ms(ef_patches p, double* ms) {
         EF_HIP(hipEventElapsedTime(&t, p->ev[0], p->ev[1]));
         *ms = t;
     }
+    EF_API_END
+}
+
+}  // extern "C"
+
+struct ef_timer_s {
+    ef_ctx ctx = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool recorded[2] = {false, false};
+    ~ef_timer_s() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+extern "C" {
+
+int ef_timer_create(ef_ctx ctx, ef_timer* out) {
+    EF_API_BEGIN
+    if (!ctx || !out) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<ef_timer_s> t(new ef_timer_s());
+    t->ctx = ctx;
+    EF_HIP(hipEventCreate(&t->ev[0]));
+    EF_HIP(hipEventCreate(&t->ev[1]));
+    *out = t.release();
+    EF_API_END
+}
+
+int ef_timer_destroy(ef_timer t) {
+    EF_API_BEGIN
+    delete t;
+    EF_API_END
+}
+
+int ef_timer_record(ef_timer t, void* stream, int which) {
+    EF_API_BEGIN
+    if (!t) throw EfError(EF_ERR_ARG, "null timer");
+    if (which != 0 && which != 1) throw EfError(EF_ERR_ARG, "which must be 0 (start) or 1 (stop)");
+    EF_HIP(hipSetDevice(t->ctx->device));
+    EF_HIP(hipEventRecord(t->ev[which], (hipStream_t)stream));
+    t->recorded[which] = true;
+    EF_API_END
+}
+
+int ef_timer_elapsed_ms(ef_timer t, double* ms) {
+    EF_API_BEGIN
+    if (!t || !ms) throw EfError(EF_ERR_ARG, "null argument");
+    if (!t->recorded[0] || !t->recorded[1]) throw EfError(EF_ERR_STATE, "timer events not recorded");
+    EF_HIP(hipEventSynchronize(t->ev[1]));
+    float v = 0.0f;
+    EF_HIP(hipEventElapsedTime(&v, t->ev[0], t->ev[1]));
+    *ms = v;
     EF_API_END
 }
 

@@ -181,6 +181,19 @@ int ef_patches_dtn(ef_patches p, double* T);
 int ef_patches_sync(ef_patches p);
 /* HIP-event time (ms) of the kernels of the last apply or dtn call on the batch's stream. */
 int ef_patches_last_ms(ef_patches p, double* ms);
+/* The stream the batch's kernels run on (its own, or the one given to ef_patches_set_stream). */
+int ef_patches_stream(ef_patches p, void** stream);
+
+/* Device timers: a pair of HIP events recorded on a stream (NULL: the null stream), for
+ * timing a sequence of launches on the GPU's clock, as the reference's Timer objects
+ * time its stages on the host (EllipticForestApp timers, src/EllipticForest.hpp). */
+typedef struct ef_timer_s* ef_timer;
+int ef_timer_create(ef_ctx ctx, ef_timer* out);
+int ef_timer_destroy(ef_timer t);
+/* which = 0: start event, 1: stop event */
+int ef_timer_record(ef_timer t, void* stream, int which);
+/* Waits for the stop event; milliseconds between the two events. */
+int ef_timer_elapsed_ms(ef_timer t, double* ms);
 
 #ifdef __cplusplus
 }

@@ -192,3 +192,15 @@ def test_emu_grid_stride_loop(emu_ctx, monkeypatch):
     bounds = random_bounds(rng, 3)
     pt = _lib.Patches(emu_ctx, bounds, 8, lambda0=-0.25)
     check_against_oracle(pt, bounds, 8, rng, nrhs=5, lam=-0.25)
+
+
+def test_emu_timer_and_stream(emu_ctx):
+    from ellipticforest_amd import _lib
+    pt = _lib.Patches(emu_ctx, [[0, 1, 0, 1]], 8)
+    t = _lib.Timer(emu_ctx)
+    with pytest.raises(_lib.EfError):
+        t.elapsed_ms()  # nothing recorded yet
+    t.start(pt.stream)
+    pt.solve(np.zeros((1, 1, 32)), np.ones((1, 1, 64)))
+    t.stop(pt.stream)
+    assert t.elapsed_ms() >= 0.0
