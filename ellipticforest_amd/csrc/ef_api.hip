@@ -205,12 +205,14 @@ struct ef_mesh_s {
     int solver_type = EF_SOLVER_FISHPACK;
     int precision = EF_PRECISION_FP64;
     // leaf classes
-    std::vector<LeafClass> classes_h;
-    std::vector<int> leaf_class;  // per leaf
+    std::vector<PatchCoef> classes_h;  // one per distinct (stencil, dx, dy)
+    std::vector<int> leaf_class;       // per leaf
     int tleaf_class0 = -1;
-    DevBuf<LeafClass> classes;
-    DevBuf<LeafDesc> leafdesc;
-    DevBuf<double> Q, Dinv, leafT;
+    DevBuf<PatchCoef> classes;
+    DevBuf<int> leaf_cls;                  // per leaf, on the device
+    DevBuf<long long> leaf_goff, leaf_hoff;  // vector-arena rows of each leaf's g and h
+    DevBuf<LeafDesc> leafdesc;             // block-tridiagonal path
+    DevBuf<double> QA, QB, mu, leafT;      // fast-diagonalisation tables, leaf DtN
     // variable-coefficient leaves (block-tridiagonal path)
     bool variable = false;
     DevBuf<double> vcoef, vstencil, vSinv, vwork;
@@ -306,13 +308,14 @@ void run_gj(const GJDesc* d_dev, int nsys, int m, int maxw, hipStream_t s) {
 }
 
 // ----- build ------------------------------------------------------------------
-// True when per-cell coefficients differ between cells or leaves.
+// True when some leaf's per-cell coefficients are not constant over the leaf (the
+// fast-diagonalisation path needs c_W = c_E and c_S = c_N constant within a patch).
 bool coefficients_vary(const Plan& P, const ef_coeffs& c) {
     if (c.solver_type != EF_SOLVER_FIVE_POINT || c.cell == nullptr) return false;
     const long long NN = (long long)P.N * P.N;
-    const double a0 = c.cell[0], b0 = c.cell[NN], l0 = c.cell[5 * NN];
     for (size_t li = 0; li < P.leaves.size(); li++) {
         const double* base = c.cell + li * 6 * NN;
+        const double a0 = base[0], b0 = base[NN], l0 = base[5 * NN];
         for (long long e = 0; e < NN; e++)
             if (base[e] != a0 || base[NN + e] != b0 || base[2 * NN + e] != b0 || base[3 * NN + e] != b0 ||
                 base[4 * NN + e] != b0 || base[5 * NN + e] != l0)
@@ -365,9 +368,9 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
         } else if (c.cell == nullptr) {
             cx = c.alpha0 * c.beta0 / (dx * dx); cy = c.alpha0 * c.beta0 / (dy * dy); lam = c.lambda0;
         } else {
-            // constant per-cell samples: the fast-diagonalisation class path
-            const double a0 = c.cell[0], b0 = c.cell[NN], l0 = c.cell[5 * NN];
-            cx = a0 * b0 / (dx * dx); cy = a0 * b0 / (dy * dy); lam = l0;
+            // per-cell samples constant over this leaf
+            const double* cc = c.cell + li * 6 * NN;
+            cx = cc[0] * cc[NN] / (dx * dx); cy = cc[0] * cc[NN] / (dy * dy); lam = cc[5 * NN];
         }
         std::vector<uint64_t> key = {bits(cx), bits(cy), bits(lam), bits(dx), bits(dy)};
         auto it = seen.find(key);
@@ -375,33 +378,42 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
         if (it == seen.end()) {
             id = (int)M->classes_h.size();
             seen[key] = id;
-            LeafClass L;
-            L.cx = cx; L.cy = cy; L.lam = lam; L.dx = dx; L.dy = dy; L.T = nullptr;
-            M->classes_h.push_back(L);
+            PatchCoef k;
+            k.cx = cx; k.cy = cy; k.lam = lam; k.dx = dx; k.dy = dy;
+            M->classes_h.push_back(k);
         } else id = it->second;
         M->leaf_class[li] = id;
     }
     M->tleaf_class0 = M->leaf_class[0];
     const int ncls = (int)M->classes_h.size();
     M->leafT.alloc((size_t)ncls * 16 * NN);
-    for (int i = 0; i < ncls; i++) M->classes_h[i].T = M->leafT.p + (long long)i * 16 * NN;
     M->classes.upload(M->classes_h, M->stream);
-    std::vector<LeafDesc> ld(nl);
+    M->leaf_cls.upload(M->leaf_class, M->stream);
+    std::vector<long long> goff(nl), hoff(nl);
     for (size_t li = 0; li < nl; li++) {
-        const NodeP& nd = P.nodes[P.leaves[li]];
-        LeafDesc& L = ld[li];
-        L.cls = M->leaf_class[li];
-        L.node = P.leaves[li];
-        L.h_off = nd.h_off;
-        L.g_off = nd.g_off;
-        L.coef = nullptr;
-        L.Sinv = nullptr;
-        L.stencil = nullptr;
-        L.T = M->classes_h[L.cls].T;
-        L.dx = (nd.xu - nd.xl) / N;
-        L.dy = (nd.yu - nd.yl) / N;
+        goff[li] = P.nodes[P.leaves[li]].g_off;
+        hoff[li] = P.nodes[P.leaves[li]].h_off;
     }
-    M->leafdesc.upload(ld, M->stream);
+    M->leaf_goff.upload(goff, M->stream);
+    M->leaf_hoff.upload(hoff, M->stream);
+    std::vector<double> QA, QB, mu;
+    patch_fd_tables(N, N <= 16 ? 1 : 2, QA, QB, mu);
+    M->QA.upload(QA, M->stream);
+    M->QB.upload(QB, M->stream);
+    M->mu.upload(mu, M->stream);
+}
+
+// Arguments of patch_fd_kernel over the mesh's constant-coefficient leaves.
+PatchFDArgs leaf_fd_args(ef_mesh_s* M, int op) {
+    PatchFDArgs a{};
+    a.N = M->P.N;
+    a.op = op;
+    a.coef = M->classes.p;
+    a.QA = M->QA.p;
+    a.QB = M->QB.p;
+    a.mu = M->mu.p;
+    a.vk = 1;
+    return a;
 }
 
 void plan_build_descriptors(ef_mesh_s* M) {
@@ -539,18 +551,12 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
             EF_LAUNCH_CHECK();
         }
     } else {
-        // leaf DtN per class (buildD2N)
-        const int ncls = (int)M->classes_h.size();
-        M->Q.alloc((size_t)N * N);
-        M->Dinv.alloc((size_t)ncls * N * N);
-        hipLaunchKernelGGL(fd_basis_kernel, dim3(blocks_for((long long)N * N)), dim3(256), 0, s, M->Q.p, N);
-        EF_LAUNCH_CHECK();
-        hipLaunchKernelGGL(fd_dinv_kernel, dim3(blocks_for((long long)N * N), ncls), dim3(256), 0, s, M->classes.p,
-                           M->Dinv.p, N);
-        EF_LAUNCH_CHECK();
-        hipLaunchKernelGGL(class_dtn_kernel, dim3(4 * N, ncls), dim3(256), (size_t)3 * N * N * sizeof(double), s,
-                           M->classes.p, M->Q.p, M->Dinv.p, N);
-        EF_LAUNCH_CHECK();
+        // leaf DtN per class (buildD2N): one "patch" per class, T written class by class
+        PatchFDArgs a = leaf_fd_args(M, PATCH_OP_DTN);
+        a.npatches = (int)M->classes_h.size();
+        a.nrhs = 1;
+        a.out = M->leafT.p;
+        launch_patch_fd(a, N <= 16 ? 1 : 2, patch_default_blocks(), s);
     }
     // merges, deepest level first (post-order dependencies)
     EF_HIP(hipMemsetAsync(M->winfo.p, 0, sizeof(int) * std::max(P.max_group, 1), s));
@@ -723,12 +729,27 @@ void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u,
     if (M->variable) {
         hipLaunchKernelGGL(bt_solve_kernel, dim3(nl, k), dim3(256), 0, M->stream, M->leafdesc.p, N, nl, k, mode, f,
                            f_zero, (const double*)M->vec.p, M->vec.p, u);
+        EF_LAUNCH_CHECK();
     } else {
-        hipLaunchKernelGGL(leaf_fd_kernel, dim3(nl, k), dim3(256), (size_t)4 * N * N * sizeof(double), M->stream,
-                           M->leafdesc.p, M->classes.p, M->Q.p, M->Dinv.p, N, nl, k, mode, f, f_zero,
-                           (const double*)M->vec.p, M->vec.p, u);
+        // mode 0: particular Neumann data (g = 0) into the h rows of every leaf;
+        // mode 1: solve with the g rows of every leaf, u out
+        PatchFDArgs a = leaf_fd_args(M, mode == 0 ? PATCH_OP_MAP_D2N : PATCH_OP_SOLVE);
+        a.npatches = nl;
+        a.nrhs = k;
+        a.cls = M->leaf_cls.p;
+        a.f = f_zero ? nullptr : f;
+        a.vk = k;
+        if (mode == 0) {
+            a.g = nullptr;
+            a.out = M->vec.p;
+            a.h_off = M->leaf_hoff.p;
+        } else {
+            a.g = M->vec.p;
+            a.g_off = M->leaf_goff.p;
+            a.out = u;
+        }
+        launch_patch_fd(a, N <= 16 ? 1 : 2, patch_default_blocks(), M->stream);
     }
-    EF_LAUNCH_CHECK();
 }
 
 void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {

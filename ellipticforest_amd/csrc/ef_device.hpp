@@ -77,17 +77,9 @@ struct CoarsenDesc {
     int m;              // fine side length
 };
 
-// Leaf class for constant coefficients: every leaf of a class has the same dx, dy and
-// stencil, hence the same DtN matrix and eigen-decomposition.
-struct LeafClass {
-    double cx, cy, lam;  // c_W = c_E = cx, c_S = c_N = cy, centre = -2cx - 2cy + lam
-    double dx, dy;
-    double* T;           // 4N x 4N
-};
-
 // Per-leaf view used by the leaf sweep kernels.
 struct LeafDesc {
-    int cls;        // constant-coefficient class (or -1 for the block-tridiagonal path)
+    int cls;        // unused by the block-tridiagonal kernels (-1)
     int node;       // node id
     long long h_off; // row offset of h in the vector arena (4N rows)
     long long g_off; // row offset of g
@@ -172,7 +164,6 @@ static_assert(offsetof(GJDesc, piv) == 80, "GJDesc.piv");
 static_assert(sizeof(MergeDesc) == 80, "MergeDesc");
 static_assert(offsetof(MergeDesc, Tc) == 8, "MergeDesc.Tc");
 static_assert(sizeof(CoarsenDesc) == 24, "CoarsenDesc");
-static_assert(sizeof(LeafClass) == 48, "LeafClass");
 static_assert(sizeof(LeafDesc) == 72, "LeafDesc");
 static_assert(offsetof(LeafDesc, h_off) == 8, "LeafDesc.h_off");
 static_assert(offsetof(LeafDesc, coef) == 24, "LeafDesc.coef");

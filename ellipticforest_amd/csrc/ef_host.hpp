@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "ef_device.hpp"
 #include "ef_hps.h"
 
 namespace efk {
@@ -17,6 +18,13 @@ void set_last_error(const std::string& msg);  // thread-local message of ef_last
 // ordered on stream s and complete on return, so no caller memory is ever page-locked.
 void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Fast-diagonalisation tables of an N x N patch for patch_fd_kernel (NB = 1 for N <= 16,
+// 2 for N <= 32): the basis in both MFMA operand layouts and the eigenvalues of K.
+void patch_fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, std::vector<double>& mu);
+// Launch patch_fd_kernel for a.op over at most max_blocks workgroups on stream s.
+void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStream_t s);
+// Workgroup cap of patch launches (EF_PATCH_MAX_BLOCKS or the default).
+long long patch_default_blocks();
 }
 
 // A context is one GPU.

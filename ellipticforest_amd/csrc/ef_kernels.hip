@@ -11,6 +11,13 @@ namespace efk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Element c (0..3) of a four-entry descriptor array by selects: a runtime index into a
+// struct copied from memory would place the struct in scratch or LDS.
+template <class T>
+__device__ inline T pick4(const T (&a)[4], int c) {
+    return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
+}
+
 // ---------------------------------------------------------------------------
 // Batched FP64 GEMM on MFMA (v_mfma_f64_16x16x4_f64).
 // Workgroup = 4 waves computing a 64x64 tile of D; each wave owns 32x32 = 2x2 MFMA
@@ -205,8 +212,7 @@ __device__ inline void gj_panel_body(const GJDesc& d, int k, double* pan, int ld
     for (int r = tid; r < m; r += 256) {
         double* Mr = d.M + (long long)r * GJ_NB;
         if (r < k) {
-            double xr[GJ_NB];
-            for (int t = 0; t < nbb; t++) xr[t] = d.X[(long long)r * d.ldx + k + t];
+            const double* xr = d.X + (long long)r * d.ldx + k;
             for (int c = 0; c < nbb; c++) {
                 double s = 0.0;
                 for (int t = 0; t < nbb; t++) s += xr[t] * Pinv[t][c];
@@ -217,11 +223,9 @@ __device__ inline void gj_panel_body(const GJDesc& d, int k, double* pan, int ld
             for (int c = 0; c < nbb; c++) Mr[c] = Pinv[i][c] - (i == c ? 1.0 : 0.0);
         } else {
             const int rr = r - k;
-            double lr[GJ_NB];
-            for (int t = 0; t < nbb; t++) lr[t] = P(rr, t);
             for (int c = 0; c < nbb; c++) {
                 double s = 0.0;
-                for (int t = c; t < nbb; t++) s += lr[t] * Li[t][c];
+                for (int t = c; t < nbb; t++) s += P(rr, t) * Li[t][c];
                 Mr[c] = -s;
             }
         }
@@ -346,13 +350,13 @@ __global__ void __launch_bounds__(256) merge_assemble_kernel(const MergeDesc* de
         const int c1 = IF_C1[ri], s1 = IF_S1[ri], c2 = IF_C2[ri], s2 = IF_S2[ri];
         double v;
         if (ri == ci) {
-            v = tblk(d.Tc[c1], n, s1, s1, i, j) - tblk(d.Tc[c2], n, s2, s2, i, j);
+            v = tblk(pick4(d.Tc, c1), n, s1, s1, i, j) - tblk(pick4(d.Tc, c2), n, s2, s2, i, j);
         } else {
             v = 0.0;
             const int sc1 = side_on_if(c1, ci);
             const int sc2 = side_on_if(c2, ci);
-            if (sc1 >= 0) v = tblk(d.Tc[c1], n, s1, sc1, i, j);
-            else if (sc2 >= 0) v = -tblk(d.Tc[c2], n, s2, sc2, i, j);
+            if (sc1 >= 0) v = tblk(pick4(d.Tc, c1), n, s1, sc1, i, j);
+            else if (sc2 >= 0) v = -tblk(pick4(d.Tc, c2), n, s2, sc2, i, j);
         }
         d.X[e] = v;
         return;
@@ -363,8 +367,8 @@ __global__ void __launch_bounds__(256) merge_assemble_kernel(const MergeDesc* de
         const int ri = r / n, i = r % n, q = c / n, j = c % n;
         const int cq = Q_CHILD[q], sq = Q_SIDE[q];
         double v = 0.0;
-        if (cq == IF_C1[ri]) v = -tblk(d.Tc[cq], n, IF_S1[ri], sq, i, j);
-        else if (cq == IF_C2[ri]) v = tblk(d.Tc[cq], n, IF_S2[ri], sq, i, j);
+        if (cq == IF_C1[ri]) v = -tblk(pick4(d.Tc, cq), n, IF_S1[ri], sq, i, j);
+        else if (cq == IF_C2[ri]) v = tblk(pick4(d.Tc, cq), n, IF_S2[ri], sq, i, j);
         d.S[e] = v;
         return;
     }
@@ -379,7 +383,7 @@ __global__ void __launch_bounds__(256) merge_assemble_kernel(const MergeDesc* de
         const int r = (int)(e / (8 * n)), c = (int)(e % (8 * n));
         const int q = r / n, i = r % n, q2 = c / n, j = c % n;
         double v = 0.0;
-        if (Q_CHILD[q] == Q_CHILD[q2]) v = tblk(d.Tc[Q_CHILD[q]], n, Q_SIDE[q], Q_SIDE[q2], i, j);
+        if (Q_CHILD[q] == Q_CHILD[q2]) v = tblk(pick4(d.Tc, Q_CHILD[q]), n, Q_SIDE[q], Q_SIDE[q2], i, j);
         d.T[e] = v;
         return;
     }
@@ -390,7 +394,7 @@ __global__ void __launch_bounds__(256) merge_assemble_kernel(const MergeDesc* de
         const int rr = (int)(ee / (2 * n)), cc = (int)(ee % (2 * n));
         const int ts = TAU_SIDE[c][rr / n];
         const int is = CH_IF_SIDE[c][cc / n];
-        d.H[e] = tblk(d.Tc[c], n, ts, is, rr % n, cc % n);
+        d.H[e] = tblk(pick4(d.Tc, c), n, ts, is, rr % n, cc % n);
     }
 }
 
@@ -435,34 +439,6 @@ __global__ void __launch_bounds__(256) coarsen_kernel(const CoarsenDesc* descs) 
     d.dst[e] = s;
 }
 
-// ---------------------------------------------------------------------------
-// Leaf fast diagonalisation (constant coefficients).  With x-major cell vectors
-// U[i][j], the 5-point operator with ghost value 2g - u is
-//   A U = cx K U + cy U K + lam U,   K = tridiag(1,-2,1) with -3 at both ends,
-// and K = Q diag(mu) Q^T with Q[i][k] = nrm_k sin(pi (k+1)(i+1/2)/N),
-// mu_k = -4 sin^2(pi (k+1)/(2N)).  So A^-1 B = Q ((Q^T B Q) ./ D) Q^T with
-// D[k][l] = cx mu_k + cy mu_l + lam.
-// ---------------------------------------------------------------------------
-__global__ void fd_basis_kernel(double* Q, int Nn) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= Nn * Nn) return;
-    const int i = e / Nn, k = e % Nn;
-    const double nrm = (k == Nn - 1) ? sqrt(1.0 / Nn) : sqrt(2.0 / Nn);
-    Q[e] = nrm * sin(M_PI * (double)(k + 1) * ((double)i + 0.5) / (double)Nn);
-}
-
-__global__ void fd_dinv_kernel(const LeafClass* cls, double* Dinv, int Nn) {
-    const int c = blockIdx.y;
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= Nn * Nn) return;
-    const int k = e / Nn, l = e % Nn;
-    const double sk = sin(M_PI * (double)(k + 1) / (2.0 * Nn));
-    const double sl = sin(M_PI * (double)(l + 1) / (2.0 * Nn));
-    const double mk = -4.0 * sk * sk, ml = -4.0 * sl * sl;
-    const LeafClass L = cls[c];
-    Dinv[(long long)c * Nn * Nn + e] = 1.0 / (L.cx * mk + L.cy * ml + L.lam);
-}
-
 // boundary cell of WESN position b (0..4N): (i, j)
 __device__ inline void bcell(int b, int Nn, int& i, int& j) {
     const int s = b / Nn, t = b % Nn;
@@ -470,134 +446,6 @@ __device__ inline void bcell(int b, int Nn, int& i, int& j) {
     else if (s == E) { i = Nn - 1; j = t; }
     else if (s == S) { i = t; j = 0; }
     else { i = t; j = Nn - 1; }
-}
-
-// Leaf DtN per class (buildD2N, FiniteVolumeSolver.cpp:355-455): column b is the
-// Neumann data of the solution driven by unit Dirichlet data at b:
-//   u = A^-1 (-2 c_b e_cell(b)),  T[a][b] = sgn_a (2/d_a) (u_cell(a) - delta_ab)
-// One workgroup per (class, output row a); G(a,b) = sum_kl Q[ia,k]Q[ja,l]Q[ib,k]Q[jb,l]/D_kl.
-__global__ void __launch_bounds__(256) class_dtn_kernel(const LeafClass* cls, const double* Q, const double* Dinv, int Nn) {
-    extern __shared__ double sm[];
-    double* sQ = sm;                 // N x N
-    double* sD = sm + Nn * Nn;       // N x N
-    double* sW = sD + Nn * Nn;       // N x N : Q[ia,k] Q[ja,l] / D[k,l]
-    const int c = blockIdx.y;
-    const int a = blockIdx.x;        // output row 0..4N
-    const LeafClass L = cls[c];
-    const int tid = threadIdx.x;
-    const int NN = Nn * Nn;
-    for (int e = tid; e < NN; e += 256) { sQ[e] = Q[e]; sD[e] = Dinv[(long long)c * NN + e]; }
-    __syncthreads();
-    int ia, ja;
-    bcell(a, Nn, ia, ja);
-    for (int e = tid; e < NN; e += 256) {
-        const int k = e / Nn, l = e % Nn;
-        sW[e] = sQ[ia * Nn + k] * sQ[ja * Nn + l] * sD[e];
-    }
-    __syncthreads();
-    const int sa = a / Nn;
-    const double da = (sa < 2) ? L.dx : L.dy;
-    const double sg = (sa == W || sa == S) ? 2.0 / da : -2.0 / da;
-    for (int b = tid; b < 4 * Nn; b += 256) {
-        int ib, jb;
-        bcell(b, Nn, ib, jb);
-        double g = 0.0;
-        for (int k = 0; k < Nn; k++) {
-            const double qk = sQ[ib * Nn + k];
-            double s = 0.0;
-            for (int l = 0; l < Nn; l++) s += sW[k * Nn + l] * sQ[jb * Nn + l];
-            g += qk * s;
-        }
-        const int sb = b / Nn;
-        const double cb = (sb < 2) ? L.cx : L.cy;
-        const double u = -2.0 * cb * g;
-        L.T[(long long)a * 4 * Nn + b] = sg * (u - (a == b ? 1.0 : 0.0));
-    }
-}
-
-// Leaf solve by fast diagonalisation for every (leaf, rhs):
-//  mode 0: particularNeumannData (FiniteVolumeSolver.cpp:457-462): g = 0, h = edges
-//  mode 1: solve (FiniteVolumeSolver.cpp:25-294) with g, writes u
-// Boundary data enter the right-hand side as -2 c g at the adjacent cell (:100-184).
-__global__ void __launch_bounds__(256) leaf_fd_kernel(const LeafDesc* leaves, const LeafClass* cls, const double* Q,
-                                                      const double* Dinv, int Nn, int nleaves, int k, int mode,
-                                                      const double* f, int f_zero, const double* garena,
-                                                      double* harena, double* u) {
-    extern __shared__ double sm[];
-    const int NN = Nn * Nn;
-    double* sQ = sm;
-    double* sD = sm + NN;
-    double* b0 = sD + NN;
-    double* b1 = b0 + NN;
-    const int leaf = blockIdx.x, rhs = blockIdx.y;
-    const LeafDesc ld = leaves[leaf];
-    const LeafClass L = cls[ld.cls];
-    const int tid = threadIdx.x;
-    const double* fl = f + ((long long)rhs * nleaves + leaf) * NN;
-    for (int e = tid; e < NN; e += 256) {
-        sQ[e] = Q[e];
-        sD[e] = Dinv[(long long)ld.cls * NN + e];
-        const int i = e / Nn, j = e % Nn;
-        double v = f_zero ? 0.0 : fl[e];
-        if (mode == 1 && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
-            double add = 0.0;
-            const double* g = garena + ld.g_off * k + rhs;
-            if (i == 0) add += L.cx * g[(long long)(0 * Nn + j) * k];
-            if (i == Nn - 1) add += L.cx * g[(long long)(1 * Nn + j) * k];
-            if (j == 0) add += L.cy * g[(long long)(2 * Nn + i) * k];
-            if (j == Nn - 1) add += L.cy * g[(long long)(3 * Nn + i) * k];
-            v += -2.0 * add;
-        }
-        b0[e] = v;
-    }
-    __syncthreads();
-    // b1 = Q^T b0
-    for (int e = tid; e < NN; e += 256) {
-        const int kk = e / Nn, j = e % Nn;
-        double s = 0.0;
-        for (int i = 0; i < Nn; i++) s += sQ[i * Nn + kk] * b0[i * Nn + j];
-        b1[e] = s;
-    }
-    __syncthreads();
-    // b0 = (b1 Q) .* Dinv
-    for (int e = tid; e < NN; e += 256) {
-        const int kk = e / Nn, l = e % Nn;
-        double s = 0.0;
-        for (int j = 0; j < Nn; j++) s += b1[kk * Nn + j] * sQ[j * Nn + l];
-        b0[e] = s * sD[e];
-    }
-    __syncthreads();
-    // b1 = Q b0
-    for (int e = tid; e < NN; e += 256) {
-        const int i = e / Nn, l = e % Nn;
-        double s = 0.0;
-        for (int kk = 0; kk < Nn; kk++) s += sQ[i * Nn + kk] * b0[kk * Nn + l];
-        b1[e] = s;
-    }
-    __syncthreads();
-    // u = b1 Q^T
-    if (mode == 1) {
-        double* ul = u + ((long long)rhs * nleaves + leaf) * NN;
-        for (int e = tid; e < NN; e += 256) {
-            const int i = e / Nn, j = e % Nn;
-            double s = 0.0;
-            for (int l = 0; l < Nn; l++) s += b1[i * Nn + l] * sQ[j * Nn + l];
-            ul[e] = s;
-        }
-    } else {
-        // only edge cells are needed: h = +-(2/d)(u_edge - 0)   (mapD2N, :296-353)
-        double* h = harena + ld.h_off * k + rhs;
-        for (int b = tid; b < 4 * Nn; b += 256) {
-            int i, j;
-            bcell(b, Nn, i, j);
-            double s = 0.0;
-            for (int l = 0; l < Nn; l++) s += b1[i * Nn + l] * sQ[j * Nn + l];
-            const int sd = b / Nn;
-            const double dd = (sd < 2) ? L.dx : L.dy;
-            const double sg = (sd == W || sd == S) ? 2.0 / dd : -2.0 / dd;
-            h[(long long)b * k] = sg * s;
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -618,18 +466,14 @@ __device__ inline double coarse_h(const double* harena, long long off, int clen,
         const double a = 0.5 * x(0) + 0.5 * x(1), b = 0.5 * x(2) + 0.5 * x(3);
         return 0.5 * a + 0.5 * b;
     }
-    // deeper coarsening: same nested pairwise order, via a carry stack per level
-    double acc[16];
-    int cnt[16];
-    for (int l = 0; l <= tag; l++) cnt[l] = 0;
-    for (int t = 0; t < w; t++) {
-        double v = x(t);
-        int l = 0;
-        while (cnt[l] == 1) { v = 0.5 * acc[l] + 0.5 * v; cnt[l] = 0; l++; }
-        acc[l] = v;
-        cnt[l] = 1;
+    // deeper coarsening: pairwise averages of blocks of four (the tag-2 order), then the
+    // mean of those; equal to the nested pairwise order up to rounding
+    double s = 0.0;
+    for (int t = 0; t < w; t += 4) {
+        const double a = 0.5 * x(t) + 0.5 * x(t + 1), b = 0.5 * x(t + 2) + 0.5 * x(t + 3);
+        s += 0.5 * a + 0.5 * b;
     }
-    return acc[tag];
+    return s * (4.0 / (double)w);
 }
 
 __global__ void __launch_bounds__(256) up_gather_kernel(const UpDesc* descs, double* harena, int k) {
@@ -643,14 +487,14 @@ __global__ void __launch_bounds__(256) up_gather_kernel(const UpDesc* descs, dou
     if (row < 4 * n) {
         const int r = row / n, i = row % n;
         const int c1 = IF_C1[r], c2 = IF_C2[r];
-        const double a = coarse_h(harena, d.ch_off[c2], d.clen[c2], d.tag[c2], IF_S2[r], i, k, rhs);
-        const double b = coarse_h(harena, d.ch_off[c1], d.clen[c1], d.tag[c1], IF_S1[r], i, k, rhs);
+        const double a = coarse_h(harena, pick4(d.ch_off, c2), pick4(d.clen, c2), pick4(d.tag, c2), IF_S2[r], i, k, rhs);
+        const double b = coarse_h(harena, pick4(d.ch_off, c1), pick4(d.clen, c1), pick4(d.tag, c1), IF_S1[r], i, k, rhs);
         harena[(d.hd_off + row) * k + rhs] = a - b;
     } else {
         const int rr = row - 4 * n;
         const int q = rr / n, i = rr % n;
         const int c = Q_CHILD[q];
-        harena[(d.h_off + rr) * k + rhs] = coarse_h(harena, d.ch_off[c], d.clen[c], d.tag[c], Q_SIDE[q], i, k, rhs);
+        harena[(d.h_off + rr) * k + rhs] = coarse_h(harena, pick4(d.ch_off, c), pick4(d.clen, c), pick4(d.tag, c), Q_SIDE[q], i, k, rhs);
     }
 }
 
@@ -693,7 +537,7 @@ __global__ void __launch_bounds__(256) down_scatter_kernel(const DownDesc* descs
     double v;
     if (q >= 0) v = garena[(d.g_off + (long long)q * n + i) * k + rhs];
     else v = garena[(d.ui_off + (long long)CH_SRC_R[c][s] * n + i) * k + rhs];
-    garena[(d.cg_off[c] + rr) * k + rhs] = v;
+    garena[(pick4(d.cg_off, c) + rr) * k + rhs] = v;
 }
 
 // Root Dirichlet data: g = r / a, the LU solve of diag(a) in solveStage

@@ -15,12 +15,6 @@ __global__ void gj_swap_kernel(const GJDesc* descs, int k);
 __global__ void gj_update_kernel(const GJDesc* descs, int k);
 __global__ void merge_assemble_kernel(const MergeDesc* descs);
 __global__ void coarsen_kernel(const CoarsenDesc* descs);
-__global__ void fd_basis_kernel(double* Q, int Nn);
-__global__ void fd_dinv_kernel(const LeafClass* cls, double* Dinv, int Nn);
-__global__ void class_dtn_kernel(const LeafClass* cls, const double* Q, const double* Dinv, int Nn);
-__global__ void leaf_fd_kernel(const LeafDesc* leaves, const LeafClass* cls, const double* Q, const double* Dinv,
-                               int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
-                               const double* garena, double* harena, double* u);
 __global__ void up_gather_kernel(const UpDesc* descs, double* harena, int k);
 __global__ void uncoarsen_kernel(const VecStepDesc* descs, double* garena, int k);
 __global__ void down_scatter_kernel(const DownDesc* descs, double* garena, int k);

@@ -50,13 +50,13 @@ struct ef_patches_s {
     }
 };
 
-namespace {
+namespace efk {
 
 // Workgroups of a patch launch: several rounds of every CU's resident waves (256 CUs x
 // 4 SIMDs x 5 waves / 4 waves per workgroup = 1,280 at a time); each wave
 // then loops over its items.  EF_PATCH_MAX_BLOCKS, read when a batch is created,
 // overrides it (tuning sweeps, tests).
-long long patch_max_blocks() {
+long long patch_default_blocks() {
     const char* e = std::getenv("EF_PATCH_MAX_BLOCKS");
     const long long x = e ? std::atoll(e) : 0;
     return x > 0 ? x : 4096LL;
@@ -65,7 +65,7 @@ long long patch_max_blocks() {
 // Basis of K = tridiag(1, -2, 1) with -3 ends (ghost 2g - u on both sides), padded to
 // 16*NB, and the MFMA lane tables: QA[x][y][s] = Q[16x + r][16y + 4s + q],
 // QB[x][y][s] = Q[16x + 4s + q][16y + r], lane = 16q + r.
-void fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, std::vector<double>& mu) {
+void patch_fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, std::vector<double>& mu) {
     const int Np = 16 * NB;
     std::vector<double> Q((size_t)Np * Np, 0.0);
     for (int i = 0; i < N; i++)
@@ -91,6 +91,55 @@ void fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, 
                 }
 }
 
+void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStream_t s) {
+    const long long waves =
+        (a.op == PATCH_OP_DTN) ? (long long)a.npatches * 4 * a.N : (long long)a.npatches * a.nrhs;
+    if (waves == 0) return;
+    const unsigned blocks = (unsigned)std::min<long long>((waves + 3) / 4, max_blocks);
+    const bool full = (a.N == 16 * NB);
+    const dim3 grid(blocks), block(256);
+    const int op = a.op;
+#define EF_PATCH_LAUNCH(NB_, FULL_)                                                                         \
+    do {                                                                                                     \
+        if (op == PATCH_OP_SOLVE) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE>), grid, block, 0, s, a); \
+        else if (op == PATCH_OP_MAP_D2N) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N>), grid, block, 0, s, a); \
+        else hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN>), grid, block, 0, s, a); \
+    } while (0)
+    if (NB == 1) {
+        if (full) EF_PATCH_LAUNCH(1, true);
+        else EF_PATCH_LAUNCH(1, false);
+    } else {
+        if (full) EF_PATCH_LAUNCH(2, true);
+        else EF_PATCH_LAUNCH(2, false);
+    }
+#undef EF_PATCH_LAUNCH
+    EF_LAUNCH_CHECK();
+}
+
+}  // namespace efk
+
+namespace {
+
+void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
+    PatchFDArgs a{};
+    a.N = p->N;
+    a.npatches = p->P;
+    a.nrhs = nrhs;
+    a.op = op;
+    a.coef = p->coef.p;
+    a.cls = nullptr;
+    a.QA = p->QA.p;
+    a.QB = p->QB.p;
+    a.mu = p->mu.p;
+    a.f = f;
+    a.g = g;
+    a.out = out;
+    a.g_off = nullptr;
+    a.h_off = nullptr;
+    a.vk = 1;
+    efk::launch_patch_fd(a, p->NB, p->max_blocks, p->stream);
+}
+
 // True when every per-cell array of the patch is constant and the four face betas agree.
 bool patch_constant(const double* c, long long NN) {
     const double a0 = c[0], b0 = c[NN], l0 = c[5 * NN];
@@ -111,45 +160,6 @@ void record_begin(ef_patches_s* p) {
 void record_end(ef_patches_s* p) {
     EF_HIP(hipEventRecord(p->ev[1], p->stream));
     p->ev_recorded = true;
-}
-
-void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
-    PatchFDArgs a{};
-    a.N = p->N;
-    a.npatches = p->P;
-    a.nrhs = nrhs;
-    a.op = op;
-    a.coef = p->coef.p;
-    a.cls = nullptr;
-    a.QA = p->QA.p;
-    a.QB = p->QB.p;
-    a.mu = p->mu.p;
-    a.f = f;
-    a.g = g;
-    a.out = out;
-    a.g_off = nullptr;
-    a.h_off = nullptr;
-    a.vk = 1;
-    const long long waves = (op == PATCH_OP_DTN) ? (long long)p->P * 4 * p->N : (long long)p->P * nrhs;
-    // enough workgroups to keep every CU busy; each wave then loops over its items
-    const unsigned blocks = (unsigned)std::min<long long>((waves + 3) / 4, p->max_blocks);
-    const bool full = (p->N == 16 * p->NB);
-    const dim3 grid(blocks), block(256);
-#define EF_PATCH_LAUNCH(NB_, FULL_)                                                                         \
-    do {                                                                                                     \
-        if (op == PATCH_OP_SOLVE) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE>), grid, block, 0, p->stream, a); \
-        else if (op == PATCH_OP_MAP_D2N) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N>), grid, block, 0, p->stream, a); \
-        else hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN>), grid, block, 0, p->stream, a); \
-    } while (0)
-    if (p->NB == 1) {
-        if (full) EF_PATCH_LAUNCH(1, true);
-        else EF_PATCH_LAUNCH(1, false);
-    } else {
-        if (full) EF_PATCH_LAUNCH(2, true);
-        else EF_PATCH_LAUNCH(2, false);
-    }
-#undef EF_PATCH_LAUNCH
-    EF_LAUNCH_CHECK();
 }
 
 // Block-tridiagonal path, one right-hand side per launch ([patch][...] layouts with k = 1).
@@ -201,7 +211,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
     p->N = b->N;
     p->P = b->npatches;
     p->NB = (b->N <= 16) ? 1 : 2;
-    p->max_blocks = patch_max_blocks();
+    p->max_blocks = efk::patch_default_blocks();
     EF_HIP(hipStreamCreate(&p->own_stream));
     p->stream = p->own_stream;
     const int N = p->N;
@@ -249,7 +259,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
         EF_LAUNCH_CHECK();
     } else {
         std::vector<double> QA, QB, mu;
-        fd_tables(N, p->NB, QA, QB, mu);
+        efk::patch_fd_tables(N, p->NB, QA, QB, mu);
         p->QA.upload(QA, p->stream);
         p->QB.upload(QB, p->stream);
         p->mu.upload(mu, p->stream);

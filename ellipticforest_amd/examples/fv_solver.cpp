@@ -1,0 +1,74 @@
+// fv_solver.cpp -- the single-grid FiniteVolumeSolver operations through the drop-in C++ API,
+// in the shape of the reference's test_finite_volume_solver.cpp: u = sin x + sin y on
+// [-1,1]^2, Dirichlet data at the faces, f = -(sin x + sin y), FISHPACK semantics; the
+// max-norm error for n = 8, 16, 32 and the convergence order of the last two.  Then, on an
+// 8 x 8 grid over [0,1] x [0.5,1.25], the DtN matrix, the Neumann data of the same problem
+// and its particular part, printed for comparison with other implementations.
+//   fv_solver
+// Output lines: "err n linf", "order p", "T i j v" (a few entries), "h i v", "hp i v".
+#include <EllipticForest.hpp>
+#include <Patches/FiniteVolume/FiniteVolume.hpp>
+
+#include <cmath>
+#include <cstdio>
+
+using namespace EllipticForest;
+
+static double uExact(double x, double y) { return std::sin(x) + std::sin(y); }
+static double fRHS(double x, double y) { return -uExact(x, y); }
+
+static void problem(FiniteVolumeGrid& grid, int n, Vector<double>& g, Vector<double>& f, Vector<double>& ue) {
+    g = Vector<double>(4 * n);
+    f = Vector<double>(n * n);
+    ue = Vector<double>(n * n);
+    for (int j = 0; j < n; j++) {
+        g[j] = uExact(grid.xLower(), grid(YDIM, j));
+        g[n + j] = uExact(grid.xUpper(), grid(YDIM, j));
+    }
+    for (int i = 0; i < n; i++) {
+        g[2 * n + i] = uExact(grid(XDIM, i), grid.yLower());
+        g[3 * n + i] = uExact(grid(XDIM, i), grid.yUpper());
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            f[j + i * n] = fRHS(grid(XDIM, i), grid(YDIM, j));
+            ue[j + i * n] = uExact(grid(XDIM, i), grid(YDIM, j));
+        }
+}
+
+int main(int argc, char** argv) {
+    EllipticForestApp app(&argc, &argv);
+    FiniteVolumeSolver solver;
+    solver.solver_type = FiniteVolumeSolverType::FISHPACK90;
+    solver.alpha_function = [](double, double) { return 1.0; };
+    solver.beta_function = [](double, double) { return 1.0; };
+    solver.lambda_function = [](double, double) { return 0.0; };
+
+    const int ns[3] = {8, 16, 32};
+    double errors[3];
+    for (int t = 0; t < 3; t++) {
+        const int n = ns[t];
+        FiniteVolumeGrid grid(MPI_COMM_WORLD, n, -1.0, 1.0, n, -1.0, 1.0);
+        Vector<double> g, f, ue;
+        problem(grid, n, g, f, ue);
+        Vector<double> u = solver.solve(grid, g, f);
+        double e = 0.0;
+        for (int k = 0; k < n * n; k++) e = std::fmax(e, std::fabs(u[k] - ue[k]));
+        errors[t] = e;
+        std::printf("err %d %.17g\n", n, e);
+    }
+    std::printf("order %.17g\n", std::log(errors[2] / errors[1]) / std::log((double)ns[1] / ns[2]));
+
+    const int n = 8;
+    FiniteVolumeGrid grid(MPI_COMM_WORLD, n, 0.0, 1.0, n, 0.5, 1.25);
+    Matrix<double> T = solver.buildD2N(grid);
+    const int picks[6][2] = {{0, 0}, {0, 1}, {3, 17}, {8, 8}, {31, 31}, {20, 5}};
+    for (auto& p : picks) std::printf("T %d %d %.17g\n", p[0], p[1], T(p[0], p[1]));
+    Vector<double> g, f, ue;
+    problem(grid, n, g, f, ue);
+    Vector<double> h = solver.mapD2N(grid, g, f);
+    Vector<double> hp = solver.particularNeumannData(grid, f);
+    for (int i = 0; i < 4 * n; i++) std::printf("h %d %.17g\n", i, h[i]);
+    for (int i = 0; i < 4 * n; i++) std::printf("hp %d %.17g\n", i, hp[i]);
+    return 0;
+}

@@ -108,67 +108,60 @@ public:
         }
     }
 
+    // The single-grid operations run as a batch of one patch (ef_patches_*).
     Vector<double> solve(FiniteVolumeGrid& grid, Vector<double>& dirichlet_data, Vector<double>& rhs_data) {
         OnePatch p(*this, grid);
-        p.upwards(rhs_data);
-        return p.solve(dirichlet_data);
+        p.checkSizes(&dirichlet_data, &rhs_data);
+        Vector<double> u((std::size_t)p.N * p.N);
+        efCheck(ef_patches_apply(p.h, EF_PATCH_SOLVE, dirichlet_data.dataPointer(), rhs_data.dataPointer(),
+                                 u.dataPointer(), 1), "FiniteVolumeSolver::solve");
+        return u;
     }
     Vector<double> mapD2N(FiniteVolumeGrid& grid, Vector<double>& dirichlet_data, Vector<double>& rhs_data) {
         OnePatch p(*this, grid);
-        p.upwards(rhs_data);
-        const Matrix<double> T = p.get(0);
-        Vector<double> h = p.getVec(7);
-        const std::size_t M = T.nRows();
-        for (std::size_t i = 0; i < M; i++) {
-            double s = 0.0;
-            for (std::size_t j = 0; j < M; j++) s += T(i, j) * dirichlet_data[j];
-            h[i] += s;
-        }
+        p.checkSizes(&dirichlet_data, &rhs_data);
+        Vector<double> h((std::size_t)4 * p.N);
+        efCheck(ef_patches_apply(p.h, EF_PATCH_MAP_D2N, dirichlet_data.dataPointer(), rhs_data.dataPointer(),
+                                 h.dataPointer(), 1), "FiniteVolumeSolver::mapD2N");
         return h;
     }
     Matrix<double> buildD2N(FiniteVolumeGrid& grid) {
         OnePatch p(*this, grid);
-        return p.get(0);
+        Matrix<double> T((std::size_t)4 * p.N, (std::size_t)4 * p.N);
+        efCheck(ef_patches_dtn(p.h, T.dataPointer()), "FiniteVolumeSolver::buildD2N");
+        return T;
     }
     Vector<double> particularNeumannData(FiniteVolumeGrid& grid, Vector<double>& rhs_data) {
         OnePatch p(*this, grid);
-        p.upwards(rhs_data);
-        return p.getVec(7);
+        p.checkSizes(nullptr, &rhs_data);
+        Vector<double> h((std::size_t)4 * p.N);
+        efCheck(ef_patches_apply(p.h, EF_PATCH_MAP_D2N, nullptr, rhs_data.dataPointer(), h.dataPointer(), 1),
+                "FiniteVolumeSolver::particularNeumannData");
+        return h;
     }
 
 private:
     struct OnePatch {
-        ef_mesh m = nullptr;
+        ef_patches h = nullptr;
         int N = 0;
         OnePatch(const FiniteVolumeSolver& s, const FiniteVolumeGrid& g) {
             if (g.nx() != g.ny()) throw std::invalid_argument("[EllipticForest::FiniteVolumeSolver] square patches only");
             N = (int)g.nx();
-            int lv = 0, ix = 0, iy = 0;
-            ef_mesh_desc d{1, &lv, &ix, &iy, N, g.xLower(), g.xUpper(), g.yLower(), g.yUpper()};
-            efCheck(ef_mesh_create(defaultContext(), &d, &m), "FiniteVolumeSolver");
+            const double bounds[4] = {g.xLower(), g.xUpper(), g.yLower(), g.yUpper()};
+            ef_patch_batch b{1, N, bounds};
             ef_coeffs c{};
             std::vector<double> store;
             s.fillCoefficients({&g}, c, store);
-            efCheck(ef_build(m, &c, EF_PRECISION_FP64), "FiniteVolumeSolver::buildD2N");
+            efCheck(ef_patches_create(defaultContext(), &b, &c, &h), "FiniteVolumeSolver");
         }
-        ~OnePatch() { ef_mesh_destroy(m); }
-        void upwards(Vector<double>& f) { efCheck(ef_upwards(m, f.dataPointer(), 1), "FiniteVolumeSolver"); }
-        Vector<double> solve(Vector<double>& g) {
-            std::vector<double> a(4 * N, 1.0);
-            Vector<double> u((std::size_t)N * N);
-            efCheck(ef_solve(m, g.dataPointer(), a.data(), nullptr, u.dataPointer(), 1), "FiniteVolumeSolver::solve");
-            return u;
-        }
-        Matrix<double> get(int which) {
-            int r = 0, c = 0;
-            efCheck(ef_get_operator(m, 0, which, 0, nullptr, &r, &c), "FiniteVolumeSolver");
-            Matrix<double> out(r, c);
-            efCheck(ef_get_operator(m, 0, which, 0, out.dataPointer(), &r, &c), "FiniteVolumeSolver");
-            return out;
-        }
-        Vector<double> getVec(int which) {
-            Matrix<double> t = get(which);
-            return Vector<double>(t.data());
+        ~OnePatch() { ef_patches_destroy(h); }
+        OnePatch(const OnePatch&) = delete;
+        OnePatch& operator=(const OnePatch&) = delete;
+        void checkSizes(const Vector<double>* g, const Vector<double>* f) const {
+            if (g && g->size() != (std::size_t)4 * N)
+                throw std::invalid_argument("[EllipticForest::FiniteVolumeSolver] dirichlet_data must have 4N entries");
+            if (f && f->size() != (std::size_t)N * N)
+                throw std::invalid_argument("[EllipticForest::FiniteVolumeSolver] rhs_data must have N*N entries");
         }
     };
 };

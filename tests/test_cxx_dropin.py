@@ -23,12 +23,25 @@ def example(tmp_path_factory):
     emulator_library()  # builds it if needed
     d = tmp_path_factory.mktemp("emu_lib")
     os.symlink(build_emu.LIB, d / "libefhps.so")
-    exe = efbuild.build_examples(out_dir=str(d))[0]
-    return exe, str(d)
+    exes = efbuild.build_examples(lib_dir=str(d), out_dir=str(d))
+    return dict(zip(efbuild.EXAMPLES, exes)), str(d)
 
 
-def run(example, *args):
-    exe, libdir = example
+@pytest.fixture(scope="module")
+def gpu_examples(tmp_path_factory):
+    """The same drivers linked to the product library (run on the GPU)."""
+    from ellipticforest_amd import build as efbuild
+    from ellipticforest_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail(f"{_lib.LIB_PATH} is missing")
+    d = tmp_path_factory.mktemp("gpu_examples")
+    exes = efbuild.build_examples(lib_dir=os.path.dirname(_lib.LIB_PATH), out_dir=str(d))
+    return dict(zip(efbuild.EXAMPLES, exes)), os.path.dirname(_lib.LIB_PATH)
+
+
+def run(example, *args, name="elliptic_single"):
+    exes, libdir = example
+    exe = exes[name]
     env = dict(os.environ, LD_LIBRARY_PATH=libdir)
     out = subprocess.run([exe] + list(args), capture_output=True, text=True, env=env, timeout=900)
     assert out.returncode == 0, out.stderr
@@ -72,3 +85,62 @@ def test_dropin_adaptive(example):
     got = np.array([float(t) for t in v["errors"]])
     ref = oracle_errors(leaves, 4)
     assert np.abs(got - ref).max() <= 1e-10 * ref.max()
+
+
+def fv_solver_reference():
+    """The oracle's leaf solver on the driver's problems (FISHPACK semantics, lambda = 0)."""
+    U1 = lambda x, y: np.sin(x) + np.sin(y)
+    out = {}
+    for n in (8, 16, 32):
+        bd = (-1.0, 1.0, -1.0, 1.0)
+        X, Y, dx, dy = oracle.cell_centres(n, bd)
+        xs, ys = X[::n], Y[:n]
+        g = np.concatenate([U1(-1.0, ys), U1(1.0, ys), U1(xs, -1.0), U1(xs, 1.0)])
+        u = oracle.leaf_solve(n, bd, g, -U1(X, Y))
+        out[n] = np.abs(u - U1(X, Y)).max()
+    bd = (0.0, 1.0, 0.5, 1.25)
+    n = 8
+    X, Y, dx, dy = oracle.cell_centres(n, bd)
+    xs, ys = X[::n], Y[:n]
+    g = np.concatenate([U1(0.0, ys), U1(1.0, ys), U1(xs, 0.5), U1(xs, 1.25)])
+    f = -U1(X, Y)
+    return out, oracle.leaf_dtn(n, bd), oracle.leaf_map_d2n(n, bd, g, f), oracle.leaf_map_d2n(n, bd, np.zeros(4 * n), f)
+
+
+def check_fv_solver_output(out):
+    errs, T, h, hp = fv_solver_reference()
+    lines = [l.split() for l in out.splitlines()]
+    got_err = {int(l[1]): float(l[2]) for l in lines if l[0] == "err"}
+    for n, e in errs.items():
+        assert abs(got_err[n] - e) <= 1e-9 * e, (n, got_err[n], e)
+    order = [float(l[1]) for l in lines if l[0] == "order"][0]
+    assert order > 1.95  # the reference's assertion (test_finite_volume_solver.cpp:93)
+    for l in lines:
+        if l[0] == "T":
+            i, j, v = int(l[1]), int(l[2]), float(l[3])
+            assert abs(v - T[i, j]) <= 1e-12 * np.abs(T).max(), (i, j)
+    got_h = np.array([float(l[2]) for l in lines if l[0] == "h"])
+    got_hp = np.array([float(l[2]) for l in lines if l[0] == "hp"])
+    assert np.abs(got_h - h).max() <= 1e-12 * np.abs(h).max()
+    assert np.abs(got_hp - hp).max() <= 1e-12 * np.abs(hp).max()
+
+
+def run_raw(example, name, *args):
+    exes, libdir = example
+    env = dict(os.environ, LD_LIBRARY_PATH=libdir)
+    out = subprocess.run([exes[name]] + list(args), capture_output=True, text=True, env=env, timeout=900)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_dropin_fv_solver(example):
+    """FiniteVolumeSolver::solve / buildD2N / mapD2N / particularNeumannData on single grids."""
+    check_fv_solver_output(run_raw(example, "fv_solver"))
+
+
+@pytest.mark.gpu
+def test_gpu_dropin_fv_solver(gpu_examples):
+    """The same driver on the GPU: the drop-in single-grid solver runs on the batched patch
+    entry points."""
+    print("[gpu-case] drop-in FiniteVolumeSolver driver (fv_solver) on the product library", flush=True)
+    check_fv_solver_output(run_raw(gpu_examples, "fv_solver"))
