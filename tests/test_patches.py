@@ -204,3 +204,64 @@ def test_emu_timer_and_stream(emu_ctx):
     pt.solve(np.zeros((1, 1, 32)), np.ones((1, 1, 64)))
     t.stop(pt.stream)
     assert t.elapsed_ms() >= 0.0
+
+
+BETA3 = lambda x, y: 1 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+
+
+def variable_cells(bounds, N, beta=BETA3, lam=-1.0):
+    return np.stack([oracle.sample_coeffs(N, b, lambda x, y: 1.0 + 0 * x, beta, lambda x, y: lam + 0 * x)
+                     for b in bounds])
+
+
+def test_emu_variable_patches_N32(emu_ctx):
+    """Largest block-tridiagonal size (N = 32) on the emulation build."""
+    from ellipticforest_amd import _lib
+    rng = np.random.default_rng(33)
+    bounds = random_bounds(rng, 2)
+    cell = variable_cells(bounds, 32)
+    pv = _lib.Patches(emu_ctx, bounds, 32, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    assert pv.variable
+    check_against_oracle(pv, bounds, 32, rng, nrhs=1, coeffs=cell)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", [8, 16, 32])
+def test_gpu_variable_patches_vs_oracle(patch_ctx, N):
+    """Five-point stencil with cfg3's beta (FiniteVolumeSolver.cpp:64-184): block-tridiagonal path."""
+    from ellipticforest_amd import _lib
+    say(f"patches five-point variable beta N={N}, 4 patches, 2 rhs")
+    rng = np.random.default_rng(300 + N)
+    bounds = random_bounds(rng, 4)
+    cell = variable_cells(bounds, N)
+    pv = _lib.Patches(patch_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    assert pv.variable
+    check_against_oracle(pv, bounds, N, rng, coeffs=cell)
+
+
+@pytest.mark.gpu
+def test_gpu_cfg3_leaf_set_properties(patch_ctx):
+    """The cfg3 leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches) with cfg3's
+    coefficients: oracle on a sample, linearity, and mapD2N = T g + particular data."""
+    from ellipticforest_amd import _lib
+    N, lv = 32, 6
+    w = 2.0 / (1 << lv)
+    bounds = np.array([(-1 + i * w, -1 + (i + 1) * w, -1 + j * w, -1 + (j + 1) * w)
+                       for i in range(1 << lv) for j in range(1 << lv)])
+    say(f"cfg3 leaf set: {len(bounds)} patches, N={N}, variable beta, lambda=-1")
+    cell = variable_cells(bounds, N)
+    pv = _lib.Patches(patch_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    assert pv.variable
+    rng = np.random.default_rng(12)
+    P = len(bounds)
+    g1, g2 = rng.standard_normal((2, 1, P, 4 * N))
+    f1, f2 = rng.standard_normal((2, 1, P, N * N))
+    u1, u2 = pv.solve(g1, f1), pv.solve(g2, f2)
+    assert rel(pv.solve(g1 - 3.0 * g2, f1 - 3.0 * f2), u1 - 3.0 * u2) < 1e-11
+    for p in rng.choice(P, 6, replace=False):
+        assert rel(u1[0, p], oracle.leaf_solve(N, bounds[p], g1[0, p], f1[0, p], coeffs=cell[p])) < TOL, p
+    sample = rng.choice(P, 4, replace=False)
+    T = _lib.Patches(patch_ctx, bounds[sample], N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell[sample]).dtn()
+    h, hp = pv.map_d2n(g1, f1), pv.map_d2n(None, f1)
+    for i, p in enumerate(sample):
+        assert rel(h[0, p], T[i] @ g1[0, p] + hp[0, p]) < 1e-11, p
