@@ -31,6 +31,10 @@ U = lambda x, y: np.sin(x) + np.sin(y)
 F = lambda x, y: -(np.sin(x) + np.sin(y))
 BOUNDS = (-10.0, 10.0, -10.0, 10.0)
 METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds"
+# HBM bytes per launch of the leaf-solve kernel on this workload (16 right-hand sides), from
+# two rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from the copy
+# kernel); profiles/patch_fd_kernel_cfg2_leafset.md has the commands and the raw values.
+TRAFFIC_PER_LAUNCH = 578.1e6
 
 
 def log(msg):
@@ -224,6 +228,24 @@ def traffic_probe(bounds, N, k, copy_doubles=32 << 20):
         f"copy reads and writes {copy_doubles * 8} bytes each")
 
 
+def copy_bandwidth(doubles=64 << 20, reps=5):
+    """Achievable HBM bandwidth of a plain copy on this GPU (the library's copy kernel through
+    ef_memcpy, 512 MiB read + 512 MiB written), best of `reps`, in GB/s."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    ctx = ef.Context(0)
+    a, b = ef.DeviceArray(ctx, (doubles,)), ef.DeviceArray(ctx, (doubles,))
+    t = ef.Timer(ctx)
+    best = None
+    for _ in range(reps):
+        t.start(None)
+        _lib.check(ctx.L.ef_memcpy(ctx.h, b.ptr, a.ptr, doubles * 8, 2), ctx.L)
+        t.stop(None)
+        ms = t.elapsed_ms()
+        best = ms if best is None else min(best, ms)
+    return 2 * doubles * 8 / (best / 1e3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,8 +309,11 @@ def main():
             cpu_baseline=dict(value=cpu_patch["value"], unit="leaf solves/s", cores=1, kind="port",
                               sample="oracle leaf solve (banded LU of the 5-point system), %d patches of the "
                                      "workload in %.1f s at 1 thread" % (cpu_patch["patches"], cpu_patch["seconds"])))]
+        copy_gbs = copy_bandwidth()
+        log(f"plain copy on this GPU: {copy_gbs:.0f} GB/s")
         out["roofline"] = dict(bound="hbm", achieved=pr["gbs"], peak=8000.0, unit="GB/s", frac=pr["gbs"] / 8000.0,
-                               traffic=None, kernel="patch_fd_kernel<1>",
+                               traffic=TRAFFIC_PER_LAUNCH if k == 16 else None, kernel="patch_fd_kernel<1>",
+                               copy_kernel_gbs=copy_gbs, frac_of_copy=pr["gbs"] / copy_gbs,
                                algorithmic_bytes_per_launch=pr["bytes"],
                                scope="leaf-solve stage only; the build stage is switched off in this library build")
     except Exception as e:

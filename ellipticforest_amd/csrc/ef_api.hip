@@ -212,7 +212,7 @@ struct ef_mesh_s {
     DevBuf<int> leaf_cls;                  // per leaf, on the device
     DevBuf<long long> leaf_goff, leaf_hoff;  // vector-arena rows of each leaf's g and h
     DevBuf<LeafDesc> leafdesc;             // block-tridiagonal path
-    DevBuf<double> QA, QB, mu, leafT;      // fast-diagonalisation tables, leaf DtN
+    DevBuf<double> QA, QB, mu, dinv, leafT;  // fast-diagonalisation tables, leaf DtN
     // variable-coefficient leaves (block-tridiagonal path)
     bool variable = false;
     DevBuf<double> vcoef, vstencil, vSinv, vwork;
@@ -396,11 +396,13 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     }
     M->leaf_goff.upload(goff, M->stream);
     M->leaf_hoff.upload(hoff, M->stream);
-    std::vector<double> QA, QB, mu;
+    std::vector<double> QA, QB, mu, dinv;
     patch_fd_tables(N, N <= 16 ? 1 : 2, QA, QB, mu);
+    patch_fd_dinv(N, N <= 16 ? 1 : 2, mu, M->classes_h, dinv);
     M->QA.upload(QA, M->stream);
     M->QB.upload(QB, M->stream);
     M->mu.upload(mu, M->stream);
+    M->dinv.upload(dinv, M->stream);
 }
 
 // Arguments of patch_fd_kernel over the mesh's constant-coefficient leaves.
@@ -412,6 +414,7 @@ PatchFDArgs leaf_fd_args(ef_mesh_s* M, int op) {
     a.QA = M->QA.p;
     a.QB = M->QB.p;
     a.mu = M->mu.p;
+    a.dinv = M->dinv.p;
     a.vk = 1;
     return a;
 }

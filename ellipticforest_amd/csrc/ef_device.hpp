@@ -130,6 +130,7 @@ struct PatchFDArgs {
     const double* QA;       // basis in the MFMA A layout, lane-swizzled: [NB*NB*4][64]
     const double* QB;       // basis in the MFMA B layout, lane-swizzled
     const double* mu;       // eigenvalues of K, padded to 16*NB entries
+    const double* dinv;     // per coefficient entry, 1/D on the padded (16 NB)^2 modes (0 where padded), or null
     const double* f;        // loads [nrhs][npatches][N*N], or null (zero)
     const double* g;        // boundary data, or null (zero); see g_off
     double* out;            // u [nrhs][npatches][N*N] | h (see h_off) | T [npatches][4N][4N]
@@ -175,8 +176,8 @@ static_assert(offsetof(DownDesc, gc_off) == 8, "DownDesc.gc_off");
 static_assert(offsetof(DownDesc, S) == 72, "DownDesc.S");
 static_assert(sizeof(VecStepDesc) == 24, "VecStepDesc");
 static_assert(sizeof(PatchCoef) == 40, "PatchCoef");
-static_assert(sizeof(PatchFDArgs) == 104, "PatchFDArgs");
+static_assert(sizeof(PatchFDArgs) == 112, "PatchFDArgs");
 static_assert(offsetof(PatchFDArgs, coef) == 16, "PatchFDArgs.coef");
-static_assert(offsetof(PatchFDArgs, vk) == 96, "PatchFDArgs.vk");
+static_assert(offsetof(PatchFDArgs, vk) == 104, "PatchFDArgs.vk");
 
 }  // namespace efk

@@ -21,6 +21,10 @@ void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Fast-diagonalisation tables of an N x N patch for patch_fd_kernel (NB = 1 for N <= 16,
 // 2 for N <= 32): the basis in both MFMA operand layouts and the eigenvalues of K.
 void patch_fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, std::vector<double>& mu);
+// 1/D of every coefficient entry on the padded (16 NB)^2 modes, D[k][l] = cx mu_k + cy mu_l + lam
+// (zero on padded modes), entry after entry.
+void patch_fd_dinv(int N, int NB, const std::vector<double>& mu, const std::vector<PatchCoef>& coef,
+                   std::vector<double>& out);
 // Launch patch_fd_kernel for a.op over at most max_blocks workgroups on stream s.
 void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStream_t s);
 // Workgroup cap of patch launches (EF_PATCH_MAX_BLOCKS or the default).

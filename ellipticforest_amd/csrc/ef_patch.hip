@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -32,9 +33,10 @@ struct ef_patches_s {
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool ev_recorded = false;
     std::vector<PatchCoef> coef_h;
-    // fast-diagonalisation tables
+    // fast-diagonalisation tables: one coefficient entry per distinct (cx, cy, lam, dx, dy)
     DevBuf<PatchCoef> coef;
-    DevBuf<double> QA, QB, mu;
+    DevBuf<int> cls;               // patch -> entry
+    DevBuf<double> QA, QB, mu, dinv;
     // block-tridiagonal path
     std::vector<LeafDesc> ld_h;
     DevBuf<LeafDesc> leafdesc;
@@ -91,6 +93,16 @@ void patch_fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>
                 }
 }
 
+void patch_fd_dinv(int N, int NB, const std::vector<double>& mu, const std::vector<PatchCoef>& coef,
+                   std::vector<double>& out) {
+    const int Np = 16 * NB;
+    out.assign(coef.size() * (size_t)Np * Np, 0.0);
+    for (size_t c = 0; c < coef.size(); c++)
+        for (int k = 0; k < N; k++)
+            for (int l = 0; l < N; l++)
+                out[(c * Np + k) * Np + l] = 1.0 / (coef[c].cx * mu[k] + coef[c].cy * mu[l] + coef[c].lam);
+}
+
 void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStream_t s) {
     const long long waves =
         (a.op == PATCH_OP_DTN) ? (long long)a.npatches * 4 * a.N : (long long)a.npatches * a.nrhs;
@@ -127,10 +139,11 @@ void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double
     a.nrhs = nrhs;
     a.op = op;
     a.coef = p->coef.p;
-    a.cls = nullptr;
+    a.cls = p->cls.p;
     a.QA = p->QA.p;
     a.QB = p->QB.p;
     a.mu = p->mu.p;
+    a.dinv = p->dinv.p;
     a.f = f;
     a.g = g;
     a.out = out;
@@ -263,7 +276,30 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
         p->QA.upload(QA, p->stream);
         p->QB.upload(QB, p->stream);
         p->mu.upload(mu, p->stream);
-        p->coef.upload(p->coef_h, p->stream);
+        // distinct coefficient entries (a leaf set has a handful: one per level and shape)
+        std::map<std::vector<uint64_t>, int> seen;
+        auto bits = [](double v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
+        std::vector<PatchCoef> entries;
+        std::vector<int> cls(p->P);
+        for (int i = 0; i < p->P; i++) {
+            const PatchCoef& k = p->coef_h[i];
+            std::vector<uint64_t> key = {bits(k.cx), bits(k.cy), bits(k.lam), bits(k.dx), bits(k.dy)};
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                cls[i] = (int)entries.size();
+                seen[key] = cls[i];
+                entries.push_back(k);
+            } else cls[i] = it->second;
+        }
+        p->coef.upload(entries, p->stream);
+        p->cls.upload(cls, p->stream);
+        // 1/D tables unless they would outweigh the loads themselves (then computed in the kernel)
+        const size_t Np = 16 * (size_t)p->NB;
+        if (entries.size() * Np * Np <= (size_t)p->P * NN / 4) {
+            std::vector<double> dinv;
+            efk::patch_fd_dinv(N, p->NB, mu, entries, dinv);
+            p->dinv.upload(dinv, p->stream);
+        }
     }
     EF_HIP(hipStreamSynchronize(p->stream));
     *out = p.release();

@@ -160,12 +160,29 @@ patch_fd_kernel(PatchFDArgs p) {
             const int patch2 = (OP == PATCH_OP_DTN) ? o2 : i2, rhs2 = (OP == PATCH_OP_DTN) ? 0 : o2;
             patch_fetch<NB, FULL, OP>(p, patch2, rhs2, lane, nxt);
         }
-        const PatchCoef cf = p.coef[p.cls ? p.cls[patch] : patch];
+        const int ce = p.cls ? p.cls[patch] : patch;
+        const PatchCoef cf = p.coef[ce];
+        const double* dinv_row = p.dinv ? p.dinv + (long long)ce * (16 * NB) * (16 * NB) : nullptr;
         const bool has_g = (OP == PATCH_OP_DTN) || p.g != nullptr;
 
         // fold the boundary data into the load: -2 (c_W g_W + c_E g_E) on the x-edge
         // cells, -2 (c_S g_S + c_N g_N) on the y-edge cells
         double BT[NB][NB][4];
+        if (NB == 1 && FULL && OP != PATCH_OP_DTN) {
+            // N = 16: element s of lane (q, r) is cell (i, j) = (4s + q, r).  x-edges: (s, q) =
+            // (0, 0) takes g_W[r] (entry r), (3, 3) takes g_E[r] (entry 16 + r); y-edges: r = 0
+            // takes g_S[4s + q] (entry 32 + 4s + q), r = 15 takes g_N[4s + q] (entry 48 + 4s + q).
+            const double xg = has_g ? __shfl(cur.G[0], q == 0 ? r : 16 + r) : 0.0;
+            const double ax = -2.0 * cf.cx, ay = (r == 0 || r == 15) ? -2.0 * cf.cy : 0.0;
+            const int ybase = (r == 0) ? 32 + q : 48 + q;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const double yg = has_g ? __shfl(cur.G[0], ybase + 4 * s) : 0.0;
+                double v = fma(ay, yg, cur.F[0][0][s]);
+                if ((s == 0 && q == 0) || (s == 3 && q == 3)) v = fma(ax, xg, v);
+                BT[0][0][s] = v;
+            }
+        } else {
 #pragma unroll
         for (int bi = 0; bi < NB; bi++) {
             const int j = 16 * bi + r;
@@ -188,6 +205,7 @@ patch_fd_kernel(PatchFDArgs p) {
                     BT[bi][bk][s] = v;
                 }
         }
+        }
 
         pd4 Z[NB][NB], Y[NB][NB];
         // Z1^T = B^T Q
@@ -204,13 +222,17 @@ patch_fd_kernel(PatchFDArgs p) {
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const int kk = 16 * bi + 4 * v + q, ll = 16 * bj + r;
-                    const double d = cf.cx * muk[bi][v] + cf.cy * mul[bj] + cf.lam;
-                    // 1/d: hardware estimate and two Newton steps (relative error ~ 1 ulp)
-                    double rc = __builtin_amdgcn_rcp(d);
-                    rc = fma(rc, fma(-d, rc, 1.0), rc);
-                    rc = fma(rc, fma(-d, rc, 1.0), rc);
-                    const double z = Y[bi][bj][v] * rc;
-                    Y[bi][bj][v] = (FULL || (kk < Nn && ll < Nn)) ? z : 0.0;
+                    if (p.dinv) {
+                        Y[bi][bj][v] *= dinv_row[kk * (16 * NB) + ll];
+                    } else {
+                        const double d = cf.cx * muk[bi][v] + cf.cy * mul[bj] + cf.lam;
+                        // 1/d: hardware estimate and two Newton steps (relative error ~ 1 ulp)
+                        double rc = __builtin_amdgcn_rcp(d);
+                        rc = fma(rc, fma(-d, rc, 1.0), rc);
+                        rc = fma(rc, fma(-d, rc, 1.0), rc);
+                        const double z = Y[bi][bj][v] * rc;
+                        Y[bi][bj][v] = (FULL || (kk < Nn && ll < Nn)) ? z : 0.0;
+                    }
                 }
         // Z4^T = Z3^T Q^T: A operand of Z3^T = accumulator of Z3 swapped; B operand of Q^T = QA swapped
         patch_mm<NB>([&](int bi, int bk, int s) { return Y[bk][bi][s]; },

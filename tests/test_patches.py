@@ -265,3 +265,13 @@ def test_gpu_cfg3_leaf_set_properties(patch_ctx):
     h, hp = pv.map_d2n(g1, f1), pv.map_d2n(None, f1)
     for i, p in enumerate(sample):
         assert rel(h[0, p], T[i] @ g1[0, p] + hp[0, p]) < 1e-11, p
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_emu_shared_coefficient_tables(emu_ctx, N):
+    """Patches of equal size share one coefficient entry and its 1/D table (the leaf-set case)."""
+    from ellipticforest_amd import _lib
+    rng = np.random.default_rng(40 + N)
+    bounds = np.array([(-1 + 0.5 * i, -0.5 + 0.5 * i, -1 + 0.5 * j, -0.5 + 0.5 * j) for i in range(4) for j in range(4)])
+    pt = _lib.Patches(emu_ctx, bounds, N, lambda0=-0.5)
+    check_against_oracle(pt, bounds, N, rng, nrhs=2, lam=-0.5)
