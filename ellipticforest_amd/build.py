@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libefhps.so")
 
-SOURCES = ["ef_kernels.hip", "ef_patch_kernels.hip", "ef_api.hip", "ef_patch.hip", "ef_mesh.cpp"]
+SOURCES = ["ef_kernels.hip", "ef_patch_kernels.hip", "ef_patch_large.hip", "ef_api.hip", "ef_patch.hip", "ef_mesh.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

@@ -141,6 +141,32 @@ struct PatchFDArgs {
     int vk;
 };
 
+// Batched GEMM with arbitrary strides (transposes are free): for every batch entry b,
+// C_b = A_b B_b, A_b(i, k) = A[b*sa + i*ra + k*ca], B_b(k, j) = B[b*sb + k*rb + j*cb],
+// C_b(i, j) = C[b*sc + i*rc + j*cc].  M, N multiples of 16, K a multiple of 4.
+struct GemmStridedArgs {
+    const double* A; long long sa, ra, ca;
+    const double* B; long long sb, rb, cb;
+    double* C; long long sc, rc, cc;
+    int M, N, K, batch;
+};
+
+// Element-wise steps of the fast diagonalisation of large patches (N > 32), on buffers of
+// `items` N x N cell arrays.  Item t is (rhs, patch) = (t / npatches, t % npatches) for the
+// solves, or DtN column `col0 + t` of patch `patch0` for DtN chunks.
+struct PatchLargeArgs {
+    int N, npatches, op, items;
+    int patch0, col0;          // DtN chunks
+    const PatchCoef* coef;
+    const int* cls;
+    const double* mu;          // N eigenvalues
+    const double* f;           // [nrhs][npatches][N*N] or null
+    const double* g;           // [nrhs][npatches][4N] or null
+    double* work;              // items x N*N
+    double* out;               // u (solve) | h | T
+    long long item0;           // first item of this chunk
+};
+
 // One interpolation step on vectors in an arena (uncoarsen_: L12 of fine size nfine per side).
 struct VecStepDesc {
     long long src_off, dst_off;

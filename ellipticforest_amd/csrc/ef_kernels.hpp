@@ -28,6 +28,10 @@ __global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double*
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)
 template <int NB, bool FULL, int OP>
 __global__ void patch_fd_kernel(PatchFDArgs p);
+__global__ void gemm_strided_f64_kernel(GemmStridedArgs a);
+__global__ void patch_large_fold_kernel(PatchLargeArgs a);
+__global__ void patch_large_scale_kernel(PatchLargeArgs a);
+__global__ void patch_large_out_kernel(PatchLargeArgs a);
 __global__ void fill_kernel(double* p, double v, long long n);
 __global__ void copy_kernel(const double* src, double* dst, long long n);
 

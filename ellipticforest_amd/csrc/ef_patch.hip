@@ -29,6 +29,8 @@ struct ef_patches_s {
     int N = 0, P = 0, NB = 1;
     long long max_blocks = 4096;
     bool variable = false;
+    bool large = false;            // N > 32: batched-GEMM fast diagonalisation
+    int Np = 0;                    // padded side of the large path
     hipStream_t stream = nullptr, own_stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool ev_recorded = false;
@@ -41,6 +43,8 @@ struct ef_patches_s {
     std::vector<LeafDesc> ld_h;
     DevBuf<LeafDesc> leafdesc;
     DevBuf<double> vcoef, vstencil, vSinv, vwork;
+    // large path: padded basis (Np x Np), work buffers
+    DevBuf<double> Qp, W1, W2;
     // staging of the host-buffer entry points
     DevBuf<double> gbuf, fbuf, obuf;
 
@@ -153,6 +157,70 @@ void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double
     efk::launch_patch_fd(a, p->NB, p->max_blocks, p->stream);
 }
 
+// Large patches: chunks of items through fold, four GEMMs, scaling and the output step.
+void launch_gemm_strided(const GemmStridedArgs& g, hipStream_t s) {
+    const long long tiles = (long long)(g.M / 16) * (g.N / 16);
+    hipLaunchKernelGGL(gemm_strided_f64_kernel, dim3((unsigned)((tiles + 3) / 4), (unsigned)g.batch), dim3(256), 0, s, g);
+    EF_LAUNCH_CHECK();
+}
+
+void launch_large(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs, int patch0 = 0) {
+    const int N = p->N, Np = p->Np;
+    const long long per = (long long)Np * Np;
+    const long long total = (op == PATCH_OP_DTN) ? 4LL * N : (long long)p->P * nrhs;
+    const long long chunk = std::max<long long>(1, std::min<long long>({65535LL, total, (16LL << 20) / per}));
+    p->W1.ensure((size_t)(chunk * per));
+    p->W2.ensure((size_t)(chunk * per));
+    for (long long first = 0; first < total; first += chunk) {
+        const int items = (int)std::min(chunk, total - first);
+        PatchLargeArgs a{};
+        a.N = N;
+        a.npatches = p->P;
+        a.op = op;
+        a.items = items;
+        a.patch0 = patch0;
+        a.col0 = (int)first;
+        a.coef = p->coef.p;
+        a.cls = p->cls.p;
+        a.mu = p->mu.p;
+        a.f = f;
+        a.g = g;
+        a.work = p->W1.p;
+        a.out = out;
+        a.item0 = (op == PATCH_OP_DTN) ? 0 : first;
+        hipLaunchKernelGGL(patch_large_fold_kernel, dim3(blocks_for(items * per)), dim3(256), 0, p->stream, a);
+        EF_LAUNCH_CHECK();
+        GemmStridedArgs m{};
+        m.M = m.N = m.K = Np;
+        m.batch = items;
+        // W2 = Q^T W1
+        m.A = p->Qp.p; m.sa = 0; m.ra = 1; m.ca = Np;
+        m.B = p->W1.p; m.sb = per; m.rb = Np; m.cb = 1;
+        m.C = p->W2.p; m.sc = per; m.rc = Np; m.cc = 1;
+        launch_gemm_strided(m, p->stream);
+        // W1 = W2 Q
+        m.A = p->W2.p; m.sa = per; m.ra = Np; m.ca = 1;
+        m.B = p->Qp.p; m.sb = 0; m.rb = Np; m.cb = 1;
+        m.C = p->W1.p;
+        launch_gemm_strided(m, p->stream);
+        hipLaunchKernelGGL(patch_large_scale_kernel, dim3(blocks_for(items * per)), dim3(256), 0, p->stream, a);
+        EF_LAUNCH_CHECK();
+        // W2 = Q W1
+        m.A = p->Qp.p; m.sa = 0; m.ra = Np; m.ca = 1;
+        m.B = p->W1.p; m.sb = per; m.rb = Np; m.cb = 1;
+        m.C = p->W2.p;
+        launch_gemm_strided(m, p->stream);
+        // W1 = W2 Q^T
+        m.A = p->W2.p; m.sa = per; m.ra = Np; m.ca = 1;
+        m.B = p->Qp.p; m.sb = 0; m.rb = 1; m.cb = Np;
+        m.C = p->W1.p;
+        launch_gemm_strided(m, p->stream);
+        const long long nout = (op == PATCH_OP_SOLVE) ? (long long)items * N * N : (long long)items * 4 * N;
+        hipLaunchKernelGGL(patch_large_out_kernel, dim3(blocks_for(nout)), dim3(256), 0, p->stream, a);
+        EF_LAUNCH_CHECK();
+    }
+}
+
 // True when every per-cell array of the patch is constant and the four face betas agree.
 bool patch_constant(const double* c, long long NN) {
     const double a0 = c[0], b0 = c[NN], l0 = c[5 * NN];
@@ -203,6 +271,7 @@ void apply(ef_patches_s* p, int op, const double* g, const double* f, double* ou
     if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
     if (!out) throw EfError(EF_ERR_ARG, "null output");
     if (p->variable) launch_bt(p, op, g, f, out, nrhs);
+    else if (p->large) launch_large(p, op, g, f, out, nrhs);
     else launch_fd(p, op, g, f, out, nrhs);
 }
 
@@ -214,7 +283,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
     EF_API_BEGIN
     if (!ctx || !b || !c || !out) throw EfError(EF_ERR_ARG, "null argument");
     if (b->npatches < 1) throw EfError(EF_ERR_ARG, "npatches must be >= 1");
-    if (b->N < 4 || b->N > 32) throw EfError(EF_ERR_ARG, "patch side N must be in [4, 32]");
+    if (b->N < 4 || b->N > 1024) throw EfError(EF_ERR_ARG, "patch side N must be in [4, 1024]");
     if (!b->bounds) throw EfError(EF_ERR_ARG, "null patch bounds");
     if (c->solver_type != EF_SOLVER_FISHPACK && c->solver_type != EF_SOLVER_FIVE_POINT)
         throw EfError(EF_ERR_ARG, "unknown solver type");
@@ -224,6 +293,8 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
     p->N = b->N;
     p->P = b->npatches;
     p->NB = (b->N <= 16) ? 1 : 2;
+    p->large = b->N > 32;
+    p->Np = (b->N + 15) / 16 * 16;
     p->max_blocks = efk::patch_default_blocks();
     EF_HIP(hipStreamCreate(&p->own_stream));
     p->stream = p->own_stream;
@@ -247,6 +318,8 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
             k.cx = cc[0] * cc[NN] / (dx * dx); k.cy = cc[0] * cc[NN] / (dy * dy); k.lam = cc[5 * NN];
         }
     }
+    if (p->variable && p->N > BT_MAX_N)
+        throw EfError(EF_ERR_ARG, "coefficients that vary within a patch need N <= " + std::to_string(BT_MAX_N));
     if (p->variable) {
         const size_t P = p->P;
         p->vcoef.alloc(P * 6 * NN);
@@ -272,9 +345,26 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
         EF_LAUNCH_CHECK();
     } else {
         std::vector<double> QA, QB, mu;
-        efk::patch_fd_tables(N, p->NB, QA, QB, mu);
-        p->QA.upload(QA, p->stream);
-        p->QB.upload(QB, p->stream);
+        if (p->large) {
+            // padded basis, row-major Np x Np, and the N eigenvalues
+            const int Np = p->Np;
+            std::vector<double> Q((size_t)Np * Np, 0.0);
+            mu.assign(N, 0.0);
+            for (int i = 0; i < N; i++)
+                for (int k = 0; k < N; k++) {
+                    const double nrm = (k == N - 1) ? std::sqrt(1.0 / N) : std::sqrt(2.0 / N);
+                    Q[(size_t)i * Np + k] = nrm * std::sin(M_PI * (double)(k + 1) * ((double)i + 0.5) / (double)N);
+                }
+            for (int k = 0; k < N; k++) {
+                const double sk = std::sin(M_PI * (double)(k + 1) / (2.0 * N));
+                mu[k] = -4.0 * sk * sk;
+            }
+            p->Qp.upload(Q, p->stream);
+        } else {
+            efk::patch_fd_tables(N, p->NB, QA, QB, mu);
+            p->QA.upload(QA, p->stream);
+            p->QB.upload(QB, p->stream);
+        }
         p->mu.upload(mu, p->stream);
         // distinct coefficient entries (a leaf set has a handful: one per level and shape)
         std::map<std::vector<uint64_t>, int> seen;
@@ -295,7 +385,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
         p->cls.upload(cls, p->stream);
         // 1/D tables unless they would outweigh the loads themselves (then computed in the kernel)
         const size_t Np = 16 * (size_t)p->NB;
-        if (entries.size() * Np * Np <= (size_t)p->P * NN / 4) {
+        if (!p->large && entries.size() * Np * Np <= (size_t)p->P * NN / 4) {
             std::vector<double> dinv;
             efk::patch_fd_dinv(N, p->NB, mu, entries, dinv);
             p->dinv.upload(dinv, p->stream);
@@ -385,6 +475,8 @@ int ef_patches_dtn(ef_patches p, double* T) {
             hipLaunchKernelGGL(bt_dtn_kernel, dim3(cnt), dim3(256), 0, p->stream, p->leafdesc.p, p->N, first, p->vwork.p);
             EF_LAUNCH_CHECK();
         }
+    } else if (p->large) {
+        for (int i = 0; i < p->P; i++) launch_large(p, PATCH_OP_DTN, nullptr, nullptr, p->obuf.p, 1, i);
     } else {
         launch_fd(p, PATCH_OP_DTN, nullptr, nullptr, p->obuf.p, 1);
     }

@@ -57,12 +57,16 @@ template <int NB>
 struct PatchItemIn {
     double F[NB][NB][4];
     double G[NB];
+    PatchCoef cf;  // the item's coefficient entry (scalar loads issued with the prefetch)
+    int ce;
 };
 
 template <int NB, bool FULL, int OP>
 __device__ inline void patch_fetch(const PatchFDArgs& p, int patch, int rhs, int lane, PatchItemIn<NB>& in) {
     const int Nn = p.N, q = lane >> 4, r = lane & 15;
     const long long slot = (long long)rhs * p.npatches + patch;
+    in.ce = p.cls ? p.cls[patch] : patch;
+    in.cf = p.coef[in.ce];
     const double* f = p.f ? p.f + slot * Nn * Nn : nullptr;
 #pragma unroll
     for (int bi = 0; bi < NB; bi++)
@@ -160,9 +164,8 @@ patch_fd_kernel(PatchFDArgs p) {
             const int patch2 = (OP == PATCH_OP_DTN) ? o2 : i2, rhs2 = (OP == PATCH_OP_DTN) ? 0 : o2;
             patch_fetch<NB, FULL, OP>(p, patch2, rhs2, lane, nxt);
         }
-        const int ce = p.cls ? p.cls[patch] : patch;
-        const PatchCoef cf = p.coef[ce];
-        const double* dinv_row = p.dinv ? p.dinv + (long long)ce * (16 * NB) * (16 * NB) : nullptr;
+        const PatchCoef cf = cur.cf;
+        const double* dinv_row = p.dinv ? p.dinv + (long long)cur.ce * (16 * NB) * (16 * NB) : nullptr;
         const bool has_g = (OP == PATCH_OP_DTN) || p.g != nullptr;
 
         // fold the boundary data into the load: -2 (c_W g_W + c_E g_E) on the x-edge

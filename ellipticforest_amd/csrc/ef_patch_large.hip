@@ -1,0 +1,140 @@
+// ef_patch_large.hip -- fast diagonalisation of large patches (N > 32) as batched GEMMs.
+//
+// Same mathematics as patch_fd_kernel (U = Q ((Q^T B Q) ./ D) Q^T, FiniteVolumeSolver.cpp
+// :25-294 with constant coefficients), but the N x N arrays no longer fit one wave's
+// registers: each step is a launch over a chunk of items held in HBM work buffers padded
+// to Np = 16 ceil(N / 16) (zero padding; padded modes are zeroed at the scaling step).
+//   fold   B = f - 2 c g on the edge cells (or a unit boundary column for DtN chunks)
+//   GEMM   W2 = Q^T W1,  W1 = W2 Q,  scale W1 ./ D,  W2 = Q W1,  W1 = W2 Q^T
+//   out    u, or the Neumann data h / DtN column from the edge cells of u
+// The GEMM is register-resident like the small-patch kernel: one wave per 16 x 16 output
+// tile, operands read straight from memory in the v_mfma_f64_16x16x4_f64 layouts
+// (A[r][k0 + q], B[k0 + q][r] for lane 16q + r), no LDS and no scratch.
+#include <hip/hip_runtime.h>
+#include "ef_device.hpp"
+#include "ef_kernels.hpp"
+
+namespace efk {
+
+typedef double ld4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) gemm_strided_f64_kernel(GemmStridedArgs a) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int q = lane >> 4, r = lane & 15;
+    const int tn = a.N / 16;
+    const long long w = (long long)blockIdx.x * 4 + wave;
+    if (w >= (long long)(a.M / 16) * tn) return;  // uniform per wave
+    const long long b = blockIdx.y;
+    const int row0 = (int)(w / tn) * 16, col0 = (int)(w % tn) * 16;
+    const double* A = a.A + b * a.sa + (long long)(row0 + r) * a.ra;
+    const double* B = a.B + b * a.sb + (long long)(col0 + r) * a.cb;
+    ld4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < a.K; k0 += 4) {
+        const double av = A[(long long)(k0 + q) * a.ca];
+        const double bv = B[(long long)(k0 + q) * a.rb];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+    double* C = a.C + b * a.sc + (long long)(col0 + r) * a.cc;
+#pragma unroll
+    for (int v = 0; v < 4; v++) C[(long long)(row0 + 4 * v + q) * a.rc] = acc[v];
+}
+
+// item t of the chunk -> (patch, rhs, DtN column or -1)
+__device__ inline void large_item(const PatchLargeArgs& a, long long t, int& patch, int& rhs, int& col) {
+    const long long it = a.item0 + t;
+    if (a.op == PATCH_OP_DTN) {
+        patch = a.patch0;
+        rhs = 0;
+        col = a.col0 + (int)t;
+    } else {
+        rhs = (int)(it / a.npatches);
+        patch = (int)(it % a.npatches);
+        col = -1;
+    }
+}
+
+__device__ inline double large_g(const PatchLargeArgs& a, const double* g, int col, int t) {
+    if (col >= 0) return (t == col) ? 1.0 : 0.0;
+    return g ? g[t] : 0.0;
+}
+
+// work[t][i][j] (Np x Np) = f - 2 c g at the edge cells, zero in the padding
+__global__ void patch_large_fold_kernel(PatchLargeArgs a) {
+    const int Nn = a.N, Np = (Nn + 15) / 16 * 16;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)a.items * Np * Np) return;
+    const long long t = e / ((long long)Np * Np);
+    const int i = (int)((e / Np) % Np), j = (int)(e % Np);
+    double v = 0.0;
+    if (i < Nn && j < Nn) {
+        int patch, rhs, col;
+        large_item(a, t, patch, rhs, col);
+        const long long slot = (long long)rhs * a.npatches + patch;
+        const PatchCoef cf = a.coef[a.cls ? a.cls[patch] : patch];
+        if (a.f && col < 0) v = a.f[slot * Nn * Nn + (long long)i * Nn + j];
+        const double* g = (a.g && col < 0) ? a.g + slot * 4 * Nn : nullptr;
+        double add = 0.0;
+        if (i == 0) add += cf.cx * large_g(a, g, col, W * Nn + j);
+        if (i == Nn - 1) add += cf.cx * large_g(a, g, col, E * Nn + j);
+        if (j == 0) add += cf.cy * large_g(a, g, col, S * Nn + i);
+        if (j == Nn - 1) add += cf.cy * large_g(a, g, col, N * Nn + i);
+        v += -2.0 * add;
+    }
+    a.work[e] = v;
+}
+
+// work[t][k][l] /= cx mu_k + cy mu_l + lam, zero on padded modes
+__global__ void patch_large_scale_kernel(PatchLargeArgs a) {
+    const int Nn = a.N, Np = (Nn + 15) / 16 * 16;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long long)a.items * Np * Np) return;
+    const long long t = e / ((long long)Np * Np);
+    const int k = (int)((e / Np) % Np), l = (int)(e % Np);
+    if (k >= Nn || l >= Nn) {
+        a.work[e] = 0.0;
+        return;
+    }
+    int patch, rhs, col;
+    large_item(a, t, patch, rhs, col);
+    const PatchCoef cf = a.coef[a.cls ? a.cls[patch] : patch];
+    a.work[e] = a.work[e] / (cf.cx * a.mu[k] + cf.cy * a.mu[l] + cf.lam);
+}
+
+// u out, or h / DtN column from the edge cells: one thread per cell (solve) or per
+// boundary position (Neumann data)
+__global__ void patch_large_out_kernel(PatchLargeArgs a) {
+    const int Nn = a.N, Np = (Nn + 15) / 16 * 16;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (a.op == PATCH_OP_SOLVE) {
+        if (e >= (long long)a.items * Nn * Nn) return;
+        const long long t = e / ((long long)Nn * Nn);
+        const int i = (int)((e / Nn) % Nn), j = (int)(e % Nn);
+        int patch, rhs, col;
+        large_item(a, t, patch, rhs, col);
+        const long long slot = (long long)rhs * a.npatches + patch;
+        a.out[slot * Nn * Nn + (long long)i * Nn + j] = a.work[t * Np * Np + (long long)i * Np + j];
+        return;
+    }
+    if (e >= (long long)a.items * 4 * Nn) return;
+    const long long t = e / (4 * Nn);
+    const int b = (int)(e % (4 * Nn));
+    int patch, rhs, col;
+    large_item(a, t, patch, rhs, col);
+    const long long slot = (long long)rhs * a.npatches + patch;
+    const PatchCoef cf = a.coef[a.cls ? a.cls[patch] : patch];
+    const double* g = (a.g && col < 0) ? a.g + slot * 4 * Nn : nullptr;
+    const int sd = b / Nn, m = b % Nn;
+    int i, j;
+    if (sd == W) { i = 0; j = m; }
+    else if (sd == E) { i = Nn - 1; j = m; }
+    else if (sd == S) { i = m; j = 0; }
+    else { i = m; j = Nn - 1; }
+    const double uu = a.work[t * Np * Np + (long long)i * Np + j];
+    const double dd = (sd < 2) ? cf.dx : cf.dy;
+    const double sg = (sd == W || sd == S) ? 2.0 / dd : -2.0 / dd;
+    const double h = sg * (uu - large_g(a, g, col, b));
+    if (a.op == PATCH_OP_DTN) a.out[(long long)patch * 16 * Nn * Nn + (long long)b * 4 * Nn + col] = h;
+    else a.out[slot * 4 * Nn + b] = h;
+}
+
+}  // namespace efk

@@ -1,16 +1,17 @@
 // fv_solver.cpp -- the single-grid FiniteVolumeSolver operations through the drop-in C++ API,
 // in the shape of the reference's test_finite_volume_solver.cpp: u = sin x + sin y on
 // [-1,1]^2, Dirichlet data at the faces, f = -(sin x + sin y), FISHPACK semantics; the
-// max-norm error for n = 8, 16, 32 and the convergence order of the last two.  Then, on an
+// max-norm error for n = 8, 16, ..., M and the convergence order of the last two.  Then, on an
 // 8 x 8 grid over [0,1] x [0.5,1.25], the DtN matrix, the Neumann data of the same problem
 // and its particular part, printed for comparison with other implementations.
-//   fv_solver
+//   fv_solver [--max-n=M]   (n = 8, 16, ..., M; default 32, the reference's test runs to 512)
 // Output lines: "err n linf", "order p", "T i j v" (a few entries), "h i v", "hp i v".
 #include <EllipticForest.hpp>
 #include <Patches/FiniteVolume/FiniteVolume.hpp>
 
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 using namespace EllipticForest;
 
@@ -44,9 +45,11 @@ int main(int argc, char** argv) {
     solver.beta_function = [](double, double) { return 1.0; };
     solver.lambda_function = [](double, double) { return 0.0; };
 
-    const int ns[3] = {8, 16, 32};
-    double errors[3];
-    for (int t = 0; t < 3; t++) {
+    const int max_n = app.options.optionExists("max-n") ? std::get<int>(app.options["max-n"]) : 32;
+    std::vector<int> ns;
+    for (int n = 8; n <= max_n; n *= 2) ns.push_back(n);
+    std::vector<double> errors(ns.size());
+    for (std::size_t t = 0; t < ns.size(); t++) {
         const int n = ns[t];
         FiniteVolumeGrid grid(MPI_COMM_WORLD, n, -1.0, 1.0, n, -1.0, 1.0);
         Vector<double> g, f, ue;
@@ -57,7 +60,8 @@ int main(int argc, char** argv) {
         errors[t] = e;
         std::printf("err %d %.17g\n", n, e);
     }
-    std::printf("order %.17g\n", std::log(errors[2] / errors[1]) / std::log((double)ns[1] / ns[2]));
+    const std::size_t L = ns.size();
+    std::printf("order %.17g\n", std::log(errors[L - 1] / errors[L - 2]) / std::log((double)ns[L - 2] / ns[L - 1]));
 
     const int n = 8;
     FiniteVolumeGrid grid(MPI_COMM_WORLD, n, 0.0, 1.0, n, 0.5, 1.25);

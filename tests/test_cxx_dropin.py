@@ -87,11 +87,11 @@ def test_dropin_adaptive(example):
     assert np.abs(got - ref).max() <= 1e-10 * ref.max()
 
 
-def fv_solver_reference():
+def fv_solver_reference(max_n=32):
     """The oracle's leaf solver on the driver's problems (FISHPACK semantics, lambda = 0)."""
     U1 = lambda x, y: np.sin(x) + np.sin(y)
     out = {}
-    for n in (8, 16, 32):
+    for n in [8 << i for i in range(8) if (8 << i) <= max_n]:
         bd = (-1.0, 1.0, -1.0, 1.0)
         X, Y, dx, dy = oracle.cell_centres(n, bd)
         xs, ys = X[::n], Y[:n]
@@ -107,8 +107,8 @@ def fv_solver_reference():
     return out, oracle.leaf_dtn(n, bd), oracle.leaf_map_d2n(n, bd, g, f), oracle.leaf_map_d2n(n, bd, np.zeros(4 * n), f)
 
 
-def check_fv_solver_output(out):
-    errs, T, h, hp = fv_solver_reference()
+def check_fv_solver_output(out, max_n=32):
+    errs, T, h, hp = fv_solver_reference(max_n)
     lines = [l.split() for l in out.splitlines()]
     got_err = {int(l[1]): float(l[2]) for l in lines if l[0] == "err"}
     for n, e in errs.items():
@@ -143,4 +143,4 @@ def test_gpu_dropin_fv_solver(gpu_examples):
     """The same driver on the GPU: the drop-in single-grid solver runs on the batched patch
     entry points."""
     print("[gpu-case] drop-in FiniteVolumeSolver driver (fv_solver) on the product library", flush=True)
-    check_fv_solver_output(run_raw(gpu_examples, "fv_solver"))
+    check_fv_solver_output(run_raw(gpu_examples, "fv_solver", "--max-n=256"), max_n=256)

@@ -275,3 +275,30 @@ def test_emu_shared_coefficient_tables(emu_ctx, N):
     bounds = np.array([(-1 + 0.5 * i, -0.5 + 0.5 * i, -1 + 0.5 * j, -0.5 + 0.5 * j) for i in range(4) for j in range(4)])
     pt = _lib.Patches(emu_ctx, bounds, N, lambda0=-0.5)
     check_against_oracle(pt, bounds, N, rng, nrhs=2, lam=-0.5)
+
+
+@pytest.mark.parametrize("N", [40, 64])
+def test_emu_large_patches(emu_ctx, N):
+    """N > 32: fast diagonalisation as batched GEMMs (padding to a multiple of 16 for N = 40)."""
+    from ellipticforest_amd import _lib
+    rng = np.random.default_rng(50 + N)
+    bounds = random_bounds(rng, 2)
+    pt = _lib.Patches(emu_ctx, bounds, N, lambda0=-1.0)
+    check_against_oracle(pt, bounds, N, rng, nrhs=1, lam=-1.0, dtn=False)  # DtN of large patches: GPU test
+    with pytest.raises(_lib.EfError):  # coefficients varying within a patch stay at N <= 32
+        cell = np.stack([oracle.sample_coeffs(N, b, lambda x, y: 1.0 + 0 * x, lambda x, y: 1.0 + x * y,
+                                              lambda x, y: 0 * x) for b in bounds])
+        _lib.Patches(emu_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("N", [48, 64, 100, 128])
+def test_gpu_large_patches_vs_oracle(patch_ctx, N):
+    """Single grids beyond one wave's registers (the reference's solver test runs n up to 512):
+    solve, mapD2N, particular data and (N <= 64) the DtN against the oracle."""
+    from ellipticforest_amd import _lib
+    say(f"large patches N={N}, 3 patches, 2 rhs, lambda=-1")
+    rng = np.random.default_rng(500 + N)
+    bounds = random_bounds(rng, 3)
+    pt = _lib.Patches(patch_ctx, bounds, N, lambda0=-1.0)
+    check_against_oracle(pt, bounds, N, rng, nrhs=2, lam=-1.0, dtn=(N <= 64))
