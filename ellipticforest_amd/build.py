@@ -51,7 +51,7 @@ def build(verbose=False):
     return LIB
 
 
-EXAMPLES = ["elliptic_single", "fv_solver"]
+EXAMPLES = ["elliptic_single", "fv_solver", "thermal"]
 
 
 def build_examples(lib_dir=None, out_dir=None, verbose=False):

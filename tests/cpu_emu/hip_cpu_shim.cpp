@@ -55,7 +55,12 @@ hipError_t hipFree(void* p) {
 static const int NT = 256;
 static std::barrier<>* g_block_bar = nullptr;
 static std::barrier<>* g_wave_bar[NT / 64];
-static double g_mfma_a[NT], g_mfma_b[NT];
+// Operand slots of the wave-collective operations, double-buffered by call parity: a lane
+// writes slot parity p, passes the wave barrier and reads slot p; the next collective uses
+// slot 1 - p, so one barrier per call suffices (no lane can write slot p again before every
+// lane has passed the next call's barrier, i.e. finished reading slot p).
+static double g_mfma_a[2][NT], g_mfma_b[2][NT];
+static thread_local int g_par = 0;
 static void* g_dyn = nullptr;
 static void* g_dyn_base = nullptr;
 static size_t g_dyn_len = 0;
@@ -66,28 +71,28 @@ void* emu_dyn_smem() { return g_dyn; }
 
 emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int) {
     const int t = threadIdx.x, w = t / 64 * 64, l = t % 64;
-    g_mfma_a[t] = a;
-    g_mfma_b[t] = b;
+    const int p = g_par;
+    g_par ^= 1;
+    g_mfma_a[p][t] = a;
+    g_mfma_b[p][t] = b;
     emu_wave_sync();
     emu_d4 out = c;
     for (int v = 0; v < 4; v++) {
         const int row = 4 * v + l / 16, col = l % 16;
         double s = 0.0;
-        for (int k = 0; k < 4; k++) s += g_mfma_a[w + row + 16 * k] * g_mfma_b[w + col + 16 * k];
+        for (int k = 0; k < 4; k++) s += g_mfma_a[p][w + row + 16 * k] * g_mfma_b[p][w + col + 16 * k];
         out[v] += s;
     }
-    emu_wave_sync();
     return out;
 }
 
-static double g_shfl[NT];
 double __shfl(double v, int src, int) {
     const int t = threadIdx.x, w = t / 64 * 64;
-    g_shfl[t] = v;
+    const int p = g_par;
+    g_par ^= 1;
+    g_mfma_a[p][t] = v;
     emu_wave_sync();
-    const double out = g_shfl[w + (src & 63)];
-    emu_wave_sync();
-    return out;
+    return g_mfma_a[p][w + (src & 63)];
 }
 
 // Persistent pool of 256 workers (one per lane of a block).  Per block the main thread
@@ -111,6 +116,7 @@ static void pool_init() {
             for (;;) {
                 g_start->arrive_and_wait();
                 if (g_quit) return;
+                g_par = 0;  // every block starts its wave collectives on slot 0
                 (*g_body)();
                 g_wave_bar[t / 64]->arrive_and_drop();
                 g_block_bar->arrive_and_drop();

@@ -145,3 +145,48 @@ def test_gpu_dropin_fv_solver(gpu_examples):
     entry points."""
     print("[gpu-case] drop-in FiniteVolumeSolver driver (fv_solver) on the product library", flush=True)
     check_fv_solver_output(run_raw(gpu_examples, "fv_solver", "--max-n=256"), max_n=256)
+
+
+def thermal_reference(level=1, nx=8, steps=3, dt=0.1):
+    """The thermal driver's loop on the oracle: one build, then per step an upwards stage with
+    f = -u_prev/dt - s (0 at the first step) and a solve with Dirichlet west/east and
+    Neumann (a = 0, b = 1) south/north data."""
+    src = lambda x, y, t: np.exp(-t) * np.cos(x) * np.sin(1.5 * y)
+    bc = {0: lambda x, y, t: np.sin(y + t), 1: lambda x, y, t: np.cos(y - t),
+          2: lambda x, y, t: 0.5 * x * np.exp(-t), 3: lambda x, y, t: -0.25 * np.cos(x + t)}
+    leaves = omesh.uniform(level)
+    one = lambda x, y: 1.0 + 0 * x
+    O = oracle.Forest(leaves, nx, (-1, 1, -1, 1), solver=oracle.FIVE_POINT,
+                      coeff_fns=(one, one, lambda x, y: -1.0 / dt + 0 * x))
+    O.build()
+    u = np.zeros((len(leaves), nx * nx))
+    for n in range(steps + 1):
+        t = n * dt
+        if n == 0:
+            f = np.zeros_like(u)
+        else:
+            f = np.stack([-(1.0 / dt) * u[k] - src(*oracle.cell_centres(nx, b)[:2], t)
+                          for k, b in enumerate(O.leaf_bounds)])
+        O.upwards(f)
+
+        def data(side, x, y):
+            r = np.select([side == s for s in range(4)], [bc[s](x, y, t) for s in range(4)])
+            a = np.where(side < 2, 1.0, 0.0)
+            return r, a, 1.0 - a
+        u = O.solve(*O.boundary_data(data))
+    return u
+
+
+def check_thermal_output(out):
+    ref = thermal_reference()
+    got = np.zeros_like(ref)
+    for line in out.splitlines():
+        p = line.split()
+        if p[0] == "u":
+            got[int(p[1]), int(p[2])] = float(p[3])
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_dropin_thermal_time_stepping(example):
+    """Time stepping that reads the previous vectorU in the upwards callback (thermal shape)."""
+    check_thermal_output(run_raw(example, "thermal", "--level=1"))

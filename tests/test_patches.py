@@ -100,7 +100,10 @@ def test_emu_five_point_constant_and_variable(emu_ctx):
 def test_emu_patch_argument_errors(emu_ctx):
     from ellipticforest_amd import _lib
     with pytest.raises(_lib.EfError) as e:
-        _lib.Patches(emu_ctx, [[0, 1, 0, 1]], 64)
+        _lib.Patches(emu_ctx, [[0, 1, 0, 1]], 2048)
+    assert e.value.code == _lib.EF_ERR_ARG
+    with pytest.raises(_lib.EfError) as e:
+        _lib.Patches(emu_ctx, [[0, 1, 0, 1]], 2)
     assert e.value.code == _lib.EF_ERR_ARG
     with pytest.raises(_lib.EfError) as e:
         _lib.Patches(emu_ctx, [[0, 0, 0, 1]], 8)
