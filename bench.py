@@ -207,6 +207,68 @@ def gpu_patch_run(bounds, N, k, steps, warmup, check=None):
                 tflops=nflops / (ms / 1e3) / 1e12)
 
 
+def time_launches(ef, ctx, pt, op, g_ptr, f_ptr, out_ptr, k, steps, warmup):
+    """Mean GPU time (ms) of `steps` queued launches between one pair of HIP events."""
+    for _ in range(warmup):
+        pt.apply_device(op, g_ptr, f_ptr, out_ptr, k)
+    pt.sync()
+    timer = ef.Timer(ctx)
+    timer.start(pt.stream)
+    for _ in range(steps):
+        pt.apply_device(op, g_ptr, f_ptr, out_ptr, k)
+    timer.stop(pt.stream)
+    return timer.elapsed_ms() / steps
+
+
+def gpu_particular_run(bounds, N, k, steps, warmup):
+    """Particular Neumann data of every leaf (the leaf step of the upwards stage,
+    FiniteVolumeSolver::particularNeumannData): f in, h out."""
+    import ellipticforest_amd as ef
+    ctx = ef.Context(0)
+    P = len(bounds)
+    pt = ef.Patches(ctx, bounds, N)
+    _, f_h = patch_inputs(P, N, k)
+    f, h = ef.DeviceArray(ctx, f_h.shape), ef.DeviceArray(ctx, (k, P, 4 * N))
+    f.upload(f_h)
+    ms = time_launches(ef, ctx, pt, ef._lib.EF_PATCH_MAP_D2N, None, f.ptr, h.ptr, k, steps, warmup)
+    nbytes = P * k * (N * N + 4 * N) * 8
+    return dict(ms=ms, value=P * k / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9)
+
+
+def cfg3_cells(bounds, N):
+    """alpha, beta at the four faces and lambda per cell (FiniteVolumeSolver.cpp:64-78) for the
+    cfg3 coefficients alpha = 1, beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1."""
+    beta = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    out = np.empty((len(bounds), 6, N * N))
+    for p, (xl, xu, yl, yu) in enumerate(bounds):
+        dx, dy = (xu - xl) / N, (yu - yl) / N
+        X, Y = np.meshgrid(xl + (np.arange(N) + 0.5) * dx, yl + (np.arange(N) + 0.5) * dy, indexing="ij")
+        X, Y = X.ravel(), Y.ravel()
+        out[p] = [np.ones_like(X), beta(X - dx / 2, Y), beta(X + dx / 2, Y), beta(X, Y - dy / 2),
+                  beta(X, Y + dy / 2), -np.ones_like(X)]
+    return out
+
+
+def gpu_variable_run(steps, warmup, level=6, N=32):
+    """Leaf solves of the cfg3 leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches,
+    five-point stencil with variable beta), one right-hand side per launch."""
+    import ellipticforest_amd as ef
+    ctx = ef.Context(0)
+    w = 2.0 / (1 << level)
+    bounds = np.array([(-1 + i * w, -1 + (i + 1) * w, -1 + j * w, -1 + (j + 1) * w)
+                       for i in range(1 << level) for j in range(1 << level)])
+    P = len(bounds)
+    pt = ef.Patches(ctx, bounds, N, solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=cfg3_cells(bounds, N))
+    g_h, f_h = patch_inputs(P, N, 1)
+    g, f, u = ef.DeviceArray(ctx, g_h.shape), ef.DeviceArray(ctx, f_h.shape), ef.DeviceArray(ctx, f_h.shape)
+    g.upload(g_h)
+    f.upload(f_h)
+    ms = time_launches(ef, ctx, pt, ef._lib.EF_PATCH_SOLVE, g.ptr, f.ptr, u.ptr, 1, steps, warmup)
+    # stored inverse pivot blocks (N^3) and stencil (5 N^2) per patch, f, u (N^2 each) and g (4N)
+    nbytes = P * (N ** 3 + 5 * N * N + 2 * N * N + 4 * N) * 8
+    return dict(ms=ms, value=P / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9, patches=P)
+
+
 def traffic_probe(bounds, N, k, copy_doubles=32 << 20):
     """Launches for the byte counters: the leaf-solve kernel, then a plain copy of known size."""
     import ellipticforest_amd as ef
@@ -316,6 +378,20 @@ def main():
                                copy_kernel_gbs=copy_gbs, frac_of_copy=pr["gbs"] / copy_gbs,
                                algorithmic_bytes_per_launch=pr["bytes"],
                                scope="leaf-solve stage only; the build stage is switched off in this library build")
+        pp = gpu_particular_run(bounds, args.N, k, args.steps, args.warmup)
+        log(f"particular Neumann data: {pp['ms']:.4f} ms per launch, {pp['gbs']:.0f} GB/s")
+        out["components"].append(dict(
+            name="upwards leaf stage (particularNeumannData on every leaf, ef_patches_apply_device MAP_D2N)",
+            kernel="patch_fd_kernel<1, true, 1>", value=pp["value"], unit="leaf maps/s", ms_per_launch=pp["ms"],
+            patches=len(bounds), nrhs=k, achieved_gbs=pp["gbs"], algorithmic_bytes_per_launch=pp["bytes"],
+            frac_of_peak=pp["gbs"] / 8000.0))
+        pv = gpu_variable_run(args.steps, args.warmup)
+        log(f"cfg3 variable-coefficient leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s")
+        out["components"].append(dict(
+            name="cfg3 leaf solves (five-point, variable beta: 4,096 patches, N = 32, 1 right-hand side)",
+            kernel="bt_solve_kernel", value=pv["value"], unit="leaf solves/s", ms_per_launch=pv["ms"],
+            patches=pv["patches"], nrhs=1, achieved_gbs=pv["gbs"], algorithmic_bytes_per_launch=pv["bytes"],
+            frac_of_peak=pv["gbs"] / 8000.0))
     except Exception as e:
         out["components_error"] = f"{type(e).__name__}: {e}"
         log(out["components_error"])

@@ -111,11 +111,11 @@ __device__ inline double patch_g(const PatchItemIn<NB>& in, int t, int col) {
     return (t < 64) ? a : b;
 }
 
-// The NB = 1 solve fits five waves per SIMD (at most 102 registers), which overlaps one
-// wave's loads with another's MFMA chain; the Neumann-data variants would spill there and
-// NB = 2 needs about 330 registers, so they keep the compiler's choice.
+// The N = 16 solve fits five waves per SIMD (at most 102 registers) and the N = 16 Neumann
+// data four, which overlaps one wave's loads with another's MFMA chain; the other variants
+// would spill there (NB = 2 needs about 330 registers), so they keep the compiler's choice.
 template <int NB, bool FULL, int OP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 && OP == PATCH_OP_SOLVE ? 5 : 1, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 && FULL && OP == PATCH_OP_SOLVE ? 5 : (NB == 1 && FULL && OP == PATCH_OP_MAP_D2N ? 4 : 1), 8)))
 patch_fd_kernel(PatchFDArgs p) {
     // the wave index is uniform: keep it (and every item index and base address derived
     // from it) in scalar registers
@@ -171,17 +171,20 @@ patch_fd_kernel(PatchFDArgs p) {
         // fold the boundary data into the load: -2 (c_W g_W + c_E g_E) on the x-edge
         // cells, -2 (c_S g_S + c_N g_N) on the y-edge cells
         double BT[NB][NB][4];
+        // N = 16: element s of lane (q, r) is cell (i, j) = (4s + q, r), both in the load (A
+        // layout of B^T) and in the solution (accumulator layout of U).  x-edges: (s, q) = (0, 0)
+        // takes g_W[r] (entry r), (3, 3) takes g_E[r] (entry 16 + r); y-edges: r = 0 takes
+        // g_S[4s + q] (entry 32 + 4s + q), r = 15 takes g_N[4s + q] (entry 48 + 4s + q).  The
+        // shuffled values serve the folding here and the Neumann data at the end.
+        double xg = 0.0, yg[4] = {0.0, 0.0, 0.0, 0.0};
         if (NB == 1 && FULL && OP != PATCH_OP_DTN) {
-            // N = 16: element s of lane (q, r) is cell (i, j) = (4s + q, r).  x-edges: (s, q) =
-            // (0, 0) takes g_W[r] (entry r), (3, 3) takes g_E[r] (entry 16 + r); y-edges: r = 0
-            // takes g_S[4s + q] (entry 32 + 4s + q), r = 15 takes g_N[4s + q] (entry 48 + 4s + q).
-            const double xg = has_g ? __shfl(cur.G[0], q == 0 ? r : 16 + r) : 0.0;
             const double ax = -2.0 * cf.cx, ay = (r == 0 || r == 15) ? -2.0 * cf.cy : 0.0;
             const int ybase = (r == 0) ? 32 + q : 48 + q;
+            if (has_g) xg = __shfl(cur.G[0], q == 0 ? r : 16 + r);
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const double yg = has_g ? __shfl(cur.G[0], ybase + 4 * s) : 0.0;
-                double v = fma(ay, yg, cur.F[0][0][s]);
+                if (has_g) yg[s] = __shfl(cur.G[0], ybase + 4 * s);
+                double v = fma(ay, yg[s], cur.F[0][0][s]);
                 if ((s == 0 && q == 0) || (s == 3 && q == 3)) v = fma(ax, xg, v);
                 BT[0][0][s] = v;
             }
@@ -273,6 +276,18 @@ patch_fd_kernel(PatchFDArgs p) {
                 ostride = 1;
             }
             const double sx = 2.0 / cf.dx, sy = 2.0 / cf.dy;
+            if (NB == 1 && FULL && OP == PATCH_OP_MAP_D2N) {
+                // edge cells of lane (q, r): (4v + q, r)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const double uu = Y[0][0][v];
+                    if (v == 0 && q == 0) h[(long long)(W * 16 + r) * ostride] = sx * (uu - xg);
+                    if (v == 3 && q == 3) h[(long long)(E * 16 + r) * ostride] = -sx * (uu - xg);
+                    if (r == 0) h[(long long)(S * 16 + 4 * v + q) * ostride] = sy * (uu - yg[v]);
+                    if (r == 15) h[(long long)(N * 16 + 4 * v + q) * ostride] = -sy * (uu - yg[v]);
+                }
+                goto next_item;
+            }
 #pragma unroll
             for (int bi = 0; bi < NB; bi++)
 #pragma unroll
@@ -293,6 +308,7 @@ patch_fd_kernel(PatchFDArgs p) {
                         if (j == Nn - 1) h[(long long)(N * Nn + i) * ostride] = -sy * (uu - gnv);
                     }
         }
+    next_item:
         o = o2;
         in_i = i2;
         cur = nxt;
