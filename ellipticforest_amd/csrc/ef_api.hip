@@ -730,8 +730,7 @@ void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u,
     const int N = M->P.N;
     const int nl = (int)M->P.leaves.size();
     if (M->variable) {
-        hipLaunchKernelGGL(bt_solve_kernel, dim3(nl, k), dim3(256), 0, M->stream, M->leafdesc.p, N, nl, k, mode, f,
-                           f_zero, (const double*)M->vec.p, M->vec.p, u);
+        launch_bt_solve(M->leafdesc.p, N, nl, k, mode, f, f_zero, (const double*)M->vec.p, M->vec.p, u, M->stream);
         EF_LAUNCH_CHECK();
     } else {
         // mode 0: particular Neumann data (g = 0) into the h rows of every leaf;

@@ -768,6 +768,148 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
     }
 }
 
+// The same substitution with one wave per (leaf, rhs), for N = NT in {8, 16, 32}.
+// Lane (j, h): j = lane % NT is the cell row index within a block row (an x-line), h =
+// lane / NT one of G = 64 / NT groups that each own C = NT / G columns of the inverse pivot
+// blocks.  The block rows of the solution sit in the wave's own slice of LDS (lane j holds
+// element j); a block product Sinv_i t is C FMAs per lane on columns fetched from lane c by
+// shuffles, then a butterfly over the G groups.  Sinv_{i+1} is loaded while step i computes
+// (two register buffers, the block-row loop unrolled by two).
+template <int NT>
+__device__ inline double bt_wave_matvec(const double (&Srow)[NT * NT / 64], double t, int h) {
+    constexpr int G = 64 / NT, C = NT / G;
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < C; m++) acc = fma(Srow[m], __shfl(t, h * C + m), acc);
+#pragma unroll
+    for (int o = NT; o < 64; o <<= 1) acc += __shfl_xor(acc, o);
+    return acc;
+}
+
+template <int NT>
+__device__ inline void bt_wave_load(const double* Si, int j, int h, double (&Srow)[NT * NT / 64]) {
+    constexpr int G = 64 / NT, C = NT / G;
+#pragma unroll
+    for (int m = 0; m < C; m++) Srow[m] = Si[j * NT + h * C + m];
+}
+
+template <int NT>
+__global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leaves, int nleaves, int k, int mode,
+                                                            const double* f, int f_zero, const double* garena,
+                                                            double* harena, double* u) {
+    constexpr int NN = NT * NT, C = NT * NT / 64;
+    __shared__ double zs_all[4][NN];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * 4 + wave;
+    if (item >= (long long)nleaves * k) return;  // uniform per wave; no barrier below
+    double* zs = zs_all[wave];
+    const int rhs = (int)(item / nleaves), leaf = (int)(item % nleaves);
+    const int j = lane % NT, h = lane / NT;
+    const LeafDesc L = leaves[leaf];
+    const double* fl = f_zero ? nullptr : f + ((long long)rhs * nleaves + leaf) * NN;
+    const double* g = (mode != 0) ? garena + L.g_off * k + rhs : nullptr;
+    const double* cWv = L.stencil;
+    const double* cEv = L.stencil + NN;
+    const double* cSv = L.stencil + 2 * NN;
+    const double* cNv = L.stencil + 3 * NN;
+    auto rhs_row = [&](int i) {
+        const int I = i * NT + j;
+        double b = fl ? fl[I] : 0.0;
+        if (g) {
+            double add = 0.0;
+            if (i == 0) add += cWv[I] * g[(long long)(0 * NT + j) * k];
+            if (i == NT - 1) add += cEv[I] * g[(long long)(1 * NT + j) * k];
+            if (j == 0) add += cSv[I] * g[(long long)(2 * NT + i) * k];
+            if (j == NT - 1) add += cNv[I] * g[(long long)(3 * NT + i) * k];
+            b += -2.0 * add;
+        }
+        return b;
+    };
+    double Sa[C], Sb[C];
+    // forward: z_i = Sinv_i (b_i - cW_i .* z_{i-1}); even i in Sa, odd i in Sb
+    bt_wave_load<NT>(L.Sinv, j, h, Sa);
+    double zprev = 0.0;
+    for (int i = 0; i < NT; i += 2) {
+        bt_wave_load<NT>(L.Sinv + (long long)(i + 1) * NN, j, h, Sb);
+        double t = rhs_row(i);
+        if (i > 0) t -= cWv[i * NT + j] * zprev;
+        zprev = bt_wave_matvec<NT>(Sa, t, h);
+        if (h == 0) zs[i * NT + j] = zprev;
+        if (i + 2 < NT) bt_wave_load<NT>(L.Sinv + (long long)(i + 2) * NN, j, h, Sa);
+        t = rhs_row(i + 1) - cWv[(i + 1) * NT + j] * zprev;
+        zprev = bt_wave_matvec<NT>(Sb, t, h);
+        if (h == 0) zs[(i + 1) * NT + j] = zprev;
+    }
+    // backward: x_i = z_i - Sinv_i (cE_i .* x_{i+1}) for i = N-2 .. 0, x_{N-1} = z_{N-1} = zprev;
+    // pairs (i, i-1) for i = N-2, N-4, .., 2 (Sinv_i in Sa, Sinv_{i-1} in Sb), then i = 0 (Sa)
+    double xnext = zprev;
+    bt_wave_load<NT>(L.Sinv + (long long)(NT - 2) * NN, j, h, Sa);
+    for (int i = NT - 2; i >= 2; i -= 2) {
+        // z_i is read before the product, whose shuffles order every lane's read before the write
+        bt_wave_load<NT>(L.Sinv + (long long)(i - 1) * NN, j, h, Sb);
+        double zi = zs[i * NT + j];
+        double x = zi - bt_wave_matvec<NT>(Sa, cEv[i * NT + j] * xnext, h);
+        if (h == 0) zs[i * NT + j] = x;
+        xnext = x;
+        bt_wave_load<NT>(L.Sinv + (long long)(i - 2) * NN, j, h, Sa);
+        zi = zs[(i - 1) * NT + j];
+        x = zi - bt_wave_matvec<NT>(Sb, cEv[(i - 1) * NT + j] * xnext, h);
+        if (h == 0) zs[(i - 1) * NT + j] = x;
+        xnext = x;
+    }
+    {
+        const double zi = zs[j];
+        const double x = zi - bt_wave_matvec<NT>(Sa, cEv[j] * xnext, h);
+        if (h == 0) zs[j] = x;
+    }
+    // the wave's LDS writes are in order before its reads below; on the CPU emulation this is
+    // where the lanes meet
+    __builtin_amdgcn_wave_barrier();
+    if (mode == 1) {
+        double* ul = u + ((long long)rhs * nleaves + leaf) * NN;
+        for (int e = lane; e < NN; e += 64) ul[e] = zs[e];
+        return;
+    }
+    // Neumann data of the edge cells: h_W = (2/dx)(u_W - g_W), h_E = -(2/dx)(u_E - g_E),
+    // h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N)
+    double* hv = harena + L.h_off * k + rhs;
+    const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
+    auto gv = [&](int t) { return g ? g[(long long)t * k] : 0.0; };
+    for (int t = lane; t < 4 * NT; t += 64) {
+        const int sd = t / NT, m = t % NT;
+        double v;
+        if (sd == W) v = sx * (zs[m] - gv(t));
+        else if (sd == E) v = -sx * (zs[(NT - 1) * NT + m] - gv(t));
+        else if (sd == S) v = sy * (zs[m * NT] - gv(t));
+        else v = -sy * (zs[m * NT + NT - 1] - gv(t));
+        hv[(long long)t * k] = v;
+    }
+}
+
+template __global__ void bt_solve_wave_kernel<8>(const LeafDesc*, int, int, int, const double*, int, const double*,
+                                                 double*, double*);
+template __global__ void bt_solve_wave_kernel<16>(const LeafDesc*, int, int, int, const double*, int, const double*,
+                                                  double*, double*);
+template __global__ void bt_solve_wave_kernel<32>(const LeafDesc*, int, int, int, const double*, int, const double*,
+                                                  double*, double*);
+
+void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
+                     const double* garena, double* harena, double* u, hipStream_t s) {
+    const unsigned blocks = (unsigned)(((long long)nleaves * k + 3) / 4);
+    if (Nn == 8)
+        hipLaunchKernelGGL(bt_solve_wave_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
+                           garena, harena, u);
+    else if (Nn == 16)
+        hipLaunchKernelGGL(bt_solve_wave_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
+                           f_zero, garena, harena, u);
+    else if (Nn == 32)
+        hipLaunchKernelGGL(bt_solve_wave_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
+                           f_zero, garena, harena, u);
+    else
+        hipLaunchKernelGGL(bt_solve_kernel, dim3(nleaves, k), dim3(256), 0, s, leaves, Nn, nleaves, k, mode, f, f_zero,
+                           garena, harena, u);
+}
+
 // Leaf DtN (buildD2N, FiniteVolumeSolver.cpp:355-455) with variable coefficients: column b is
 // driven by -2 c_b at the boundary cell of b; Z (N blocks of N x 4N per leaf) is global work.
 __global__ void __launch_bounds__(256) bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work) {

@@ -25,6 +25,13 @@ __global__ void bt_factor_kernel(const LeafDesc* leaves, int Nn);
 __global__ void bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f,
                                 int f_zero, const double* garena, double* harena, double* u);
 __global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work);
+template <int NT>
+__global__ void bt_solve_wave_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f,
+                                     int f_zero, const double* garena, double* harena, double* u);
+// Leaf solves with variable coefficients: one wave per (leaf, rhs) for N in {8, 16, 32},
+// the workgroup-per-leaf kernel otherwise.  Modes as bt_solve_kernel.
+void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
+                     const double* garena, double* harena, double* u, hipStream_t s);
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)
 template <int NB, bool FULL, int OP>
 __global__ void patch_fd_kernel(PatchFDArgs p);

@@ -248,9 +248,8 @@ void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double
     const long long NN = (long long)p->N * p->N, B4 = 4LL * p->N;
     if (op == PATCH_OP_MAP_D2N && g == nullptr) {
         for (int c = 0; c < nrhs; c++) {
-            hipLaunchKernelGGL(bt_solve_kernel, dim3(p->P, 1), dim3(256), 0, p->stream, p->leafdesc.p, p->N, p->P, 1,
-                               0, f ? f + c * p->P * NN : nullptr, f ? 0 : 1, (const double*)nullptr,
-                               out + c * p->P * B4, (double*)nullptr);
+            launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, 0, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
+                            (const double*)nullptr, out + c * p->P * B4, (double*)nullptr, p->stream);
             EF_LAUNCH_CHECK();
         }
         return;
@@ -258,10 +257,9 @@ void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double
     if (g == nullptr) throw EfError(EF_ERR_ARG, "boundary data g is required");
     const int mode = (op == PATCH_OP_SOLVE) ? 1 : 2;
     for (int c = 0; c < nrhs; c++) {
-        hipLaunchKernelGGL(bt_solve_kernel, dim3(p->P, 1), dim3(256), 0, p->stream, p->leafdesc.p, p->N, p->P, 1, mode,
-                           f ? f + c * p->P * NN : nullptr, f ? 0 : 1, g + c * p->P * B4,
-                           mode == 2 ? out + c * p->P * B4 : (double*)nullptr,
-                           mode == 1 ? out + c * p->P * NN : (double*)nullptr);
+        launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, mode, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
+                        g + c * p->P * B4, mode == 2 ? out + c * p->P * B4 : (double*)nullptr,
+                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream);
         EF_LAUNCH_CHECK();
     }
 }

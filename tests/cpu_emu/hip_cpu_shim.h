@@ -85,8 +85,11 @@ typedef double emu_d4 __attribute__((ext_vector_type(4)));
 emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);
 // cross-lane read within a wave (ds_bpermute semantics: lane reads src % 64 of its wave)
 double __shfl(double v, int src, int width = 64);
+inline double __shfl_xor(double v, int mask, int width = 64) { return __shfl(v, (int)(threadIdx.x % 64) ^ mask, width); }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 emu_mfma_f64
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+// a scheduling barrier on the GPU (lanes of a wave run in lockstep); here the lanes meet
+#define __builtin_amdgcn_wave_barrier() emu_wave_sync()
 // only applied to values that are uniform across the wave by construction
 #define __builtin_amdgcn_readfirstlane(x) (x)
 
