@@ -150,14 +150,16 @@ int ef_mesh_stage_ms(ef_mesh mesh, double* ms);
  * (FiniteVolumeSolver.hpp:122-149, FiniteVolumeSolver.cpp:25-462) applied to many
  * independent N x N patches at once.  Patches with constant coefficients (FISHPACK
  * semantics, or five-point with alpha, beta, lambda constant within the patch) use fast
- * diagonalisation; otherwise the five-point matrix is factorised once at creation.
+ * diagonalisation (in registers for N <= 32, as batched GEMMs beyond); otherwise the
+ * five-point matrix is factorised once at creation (block-tridiagonal, N <= 32).
  * Layouts: g and h are [rhs][patch][4N] (WESN), f and u are [rhs][patch][N*N].
  * ------------------------------------------------------------------------------------ */
 typedef struct ef_patches_s* ef_patches;
 
 typedef struct ef_patch_batch {
     int npatches;
-    int N;                  /* cells per side, 4 <= N <= 32                  */
+    int N;                  /* cells per side, 4 <= N <= 1024 (N <= 32 when the
+                               coefficients vary within a patch)             */
     const double* bounds;   /* [npatches][4]: xl, xu, yl, yu of each patch   */
 } ef_patch_batch;
 
