@@ -826,40 +826,53 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
         return b;
     };
     double Sa[C], Sb[C];
-    // forward: z_i = Sinv_i (b_i - cW_i .* z_{i-1}); even i in Sa, odd i in Sb
+    // forward: z_i = Sinv_i (b_i - cW_i .* z_{i-1}); even i in Sa, odd i in Sb.  The next block
+    // row's inverse, right-hand side and west coefficients are loaded while this one computes.
     bt_wave_load<NT>(L.Sinv, j, h, Sa);
     double zprev = 0.0;
+    double bn = rhs_row(0), cwn = 0.0;
     for (int i = 0; i < NT; i += 2) {
+        double b = bn, cw = cwn;
         bt_wave_load<NT>(L.Sinv + (long long)(i + 1) * NN, j, h, Sb);
-        double t = rhs_row(i);
-        if (i > 0) t -= cWv[i * NT + j] * zprev;
-        zprev = bt_wave_matvec<NT>(Sa, t, h);
+        bn = rhs_row(i + 1);
+        cwn = cWv[(i + 1) * NT + j];
+        zprev = bt_wave_matvec<NT>(Sa, (i > 0) ? b - cw * zprev : b, h);
         if (h == 0) zs[i * NT + j] = zprev;
-        if (i + 2 < NT) bt_wave_load<NT>(L.Sinv + (long long)(i + 2) * NN, j, h, Sa);
-        t = rhs_row(i + 1) - cWv[(i + 1) * NT + j] * zprev;
-        zprev = bt_wave_matvec<NT>(Sb, t, h);
+        b = bn;
+        cw = cwn;
+        if (i + 2 < NT) {
+            bt_wave_load<NT>(L.Sinv + (long long)(i + 2) * NN, j, h, Sa);
+            bn = rhs_row(i + 2);
+            cwn = cWv[(i + 2) * NT + j];
+        }
+        zprev = bt_wave_matvec<NT>(Sb, b - cw * zprev, h);
         if (h == 0) zs[(i + 1) * NT + j] = zprev;
     }
     // backward: x_i = z_i - Sinv_i (cE_i .* x_{i+1}) for i = N-2 .. 0, x_{N-1} = z_{N-1} = zprev;
-    // pairs (i, i-1) for i = N-2, N-4, .., 2 (Sinv_i in Sa, Sinv_{i-1} in Sb), then i = 0 (Sa)
+    // pairs (i, i-1) for i = N-2, N-4, .., 2 (Sinv_i in Sa, Sinv_{i-1} in Sb), then i = 0 (Sa).
+    // z_i is read before the product, whose shuffles order every lane's read before the write.
     double xnext = zprev;
     bt_wave_load<NT>(L.Sinv + (long long)(NT - 2) * NN, j, h, Sa);
+    double cen = cEv[(NT - 2) * NT + j];
     for (int i = NT - 2; i >= 2; i -= 2) {
-        // z_i is read before the product, whose shuffles order every lane's read before the write
+        double ce = cen;
         bt_wave_load<NT>(L.Sinv + (long long)(i - 1) * NN, j, h, Sb);
+        cen = cEv[(i - 1) * NT + j];
         double zi = zs[i * NT + j];
-        double x = zi - bt_wave_matvec<NT>(Sa, cEv[i * NT + j] * xnext, h);
+        double x = zi - bt_wave_matvec<NT>(Sa, ce * xnext, h);
         if (h == 0) zs[i * NT + j] = x;
         xnext = x;
+        ce = cen;
         bt_wave_load<NT>(L.Sinv + (long long)(i - 2) * NN, j, h, Sa);
+        cen = cEv[(i - 2) * NT + j];
         zi = zs[(i - 1) * NT + j];
-        x = zi - bt_wave_matvec<NT>(Sb, cEv[(i - 1) * NT + j] * xnext, h);
+        x = zi - bt_wave_matvec<NT>(Sb, ce * xnext, h);
         if (h == 0) zs[(i - 1) * NT + j] = x;
         xnext = x;
     }
     {
         const double zi = zs[j];
-        const double x = zi - bt_wave_matvec<NT>(Sa, cEv[j] * xnext, h);
+        const double x = zi - bt_wave_matvec<NT>(Sa, cen * xnext, h);
         if (h == 0) zs[j] = x;
     }
     // the wave's LDS writes are in order before its reads below; on the CPU emulation this is
