@@ -1,6 +1,9 @@
 // hip_cpu_shim.cpp -- runtime of the CPU emulation (see hip_cpu_shim.h).
 #include "hip_cpu_shim.h"
 
+#include <dlfcn.h>
+
+#include <cstdint>
 #include <map>
 #include <mutex>
 
@@ -27,7 +30,21 @@ static void* guarded_alloc(size_t bytes, void** base_out, size_t* len_out) {
     const bool front = mode && std::strcmp(mode, "front") == 0;
     *base_out = base;
     *len_out = len;
-    std::memset(rw, 0xff, body);  // NaN pattern: reads of unwritten memory show up in results
+    // Fill of new memory: all-ones by default (a NaN pattern, so reads of unwritten memory show
+    // up in results; reads as -1 where an int is read); EF_EMU_FILL=zero or =rand instead, since a
+    // GPU's fresh memory holds arbitrary values and an unwritten index read there is wild.
+    const char* fill = getenv("EF_EMU_FILL");
+    if (fill && std::strcmp(fill, "zero") == 0) {
+        std::memset(rw, 0, body);
+    } else if (fill && std::strcmp(fill, "rand") == 0) {
+        static uint64_t x = 0x9e3779b97f4a7c15ULL;
+        for (size_t i = 0; i < body; i += 8) {
+            x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+            std::memcpy(rw + i, &x, 8);
+        }
+    } else {
+        std::memset(rw, 0xff, body);
+    }
     if (front) return rw;
     return rw + body - ((bytes + 7) / 8) * 8;  // end of the buffer touches the back guard
 }
@@ -42,6 +59,37 @@ hipError_t hipMalloc(void** p, size_t bytes) {
     *p = q;
     return 0;
 }
+bool emu_ptrcheck_on() {
+    static const bool on = getenv("EF_EMU_PTRCHECK") && std::strcmp(getenv("EF_EMU_PTRCHECK"), "0") != 0;
+    return on;
+}
+
+static bool in_device_allocation(uintptr_t w) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto& m : g_maps) {
+        const uintptr_t b = (uintptr_t)m.second.first;
+        if (w >= b && w < b + m.second.second) return true;
+    }
+    return false;
+}
+
+void emu_ptrcheck(const void* p, size_t n, const char* what, const void* kernel, int index) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t off = 0; off + 8 <= n; off += 8) {
+        uintptr_t w;
+        std::memcpy(&w, c + off, 8);
+        if (w < 0x10000 || w > 0x7fffffffffffULL || (w & 7)) continue;
+        if (in_device_allocation(w)) continue;
+        void* page = (void*)(w & ~(uintptr_t)(PAGE - 1));
+        if (msync(page, PAGE, MS_ASYNC) != 0) continue;  // not mapped: an ordinary integer
+        Dl_info info{};
+        const char* name = (kernel && dladdr(kernel, &info) && info.dli_sname) ? info.dli_sname : "-";
+        fprintf(stderr, "EMU-PTRCHECK host address 0x%llx in %s %d at byte %zu of %zu (kernel %s)\n",
+                (unsigned long long)w, what, index, off, n, name);
+        if (std::strcmp(getenv("EF_EMU_PTRCHECK"), "abort") == 0) abort();
+    }
+}
+
 hipError_t hipFree(void* p) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_maps.find(p);
@@ -133,17 +181,31 @@ void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body)
     blockDim = b;
     if (shm) g_dyn = guarded_alloc(shm, &g_dyn_base, &g_dyn_len);
     g_body = &body;
+    // Block order: in order by default; EF_EMU_ORDER=reverse or =shuffle runs the blocks of a
+    // grid in another order, so results that depend on the order (a race between blocks of one
+    // launch, which a GPU runs concurrently) change.
+    std::vector<emu_idx> order;
     for (unsigned z = 0; z < g.z; z++)
         for (unsigned y = 0; y < g.y; y++)
-            for (unsigned x = 0; x < g.x; x++) {
-                blockIdx = {x, y, z};
-                std::barrier<> bb(NT);
-                g_block_bar = &bb;
-                for (int i = 0; i < NT / 64; i++) g_wave_bar[i] = new std::barrier<>(64);
-                if (shm) std::memset(g_dyn, 0xff, shm);
-                g_start->arrive_and_wait();
-                g_end->arrive_and_wait();
-                for (int i = 0; i < NT / 64; i++) delete g_wave_bar[i];
-            }
+            for (unsigned x = 0; x < g.x; x++) order.push_back({x, y, z});
+    const char* ord = getenv("EF_EMU_ORDER");
+    if (ord && std::strcmp(ord, "reverse") == 0) std::reverse(order.begin(), order.end());
+    if (ord && std::strcmp(ord, "shuffle") == 0) {
+        static uint64_t x = 0x2545f4914f6cdd1dULL;
+        for (size_t i = order.size(); i > 1; i--) {
+            x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+            std::swap(order[i - 1], order[x % i]);
+        }
+    }
+    for (const emu_idx& bi : order) {
+        blockIdx = bi;
+        std::barrier<> bb(NT);
+        g_block_bar = &bb;
+        for (int i = 0; i < NT / 64; i++) g_wave_bar[i] = new std::barrier<>(64);
+        if (shm) std::memset(g_dyn, 0xff, shm);
+        g_start->arrive_and_wait();
+        g_end->arrive_and_wait();
+        for (int i = 0; i < NT / 64; i++) delete g_wave_bar[i];
+    }
     if (shm) { munmap(g_dyn_base, g_dyn_len); g_dyn = nullptr; }
 }

@@ -53,7 +53,16 @@ inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
 hipError_t hipMalloc(void** p, size_t bytes);
 template <class T> hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc((void**)p, bytes); }
 hipError_t hipFree(void* p);
-inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memcpy(d, s, n); return 0; }
+// EF_EMU_PTRCHECK=1: report every aligned 8-byte word of a kernel argument or of a host-to-device
+// copy that points into mapped memory outside every device allocation.  Such a word is a host
+// address handed to the device, which works here (one address space) and faults a GPU.
+bool emu_ptrcheck_on();
+void emu_ptrcheck(const void* p, size_t n, const char* what, const void* kernel, int index);
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k) {
+    if (n) std::memcpy(d, s, n);
+    if (n && k == hipMemcpyHostToDevice && emu_ptrcheck_on()) emu_ptrcheck(d, n, "host-to-device copy", nullptr, -1);
+    return 0;
+}
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
 inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return 0; }
@@ -96,6 +105,10 @@ inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body);
 template <class F, class... A>
 void emu_launch(F* f, dim3 g, dim3 b, size_t shm, A... args) {
+    if (emu_ptrcheck_on()) {
+        int i = 0;
+        (emu_ptrcheck(&args, sizeof(args), "kernel argument", (const void*)f, i++), ...);
+    }
     emu_run_grid(g, b, shm, [=]() { f(args...); });
 }
 #define hipLaunchKernelGGL(k, g, b, shm, s, ...) emu_launch(k, dim3(g), dim3(b), shm, ##__VA_ARGS__)
