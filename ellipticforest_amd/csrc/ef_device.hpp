@@ -20,6 +20,20 @@
 
 namespace efk {
 
+// Pointers read from descriptors are generic to the compiler; loads through them become FLAT
+// instructions, which also count against the LDS counter, so every wait on a shuffle or LDS
+// read would wait for outstanding HBM loads as well.  Kernels cast such pointers to the
+// global address space (device compilation only; the host pass never dereferences them).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define EF_GLOBAL __attribute__((address_space(1)))
+#else
+#define EF_GLOBAL
+#endif
+template <class T>
+__host__ __device__ inline const EF_GLOBAL T* gmem(const T* p) { return (const EF_GLOBAL T*)p; }
+template <class T>
+__host__ __device__ inline EF_GLOBAL T* gmem(T* p) { return (EF_GLOBAL T*)p; }
+
 enum Side { W = 0, E = 1, S = 2, N = 3 };
 
 // Batched GEMM problem: D = alpha * (A B + A2 B2) + beta * Cin, row-major.

@@ -789,8 +789,9 @@ __device__ inline double bt_wave_matvec(const double (&Srow)[NT * NT / 64], doub
 template <int NT>
 __device__ inline void bt_wave_load(const double* Si, int j, int h, double (&Srow)[NT * NT / 64]) {
     constexpr int G = 64 / NT, C = NT / G;
+    const auto* row = gmem(Si) + j * NT + h * C;
 #pragma unroll
-    for (int m = 0; m < C; m++) Srow[m] = Si[j * NT + h * C + m];
+    for (int m = 0; m < C; m++) Srow[m] = row[m];
 }
 
 template <int NT>
@@ -806,12 +807,12 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
     const int rhs = (int)(item / nleaves), leaf = (int)(item % nleaves);
     const int j = lane % NT, h = lane / NT;
     const LeafDesc L = leaves[leaf];
-    const double* fl = f_zero ? nullptr : f + ((long long)rhs * nleaves + leaf) * NN;
-    const double* g = (mode != 0) ? garena + L.g_off * k + rhs : nullptr;
-    const double* cWv = L.stencil;
-    const double* cEv = L.stencil + NN;
-    const double* cSv = L.stencil + 2 * NN;
-    const double* cNv = L.stencil + 3 * NN;
+    const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
+    const auto* g = (mode != 0) ? gmem(garena) + L.g_off * k + rhs : nullptr;
+    const auto* cWv = gmem(L.stencil);
+    const auto* cEv = cWv + NN;
+    const auto* cSv = cWv + 2 * NN;
+    const auto* cNv = cWv + 3 * NN;
     auto rhs_row = [&](int i) {
         const int I = i * NT + j;
         double b = fl ? fl[I] : 0.0;
@@ -879,13 +880,13 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
     // where the lanes meet
     __builtin_amdgcn_wave_barrier();
     if (mode == 1) {
-        double* ul = u + ((long long)rhs * nleaves + leaf) * NN;
+        auto* ul = gmem(u) + ((long long)rhs * nleaves + leaf) * NN;
         for (int e = lane; e < NN; e += 64) ul[e] = zs[e];
         return;
     }
     // Neumann data of the edge cells: h_W = (2/dx)(u_W - g_W), h_E = -(2/dx)(u_E - g_E),
     // h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N)
-    double* hv = harena + L.h_off * k + rhs;
+    auto* hv = gmem(harena) + L.h_off * k + rhs;
     const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
     auto gv = [&](int t) { return g ? g[(long long)t * k] : 0.0; };
     for (int t = lane; t < 4 * NT; t += 64) {
