@@ -264,25 +264,44 @@ def gpu_variable_run(steps, warmup, level=6, N=32):
     g.upload(g_h)
     f.upload(f_h)
     ms = time_launches(ef, ctx, pt, ef._lib.EF_PATCH_SOLVE, g.ptr, f.ptr, u.ptr, 1, steps, warmup)
-    # stored inverse pivot blocks (N^3) and stencil (5 N^2) per patch, f, u (N^2 each) and g (4N)
-    nbytes = P * (N ** 3 + 5 * N * N + 2 * N * N + 4 * N) * 8
-    return dict(ms=ms, value=P / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9, patches=P)
+    # per patch: the inverse pivot blocks, read by the forward and again by the backward sweep
+    # (N blocks of N (N + 1) / 2 packed entries for a symmetric matrix, N^2 otherwise), the
+    # stencil's cW and cE (N^2 each) and cS, cN on the two edge columns, f and u (N^2 each), g (4N)
+    block = N * (N + 1) // 2 if pt.packed else N * N
+    nbytes = P * (2 * N * block + 2 * N * N + 2 * N + 2 * N * N + 4 * N) * 8
+    return dict(ms=ms, value=P / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9, patches=P,
+                packed=pt.packed)
 
 
-def traffic_probe(bounds, N, k, copy_doubles=32 << 20):
-    """Launches for the byte counters: the leaf-solve kernel, then a plain copy of known size."""
+def traffic_probe(bounds, N, k, copy_doubles=32 << 20, level=6, NV=32):
+    """Launches for the byte counters: 5 leaf solves and 5 particular-Neumann-data maps of the
+    leaf set, 5 leaf solves of the cfg3 leaf set, then 3 plain copies of known size."""
     import ellipticforest_amd as ef
     from ellipticforest_amd import _lib
     ctx = ef.Context(0)
     P = len(bounds)
     g_h, f_h = patch_inputs(P, N, k)
     g, f, u = ef.DeviceArray(ctx, g_h.shape), ef.DeviceArray(ctx, f_h.shape), ef.DeviceArray(ctx, f_h.shape)
+    h = ef.DeviceArray(ctx, g_h.shape)
     g.upload(g_h)
     f.upload(f_h)
     pt = ef.Patches(ctx, bounds, N)
     for _ in range(5):
         pt.apply_device(_lib.EF_PATCH_SOLVE, g.ptr, f.ptr, u.ptr, k)
+    for _ in range(5):
+        pt.apply_device(_lib.EF_PATCH_MAP_D2N, None, f.ptr, h.ptr, k)
     pt.sync()
+    w = 2.0 / (1 << level)
+    vb = np.array([(-1 + i * w, -1 + (i + 1) * w, -1 + j * w, -1 + (j + 1) * w)
+                   for i in range(1 << level) for j in range(1 << level)])
+    pv = ef.Patches(ctx, vb, NV, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cfg3_cells(vb, NV))
+    vg_h, vf_h = patch_inputs(len(vb), NV, 1)
+    vg, vf, vu = ef.DeviceArray(ctx, vg_h.shape), ef.DeviceArray(ctx, vf_h.shape), ef.DeviceArray(ctx, vf_h.shape)
+    vg.upload(vg_h)
+    vf.upload(vf_h)
+    for _ in range(5):
+        pv.apply_device(_lib.EF_PATCH_SOLVE, vg.ptr, vf.ptr, vu.ptr, 1)
+    pv.sync()
     a, b = ef.DeviceArray(ctx, (copy_doubles,)), ef.DeviceArray(ctx, (copy_doubles,))
     for _ in range(3):
         _lib.check(ctx.L.ef_memcpy(ctx.h, b.ptr, a.ptr, copy_doubles * 8, 2), ctx.L)
@@ -320,8 +339,9 @@ def main():
     ap.add_argument("--patch-nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="only 5 leaf-solve launches and 3 device-to-device copies of 256 MiB on the library's "
-                         "copy kernel (the calibration of HBM byte counters), then exit")
+                    help="only 5 leaf-solve, 5 particular-Neumann-data and 5 cfg3 leaf-solve launches and 3 "
+                         "device-to-device copies of 256 MiB on the library's copy kernel (the calibration of "
+                         "HBM byte counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the HPS path of this build runs on a single MI355X; "
@@ -389,9 +409,10 @@ def main():
         log(f"cfg3 variable-coefficient leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s")
         out["components"].append(dict(
             name="cfg3 leaf solves (five-point, variable beta: 4,096 patches, N = 32, 1 right-hand side)",
-            kernel="bt_solve_kernel", value=pv["value"], unit="leaf solves/s", ms_per_launch=pv["ms"],
-            patches=pv["patches"], nrhs=1, achieved_gbs=pv["gbs"], algorithmic_bytes_per_launch=pv["bytes"],
-            frac_of_peak=pv["gbs"] / 8000.0))
+            kernel="bt_solve_sym_kernel<32>" if pv["packed"] else "bt_solve_wave_kernel<32>", value=pv["value"],
+            unit="leaf solves/s", ms_per_launch=pv["ms"], patches=pv["patches"], nrhs=1,
+            factor_storage="packed symmetric" if pv["packed"] else "full", achieved_gbs=pv["gbs"],
+            algorithmic_bytes_per_launch=pv["bytes"], frac_of_peak=pv["gbs"] / 8000.0))
     except Exception as e:
         out["components_error"] = f"{type(e).__name__}: {e}"
         log(out["components_error"])

@@ -346,6 +346,7 @@ class Patches:
         info = np.zeros(3, np.int32)
         check(self.L.ef_patches_info(self.h, iptr(info)), self.L)
         self.variable = bool(info[2])
+        self.packed = info[2] == 2  # symmetric: packed inverse pivot blocks
 
     def __del__(self):
         if getattr(self, "h", None) is not None:

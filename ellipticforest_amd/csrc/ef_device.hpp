@@ -101,6 +101,7 @@ struct LeafDesc {
     double* Sinv;        // variable coefficients: N blocks of N x N, inverses of the block-LU pivots
     double* stencil;     // variable coefficients: 5 N*N arrays cW, cE, cS, cN, cC
     double* T;           // leaf DtN (4N x 4N)
+    double* SinvP;       // symmetric matrices: the inverse pivot blocks packed (upper triangles), or null
     double dx, dy;
 };
 
@@ -205,7 +206,7 @@ static_assert(offsetof(GJDesc, piv) == 80, "GJDesc.piv");
 static_assert(sizeof(MergeDesc) == 80, "MergeDesc");
 static_assert(offsetof(MergeDesc, Tc) == 8, "MergeDesc.Tc");
 static_assert(sizeof(CoarsenDesc) == 24, "CoarsenDesc");
-static_assert(sizeof(LeafDesc) == 72, "LeafDesc");
+static_assert(sizeof(LeafDesc) == 80, "LeafDesc");
 static_assert(offsetof(LeafDesc, h_off) == 8, "LeafDesc.h_off");
 static_assert(offsetof(LeafDesc, coef) == 24, "LeafDesc.coef");
 static_assert(sizeof(UpDesc) == 112, "UpDesc");

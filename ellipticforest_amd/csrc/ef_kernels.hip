@@ -907,10 +907,228 @@ template __global__ void bt_solve_wave_kernel<16>(const LeafDesc*, int, int, int
 template __global__ void bt_solve_wave_kernel<32>(const LeafDesc*, int, int, int, const double*, int, const double*,
                                                   double*, double*);
 
+// ---------------------------------------------------------------------------
+// Symmetric matrices (alpha beta the same seen from both cells of every interior face, which
+// holds for constant alpha): the pivot blocks S_i and their inverses are symmetric, and the
+// solve reads only the upper triangles, NT (NT + 1) / 2 entries per block instead of NT^2.
+// Memory layout of one block: NT / 2 pairs of rows, pair p = row p from the diagonal on
+// (NT - p entries), then row NT - 1 - p from the diagonal on (p + 1 entries): NT + 1 entries
+// per pair.  In LDS the pairs sit NT + 2 entries apart, so that the rows a lane reads along
+// lie in different banks.
+// ---------------------------------------------------------------------------
+template <int NT>
+struct BtPack {
+    static constexpr int NP = NT * (NT + 1) / 2;  // entries per block
+    static constexpr int PAIR = NT + 1;           // entries per pair of rows in memory
+    static constexpr int LSIZE = NT / 2 * (NT + 2);  // LDS entries per block
+    static constexpr int R = (NP + 63) / 64;      // entries per lane when a wave loads a block
+    // LDS index of S[a][b], a <= b
+    static __device__ inline int lds(int a, int b) {
+        return a < NT / 2 ? (NT + 1) * a + b : (NT + 1) * (NT - 1) + NT - (NT + 2) * a + b;
+    }
+};
+
+// SinvP of every leaf from its Sinv: entry (a, b) of the upper triangle = (S[a][b] + S[b][a]) / 2
+// (the computed inverse of a symmetric block is symmetric up to rounding).
+__global__ void bt_pack_kernel(const LeafDesc* leaves, int nleaves, int Nn) {
+    const int NP = Nn * (Nn + 1) / 2, PAIR = Nn + 1;
+    const long long per = (long long)Nn * NP;
+    const long long total = (long long)nleaves * per;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int leaf = (int)(e / per);
+        const long long r = e % per;
+        const int i = (int)(r / NP), q = (int)(r % NP);
+        const int p = q / PAIR, o = q % PAIR;
+        int a, b;
+        if (o < Nn - p) {
+            a = p;
+            b = p + o;
+        } else {
+            a = Nn - 1 - p;
+            b = a + (o - (Nn - p));
+        }
+        const double* S = leaves[leaf].Sinv + (long long)i * Nn * Nn;
+        leaves[leaf].SinvP[(long long)i * NP + q] = 0.5 * (S[a * Nn + b] + S[b * Nn + a]);
+    }
+}
+
+// x[l] + x[l ^ 32] in every lane: the two halves of the wave exchanged by v_permlane32_swap
+__device__ inline double wave_half_sum(double x) {
+    const unsigned lo = (unsigned)__double2loint(x), hi = (unsigned)__double2hiint(x);
+    const auto sl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto sh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return __hiloint2double((int)sh[0], (int)sl[0]) + __hiloint2double((int)sh[1], (int)sl[1]);
+}
+
+// One wave per (leaf, rhs) as bt_solve_wave_kernel, on the packed blocks.  The 2 NT - 1 block
+// steps (forward i = 0 .. NT-1, then backward i = NT-2 .. 0) form one sequence; block s + 1 and
+// s + 2 of the sequence are in registers while step s computes on block s in the wave's LDS
+// slice, so the backward sweep's first blocks arrive during the last forward steps.  A block
+// product: group 0 publishes the vector in LDS, lane (j, h) gathers S[j][c] for its C columns
+// c from the packed upper triangle, and the G groups are summed.
+template <int NT>
+__global__ void __launch_bounds__(256) bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode,
+                                                           const double* f, int f_zero, const double* garena,
+                                                           double* harena, double* u) {
+    using PK = BtPack<NT>;
+    constexpr int NN = NT * NT, G = 64 / NT, C = NT / G, R = PK::R, STEPS = 2 * NT - 1;
+    __shared__ double zs_all[4][NN];
+    __shared__ double sp_all[4][PK::LSIZE];
+    __shared__ __attribute__((aligned(16))) double tv_all[4][NT];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const long long item = (long long)blockIdx.x * 4 + wave;
+    if (item >= (long long)nleaves * k) return;  // uniform per wave; no barrier below
+    double* zs = zs_all[wave];
+    double* sp = sp_all[wave];
+    double* tv = tv_all[wave];
+    const int rhs = (int)(item / nleaves), leaf = (int)(item % nleaves);
+    const int j = lane % NT, h = lane / NT;
+    const LeafDesc L = leaves[leaf];
+    const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
+    const auto* g = (mode != 0) ? gmem(garena) + L.g_off * k + rhs : nullptr;
+    const auto* cWv = gmem(L.stencil);
+    const auto* cEv = cWv + NN;
+    const auto* cSv = cWv + 2 * NN;
+    const auto* cNv = cWv + 3 * NN;
+    const auto* SP = gmem((const double*)L.SinvP);
+    auto rhs_row = [&](int i) {
+        const int I = i * NT + j;
+        double b = fl ? fl[I] : 0.0;
+        if (g) {
+            double add = 0.0;
+            if (i == 0) add += cWv[I] * g[(long long)(0 * NT + j) * k];
+            if (i == NT - 1) add += cEv[I] * g[(long long)(1 * NT + j) * k];
+            if (j == 0) add += cSv[I] * g[(long long)(2 * NT + i) * k];
+            if (j == NT - 1) add += cNv[I] * g[(long long)(3 * NT + i) * k];
+            b += -2.0 * add;
+        }
+        return b;
+    };
+    auto blk = [](int s) { return s < NT ? s : 2 * NT - 2 - s; };
+    auto load = [&](int s, double (&Rr)[R]) {
+        const auto* src = SP + (long long)blk(s) * PK::NP;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int e = lane + 64 * r;
+            Rr[r] = (e < PK::NP) ? src[e] : 0.0;
+        }
+    };
+    auto stage = [&](const double (&Rr)[R]) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int e = lane + 64 * r;
+            if (e < PK::NP) sp[e + e / PK::PAIR] = Rr[r];
+        }
+    };
+    // gather indices of this lane's columns c = h C + m: S[j][c] = sp[row + c] for c >= j (row j of
+    // the upper triangle), sp[col0 + m col1 + j] for c < j (row c)
+    const int cfirst = h * C;
+    const int row = PK::lds(j, 0);
+    const int col0 = (cfirst < NT / 2) ? (NT + 1) * cfirst : (NT + 1) * (NT - 1) + NT - (NT + 2) * cfirst;
+    const int col1 = (cfirst < NT / 2) ? (NT + 1) : -(NT + 2);
+    auto matvec = [&](double t) {
+        if (h == 0) tv[j] = t;
+        __builtin_amdgcn_wave_barrier();
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int m = 0; m < C; m++) {
+            const int c = cfirst + m;
+            const int idx = (c >= j) ? row + c : col0 + m * col1 + j;
+            const double v = sp[idx] * tv[c];
+            if (m & 1) acc1 += v;
+            else acc0 += v;
+        }
+        double acc = acc0 + acc1;
+#pragma unroll
+        for (int o = NT; o < 32; o <<= 1) acc += __shfl_xor(acc, o);
+        return wave_half_sum(acc);
+    };
+    double Ra[R], Rb[R];
+    load(0, Ra);
+    load(1, Rb);
+    stage(Ra);
+    load(2, Ra);
+    double zprev = 0.0, xnext = 0.0;
+    double bn = rhs_row(0), cwn = 0.0, cen = 0.0;
+    // step s on the block in LDS: forward z_i = Sinv_i (b_i - cW_i .* z_{i-1}) for s < NT, backward
+    // x_i = z_i - Sinv_i (cE_i .* x_{i+1}) after (x_{N-1} = z_{N-1})
+    auto step = [&](int s) {
+        if (s < NT) {
+            const int i = s;
+            const double b = bn, cw = cwn;
+            if (i + 1 < NT) {
+                bn = rhs_row(i + 1);
+                cwn = cWv[(i + 1) * NT + j];
+            } else {
+                cen = cEv[(NT - 2) * NT + j];
+            }
+            zprev = matvec((i > 0) ? b - cw * zprev : b);
+            if (h == 0) zs[i * NT + j] = zprev;
+            if (i == NT - 1) xnext = zprev;
+        } else {
+            const int i = 2 * NT - 2 - s;
+            const double ce = cen;
+            if (i > 0) cen = cEv[(i - 1) * NT + j];
+            const double zi = zs[i * NT + j];
+            const double x = zi - matvec(ce * xnext);
+            if (h == 0) zs[i * NT + j] = x;
+            xnext = x;
+        }
+    };
+    for (int s = 0; s < STEPS; s += 2) {
+        step(s);
+        if (s + 1 >= STEPS) break;
+        stage(Rb);
+        if (s + 3 < STEPS) load(s + 3, Rb);
+        step(s + 1);
+        if (s + 2 < STEPS) {
+            stage(Ra);
+            if (s + 4 < STEPS) load(s + 4, Ra);
+        }
+    }
+    // the wave's LDS writes are in order before its reads below; on the CPU emulation this is
+    // where the lanes meet
+    __builtin_amdgcn_wave_barrier();
+    if (mode == 1) {
+        auto* ul = gmem(u) + ((long long)rhs * nleaves + leaf) * NN;
+        for (int e = lane; e < NN; e += 64) ul[e] = zs[e];
+        return;
+    }
+    auto* hv = gmem(harena) + L.h_off * k + rhs;
+    const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
+    auto gv = [&](int t) { return g ? g[(long long)t * k] : 0.0; };
+    for (int t = lane; t < 4 * NT; t += 64) {
+        const int sd = t / NT, mm = t % NT;
+        double v;
+        if (sd == W) v = sx * (zs[mm] - gv(t));
+        else if (sd == E) v = -sx * (zs[(NT - 1) * NT + mm] - gv(t));
+        else if (sd == S) v = sy * (zs[mm * NT] - gv(t));
+        else v = -sy * (zs[mm * NT + NT - 1] - gv(t));
+        hv[(long long)t * k] = v;
+    }
+}
+
+template __global__ void bt_solve_sym_kernel<8>(const LeafDesc*, int, int, int, const double*, int, const double*,
+                                                double*, double*);
+template __global__ void bt_solve_sym_kernel<16>(const LeafDesc*, int, int, int, const double*, int, const double*,
+                                                 double*, double*);
+template __global__ void bt_solve_sym_kernel<32>(const LeafDesc*, int, int, int, const double*, int, const double*,
+                                                 double*, double*);
+
 void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
-                     const double* garena, double* harena, double* u, hipStream_t s) {
+                     const double* garena, double* harena, double* u, hipStream_t s, bool packed) {
     const unsigned blocks = (unsigned)(((long long)nleaves * k + 3) / 4);
-    if (Nn == 8)
+    // the full blocks (Sinv) are always kept: sides without a packed kernel use them
+    if (packed && Nn == 8)
+        hipLaunchKernelGGL(bt_solve_sym_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
+                           garena, harena, u);
+    else if (packed && Nn == 16)
+        hipLaunchKernelGGL(bt_solve_sym_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
+                           f_zero, garena, harena, u);
+    else if (packed && Nn == 32)
+        hipLaunchKernelGGL(bt_solve_sym_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
+                           f_zero, garena, harena, u);
+    else if (Nn == 8)
         hipLaunchKernelGGL(bt_solve_wave_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
                            garena, harena, u);
     else if (Nn == 16)

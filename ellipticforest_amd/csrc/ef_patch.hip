@@ -29,6 +29,7 @@ struct ef_patches_s {
     int N = 0, P = 0, NB = 1;
     long long max_blocks = 4096;
     bool variable = false;
+    bool packed = false;           // variable and symmetric: packed inverse pivot blocks (N 8, 16, 32)
     bool large = false;            // N > 32: batched-GEMM fast diagonalisation
     int Np = 0;                    // padded side of the large path
     hipStream_t stream = nullptr, own_stream = nullptr;
@@ -42,7 +43,7 @@ struct ef_patches_s {
     // block-tridiagonal path
     std::vector<LeafDesc> ld_h;
     DevBuf<LeafDesc> leafdesc;
-    DevBuf<double> vcoef, vstencil, vSinv, vwork;
+    DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork;
     // large path: padded basis (Np x Np), work buffers
     DevBuf<double> Qp, W1, W2;
     // staging of the host-buffer entry points
@@ -231,6 +232,23 @@ bool patch_constant(const double* c, long long NN) {
     return true;
 }
 
+// True when the five-point matrix of the patch (FiniteVolumeSolver.cpp:64-85, cell I = j + i N
+// with i along x) is symmetric to 1e-14 relative: across every interior face the coupling
+// alpha beta / h^2 is the same seen from both cells.  The physical-side terms change the
+// diagonal only.
+bool cells_symmetric(const double* c, int N) {
+    const long long NN = (long long)N * N;
+    const double *a = c, *bW = c + NN, *bE = c + 2 * NN, *bS = c + 3 * NN, *bN = c + 4 * NN;
+    auto close = [](double x, double y) { return std::fabs(x - y) <= 1e-14 * std::max(std::fabs(x), std::fabs(y)); };
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            const long long I = (long long)i * N + j;
+            if (i > 0 && !close(a[I] * bW[I], a[I - N] * bE[I - N])) return false;
+            if (j > 0 && !close(a[I] * bS[I], a[I - 1] * bN[I - 1])) return false;
+        }
+    return true;
+}
+
 void record_begin(ef_patches_s* p) {
     if (!p->ev[0]) {
         EF_HIP(hipEventCreate(&p->ev[0]));
@@ -249,7 +267,7 @@ void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double
     if (op == PATCH_OP_MAP_D2N && g == nullptr) {
         for (int c = 0; c < nrhs; c++) {
             launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, 0, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
-                            (const double*)nullptr, out + c * p->P * B4, (double*)nullptr, p->stream);
+                            (const double*)nullptr, out + c * p->P * B4, (double*)nullptr, p->stream, p->packed);
             EF_LAUNCH_CHECK();
         }
         return;
@@ -259,7 +277,7 @@ void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double
     for (int c = 0; c < nrhs; c++) {
         launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, mode, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
                         g + c * p->P * B4, mode == 2 ? out + c * p->P * B4 : (double*)nullptr,
-                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream);
+                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream, p->packed);
         EF_LAUNCH_CHECK();
     }
 }
@@ -324,6 +342,10 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
         efk::copy_h2d(p->vcoef.p, c->cell, sizeof(double) * P * 6 * NN, p->stream);
         p->vstencil.alloc(P * 5 * NN);
         p->vSinv.alloc(P * N * NN);
+        p->packed = (N == 8 || N == 16 || N == 32);
+        for (size_t i = 0; i < P && p->packed; i++) p->packed = cells_symmetric(c->cell + i * 6 * NN, N);
+        const long long NP = bt_packed_entries(N);
+        if (p->packed) p->vSinvP.alloc(P * N * NP);
         p->ld_h.assign(P, LeafDesc{});
         for (size_t i = 0; i < P; i++) {
             LeafDesc& L = p->ld_h[i];
@@ -335,12 +357,18 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
             L.stencil = p->vstencil.p + i * 5 * NN;
             L.Sinv = p->vSinv.p + i * N * NN;
             L.T = nullptr;
+            L.SinvP = p->packed ? p->vSinvP.p + i * N * NP : nullptr;
             L.dx = p->coef_h[i].dx;
             L.dy = p->coef_h[i].dy;
         }
         p->leafdesc.upload(p->ld_h, p->stream);
         hipLaunchKernelGGL(bt_factor_kernel, dim3(p->P), dim3(256), 0, p->stream, p->leafdesc.p, N);
         EF_LAUNCH_CHECK();
+        if (p->packed) {
+            hipLaunchKernelGGL(bt_pack_kernel, dim3((unsigned)std::min<long long>(blocks_for(P * N * NP), 65535)),
+                               dim3(256), 0, p->stream, p->leafdesc.p, p->P, N);
+            EF_LAUNCH_CHECK();
+        }
     } else {
         std::vector<double> QA, QB, mu;
         if (p->large) {
@@ -415,7 +443,7 @@ int ef_patches_info(ef_patches p, int* info) {
     if (!p || !info) throw EfError(EF_ERR_ARG, "null argument");
     info[0] = p->P;
     info[1] = p->N;
-    info[2] = p->variable ? 1 : 0;
+    info[2] = p->variable ? (p->packed ? 2 : 1) : 0;
     EF_API_END
 }
 

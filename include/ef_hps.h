@@ -172,7 +172,9 @@ typedef struct ef_patch_batch {
 int ef_patches_create(ef_ctx ctx, const ef_patch_batch* batch, const ef_coeffs* coeffs, ef_patches* out);
 int ef_patches_destroy(ef_patches p);
 int ef_patches_set_stream(ef_patches p, void* stream);
-/* info[0] npatches, info[1] N, info[2] 1 if the factorised (variable-coefficient) path is used */
+/* info[0] npatches, info[1] N, info[2] 0 for the fast-diagonalisation path (constant coefficients
+ * per patch), 1 for the factorised (variable-coefficient) path, 2 for the factorised path with
+ * packed symmetric inverse pivot blocks (symmetric five-point matrices, N in {8, 16, 32}) */
 int ef_patches_info(ef_patches p, int* info);
 /* op = EF_PATCH_SOLVE (out = u) or EF_PATCH_MAP_D2N (out = h); g or f may be NULL (zero). */
 int ef_patches_apply(ef_patches p, int op, const double* g, const double* f, double* out, int nrhs);

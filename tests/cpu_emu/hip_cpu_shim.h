@@ -96,6 +96,26 @@ emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);
 double __shfl(double v, int src, int width = 64);
 inline double __shfl_xor(double v, int mask, int width = 64) { return __shfl(v, (int)(threadIdx.x % 64) ^ mask, width); }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 emu_mfma_f64
+inline int __double2loint(double x) { unsigned long long u; std::memcpy(&u, &x, 8); return (int)(unsigned)u; }
+inline int __double2hiint(double x) { unsigned long long u; std::memcpy(&u, &x, 8); return (int)(unsigned)(u >> 32); }
+inline double __hiloint2double(int hi, int lo) {
+    const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+    double x;
+    std::memcpy(&x, &u, 8);
+    return x;
+}
+// v_permlane32_swap: lanes 32..63 of the first operand trade places with lanes 0..31 of the
+// second; element 0 of the result is the first operand afterwards, element 1 the second
+struct emu_u2 {
+    unsigned v[2];
+    unsigned operator[](int i) const { return v[i]; }
+};
+inline emu_u2 __builtin_amdgcn_permlane32_swap(unsigned old, unsigned src, bool, bool) {
+    const int l = (int)(threadIdx.x % 64);
+    const unsigned from_src = (unsigned)__shfl((double)src, l ^ 32);  // exact: 32-bit values
+    const unsigned from_old = (unsigned)__shfl((double)old, l ^ 32);
+    return {{l >= 32 ? from_src : old, l < 32 ? from_old : src}};
+}
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 // a scheduling barrier on the GPU (lanes of a wave run in lockstep); here the lanes meet
 #define __builtin_amdgcn_wave_barrier() emu_wave_sync()

@@ -79,10 +79,12 @@ def test_emu_fishpack_patches_N32(emu_ctx):
     check_against_oracle(pt, bounds, 32, rng, nrhs=1, lam=-0.5, dtn=False)
 
 
-def test_emu_five_point_constant_and_variable(emu_ctx):
+@pytest.mark.parametrize("N", [8, 16])
+def test_emu_five_point_constant_and_variable(emu_ctx, N):
+    """Constant coefficients (fast diagonalisation), variable beta (a symmetric matrix: packed
+    inverse pivot blocks) and variable alpha (rows scaled by alpha, not symmetric: full blocks)."""
     from ellipticforest_amd import _lib
-    rng = np.random.default_rng(5)
-    N = 8
+    rng = np.random.default_rng(5 + N)
     bounds = random_bounds(rng, 3)
     const = np.stack([oracle.sample_coeffs(N, b, lambda x, y: 2.0 + 0 * x, lambda x, y: 0.5 + 0 * x,
                                            lambda x, y: -0.7 + 0 * x) for b in bounds])
@@ -93,8 +95,12 @@ def test_emu_five_point_constant_and_variable(emu_ctx):
     var = np.stack([oracle.sample_coeffs(N, b, lambda x, y: 1.0 + 0 * x, beta, lambda x, y: -1.0 + 0 * x)
                     for b in bounds])
     pv = _lib.Patches(emu_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=var)
-    assert pv.variable
+    assert pv.variable and pv.packed
     check_against_oracle(pv, bounds, N, rng, coeffs=var)
+    nonsym = alpha_cells(bounds, N)
+    pn = _lib.Patches(emu_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=nonsym)
+    assert pn.variable and not pn.packed
+    check_against_oracle(pn, bounds, N, rng, coeffs=nonsym)
 
 
 def test_emu_patch_argument_errors(emu_ctx):
@@ -217,6 +223,12 @@ def variable_cells(bounds, N, beta=BETA3, lam=-1.0):
                      for b in bounds])
 
 
+def alpha_cells(bounds, N):
+    """alpha = 1 + 0.3 sin(x + 2y) scales the rows of the five-point matrix: not symmetric."""
+    return np.stack([oracle.sample_coeffs(N, b, lambda x, y: 1.0 + 0.3 * np.sin(x + 2 * y), BETA3,
+                                          lambda x, y: -1.0 + 0 * x) for b in bounds])
+
+
 def test_emu_variable_patches_N32(emu_ctx):
     """Largest block-tridiagonal size (N = 32) on the emulation build."""
     from ellipticforest_amd import _lib
@@ -224,21 +236,23 @@ def test_emu_variable_patches_N32(emu_ctx):
     bounds = random_bounds(rng, 2)
     cell = variable_cells(bounds, 32)
     pv = _lib.Patches(emu_ctx, bounds, 32, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
-    assert pv.variable
+    assert pv.variable and pv.packed
     check_against_oracle(pv, bounds, 32, rng, nrhs=1, coeffs=cell, dtn=False)  # N = 32 DtN: GPU test
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("N", [8, 16, 32])
-def test_gpu_variable_patches_vs_oracle(patch_ctx, N):
-    """Five-point stencil with cfg3's beta (FiniteVolumeSolver.cpp:64-184): block-tridiagonal path."""
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_gpu_variable_patches_vs_oracle(patch_ctx, N, symmetric):
+    """Five-point stencil (FiniteVolumeSolver.cpp:64-184), block-tridiagonal path: cfg3's beta
+    (symmetric matrix, packed inverse pivot blocks) and a variable alpha (full blocks)."""
     from ellipticforest_amd import _lib
-    say(f"patches five-point variable beta N={N}, 4 patches, 2 rhs")
-    rng = np.random.default_rng(300 + N)
+    say(f"patches five-point variable {'beta' if symmetric else 'alpha'} N={N}, 4 patches, 2 rhs")
+    rng = np.random.default_rng(300 + N + (0 if symmetric else 1000))
     bounds = random_bounds(rng, 4)
-    cell = variable_cells(bounds, N)
+    cell = variable_cells(bounds, N) if symmetric else alpha_cells(bounds, N)
     pv = _lib.Patches(patch_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
-    assert pv.variable
+    assert pv.variable and pv.packed == symmetric
     check_against_oracle(pv, bounds, N, rng, coeffs=cell)
 
 
@@ -254,7 +268,7 @@ def test_gpu_cfg3_leaf_set_properties(patch_ctx):
     say(f"cfg3 leaf set: {len(bounds)} patches, N={N}, variable beta, lambda=-1")
     cell = variable_cells(bounds, N)
     pv = _lib.Patches(patch_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
-    assert pv.variable
+    assert pv.variable and pv.packed
     rng = np.random.default_rng(12)
     P = len(bounds)
     g1, g2 = rng.standard_normal((2, 1, P, 4 * N))
