@@ -103,6 +103,7 @@ struct LeafDesc {
     double* T;           // leaf DtN (4N x 4N)
     double* SinvP;       // symmetric matrices: the inverse pivot blocks packed (upper triangles), or null
     double dx, dy;
+    long long pstride;   // entries from one packed block of the leaf to its next
 };
 
 // Upward family step (upwards4to1, HPSAlgorithm.hpp:529-551) for one merge.
@@ -206,7 +207,7 @@ static_assert(offsetof(GJDesc, piv) == 80, "GJDesc.piv");
 static_assert(sizeof(MergeDesc) == 80, "MergeDesc");
 static_assert(offsetof(MergeDesc, Tc) == 8, "MergeDesc.Tc");
 static_assert(sizeof(CoarsenDesc) == 24, "CoarsenDesc");
-static_assert(sizeof(LeafDesc) == 80, "LeafDesc");
+static_assert(sizeof(LeafDesc) == 88, "LeafDesc");
 static_assert(offsetof(LeafDesc, h_off) == 8, "LeafDesc.h_off");
 static_assert(offsetof(LeafDesc, coef) == 24, "LeafDesc.coef");
 static_assert(sizeof(UpDesc) == 112, "UpDesc");

@@ -929,7 +929,10 @@ struct BtPack {
 };
 
 // SinvP of every leaf from its Sinv: entry (a, b) of the upper triangle = (S[a][b] + S[b][a]) / 2
-// (the computed inverse of a symmetric block is symmetric up to rounding).
+// (the computed inverse of a symmetric block is symmetric up to rounding).  Block i of a leaf
+// starts i * pstride entries after its block 0: with the blocks of all leaves interleaved
+// (block i of leaf l at (i * nleaves + l) * NP), the waves of a launch, which step through
+// their leaves' blocks at about the same pace, read neighbouring memory.
 __global__ void bt_pack_kernel(const LeafDesc* leaves, int nleaves, int Nn) {
     const int NP = Nn * (Nn + 1) / 2, PAIR = Nn + 1;
     const long long per = (long long)Nn * NP;
@@ -948,7 +951,7 @@ __global__ void bt_pack_kernel(const LeafDesc* leaves, int nleaves, int Nn) {
             b = a + (o - (Nn - p));
         }
         const double* S = leaves[leaf].Sinv + (long long)i * Nn * Nn;
-        leaves[leaf].SinvP[(long long)i * NP + q] = 0.5 * (S[a * Nn + b] + S[b * Nn + a]);
+        leaves[leaf].SinvP[(long long)i * leaves[leaf].pstride + q] = 0.5 * (S[a * Nn + b] + S[b * Nn + a]);
     }
 }
 
@@ -961,31 +964,33 @@ __device__ inline double wave_half_sum(double x) {
 }
 
 // One wave per (leaf, rhs) as bt_solve_wave_kernel, on the packed blocks.  The 2 NT - 1 block
-// steps (forward i = 0 .. NT-1, then backward i = NT-2 .. 0) form one sequence; block s + 1 and
-// s + 2 of the sequence are in registers while step s computes on block s in the wave's LDS
-// slice, so the backward sweep's first blocks arrive during the last forward steps.  A block
-// product: group 0 publishes the vector in LDS, lane (j, h) gathers S[j][c] for its C columns
-// c from the packed upper triangle, and the G groups are summed.
+// steps (forward i = 0 .. NT-1, then backward i = NT-2 .. 0) form one sequence; while step s
+// computes on block s in one of the wave's two LDS slots, block s + 1 waits in the other and
+// block s + 2 is on its way into registers, so the backward sweep's first blocks arrive during
+// the last forward steps.  A block product: group 0 publishes the vector in LDS, lane (j, h)
+// gathers S[j][c] for its C columns c from the packed upper triangle, and the G groups are
+// summed.  The rows z_i of the forward sweep go to zbuf ([rhs][leaf][N*N]; the solution itself
+// when mode == 1), every lane writing and later reading back element j, and the backward sweep
+// overwrites them with x_i; LDS holds two blocks and the vector (8.9 KB per wave at N = 32), so
+// four waves per SIMD fit.
 template <int NT>
-__global__ void __launch_bounds__(256) bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode,
-                                                           const double* f, int f_zero, const double* garena,
-                                                           double* harena, double* u) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f, int f_zero,
+                    const double* garena, double* harena, double* zbuf) {
     using PK = BtPack<NT>;
     constexpr int NN = NT * NT, G = 64 / NT, C = NT / G, R = PK::R, STEPS = 2 * NT - 1;
-    __shared__ double zs_all[4][NN];
-    __shared__ double sp_all[4][PK::LSIZE];
+    __shared__ double sp_all[4][2][PK::LSIZE];
     __shared__ __attribute__((aligned(16))) double tv_all[4][NT];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const long long item = (long long)blockIdx.x * 4 + wave;
     if (item >= (long long)nleaves * k) return;  // uniform per wave; no barrier below
-    double* zs = zs_all[wave];
-    double* sp = sp_all[wave];
     double* tv = tv_all[wave];
     const int rhs = (int)(item / nleaves), leaf = (int)(item % nleaves);
     const int j = lane % NT, h = lane / NT;
     const LeafDesc L = leaves[leaf];
     const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
     const auto* g = (mode != 0) ? gmem(garena) + L.g_off * k + rhs : nullptr;
+    auto* zl = gmem(zbuf) + ((long long)rhs * nleaves + leaf) * NN;
     const auto* cWv = gmem(L.stencil);
     const auto* cEv = cWv + NN;
     const auto* cSv = cWv + 2 * NN;
@@ -1006,14 +1011,15 @@ __global__ void __launch_bounds__(256) bt_solve_sym_kernel(const LeafDesc* leave
     };
     auto blk = [](int s) { return s < NT ? s : 2 * NT - 2 - s; };
     auto load = [&](int s, double (&Rr)[R]) {
-        const auto* src = SP + (long long)blk(s) * PK::NP;
+        const auto* src = SP + (long long)blk(s) * L.pstride;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int e = lane + 64 * r;
             Rr[r] = (e < PK::NP) ? src[e] : 0.0;
         }
     };
-    auto stage = [&](const double (&Rr)[R]) {
+    auto stage = [&](const double (&Rr)[R], int slot) {
+        double* sp = sp_all[wave][slot];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int e = lane + 64 * r;
@@ -1026,33 +1032,41 @@ __global__ void __launch_bounds__(256) bt_solve_sym_kernel(const LeafDesc* leave
     const int row = PK::lds(j, 0);
     const int col0 = (cfirst < NT / 2) ? (NT + 1) * cfirst : (NT + 1) * (NT - 1) + NT - (NT + 2) * cfirst;
     const int col1 = (cfirst < NT / 2) ? (NT + 1) : -(NT + 2);
-    auto matvec = [&](double t) {
+    auto matvec = [&](const double* sp, double t) {
         if (h == 0) tv[j] = t;
         __builtin_amdgcn_wave_barrier();
         double acc0 = 0.0, acc1 = 0.0;
-#pragma unroll
-        for (int m = 0; m < C; m++) {
-            const int c = cfirst + m;
-            const int idx = (c >= j) ? row + c : col0 + m * col1 + j;
-            const double v = sp[idx] * tv[c];
-            if (m & 1) acc1 += v;
-            else acc0 += v;
+        if constexpr (C == 1) {
+            acc0 = sp[(cfirst >= j) ? row + cfirst : col0 + j] * tv[cfirst];
+        } else {
+            // two columns per iteration, unrolled by two: fewer gathers in flight at a time keep
+            // the register count within four waves per SIMD
+#pragma unroll 2
+            for (int m = 0; m < C; m += 2) {
+                const int c = cfirst + m;
+                const int i0 = (c >= j) ? row + c : col0 + m * col1 + j;
+                const int i1 = (c + 1 >= j) ? row + c + 1 : col0 + (m + 1) * col1 + j;
+                acc0 += sp[i0] * tv[c];
+                acc1 += sp[i1] * tv[c + 1];
+            }
         }
         double acc = acc0 + acc1;
 #pragma unroll
         for (int o = NT; o < 32; o <<= 1) acc += __shfl_xor(acc, o);
         return wave_half_sum(acc);
     };
-    double Ra[R], Rb[R];
-    load(0, Ra);
-    load(1, Rb);
-    stage(Ra);
-    load(2, Ra);
+    double Rr[R];
+    load(0, Rr);
+    stage(Rr, 0);
+    load(1, Rr);
+    stage(Rr, 1);
+    load(2, Rr);
     double zprev = 0.0, xnext = 0.0;
     double bn = rhs_row(0), cwn = 0.0, cen = 0.0;
     // step s on the block in LDS: forward z_i = Sinv_i (b_i - cW_i .* z_{i-1}) for s < NT, backward
     // x_i = z_i - Sinv_i (cE_i .* x_{i+1}) after (x_{N-1} = z_{N-1})
     auto step = [&](int s) {
+        const double* sp = sp_all[wave][s & 1];
         if (s < NT) {
             const int i = s;
             const double b = bn, cw = cwn;
@@ -1062,48 +1076,43 @@ __global__ void __launch_bounds__(256) bt_solve_sym_kernel(const LeafDesc* leave
             } else {
                 cen = cEv[(NT - 2) * NT + j];
             }
-            zprev = matvec((i > 0) ? b - cw * zprev : b);
-            if (h == 0) zs[i * NT + j] = zprev;
+            zprev = matvec(sp, (i > 0) ? b - cw * zprev : b);
+            zl[i * NT + j] = zprev;
             if (i == NT - 1) xnext = zprev;
         } else {
             const int i = 2 * NT - 2 - s;
             const double ce = cen;
             if (i > 0) cen = cEv[(i - 1) * NT + j];
-            const double zi = zs[i * NT + j];
-            const double x = zi - matvec(ce * xnext);
-            if (h == 0) zs[i * NT + j] = x;
+            const double zi = zl[i * NT + j];
+            const double x = zi - matvec(sp, ce * xnext);
+            zl[i * NT + j] = x;
             xnext = x;
         }
     };
-    for (int s = 0; s < STEPS; s += 2) {
+    for (int s = 0; s < STEPS; s++) {
         step(s);
-        if (s + 1 >= STEPS) break;
-        stage(Rb);
-        if (s + 3 < STEPS) load(s + 3, Rb);
-        step(s + 1);
+        // slot s & 1 is free once step s has read it (the wave's LDS accesses run in order)
         if (s + 2 < STEPS) {
-            stage(Ra);
-            if (s + 4 < STEPS) load(s + 4, Ra);
+            stage(Rr, s & 1);
+            if (s + 3 < STEPS) load(s + 3, Rr);
         }
     }
-    // the wave's LDS writes are in order before its reads below; on the CPU emulation this is
-    // where the lanes meet
+    if (mode == 1) return;  // the solution is in zbuf
+    // Neumann data of the edge cells: h_W = (2/dx)(u_W - g_W), h_E = -(2/dx)(u_E - g_E),
+    // h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N), from elements other lanes wrote
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    if (mode == 1) {
-        auto* ul = gmem(u) + ((long long)rhs * nleaves + leaf) * NN;
-        for (int e = lane; e < NN; e += 64) ul[e] = zs[e];
-        return;
-    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     auto* hv = gmem(harena) + L.h_off * k + rhs;
     const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
     auto gv = [&](int t) { return g ? g[(long long)t * k] : 0.0; };
     for (int t = lane; t < 4 * NT; t += 64) {
         const int sd = t / NT, mm = t % NT;
         double v;
-        if (sd == W) v = sx * (zs[mm] - gv(t));
-        else if (sd == E) v = -sx * (zs[(NT - 1) * NT + mm] - gv(t));
-        else if (sd == S) v = sy * (zs[mm * NT] - gv(t));
-        else v = -sy * (zs[mm * NT + NT - 1] - gv(t));
+        if (sd == W) v = sx * (zl[mm] - gv(t));
+        else if (sd == E) v = -sx * (zl[(NT - 1) * NT + mm] - gv(t));
+        else if (sd == S) v = sy * (zl[mm * NT] - gv(t));
+        else v = -sy * (zl[mm * NT + NT - 1] - gv(t));
         hv[(long long)t * k] = v;
     }
 }
@@ -1116,18 +1125,19 @@ template __global__ void bt_solve_sym_kernel<32>(const LeafDesc*, int, int, int,
                                                  double*, double*);
 
 void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
-                     const double* garena, double* harena, double* u, hipStream_t s, bool packed) {
+                     const double* garena, double* harena, double* u, hipStream_t s, bool packed, double* zwork) {
     const unsigned blocks = (unsigned)(((long long)nleaves * k + 3) / 4);
     // the full blocks (Sinv) are always kept: sides without a packed kernel use them
+    double* zbuf = (mode == 1) ? u : zwork;
     if (packed && Nn == 8)
         hipLaunchKernelGGL(bt_solve_sym_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
-                           garena, harena, u);
+                           garena, harena, zbuf);
     else if (packed && Nn == 16)
         hipLaunchKernelGGL(bt_solve_sym_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
-                           f_zero, garena, harena, u);
+                           f_zero, garena, harena, zbuf);
     else if (packed && Nn == 32)
         hipLaunchKernelGGL(bt_solve_sym_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
-                           f_zero, garena, harena, u);
+                           f_zero, garena, harena, zbuf);
     else if (Nn == 8)
         hipLaunchKernelGGL(bt_solve_wave_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
                            garena, harena, u);

@@ -30,14 +30,16 @@ __global__ void bt_solve_wave_kernel(const LeafDesc* leaves, int nleaves, int k,
                                      int f_zero, const double* garena, double* harena, double* u);
 template <int NT>
 __global__ void bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f,
-                                    int f_zero, const double* garena, double* harena, double* u);
+                                    int f_zero, const double* garena, double* harena, double* zbuf);
 __global__ void bt_pack_kernel(const LeafDesc* leaves, int nleaves, int Nn);
 // Leaf solves with variable coefficients: one wave per (leaf, rhs) for N in {8, 16, 32},
 // the workgroup-per-leaf kernel otherwise.  Modes as bt_solve_kernel.  With `packed`, every
 // leaf's SinvP holds its packed symmetric inverse pivot blocks (bt_pack_kernel), which the
-// kernel for N in {8, 16, 32} reads instead of Sinv.
+// kernel for N in {8, 16, 32} reads instead of Sinv; in modes 0 and 2 it needs zwork,
+// nleaves * k * N * N doubles of work space.
 void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
-                     const double* garena, double* harena, double* u, hipStream_t s, bool packed = false);
+                     const double* garena, double* harena, double* u, hipStream_t s, bool packed = false,
+                     double* zwork = nullptr);
 // Packed storage of the symmetric inverse pivot blocks: entries per block.
 inline long long bt_packed_entries(int Nn) { return (long long)Nn * (Nn + 1) / 2; }
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)

@@ -43,7 +43,7 @@ struct ef_patches_s {
     // block-tridiagonal path
     std::vector<LeafDesc> ld_h;
     DevBuf<LeafDesc> leafdesc;
-    DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork;
+    DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork, vz;  // vz: forward rows of the packed solve
     // large path: padded basis (Np x Np), work buffers
     DevBuf<double> Qp, W1, W2;
     // staging of the host-buffer entry points
@@ -264,10 +264,12 @@ void record_end(ef_patches_s* p) {
 // Block-tridiagonal path, one right-hand side per launch ([patch][...] layouts with k = 1).
 void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
     const long long NN = (long long)p->N * p->N, B4 = 4LL * p->N;
+    if (p->packed && op == PATCH_OP_MAP_D2N) p->vz.ensure((size_t)p->P * NN);
     if (op == PATCH_OP_MAP_D2N && g == nullptr) {
         for (int c = 0; c < nrhs; c++) {
             launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, 0, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
-                            (const double*)nullptr, out + c * p->P * B4, (double*)nullptr, p->stream, p->packed);
+                            (const double*)nullptr, out + c * p->P * B4, (double*)nullptr, p->stream, p->packed,
+                            p->vz.p);
             EF_LAUNCH_CHECK();
         }
         return;
@@ -277,7 +279,7 @@ void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double
     for (int c = 0; c < nrhs; c++) {
         launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, mode, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
                         g + c * p->P * B4, mode == 2 ? out + c * p->P * B4 : (double*)nullptr,
-                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream, p->packed);
+                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream, p->packed, p->vz.p);
         EF_LAUNCH_CHECK();
     }
 }
@@ -357,7 +359,8 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
             L.stencil = p->vstencil.p + i * 5 * NN;
             L.Sinv = p->vSinv.p + i * N * NN;
             L.T = nullptr;
-            L.SinvP = p->packed ? p->vSinvP.p + i * N * NP : nullptr;
+            L.SinvP = p->packed ? p->vSinvP.p + i * NP : nullptr;  // block-major: block b at b * P * NP
+            L.pstride = (long long)P * NP;
             L.dx = p->coef_h[i].dx;
             L.dy = p->coef_h[i].dy;
         }

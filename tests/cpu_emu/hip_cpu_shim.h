@@ -119,6 +119,13 @@ inline emu_u2 __builtin_amdgcn_permlane32_swap(unsigned old, unsigned src, bool,
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 // a scheduling barrier on the GPU (lanes of a wave run in lockstep); here the lanes meet
 #define __builtin_amdgcn_wave_barrier() emu_wave_sync()
+// memory fences: the emulated lanes share the host's coherent memory; ordering between lanes
+// comes from the wave and block barriers
+#define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
+#ifndef __ATOMIC_RELEASE
+#define __ATOMIC_RELEASE 3
+#define __ATOMIC_ACQUIRE 2
+#endif
 // only applied to values that are uniform across the wave by construction
 #define __builtin_amdgcn_readfirstlane(x) (x)
 
