@@ -31,10 +31,12 @@ U = lambda x, y: np.sin(x) + np.sin(y)
 F = lambda x, y: -(np.sin(x) + np.sin(y))
 BOUNDS = (-10.0, 10.0, -10.0, 10.0)
 METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds"
-# HBM bytes per launch of the leaf-solve kernel on this workload (16 right-hand sides), from
-# two rocprofv3 counter passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from the copy
-# kernel); profiles/patch_fd_kernel_cfg2_leafset.md has the commands and the raw values.
-TRAFFIC_PER_LAUNCH = 578.1e6
+# HBM bytes per launch measured by two rocprofv3 counter passes (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, factors from the copy kernel); profiles/patch_fd_kernel_cfg2_leafset.md and
+# profiles/bt_solve_sym_kernel_cfg3_leafset.md have the commands and the raw values.
+TRAFFIC_PER_LAUNCH = 573.4e6          # leaf solves of this workload, 16 right-hand sides
+TRAFFIC_MAP_D2N = 319.1e6             # particular Neumann data, same leaf set and k
+TRAFFIC_CFG3 = 1329.8e6               # cfg3 leaf solves, packed symmetric blocks
 
 
 def log(msg):
@@ -404,7 +406,7 @@ def main():
             name="upwards leaf stage (particularNeumannData on every leaf, ef_patches_apply_device MAP_D2N)",
             kernel="patch_fd_kernel<1, true, 1>", value=pp["value"], unit="leaf maps/s", ms_per_launch=pp["ms"],
             patches=len(bounds), nrhs=k, achieved_gbs=pp["gbs"], algorithmic_bytes_per_launch=pp["bytes"],
-            frac_of_peak=pp["gbs"] / 8000.0))
+            traffic=TRAFFIC_MAP_D2N if k == 16 else None, frac_of_peak=pp["gbs"] / 8000.0))
         pv = gpu_variable_run(args.steps, args.warmup)
         log(f"cfg3 variable-coefficient leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s")
         out["components"].append(dict(
@@ -412,7 +414,8 @@ def main():
             kernel="bt_solve_sym_kernel<32>" if pv["packed"] else "bt_solve_wave_kernel<32>", value=pv["value"],
             unit="leaf solves/s", ms_per_launch=pv["ms"], patches=pv["patches"], nrhs=1,
             factor_storage="packed symmetric" if pv["packed"] else "full", achieved_gbs=pv["gbs"],
-            algorithmic_bytes_per_launch=pv["bytes"], frac_of_peak=pv["gbs"] / 8000.0))
+            algorithmic_bytes_per_launch=pv["bytes"], traffic=TRAFFIC_CFG3 if pv["packed"] else None,
+            frac_of_peak=pv["gbs"] / 8000.0))
     except Exception as e:
         out["components_error"] = f"{type(e).__name__}: {e}"
         log(out["components_error"])
