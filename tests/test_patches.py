@@ -121,8 +121,10 @@ def test_emu_patch_argument_errors(emu_ctx):
 
 # ----- GPU: the product library through the C ABI -----------------------------------
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [8, 16, 32])
+@pytest.mark.parametrize("N", [4, 8, 12, 16, 24, 32])
 def test_gpu_fishpack_patches_vs_oracle(patch_ctx, N):
+    """Every register-resident variant: one MFMA block (N <= 16) or two by two (N <= 32), each
+    with N filling the blocks (16, 32) or padded (4, 8, 12, 24)."""
     from ellipticforest_amd import _lib
     say(f"patches FISHPACK N={N}, 5 patches, 3 rhs, lambda=-1")
     rng = np.random.default_rng(100 + N)
@@ -241,18 +243,19 @@ def test_emu_variable_patches_N32(emu_ctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [8, 16, 32])
+@pytest.mark.parametrize("N", [4, 8, 12, 16, 24, 32])
 @pytest.mark.parametrize("symmetric", [True, False])
 def test_gpu_variable_patches_vs_oracle(patch_ctx, N, symmetric):
     """Five-point stencil (FiniteVolumeSolver.cpp:64-184), block-tridiagonal path: cfg3's beta
-    (symmetric matrix, packed inverse pivot blocks) and a variable alpha (full blocks)."""
+    (symmetric matrix, packed inverse pivot blocks) and a variable alpha (full blocks); N in
+    {8, 16, 32} on the wave kernels, other sides on the workgroup-per-patch kernel."""
     from ellipticforest_amd import _lib
     say(f"patches five-point variable {'beta' if symmetric else 'alpha'} N={N}, 4 patches, 2 rhs")
     rng = np.random.default_rng(300 + N + (0 if symmetric else 1000))
     bounds = random_bounds(rng, 4)
     cell = variable_cells(bounds, N) if symmetric else alpha_cells(bounds, N)
     pv = _lib.Patches(patch_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
-    assert pv.variable and pv.packed == symmetric
+    assert pv.variable and pv.packed == (symmetric and N in (8, 16, 32))
     check_against_oracle(pv, bounds, N, rng, coeffs=cell)
 
 
