@@ -546,13 +546,10 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
         hipLaunchKernelGGL(bt_factor_kernel, dim3(nl), dim3(256), 0, s, M->leafdesc.p, N);
         EF_LAUNCH_CHECK();
         const long long per = (long long)N * N * 4 * N;
-        const int chunk = (int)std::max<long long>(1, std::min<long long>(nl, (512LL << 20) / (per * 8)));
-        M->vwork.ensure((size_t)chunk * per);
-        for (int first = 0; first < nl; first += chunk) {
-            const int cnt = std::min(chunk, nl - first);
-            hipLaunchKernelGGL(bt_dtn_kernel, dim3(cnt), dim3(256), 0, s, M->leafdesc.p, N, first, M->vwork.p);
-            EF_LAUNCH_CHECK();
-        }
+        const long long chunk = std::max<long long>(1, std::min<long long>(nl, (512LL << 20) / (per * 8)));
+        M->vwork.ensure((size_t)(chunk * per));
+        launch_bt_dtn(M->leafdesc.p, N, nl, M->vwork.p, chunk, s);
+        EF_LAUNCH_CHECK();
     } else {
         // leaf DtN per class (buildD2N): one "patch" per class, T written class by class
         PatchFDArgs a = leaf_fd_args(M, PATCH_OP_DTN);

@@ -3,6 +3,7 @@
 // Every kernel is batched over all patches of one tree level (or all leaves) so a
 // level is a handful of launches, whatever the number of patches.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include "ef_device.hpp"
 #include "ef_kernels.hpp"
@@ -906,6 +907,141 @@ template __global__ void bt_solve_wave_kernel<16>(const LeafDesc*, int, int, int
                                                   double*, double*);
 template __global__ void bt_solve_wave_kernel<32>(const LeafDesc*, int, int, int, const double*, int, const double*,
                                                   double*, double*);
+
+// Leaf DtN with variable coefficients on the matrix cores, N = NT in {16, 32}: the same
+// substitution as bt_dtn_kernel for all 4N boundary columns at once, as block products
+// Z_i = Sinv_i (R_i - cW_i .* Z_{i-1}) and X_i = Z_i - Sinv_i (cE_i .* X_{i+1}) on
+// v_mfma_f64_16x16x4.  Wave w of the workgroup (one leaf per workgroup) owns the N columns of
+// side w, for all rows, so the waves never meet.  An accumulator tile (slot v of lane (q, r)
+// = row 4v + q, column r) is the B operand of the next product as it stands (k-step ks takes
+// rows 4 ks + q: tile ks / 4, slot ks % 4): the column scaling by cW or cE and the boundary
+// forcing R are applied in registers.  Z_i goes to the leaf's slice of `work` ([i][row][4N
+// columns]; each lane reads back only what it wrote) for the backward sweep, and each X_i's
+// edge entries go straight into T as the sweep produces them.
+template <int NT>
+__global__ void __launch_bounds__(256) bt_dtn_mfma_kernel(const LeafDesc* leaves, int first, double* work) {
+    constexpr int NN = NT * NT, NB = 4 * NT, RT = NT / 16, CT = NT / 16, KS = NT / 4;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int q = lane >> 4, r = lane & 15;
+    const LeafDesc L = leaves[first + blockIdx.x];
+    auto* Zw = gmem(work) + (long long)blockIdx.x * NT * NN * 4;
+    const auto* st = gmem((const double*)L.stencil);
+    auto* Tl = gmem(L.T);
+    const int c0 = wave * NT;  // columns of side `wave`
+    // forcing of this lane's columns c = c0 + 16 t + r (position m on side `wave`): the boundary
+    // cell (fi, fk) of the column and -2 c_side there (FiniteVolumeSolver.cpp:100-184)
+    int fi[CT], fk[CT];
+    double fv[CT];
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+        const int m = 16 * t + r;
+        fi[t] = (wave == W) ? 0 : (wave == E) ? NT - 1 : m;
+        fk[t] = (wave == S) ? 0 : (wave == N) ? NT - 1 : m;
+        fv[t] = -2.0 * st[wave * NN + fi[t] * NT + fk[t]];
+    }
+    const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
+    // T entries of block row i: W (i = 0) and E (i = N-1) rows, S (k = 0) and N (k = N-1) entries
+    auto emit = [&](int i, const d4 (&X)[RT][CT]) {
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int t = 0; t < CT; t++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int k = rt * 16 + 4 * v + q, c = c0 + 16 * t + r;
+                    const double x = X[rt][t][v];
+                    auto put = [&](int a, double sg) { Tl[(long long)a * NB + c] = sg * (x - (a == c ? 1.0 : 0.0)); };
+                    if (i == 0) put(W * NT + k, sx);
+                    if (i == NT - 1) put(E * NT + k, -sx);
+                    if (k == 0) put(S * NT + i, sy);
+                    if (k == NT - 1) put(N * NT + i, -sy);
+                }
+    };
+    auto zaddr = [&](int i, int rt, int t, int v) {
+        return ((long long)i * NT + rt * 16 + 4 * v + q) * NB + c0 + 16 * t + r;
+    };
+    d4 Zp[RT][CT], Zc[RT][CT];
+    // forward
+    for (int i = 0; i < NT; i++) {
+        const auto* Si = gmem((const double*)L.Sinv) + (long long)i * NN;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int t = 0; t < CT; t++) Zc[rt][t] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + q;
+            const double cw = (i > 0) ? st[i * NT + k] : 0.0;
+            double a[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) a[rt] = Si[(rt * 16 + r) * NT + 4 * ks + q];
+#pragma unroll
+            for (int t = 0; t < CT; t++) {
+                double b = (i > 0) ? -cw * Zp[ks / 4][t][ks % 4] : 0.0;
+                if (fi[t] == i && fk[t] == k) b += fv[t];
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) Zc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b, Zc[rt][t], 0, 0, 0);
+            }
+        }
+        if (i < NT - 1) {
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+                for (int t = 0; t < CT; t++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) Zw[zaddr(i, rt, t, v)] = Zc[rt][t][v];
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int t = 0; t < CT; t++) Zp[rt][t] = Zc[rt][t];
+    }
+    // backward: X_{N-1} = Z_{N-1}
+    emit(NT - 1, Zp);
+    for (int i = NT - 2; i >= 0; i--) {
+        const auto* Si = gmem((const double*)L.Sinv) + (long long)i * NN;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int t = 0; t < CT; t++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) Zc[rt][t][v] = Zw[zaddr(i, rt, t, v)];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int k = 4 * ks + q;
+            const double ce = st[NN + i * NT + k];
+            double a[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) a[rt] = Si[(rt * 16 + r) * NT + 4 * ks + q];
+#pragma unroll
+            for (int t = 0; t < CT; t++) {
+                const double b = -ce * Zp[ks / 4][t][ks % 4];
+#pragma unroll
+                for (int rt = 0; rt < RT; rt++) Zc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b, Zc[rt][t], 0, 0, 0);
+            }
+        }
+        emit(i, Zc);
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int t = 0; t < CT; t++) Zp[rt][t] = Zc[rt][t];
+    }
+}
+
+template __global__ void bt_dtn_mfma_kernel<16>(const LeafDesc*, int, double*);
+template __global__ void bt_dtn_mfma_kernel<32>(const LeafDesc*, int, double*);
+
+void launch_bt_dtn(const LeafDesc* leaves, int Nn, int nleaves, double* work, long long chunk, hipStream_t s) {
+    for (int first = 0; first < nleaves; first += (int)chunk) {
+        const unsigned cnt = (unsigned)std::min<long long>(chunk, nleaves - first);
+        if (Nn == 16)
+            hipLaunchKernelGGL(bt_dtn_mfma_kernel<16>, dim3(cnt), dim3(256), 0, s, leaves, first, work);
+        else if (Nn == 32)
+            hipLaunchKernelGGL(bt_dtn_mfma_kernel<32>, dim3(cnt), dim3(256), 0, s, leaves, first, work);
+        else
+            hipLaunchKernelGGL(bt_dtn_kernel, dim3(cnt), dim3(256), 0, s, leaves, Nn, first, work);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Symmetric matrices (alpha beta the same seen from both cells of every interior face, which

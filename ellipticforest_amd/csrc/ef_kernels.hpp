@@ -26,6 +26,11 @@ __global__ void bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int
                                 int f_zero, const double* garena, double* harena, double* u);
 __global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work);
 template <int NT>
+__global__ void bt_dtn_mfma_kernel(const LeafDesc* leaves, int first, double* work);
+// DtN of every leaf with variable coefficients (through each leaf's T) in launches of `chunk`
+// leaves; `work` holds chunk * N * N * 4N doubles.  N in {16, 32} on the matrix cores.
+void launch_bt_dtn(const LeafDesc* leaves, int Nn, int nleaves, double* work, long long chunk, hipStream_t s);
+template <int NT>
 __global__ void bt_solve_wave_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f,
                                      int f_zero, const double* garena, double* harena, double* u);
 template <int NT>

@@ -496,14 +496,13 @@ int ef_patches_dtn(ef_patches p, double* T) {
         // bt_dtn_kernel writes each leaf's T through its descriptor
         for (int i = 0; i < p->P; i++) p->ld_h[i].T = p->obuf.p + (size_t)i * per;
         p->leafdesc.upload(p->ld_h, p->stream);
+        // work space of the forward sweep: N * N * 4N doubles per leaf, up to 2 GiB per launch
+        // (2,048 leaves at N = 32, enough workgroups to fill the GPU)
         const long long zper = (long long)p->N * p->N * 4 * p->N;
-        const int chunk = (int)std::max<long long>(1, std::min<long long>(p->P, (256LL << 20) / (zper * 8)));
-        p->vwork.ensure((size_t)chunk * zper);
-        for (int first = 0; first < p->P; first += chunk) {
-            const int cnt = std::min(chunk, p->P - first);
-            hipLaunchKernelGGL(bt_dtn_kernel, dim3(cnt), dim3(256), 0, p->stream, p->leafdesc.p, p->N, first, p->vwork.p);
-            EF_LAUNCH_CHECK();
-        }
+        const long long chunk = std::max<long long>(1, std::min<long long>(p->P, (2048LL << 20) / (zper * 8)));
+        p->vwork.ensure((size_t)(chunk * zper));
+        launch_bt_dtn(p->leafdesc.p, p->N, p->P, p->vwork.p, chunk, p->stream);
+        EF_LAUNCH_CHECK();
     } else if (p->large) {
         for (int i = 0; i < p->P; i++) launch_large(p, PATCH_OP_DTN, nullptr, nullptr, p->obuf.p, 1, i);
     } else {
