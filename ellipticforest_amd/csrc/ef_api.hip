@@ -543,7 +543,7 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     if (M->variable) {
         // block-tridiagonal factors, then the DtN of every leaf (buildD2N) in chunks of leaves
         const int nl = (int)P.leaves.size();
-        hipLaunchKernelGGL(bt_factor_kernel, dim3(nl), dim3(256), 0, s, M->leafdesc.p, N);
+        launch_bt_factor(M->leafdesc.p, N, nl, s);
         EF_LAUNCH_CHECK();
         const long long per = (long long)N * N * 4 * N;
         const long long chunk = std::max<long long>(1, std::min<long long>(nl, (512LL << 20) / (per * 8)));

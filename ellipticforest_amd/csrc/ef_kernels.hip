@@ -691,6 +691,139 @@ __global__ void __launch_bounds__(256) bt_factor_kernel(const LeafDesc* leaves, 
     }
 }
 
+// x of lane `src` in every lane (src uniform)
+__device__ inline double wave_read_lane(double x, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+
+// The same factorisation with one wave per leaf, N = NT in {8, 16, 32}: lane c holds column c
+// of [S_i | I] (2 NT columns, rows in registers).  Partial pivoting by pivot order instead of
+// row exchanges: step k takes the largest |S(r, k)| among the rows not yet used, divides that
+// row by it and removes column k from every other row, so no register is ever indexed by a
+// run-time value; the row that pivots at step k holds row k of the inverse at the end, and the
+// inverse goes to LDS in row order (for the next block's S) and from there to Sinv.  The stencil
+// of each block row is computed one cell per lane (as bt_stencil) into LDS and written out.
+template <int NT>
+__global__ void __launch_bounds__(256) bt_factor_wave_kernel(const LeafDesc* leaves, int nleaves) {
+    constexpr int NN = NT * NT;
+    __shared__ double inv_all[4][NT][NT + 1];
+    __shared__ double row_all[4][6][NT];  // cW, cE, cS, cN, cC of cells (i, r); cE of (i - 1, r)
+    __shared__ int pinv_all[4][NT];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int leaf = blockIdx.x * 4 + wave;
+    if (leaf >= nleaves) return;  // uniform per wave; no barrier below
+    double (*inv)[NT + 1] = inv_all[wave];
+    double (*srow)[NT] = row_all[wave];
+    int* pinv = pinv_all[wave];
+    const LeafDesc L = leaves[leaf];
+    const double dx2 = L.dx * L.dx, dy2 = L.dy * L.dy;
+    const auto* cf = gmem((const double*)L.coef);
+    auto* stw = gmem(L.stencil);
+    // stencil of cell I (FiniteVolumeSolver.cpp:64-85), as bt_stencil
+    auto stencil = [&](int I, double& cW, double& cE, double& cS, double& cN, double& cC) {
+        const double a = cf[I], bW = cf[NN + I], bE = cf[2 * NN + I], bS = cf[3 * NN + I], bN = cf[4 * NN + I];
+        const double lam = cf[5 * NN + I];
+        cW = a * bW / dx2; cE = a * bE / dx2; cS = a * bS / dy2; cN = a * bN / dy2;
+        cC = -a * ((bE + bW) / dx2 + (bN + bS) / dy2) + lam;
+    };
+    const int c = (lane < NT) ? lane : lane - NT;  // column of S (lane < NT) or of I
+    double col[NT];
+    for (int i = 0; i < NT; i++) {
+        // the stencil of block row i, one cell per lane (written out for the solves too), and cE
+        // of block row i - 1 for the coupling: into LDS, read below by every lane
+        if (lane < NT) {
+            double cW, cE, cS, cN, cC;
+            const int I = i * NT + lane;
+            stencil(I, cW, cE, cS, cN, cC);
+            srow[0][lane] = cW; srow[1][lane] = cE; srow[2][lane] = cS; srow[3][lane] = cN; srow[4][lane] = cC;
+            stw[I] = cW; stw[NN + I] = cE; stw[2 * NN + I] = cS; stw[3 * NN + I] = cN; stw[4 * NN + I] = cC;
+        } else if (lane < 2 * NT && i > 0) {
+            double cW, cE, cS, cN, cC;
+            stencil((i - 1) * NT + c, cW, cE, cS, cN, cC);
+            srow[5][c] = cE;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const double ce_prev = (i > 0) ? srow[5][c] : 0.0;
+#pragma unroll
+        for (int r = 0; r < NT; r++) {
+            const double cW = srow[0][r], cE = srow[1][r], cS = srow[2][r], cN = srow[3][r], cC = srow[4][r];
+            double v;
+            if (lane < NT) {
+                v = 0.0;
+                if (c == r) {
+                    v = cC;
+                    if (i == 0) v -= cW;
+                    if (i == NT - 1) v -= cE;
+                    if (r == 0) v -= cS;
+                    if (r == NT - 1) v -= cN;
+                } else if (c == r - 1) {
+                    v = cS;
+                } else if (c == r + 1) {
+                    v = cN;
+                }
+                if (i > 0) v -= cW * inv[r][c] * ce_prev;
+            } else {
+                v = (lane < 2 * NT && c == r) ? 1.0 : 0.0;
+            }
+            col[r] = v;
+        }
+        unsigned used = 0;
+        for (int k = 0; k < NT; k++) {
+            // pivot: the largest unused |S(r, k)|, found by lane k on its own column
+            double best = -1.0;
+            int bi = 0;
+#pragma unroll
+            for (int r = 0; r < NT; r++) {
+                const double a = fabs(col[r]);
+                const bool take = !((used >> r) & 1u) && a > best;
+                best = take ? a : best;
+                bi = take ? r : bi;
+            }
+            const int pr = __builtin_amdgcn_readlane(bi, k);
+            used |= 1u << pr;
+            double rowp = 0.0;
+#pragma unroll
+            for (int r = 0; r < NT; r++) rowp = (r == pr) ? col[r] : rowp;
+            const double pk = wave_read_lane(rowp, k);
+            const double ip = (pk != 0.0) ? 1.0 / pk : 0.0;
+            rowp *= ip;
+#pragma unroll
+            for (int r = 0; r < NT; r++) {
+                const double l = wave_read_lane(col[r], k);
+                col[r] = (r == pr) ? rowp : fma(-l, rowp, col[r]);
+            }
+            if (lane == 0) pinv[pr] = k;
+        }
+        // the inverse in row order: row pinv[r] of Sinv_i is row r of the right half
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < NT; r++)
+            if (lane >= NT && lane < 2 * NT) inv[pinv[r]][c] = col[r];
+        __builtin_amdgcn_wave_barrier();
+        auto* Si = gmem(L.Sinv) + (long long)i * NN;
+        for (int e = lane; e < NN; e += 64) Si[e] = inv[e / NT][e % NT];
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+template __global__ void bt_factor_wave_kernel<8>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<16>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<32>(const LeafDesc*, int);
+
+void launch_bt_factor(const LeafDesc* leaves, int Nn, int nleaves, hipStream_t s) {
+    const unsigned blocks = (unsigned)((nleaves + 3) / 4);
+    if (Nn == 8)
+        hipLaunchKernelGGL(bt_factor_wave_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves);
+    else if (Nn == 16)
+        hipLaunchKernelGGL(bt_factor_wave_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves);
+    else if (Nn == 32)
+        hipLaunchKernelGGL(bt_factor_wave_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves);
+    else
+        hipLaunchKernelGGL(bt_factor_kernel, dim3(nleaves), dim3(256), 0, s, leaves, Nn);
+}
+
 // Block substitution of one right-hand side b (N x N, in LDS) in place -> solution.
 __device__ inline void bt_substitute(const LeafDesc& L, int Nn, double* b, double* tmp) {
     const int tid = threadIdx.x;

@@ -365,7 +365,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
             L.dy = p->coef_h[i].dy;
         }
         p->leafdesc.upload(p->ld_h, p->stream);
-        hipLaunchKernelGGL(bt_factor_kernel, dim3(p->P), dim3(256), 0, p->stream, p->leafdesc.p, N);
+        launch_bt_factor(p->leafdesc.p, N, p->P, p->stream);
         EF_LAUNCH_CHECK();
         if (p->packed) {
             hipLaunchKernelGGL(bt_pack_kernel, dim3((unsigned)std::min<long long>(blocks_for(P * N * NP), 65535)),

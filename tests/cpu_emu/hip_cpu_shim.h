@@ -104,6 +104,8 @@ inline double __hiloint2double(int hi, int lo) {
     std::memcpy(&x, &u, 8);
     return x;
 }
+// v_readlane_b32: lane src's value in every lane (src uniform); 32-bit values are exact in a double
+inline int __builtin_amdgcn_readlane(int x, int src) { return (int)__shfl((double)x, src); }
 // v_permlane32_swap: lanes 32..63 of the first operand trade places with lanes 0..31 of the
 // second; element 0 of the result is the first operand afterwards, element 1 the second
 struct emu_u2 {
