@@ -6,6 +6,9 @@
 // and its particular part, printed for comparison with other implementations.
 //   fv_solver [--max-n=M]   (n = 8, 16, ..., M; default 32, the reference's test runs to 512)
 // Output lines: "err n linf", "order p", "T i j v" (a few entries), "h i v", "hp i v".
+//   fv_solver --five-point [--max-n=M]   the same with the five-point stencil and variable
+// coefficients alpha = 1, beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1 (M <= 32), the
+// manufactured solution u = sin(3x) cosh(y) and f = div(beta grad u) - u.
 #include <EllipticForest.hpp>
 #include <Patches/FiniteVolume/FiniteVolume.hpp>
 
@@ -15,8 +18,20 @@
 
 using namespace EllipticForest;
 
-static double uExact(double x, double y) { return std::sin(x) + std::sin(y); }
-static double fRHS(double x, double y) { return -uExact(x, y); }
+static bool fivePoint = false;
+static double uPoisson(double x, double y) { return std::sin(x) + std::sin(y); }
+static double uVar(double x, double y) { return std::sin(3.0 * x) * std::cosh(y); }
+static double betaVar(double x, double y) { return 1.0 + 0.5 * std::sin(2.0 * M_PI * x) * std::cos(2.0 * M_PI * y); }
+static double uExact(double x, double y) { return fivePoint ? uVar(x, y) : uPoisson(x, y); }
+// f = div(beta grad u) + lambda u = beta lap(u) + grad(beta) . grad(u) - u for the variable problem
+static double fRHS(double x, double y) {
+    if (!fivePoint) return -uPoisson(x, y);
+    const double ux = 3.0 * std::cos(3.0 * x) * std::cosh(y), uy = std::sin(3.0 * x) * std::sinh(y);
+    const double lap = -8.0 * std::sin(3.0 * x) * std::cosh(y);
+    const double bx = M_PI * std::cos(2.0 * M_PI * x) * std::cos(2.0 * M_PI * y);
+    const double by = -M_PI * std::sin(2.0 * M_PI * x) * std::sin(2.0 * M_PI * y);
+    return betaVar(x, y) * lap + bx * ux + by * uy - uVar(x, y);
+}
 
 static void problem(FiniteVolumeGrid& grid, int n, Vector<double>& g, Vector<double>& f, Vector<double>& ue) {
     g = Vector<double>(4 * n);
@@ -39,11 +54,17 @@ static void problem(FiniteVolumeGrid& grid, int n, Vector<double>& g, Vector<dou
 
 int main(int argc, char** argv) {
     EllipticForestApp app(&argc, &argv);
+    fivePoint = app.options.optionExists("five-point");
     FiniteVolumeSolver solver;
-    solver.solver_type = FiniteVolumeSolverType::FISHPACK90;
+    solver.solver_type = fivePoint ? FiniteVolumeSolverType::FivePointStencil : FiniteVolumeSolverType::FISHPACK90;
     solver.alpha_function = [](double, double) { return 1.0; };
-    solver.beta_function = [](double, double) { return 1.0; };
-    solver.lambda_function = [](double, double) { return 0.0; };
+    if (fivePoint) {
+        solver.beta_function = betaVar;
+        solver.lambda_function = [](double, double) { return -1.0; };
+    } else {
+        solver.beta_function = [](double, double) { return 1.0; };
+        solver.lambda_function = [](double, double) { return 0.0; };
+    }
 
     const int max_n = app.options.optionExists("max-n") ? std::get<int>(app.options["max-n"]) : 32;
     std::vector<int> ns;
