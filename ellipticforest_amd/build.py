@@ -51,7 +51,7 @@ def build(verbose=False):
     return LIB
 
 
-EXAMPLES = ["elliptic_single", "fv_solver", "thermal"]
+EXAMPLES = ["elliptic_single", "fv_solver", "thermal", "vtk_mesh"]
 
 
 def build_examples(lib_dir=None, out_dir=None, verbose=False):

@@ -18,7 +18,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <algorithm>
 #include <cstdio>
+#include <deque>
+#include <fstream>
 #include <functional>
 #include <map>
 #include <memory>
@@ -84,6 +87,21 @@ public:
     std::string& name() { return name_; }
     void append(const Vector<T>& v) { d_.insert(d_.end(), v.d_.begin(), v.d_.end()); }
     void append(T v) { d_.push_back(v); }
+    // VTK data-array view (Vector.hpp:670-757): type, name, components, ASCII data written with
+    // std::to_string and followed by a space, range from std::to_string of the extremes
+    void setType(std::string t) { vtkType_ = std::move(t); }
+    std::string getType() const { return vtkType_; }
+    std::string getName() const { return name_; }
+    void setNumberOfComponents(std::string n) { vtkComponents_ = std::move(n); }
+    std::string getNumberOfComponents() const { return vtkComponents_; }
+    std::string getFormat() const { return "ascii"; }
+    std::string getRangeMin() const { return std::to_string(d_.empty() ? T(0) : *std::min_element(d_.begin(), d_.end())); }
+    std::string getRangeMax() const { return std::to_string(d_.empty() ? T(0) : *std::max_element(d_.begin(), d_.end())); }
+    std::string getData() const {
+        std::string str;
+        for (const T& v : d_) str += std::to_string(v) + " ";
+        return str;
+    }
     Vector<T> getSegment(std::size_t start, std::size_t len) const {
         if (start + len > d_.size()) throw std::out_of_range("[EllipticForest::Vector::getSegment] out of range");
         return Vector<T>(std::vector<T>(d_.begin() + start, d_.begin() + start + len));
@@ -94,8 +112,56 @@ public:
     }
 private:
     std::vector<T> d_;
-    std::string name_;
+    std::string name_ = "Vector";
+    std::string vtkType_ = "Float32";
+    std::string vtkComponents_ = "1";
 };
+
+// ---------------------------------------------------------------------------
+// VTK output: an XML tree written as XMLTree.cpp:27-66 writes it (two spaces of indent per
+// level, attributes in insertion order, a node's data on its own line before its children).
+// ---------------------------------------------------------------------------
+namespace vtk {
+struct XmlNode {
+    std::string name;
+    std::vector<std::pair<std::string, std::string>> attributes;
+    std::string data;
+    std::vector<XmlNode> children;
+};
+inline void writeNode(std::ofstream& file, const XmlNode& node, const std::string& prefix) {
+    std::string header = prefix + "<" + node.name + " ";
+    for (auto& a : node.attributes) header += a.first + "=\"" + a.second + "\" ";
+    header.pop_back();
+    file << header << ">" << std::endl;
+    if (!node.data.empty()) file << node.data << std::endl;
+    for (auto& c : node.children) writeNode(file, c, prefix + "  ");
+    file << prefix << "</" << node.name << ">" << std::endl;
+}
+inline void writeTree(const std::string& filename, const XmlNode& root) {
+    std::ofstream file(filename);
+    if (!file) throw std::runtime_error("[EllipticForest::vtk] cannot open " + filename);
+    file << "<?xml version=\"1.0\"?>" << std::endl;
+    writeNode(file, root, "");
+}
+// DataArray / PDataArray nodes of a vector (VTK.cpp:5-24)
+template <typename T>
+inline XmlNode dataArray(const Vector<T>& v) {
+    return XmlNode{"DataArray",
+                   {{"type", v.getType()}, {"Name", v.getName()}, {"NumberOfComponents", v.getNumberOfComponents()},
+                    {"format", v.getFormat()}, {"RangeMin", v.getRangeMin()}, {"RangeMax", v.getRangeMax()}},
+                   v.getData(), {}};
+}
+template <typename T>
+inline XmlNode pDataArray(const Vector<T>& v) {
+    return XmlNode{"PDataArray",
+                   {{"type", v.getType()}, {"Name", v.getName()}, {"NumberOfComponents", v.getNumberOfComponents()},
+                    {"format", v.getFormat()}},
+                   "", {}};
+}
+inline XmlNode vtkFile() {
+    return XmlNode{"VTKFile", {{"type", "UnstructuredGrid"}, {"version", "0.1"}, {"byte_order", "LittleEndian"}}, "", {}};
+}
+}  // namespace vtk
 
 template <typename T>
 inline double vectorInfNorm(const Vector<T>& a, const Vector<T>& b) {
@@ -312,7 +378,145 @@ public:
     }
     int nLeaves() const { return (int)leaf_level.size(); }
 
+    // Mesh functions written as cell data by toVTK (Mesh.hpp:352-385): a vector over all leaf
+    // cells in iterateCells order, or a function sampled at the cell centres
+    void addMeshFunction(Vector<double>& meshFunction) { mesh_functions.push_back(&meshFunction); }
+    void addMeshFunction(std::function<double(double x, double y)> meshFunction, std::string meshFunctionName) {
+        owned_functions_.emplace_back();
+        Vector<double>& v = owned_functions_.back();
+        v.name() = meshFunctionName;
+        iterateCells([&](double xc, double yc) { v.append(meshFunction(xc, yc)); });
+        mesh_functions.push_back(&v);
+    }
+    void clearMeshFunctions() {
+        mesh_functions.clear();
+        owned_functions_.clear();
+    }
+
+    // Mesh.hpp:393-424: "<base>-mesh[-NNNN].pvtu" with "<base>-mesh[-NNNN]_0000.vtu" (every leaf
+    // cell as its own quadrilateral, the mesh functions as cell data; the same text as the
+    // reference's writer), and "<base>-quadtree[-NNNN].pvtu" with its _0000.vtu (one pixel per
+    // leaf patch: the geometry of p4est's quadtree file, written here in ASCII)
+    void toVTK(std::string baseFilename, int number = -1) {
+        auto numbered = [&](std::string f) {
+            if (number != -1) {
+                char buffer[32];
+                std::snprintf(buffer, 32, "%04i", number);
+                f += "-" + std::string(buffer);
+            }
+            return f;
+        };
+        writeMeshVTK(numbered(baseFilename + "-mesh"));
+        writeQuadtreeVTK(numbered(baseFilename + "-quadtree"));
+    }
+
+    std::vector<Vector<double>*> mesh_functions;
+
 private:
+    std::deque<Vector<double>> owned_functions_;
+
+    // points, connectivity, offsets and types as Mesh.hpp:186-265 builds them: leaves in
+    // pre-order, cells i-major, four corners per cell counter-clockwise from (x_i, y_j)
+    void writeMeshVTK(const std::string& name) {
+        Vector<double> points;
+        points.name() = "points";
+        points.setNumberOfComponents("3");
+        Vector<int> connectivity, offsets, types;
+        connectivity.name() = "connectivity";
+        connectivity.setType("Int64");
+        offsets.name() = "offsets";
+        offsets.setType("Int64");
+        types.name() = "types";
+        types.setType("Int64");
+        int index = 0, npoints = 0, ncells = 0;
+        for (auto& nd : quadtree.nodes) {
+            if (!nd.leaf) continue;
+            auto& g = nd.data.grid();
+            const double xLower = g.xLower(), yLower = g.yLower(), dx = g.dx(), dy = g.dy();
+            const int nx = (int)g.nx(), ny = (int)g.ny();
+            for (int i = 0; i < nx; i++)
+                for (int j = 0; j < ny; j++) {
+                    const double x0 = xLower + i * dx, y0 = yLower + j * dy;
+                    const double x1 = xLower + (i + 1) * dx, y1 = yLower + (j + 1) * dy;
+                    points.append(Vector<double>{x0, y0, 0.0, x1, y0, 0.0, x1, y1, 0.0, x0, y1, 0.0});
+                    connectivity.append(Vector<int>{index, index + 1, index + 2, index + 3});
+                    offsets.append(Vector<int>{(index + 1) * 4, (index + 2) * 4, (index + 3) * 4, (index + 4) * 4});
+                    types.append(Vector<int>{9, 9, 9, 9});
+                    index += 4;
+                    npoints += 4;
+                    ncells += 1;
+                }
+        }
+        std::string names;
+        for (auto* f : mesh_functions) names += f->getName() + " ";
+        // the piece (VTK.cpp:160-207)
+        vtk::XmlNode vtu = vtk::vtkFile();
+        vtk::XmlNode piece{"Piece", {{"NumberOfPoints", std::to_string(npoints)}, {"NumberOfCells", std::to_string(ncells)}}, "", {}};
+        piece.children.push_back(vtk::XmlNode{"Points", {}, "", {vtk::dataArray(points)}});
+        piece.children.push_back(vtk::XmlNode{"Cells", {}, "", {vtk::dataArray(connectivity), vtk::dataArray(offsets), vtk::dataArray(types)}});
+        if (!mesh_functions.empty()) {
+            vtk::XmlNode cd{"CellData", {{"Scalars", names}}, "", {}};
+            for (auto* f : mesh_functions) cd.children.push_back(vtk::dataArray(*f));
+            piece.children.push_back(cd);
+        }
+        vtu.children.push_back(vtk::XmlNode{"UnstructuredGrid", {}, "", {piece}});
+        char buffer[256];
+        std::snprintf(buffer, 256, (name + "_%04i.vtu").c_str(), 0);
+        const std::string source(buffer);
+        vtk::writeTree(source, vtu);
+        // the parallel header (VTK.cpp:239-300), one piece per rank
+        vtk::XmlNode pvtu = vtk::vtkFile();
+        vtk::XmlNode pgrid{"PUnstructuredGrid", {{"GhostLevel", "0"}}, "", {}};
+        pgrid.children.push_back(vtk::XmlNode{"PPoints", {}, "", {vtk::pDataArray(points)}});
+        pgrid.children.push_back(vtk::XmlNode{"PPointData", {}, "", {}});
+        vtk::XmlNode pcd{"PCellData", {}, "", {}};
+        if (!mesh_functions.empty()) pcd.attributes.push_back({"Scalars", names});
+        for (auto* f : mesh_functions) pcd.children.push_back(vtk::pDataArray(*f));
+        pgrid.children.push_back(pcd);
+        pgrid.children.push_back(vtk::XmlNode{"Piece", {{"Source", source}}, "", {}});
+        pvtu.children.push_back(pgrid);
+        vtk::writeTree(name + ".pvtu", pvtu);
+    }
+
+    // one VTK_PIXEL (type 8, corners in z-order) per leaf patch, as p4est_vtk_write_header writes
+    // the quadrants with scale 1
+    void writeQuadtreeVTK(const std::string& name) {
+        Vector<double> points;
+        points.name() = "Position";
+        points.setNumberOfComponents("3");
+        Vector<int> connectivity, offsets, types;
+        connectivity.name() = "connectivity";
+        connectivity.setType("Int32");
+        offsets.name() = "offsets";
+        offsets.setType("Int32");
+        types.name() = "types";
+        types.setType("UInt8");
+        int k = 0;
+        for (auto& nd : quadtree.nodes) {
+            if (!nd.leaf) continue;
+            auto& g = nd.data.grid();
+            const double xl = g.xLower(), xu = g.xUpper(), yl = g.yLower(), yu = g.yUpper();
+            points.append(Vector<double>{xl, yl, 0.0, xu, yl, 0.0, xl, yu, 0.0, xu, yu, 0.0});
+            connectivity.append(Vector<int>{4 * k, 4 * k + 1, 4 * k + 2, 4 * k + 3});
+            offsets.append(4 * (k + 1));
+            types.append(8);
+            k++;
+        }
+        vtk::XmlNode vtu = vtk::vtkFile();
+        vtk::XmlNode piece{"Piece", {{"NumberOfPoints", std::to_string(4 * k)}, {"NumberOfCells", std::to_string(k)}}, "", {}};
+        piece.children.push_back(vtk::XmlNode{"Points", {}, "", {vtk::dataArray(points)}});
+        piece.children.push_back(vtk::XmlNode{"Cells", {}, "", {vtk::dataArray(connectivity), vtk::dataArray(offsets), vtk::dataArray(types)}});
+        vtu.children.push_back(vtk::XmlNode{"UnstructuredGrid", {}, "", {piece}});
+        const std::string source = name + "_0000.vtu";
+        vtk::writeTree(source, vtu);
+        vtk::XmlNode pvtu = vtk::vtkFile();
+        vtk::XmlNode pgrid{"PUnstructuredGrid", {{"GhostLevel", "0"}}, "", {}};
+        pgrid.children.push_back(vtk::XmlNode{"PPoints", {}, "", {vtk::pDataArray(points)}});
+        pgrid.children.push_back(vtk::XmlNode{"Piece", {{"Source", source}}, "", {}});
+        pvtu.children.push_back(pgrid);
+        vtk::writeTree(name + ".pvtu", pvtu);
+    }
+
     void zorderUniform(int l, int x, int y, int level) {
         if (l == level) { leaf_level.push_back(l); leaf_ix.push_back(x); leaf_iy.push_back(y); return; }
         for (int c = 0; c < 4; c++) zorderUniform(l + 1, 2 * x + (c & 1), 2 * y + (c >> 1), level);

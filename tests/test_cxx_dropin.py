@@ -234,3 +234,121 @@ def check_thermal_output(out):
 def test_dropin_thermal_time_stepping(example):
     """Time stepping that reads the previous vectorU in the upwards callback (thermal shape)."""
     check_thermal_output(run_raw(example, "thermal", "--level=1"))
+
+
+# ----- VTK output (Mesh::toVTK, Mesh.hpp:393-424; VTK.cpp; XMLTree.cpp) --------------------
+def _leaf_bounds_by_halving(leaves, xl, xu, yl, yu):
+    """Patch bounds as the node factory makes them: halves at the midpoint, level by level."""
+    out = []
+    for (lv, ix, iy) in leaves:
+        a, b, c, d = xl, xu, yl, yu
+        for k in range(lv - 1, -1, -1):
+            xm, ym = (a + b) / 2.0, (c + d) / 2.0
+            a, b = (xm, b) if (ix >> k) & 1 else (a, xm)
+            c, d = (ym, d) if (iy >> k) & 1 else (c, ym)
+        out.append((a, b, c, d))
+    return out
+
+
+def _xml(node, prefix=""):
+    """XMLTree::write_ (XMLTree.cpp:44-66) of (name, [(key, value)], data, [children])."""
+    name, attrs, data, children = node
+    head = prefix + "<" + name + " " + "".join(f'{k}="{v}" ' for k, v in attrs)
+    s = head[:-1] + ">\n"
+    if data:
+        s += data + "\n"
+    for c in children:
+        s += _xml(c, prefix + "  ")
+    return s + prefix + "</" + name + ">\n"
+
+
+def _array(name, values, typ="Float32", comps="1", fmt="%f"):
+    vals = list(values)
+    return ("DataArray", [("type", typ), ("Name", name), ("NumberOfComponents", comps), ("format", "ascii"),
+                          ("RangeMin", fmt % min(vals)), ("RangeMax", fmt % max(vals))],
+            "".join(fmt % v + " " for v in vals), [])
+
+
+def _parray(name, typ="Float32", comps="1"):
+    return ("PDataArray", [("type", typ), ("Name", name), ("NumberOfComponents", comps), ("format", "ascii")], "", [])
+
+
+def expected_mesh_vtk(base, leaves, nx, functions):
+    """The reference writer's mesh files (Mesh.hpp:186-265 cells, VTK.cpp:160-300 layout)."""
+    pts, conn, offs, types = [], [], [], []
+    index = 0
+    for (a, b, c, d) in _leaf_bounds_by_halving(leaves, -10.0, 10.0, -10.0, 10.0):
+        dx, dy = (b - a) / nx, (d - c) / nx
+        for i in range(nx):
+            for j in range(nx):
+                x0, y0, x1, y1 = a + i * dx, c + j * dy, a + (i + 1) * dx, c + (j + 1) * dy
+                pts += [x0, y0, 0.0, x1, y0, 0.0, x1, y1, 0.0, x0, y1, 0.0]
+                conn += [index, index + 1, index + 2, index + 3]
+                offs += [(index + 1) * 4, (index + 2) * 4, (index + 3) * 4, (index + 4) * 4]
+                types += [9, 9, 9, 9]
+                index += 4
+    ncells = index // 4
+    names = "".join(n + " " for n, _ in functions)
+    piece = ("Piece", [("NumberOfPoints", str(4 * ncells)), ("NumberOfCells", str(ncells))], "", [
+        ("Points", [], "", [_array("points", pts, comps="3")]),
+        ("Cells", [], "", [_array("connectivity", conn, "Int64", fmt="%d"), _array("offsets", offs, "Int64", fmt="%d"),
+                           _array("types", types, "Int64", fmt="%d")]),
+        ("CellData", [("Scalars", names)], "", [_array(n, v) for n, v in functions])])
+    root = [("type", "UnstructuredGrid"), ("version", "0.1"), ("byte_order", "LittleEndian")]
+    vtu = '<?xml version="1.0"?>\n' + _xml(("VTKFile", root, "", [("UnstructuredGrid", [], "", [piece])]))
+    pvtu = '<?xml version="1.0"?>\n' + _xml(("VTKFile", root, "", [("PUnstructuredGrid", [("GhostLevel", "0")], "", [
+        ("PPoints", [], "", [_parray("points", comps="3")]), ("PPointData", [], "", []),
+        ("PCellData", [("Scalars", names)], "", [_parray(n) for n, _ in functions]),
+        ("Piece", [("Source", base + "-mesh_0000.vtu")], "", [])])]))
+    return vtu, pvtu, ncells
+
+
+def test_dropin_vtk_output(example, tmp_path):
+    """Mesh::toVTK through the drop-in header: the mesh files match, text for text, what the
+    reference's writer produces for the same mesh and mesh functions (the function sampled with
+    sin is compared to 2e-6, since C++ and numpy may differ in the last bit before printing);
+    the quadtree files hold one pixel per leaf patch."""
+    import re
+    import xml.etree.ElementTree as ET
+    nx, max_level = 4, 3
+    base = str(tmp_path / "ef")
+    out = run_raw(example, "vtk_mesh", f"--out={base}", f"--nx={nx}", f"--max-level={max_level}")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, nx, nx, -10, 10, -10, 10, 0, max_level)
+    bounds = _leaf_bounds_by_halving(leaves, -10.0, 10.0, -10.0, 10.0)
+    xc, fv, xy = [], [], []
+    for (a, b, c, d) in bounds:
+        dx, dy = (b - a) / nx, (d - c) / nx
+        for i in range(nx):
+            for j in range(nx):
+                x, y = (a + dx / 2) + i * dx, (c + dy / 2) + j * dy
+                xc.append(x)
+                fv.append(float(F(x, y)))
+                xy.append(x * y)
+    vtu, pvtu, ncells = expected_mesh_vtk(base, leaves, nx, [("x_centre", xc), ("f_rhs", fv), ("xy", xy)])
+    assert f"leaves {len(leaves)}" in out and f"cells {ncells}" in out
+    for suffix in ("", "-0007"):
+        got_pvtu = open(base + f"-mesh{suffix}.pvtu").read()
+        assert got_pvtu == pvtu.replace(base + "-mesh_0000.vtu", base + f"-mesh{suffix}_0000.vtu")
+        got = open(base + f"-mesh{suffix}_0000.vtu").read().split("\n")
+        exp = vtu.split("\n")
+        assert len(got) == len(exp)
+        f_at = [k for k, l in enumerate(exp) if 'Name="f_rhs"' in l][0]
+        for k, (g, e) in enumerate(zip(got, exp)):
+            if k == f_at:  # RangeMin / RangeMax of the sin-sampled function
+                assert re.sub(r'Range(Min|Max)="[^"]*"', "", g) == re.sub(r'Range(Min|Max)="[^"]*"', "", e)
+            elif k == f_at + 1:
+                gv, ev = np.array(g.split(), float), np.array(e.split(), float)
+                assert len(gv) == ncells and np.abs(gv - ev).max() <= 2e-6
+                assert all(re.fullmatch(r"-?\d+\.\d{6}", t) for t in g.split())
+            else:
+                assert g == e, (k, g[:120], e[:120])
+        # quadtree: one VTK_PIXEL per leaf patch with its four corners
+        q = ET.parse(base + f"-quadtree{suffix}_0000.vtu").getroot()
+        piece = q.find("UnstructuredGrid/Piece")
+        assert piece.get("NumberOfCells") == str(len(leaves))
+        P = np.array(piece.find("Points/DataArray").text.split(), float).reshape(-1, 4, 3)
+        for (a, b, c, d), corners in zip(bounds, P):
+            assert np.allclose(corners[:, :2], [[a, c], [b, c], [a, d], [b, d]], atol=1e-6)
+        assert set(piece.find("Cells/DataArray[@Name='types']").text.split()) == {"8"}
+        assert ET.parse(base + f"-quadtree{suffix}.pvtu").getroot().find("PUnstructuredGrid/Piece").get(
+            "Source") == base + f"-quadtree{suffix}_0000.vtu"
