@@ -32,11 +32,11 @@ F = lambda x, y: -(np.sin(x) + np.sin(y))
 BOUNDS = (-10.0, 10.0, -10.0, 10.0)
 METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds"
 # HBM bytes per launch measured by two rocprofv3 counter passes (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, factors from the copy kernel); profiles/patch_fd_kernel_cfg2_leafset.md and
-# profiles/bt_solve_sym_kernel_cfg3_leafset.md have the commands and the raw values.
+# WRITE_SIZE x 1.0, factors from the copy kernel); profiles/patch_fd_kernel_adaptive_poisson_leafset.md and
+# profiles/bt_solve_sym_kernel_variable_beta_leafset.md have the commands and the raw values.
 TRAFFIC_PER_LAUNCH = 573.4e6          # leaf solves of this workload, 16 right-hand sides
 TRAFFIC_MAP_D2N = 319.1e6             # particular Neumann data, same leaf set and k
-TRAFFIC_CFG3 = 1329.8e6               # cfg3 leaf solves, packed symmetric blocks
+TRAFFIC_VAR_BETA = 1329.8e6           # variable-beta leaf solves, packed symmetric blocks
 
 
 def log(msg):
@@ -237,9 +237,9 @@ def gpu_particular_run(bounds, N, k, steps, warmup):
     return dict(ms=ms, value=P * k / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9)
 
 
-def cfg3_cells(bounds, N):
+def var_beta_cells(bounds, N):
     """alpha, beta at the four faces and lambda per cell (FiniteVolumeSolver.cpp:64-78) for the
-    cfg3 coefficients alpha = 1, beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1."""
+    variable-beta coefficients alpha = 1, beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1."""
     beta = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
     out = np.empty((len(bounds), 6, N * N))
     for p, (xl, xu, yl, yu) in enumerate(bounds):
@@ -252,7 +252,7 @@ def cfg3_cells(bounds, N):
 
 
 def gpu_variable_run(steps, warmup, level=6, N=32):
-    """Leaf solves of the cfg3 leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches,
+    """Leaf solves of the variable-beta leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches,
     five-point stencil with variable beta), one right-hand side per launch."""
     import ellipticforest_amd as ef
     ctx = ef.Context(0)
@@ -260,7 +260,7 @@ def gpu_variable_run(steps, warmup, level=6, N=32):
     bounds = np.array([(-1 + i * w, -1 + (i + 1) * w, -1 + j * w, -1 + (j + 1) * w)
                        for i in range(1 << level) for j in range(1 << level)])
     P = len(bounds)
-    pt = ef.Patches(ctx, bounds, N, solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=cfg3_cells(bounds, N))
+    pt = ef.Patches(ctx, bounds, N, solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=var_beta_cells(bounds, N))
     g_h, f_h = patch_inputs(P, N, 1)
     g, f, u = ef.DeviceArray(ctx, g_h.shape), ef.DeviceArray(ctx, f_h.shape), ef.DeviceArray(ctx, f_h.shape)
     g.upload(g_h)
@@ -282,7 +282,7 @@ def gpu_variable_run(steps, warmup, level=6, N=32):
 
 def traffic_probe(bounds, N, k, copy_doubles=32 << 20, level=6, NV=32):
     """Launches for the byte counters: 5 leaf solves and 5 particular-Neumann-data maps of the
-    leaf set, 5 leaf solves of the cfg3 leaf set, then 3 plain copies of known size."""
+    leaf set, 5 leaf solves of the variable-beta leaf set, then 3 plain copies of known size."""
     import ellipticforest_amd as ef
     from ellipticforest_amd import _lib
     ctx = ef.Context(0)
@@ -301,7 +301,7 @@ def traffic_probe(bounds, N, k, copy_doubles=32 << 20, level=6, NV=32):
     w = 2.0 / (1 << level)
     vb = np.array([(-1 + i * w, -1 + (i + 1) * w, -1 + j * w, -1 + (j + 1) * w)
                    for i in range(1 << level) for j in range(1 << level)])
-    pv = ef.Patches(ctx, vb, NV, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cfg3_cells(vb, NV))
+    pv = ef.Patches(ctx, vb, NV, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=var_beta_cells(vb, NV))
     vg_h, vf_h = patch_inputs(len(vb), NV, 1)
     vg, vf, vu = ef.DeviceArray(ctx, vg_h.shape), ef.DeviceArray(ctx, vf_h.shape), ef.DeviceArray(ctx, vf_h.shape)
     vg.upload(vg_h)
@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--patch-nrhs", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
-                    help="only 5 leaf-solve, 5 particular-Neumann-data and 5 cfg3 leaf-solve launches and 3 "
+                    help="only 5 leaf-solve, 5 particular-Neumann-data and 5 variable-beta leaf-solve launches and 3 "
                          "device-to-device copies of 256 MiB on the library's copy kernel (the calibration of "
                          "HBM byte counters), then exit")
     args = ap.parse_args()
@@ -413,13 +413,13 @@ def main():
             patches=len(bounds), nrhs=k, achieved_gbs=pp["gbs"], algorithmic_bytes_per_launch=pp["bytes"],
             traffic=TRAFFIC_MAP_D2N if k == 16 else None, frac_of_peak=pp["gbs"] / 8000.0))
         pv = gpu_variable_run(args.steps, args.warmup)
-        log(f"cfg3 variable-coefficient leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s")
+        log(f"variable-beta leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s")
         out["components"].append(dict(
-            name="cfg3 leaf solves (five-point, variable beta: 4,096 patches, N = 32, 1 right-hand side)",
+            name="variable-beta leaf solves (five-point, [-1,1]^2 uniform level 6: 4,096 patches, N = 32, 1 right-hand side)",
             kernel="bt_solve_sym_kernel<32>" if pv["packed"] else "bt_solve_wave_kernel<32>", value=pv["value"],
             unit="leaf solves/s", ms_per_launch=pv["ms"], patches=pv["patches"], nrhs=1,
             factor_storage="packed symmetric" if pv["packed"] else "full", achieved_gbs=pv["gbs"],
-            algorithmic_bytes_per_launch=pv["bytes"], traffic=TRAFFIC_CFG3 if pv["packed"] else None,
+            algorithmic_bytes_per_launch=pv["bytes"], traffic=TRAFFIC_VAR_BETA if pv["packed"] else None,
             frac_of_peak=pv["gbs"] / 8000.0, dtn_wall_ms_incl_copy_to_host=pv["dtn_wall_ms"]))
     except Exception as e:
         out["components_error"] = f"{type(e).__name__}: {e}"

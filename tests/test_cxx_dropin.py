@@ -177,7 +177,7 @@ def test_dropin_fv_solver(example):
 
 
 def test_dropin_fv_solver_five_point(example):
-    """The same operations with the five-point stencil and variable beta (cfg3's coefficients)."""
+    """The same operations with the five-point stencil and variable beta (beta = 1 + 0.5 sin 2pi x cos 2pi y, lambda = -1)."""
     check_fv_solver_output(run_raw(example, "fv_solver", "--five-point", "--max-n=16"), max_n=16, five_point=True)
 
 

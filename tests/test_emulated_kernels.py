@@ -46,7 +46,7 @@ def test_emu_homogeneous_rhs(emu_ctx):
 
 
 def test_emu_variable_coefficients(emu_ctx):
-    """Block-tridiagonal leaves (bt_factor / bt_dtn / bt_solve) with cfg3's beta and lambda."""
+    """Block-tridiagonal leaves (bt_factor / bt_dtn / bt_solve) with beta = 1 + 0.5 sin 2pi x cos 2pi y, lambda = -1."""
     uo, ug, err = gp.variable_both(emu_ctx, omesh.uniform(1), 8, check_ops=True)
     assert gp.rel(ug, uo) < gp.TOL_U
 

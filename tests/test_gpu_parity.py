@@ -76,9 +76,9 @@ def test_uniform_equals_global_solve(ctx, name):
     assert rel(u, d["u"]) < 1e-11
 
 
-def test_cfg1_operators_and_error(ctx):
+def test_uniform_level3_operators_and_error(ctx):
     """[-10,10]^2, uniform level 3, 16x16 cells per patch, FISHPACK Poisson."""
-    say("cfg1: uniform L3 N16 [-10,10]^2, operators of every node")
+    say("uniform L3 N16 [-10,10]^2, operators of every node")
     O, D, uo, ug = solve_both(ctx, omesh.uniform(3), 16, (-10, 10, -10, 10), check_ops=True)
     assert rel(ug, uo) < TOL_U
     eo = eg = 0.0
@@ -178,10 +178,10 @@ def test_robin_root_condition(ctx):
     assert rel(D.solve(r, a, b), uo) < TOL_U
 
 
-def test_cfg2_full_size(ctx):
-    """cfg2 at full size: 7,750 leaves on levels 5..7, 16x16 cells, against the oracle."""
+def test_adaptive_poisson_full_size(ctx):
+    """The adaptive Poisson mesh at full size (|f| > 1.2, levels 0..7): 7,750 leaves, 16x16 cells, against the oracle."""
     import ellipticforest_amd as ef
-    say("cfg2 full size: |f|>1.2 levels 0..7, N=16, 7750 leaves")
+    say("adaptive Poisson full size: |f|>1.2 levels 0..7, N=16, 7750 leaves")
     oracle.set_threads(min(16, os.cpu_count() or 1))
     leaves = ef.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 16, 16, (-10, 10, -10, 10), 0, 7)
     assert len(leaves) == 7750
@@ -189,7 +189,7 @@ def test_cfg2_full_size(ctx):
     assert rel(ug, uo) < TOL_U
 
 
-# ----- variable coefficients (cfg3's operator at reduced sizes) -----------------------
+# ----- variable coefficients (the variable-beta operator at reduced sizes) ------------
 ALPHA = lambda x, y: np.ones_like(x)
 BETA = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
 LAMBDA = lambda x, y: -np.ones_like(x)
@@ -235,9 +235,9 @@ def test_variable_coefficients_uniform(ctx):
     assert rel(ug, uo) < TOL_U
 
 
-def test_variable_coefficients_cfg3_shape(ctx):
-    """cfg3 operator (N = 32 patches) at level 3: parity and second-order error."""
-    say("variable beta cfg3 shape: uniform level 3, N=32")
+def test_variable_coefficients_N32_shape(ctx):
+    """The variable-beta operator on N = 32 patches at level 3: parity and second-order error."""
+    say("variable beta: uniform level 3, N=32")
     uo, ug, err = variable_both(ctx, omesh.uniform(3), 32)
     assert rel(ug, uo) < TOL_U
     assert err < 2e-4

@@ -173,7 +173,7 @@ def test_robin_root_condition_dirichlet_limit():
     assert np.abs(u_r - u_d).max() < 1e-10
 
 
-def test_mesh_cfg2_leaf_counts():
+def test_mesh_adaptive_poisson_leaf_counts():
     """refineByFunction(|f| > 1.2, 0..7) with corner balance: 262/1568/5920 leaves on levels 5/6/7."""
     leaves = mesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 16, 16, -10, 10, -10, 10, 0, 7)
     levels = np.bincount([l for l, _, _ in leaves])
@@ -188,7 +188,7 @@ def test_mesh_cfg2_leaf_counts():
 
 
 def test_variable_coefficient_second_order():
-    """cfg3's operator (beta = 1 + 0.5 sin 2pi x cos 2pi y, lambda = -1) with the manufactured
+    """The variable-beta operator (beta = 1 + 0.5 sin 2pi x cos 2pi y, lambda = -1) with the manufactured
     u = sin(3x) cosh(y): the five-point HPS solution converges at second order."""
     import test_gpu_parity as gp
     errs = []

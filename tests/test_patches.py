@@ -8,7 +8,7 @@ on u, h and T against the oracle for every patch and right-hand side.
 
 The emulator cases run on the CPU (the HIP sources compiled against tests/cpu_emu);
 the gpu cases call the product library through the C ABI on an MI355X, at the oracle's
-sizes and at the full leaf set of the cfg2 mesh (7,750 patches, N = 16), where
+sizes and at the full leaf set of the adaptive Poisson mesh (|f| > 1.2, levels 0..7: 7,750 patches, N = 16), where
 size-independent properties are checked: linearity in (g, f), agreement of mapD2N with
 T g + particular data, and the exact Neumann data of a linear solution.
 """
@@ -147,7 +147,7 @@ def test_gpu_five_point_constant_vs_oracle(patch_ctx):
     check_against_oracle(pt, bounds, N, rng, coeffs=const)
 
 
-def cfg2_leaf_bounds():
+def adaptive_poisson_leaf_bounds():
     leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 16, 16, -10, 10, -10, 10, 0, 7)
     out = []
     for lv, ix, iy in leaves:
@@ -157,14 +157,14 @@ def cfg2_leaf_bounds():
 
 
 @pytest.mark.gpu
-def test_gpu_cfg2_leaf_set_properties(patch_ctx):
-    """All 7,750 leaf patches of the cfg2 mesh, 4 right-hand sides: a sample of patches
+def test_gpu_adaptive_poisson_leaf_set_properties(patch_ctx):
+    """All 7,750 leaf patches of the adaptive Poisson mesh, 4 right-hand sides: a sample of patches
     against the oracle, linearity, mapD2N = T g + particular data, and the exact Neumann
     data of u = 3x - 2y + 1 (the 5-point scheme is exact on linear functions)."""
     from ellipticforest_amd import _lib
-    bounds = cfg2_leaf_bounds()
+    bounds = adaptive_poisson_leaf_bounds()
     P, N, k = len(bounds), 16, 4
-    say(f"cfg2 leaf set: {P} patches, N={N}, {k} rhs")
+    say(f"adaptive Poisson leaf set: {P} patches, N={N}, {k} rhs")
     assert P == 7750
     pt = _lib.Patches(patch_ctx, bounds, N)
     rng = np.random.default_rng(11)
@@ -246,7 +246,7 @@ def test_emu_variable_patches_N32(emu_ctx):
 @pytest.mark.parametrize("N", [4, 8, 12, 16, 24, 32])
 @pytest.mark.parametrize("symmetric", [True, False])
 def test_gpu_variable_patches_vs_oracle(patch_ctx, N, symmetric):
-    """Five-point stencil (FiniteVolumeSolver.cpp:64-184), block-tridiagonal path: cfg3's beta
+    """Five-point stencil (FiniteVolumeSolver.cpp:64-184), block-tridiagonal path: the variable beta
     (symmetric matrix, packed inverse pivot blocks) and a variable alpha (full blocks); N in
     {8, 16, 32} on the wave kernels, other sides on the workgroup-per-patch kernel."""
     from ellipticforest_amd import _lib
@@ -260,15 +260,15 @@ def test_gpu_variable_patches_vs_oracle(patch_ctx, N, symmetric):
 
 
 @pytest.mark.gpu
-def test_gpu_cfg3_leaf_set_properties(patch_ctx):
-    """The cfg3 leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches) with cfg3's
+def test_gpu_variable_beta_leaf_set_properties(patch_ctx):
+    """The variable-beta leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches) with its
     coefficients: oracle on a sample, linearity, and mapD2N = T g + particular data."""
     from ellipticforest_amd import _lib
     N, lv = 32, 6
     w = 2.0 / (1 << lv)
     bounds = np.array([(-1 + i * w, -1 + (i + 1) * w, -1 + j * w, -1 + (j + 1) * w)
                        for i in range(1 << lv) for j in range(1 << lv)])
-    say(f"cfg3 leaf set: {len(bounds)} patches, N={N}, variable beta, lambda=-1")
+    say(f"variable-beta leaf set: {len(bounds)} patches, N={N}, variable beta, lambda=-1")
     cell = variable_cells(bounds, N)
     pv = _lib.Patches(patch_ctx, bounds, N, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
     assert pv.variable and pv.packed
