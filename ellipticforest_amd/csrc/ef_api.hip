@@ -215,7 +215,8 @@ struct ef_mesh_s {
     DevBuf<double> QA, QB, mu, dinv, leafT;  // fast-diagonalisation tables, leaf DtN
     // variable-coefficient leaves (block-tridiagonal path)
     bool variable = false;
-    DevBuf<double> vcoef, vstencil, vSinv, vwork;
+    bool packed = false;                   // variable and symmetric: packed inverse pivot blocks
+    DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork, vz;
     // operator arenas
     DevBuf<double> Tar, Sar, Xiar, Har;
     // work
@@ -338,6 +339,10 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
         efk::copy_h2d(M->vcoef.p, c.cell, sizeof(double) * nl * 6 * NN, M->stream);
         M->vstencil.alloc(nl * 5 * NN);
         M->vSinv.alloc(nl * (size_t)N * NN);
+        M->packed = (N == 8 || N == 16 || N == 32);
+        for (size_t li = 0; li < nl && M->packed; li++) M->packed = efk::cells_symmetric(c.cell + li * 6 * NN, N);
+        const long long NP = bt_packed_entries(N);
+        if (M->packed) M->vSinvP.alloc(nl * (size_t)N * NP);
         M->leafT.alloc(nl * 16 * NN);
         std::vector<LeafDesc> ld(nl);
         for (size_t li = 0; li < nl; li++) {
@@ -351,6 +356,8 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
             L.stencil = M->vstencil.p + li * 5 * NN;
             L.Sinv = M->vSinv.p + li * (size_t)N * NN;
             L.T = M->leafT.p + li * 16 * NN;
+            L.SinvP = M->packed ? M->vSinvP.p + li * NP : nullptr;  // block-major: block b at b * nl * NP
+            L.pstride = (long long)nl * NP;
             L.dx = (nd.xu - nd.xl) / N;
             L.dy = (nd.yu - nd.yl) / N;
         }
@@ -545,6 +552,11 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
         const int nl = (int)P.leaves.size();
         launch_bt_factor(M->leafdesc.p, N, nl, s);
         EF_LAUNCH_CHECK();
+        if (M->packed) {
+            hipLaunchKernelGGL(bt_pack_kernel, dim3((unsigned)std::min<long long>(blocks_for((long long)nl * N * bt_packed_entries(N)), 65535)),
+                               dim3(256), 0, s, M->leafdesc.p, nl, N);
+            EF_LAUNCH_CHECK();
+        }
         const long long per = (long long)N * N * 4 * N;
         const long long chunk = std::max<long long>(1, std::min<long long>(nl, (512LL << 20) / (per * 8)));
         M->vwork.ensure((size_t)(chunk * per));
@@ -727,7 +739,9 @@ void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u,
     const int N = M->P.N;
     const int nl = (int)M->P.leaves.size();
     if (M->variable) {
-        launch_bt_solve(M->leafdesc.p, N, nl, k, mode, f, f_zero, (const double*)M->vec.p, M->vec.p, u, M->stream);
+        if (M->packed && mode != 1) M->vz.ensure((size_t)nl * k * N * N);
+        launch_bt_solve(M->leafdesc.p, N, nl, k, mode, f, f_zero, (const double*)M->vec.p, M->vec.p, u, M->stream,
+                        M->packed, M->vz.p);
         EF_LAUNCH_CHECK();
     } else {
         // mode 0: particular Neumann data (g = 0) into the h rows of every leaf;

@@ -29,6 +29,9 @@ void patch_fd_dinv(int N, int NB, const std::vector<double>& mu, const std::vect
 void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStream_t s);
 // Workgroup cap of patch launches (EF_PATCH_MAX_BLOCKS or the default).
 long long patch_default_blocks();
+// True when the five-point matrix of a patch with per-cell coefficients c ([6][N*N]) is
+// symmetric to 1e-14 relative (then its inverse pivot blocks can be kept packed).
+bool cells_symmetric(const double* c, int N);
 }
 
 // A context is one GPU.

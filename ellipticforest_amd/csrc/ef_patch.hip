@@ -59,6 +59,24 @@ struct ef_patches_s {
 
 namespace efk {
 
+// True when the five-point matrix of the patch (FiniteVolumeSolver.cpp:64-85, cell I = j + i N
+// with i along x) is symmetric to 1e-14 relative: across every interior face the coupling
+// alpha beta / h^2 is the same seen from both cells.  The physical-side terms change the
+// diagonal only.
+bool cells_symmetric(const double* c, int N) {
+    const long long NN = (long long)N * N;
+    const double *a = c, *bW = c + NN, *bE = c + 2 * NN, *bS = c + 3 * NN, *bN = c + 4 * NN;
+    auto close = [](double x, double y) { return std::fabs(x - y) <= 1e-14 * std::max(std::fabs(x), std::fabs(y)); };
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            const long long I = (long long)i * N + j;
+            if (i > 0 && !close(a[I] * bW[I], a[I - N] * bE[I - N])) return false;
+            if (j > 0 && !close(a[I] * bS[I], a[I - 1] * bN[I - 1])) return false;
+        }
+    return true;
+}
+
+
 // Workgroups of a patch launch: several rounds of every CU's resident waves (256 CUs x
 // 4 SIMDs x 5 waves / 4 waves per workgroup = 1,280 at a time); each wave
 // then loops over its items.  EF_PATCH_MAX_BLOCKS, read when a batch is created,
@@ -232,23 +250,6 @@ bool patch_constant(const double* c, long long NN) {
     return true;
 }
 
-// True when the five-point matrix of the patch (FiniteVolumeSolver.cpp:64-85, cell I = j + i N
-// with i along x) is symmetric to 1e-14 relative: across every interior face the coupling
-// alpha beta / h^2 is the same seen from both cells.  The physical-side terms change the
-// diagonal only.
-bool cells_symmetric(const double* c, int N) {
-    const long long NN = (long long)N * N;
-    const double *a = c, *bW = c + NN, *bE = c + 2 * NN, *bS = c + 3 * NN, *bN = c + 4 * NN;
-    auto close = [](double x, double y) { return std::fabs(x - y) <= 1e-14 * std::max(std::fabs(x), std::fabs(y)); };
-    for (int i = 0; i < N; i++)
-        for (int j = 0; j < N; j++) {
-            const long long I = (long long)i * N + j;
-            if (i > 0 && !close(a[I] * bW[I], a[I - N] * bE[I - N])) return false;
-            if (j > 0 && !close(a[I] * bS[I], a[I - 1] * bN[I - 1])) return false;
-        }
-    return true;
-}
-
 void record_begin(ef_patches_s* p) {
     if (!p->ev[0]) {
         EF_HIP(hipEventCreate(&p->ev[0]));
@@ -345,7 +346,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
         p->vstencil.alloc(P * 5 * NN);
         p->vSinv.alloc(P * N * NN);
         p->packed = (N == 8 || N == 16 || N == 32);
-        for (size_t i = 0; i < P && p->packed; i++) p->packed = cells_symmetric(c->cell + i * 6 * NN, N);
+        for (size_t i = 0; i < P && p->packed; i++) p->packed = efk::cells_symmetric(c->cell + i * 6 * NN, N);
         const long long NP = bt_packed_entries(N);
         if (p->packed) p->vSinvP.alloc(P * N * NP);
         p->ld_h.assign(P, LeafDesc{});
