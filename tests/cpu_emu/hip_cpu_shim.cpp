@@ -202,7 +202,21 @@ void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body)
         std::barrier<> bb(NT);
         g_block_bar = &bb;
         for (int i = 0; i < NT / 64; i++) g_wave_bar[i] = new std::barrier<>(64);
-        if (shm) std::memset(g_dyn, 0xff, shm);
+        if (shm) {
+            // fresh dynamic shared memory per block, filled as EF_EMU_FILL says (see guarded_alloc)
+            const char* fill = getenv("EF_EMU_FILL");
+            if (fill && std::strcmp(fill, "zero") == 0) {
+                std::memset(g_dyn, 0, shm);
+            } else if (fill && std::strcmp(fill, "rand") == 0) {
+                static uint64_t x = 0x853c49e6748fea9bULL;
+                for (size_t i = 0; i + 8 <= shm; i += 8) {
+                    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+                    std::memcpy((char*)g_dyn + i, &x, 8);
+                }
+            } else {
+                std::memset(g_dyn, 0xff, shm);
+            }
+        }
         g_start->arrive_and_wait();
         g_end->arrive_and_wait();
         for (int i = 0; i < NT / 64; i++) delete g_wave_bar[i];
