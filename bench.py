@@ -393,16 +393,17 @@ def main():
             f"{pr['tflops']:.2f} TFLOP/s, rel err {pr['rel_err']}")
         out["components"] = [dict(
             name="leaf solve stage (FiniteVolumeSolver::solve on every leaf, ef_patches_apply_device)",
-            kernel="patch_fd_kernel<1>", value=pr["value"], unit="leaf solves/s", ms_per_launch=pr["ms"],
+            kernel="patch_fd_kernel<1, true, 0>", value=pr["value"], unit="leaf solves/s", ms_per_launch=pr["ms"],
             ms_min_single=pr["ms_min"], ms_single_mean=pr["ms_single_mean"], wall_ms_per_step=pr["wall_ms"], patches=len(bounds), nrhs=k,
-            rel_err_vs_fp64_oracle=pr["rel_err"], tflops=pr["tflops"],
+            rel_err_vs_fp64_oracle=pr["rel_err"], tflops=pr["tflops"], algorithmic_bytes_per_launch=pr["bytes"],
+            traffic=TRAFFIC_PER_LAUNCH if k == 16 else None,
             cpu_baseline=dict(value=cpu_patch["value"], unit="leaf solves/s", cores=1, kind="port",
                               sample="oracle leaf solve (banded LU of the 5-point system), %d patches of the "
                                      "workload in %.1f s at 1 thread" % (cpu_patch["patches"], cpu_patch["seconds"])))]
         copy_gbs = copy_bandwidth()
         log(f"plain copy on this GPU: {copy_gbs:.0f} GB/s")
         out["roofline"] = dict(bound="hbm", achieved=pr["gbs"], peak=8000.0, unit="GB/s", frac=pr["gbs"] / 8000.0,
-                               traffic=TRAFFIC_PER_LAUNCH if k == 16 else None, kernel="patch_fd_kernel<1>",
+                               traffic=TRAFFIC_PER_LAUNCH if k == 16 else None, kernel="patch_fd_kernel<1, true, 0>",
                                copy_kernel_gbs=copy_gbs, frac_of_copy=pr["gbs"] / copy_gbs,
                                algorithmic_bytes_per_launch=pr["bytes"],
                                scope="leaf-solve stage only; the build stage is switched off in this library build")
