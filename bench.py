@@ -162,6 +162,25 @@ def cpu_patch_baseline(bounds, N, k, sample, seconds=10.0):
     return dict(value=n / dt, patches=n, seconds=dt, check=check)
 
 
+def cpu_leaf_op_baseline(op, bounds, N, cells=None, seconds=5.0, sample=2000):
+    """Oracle leaf operations at one thread on the first patches of a leaf set (a bounded
+    sample): "map" = particular Neumann data of f, "solve" = the leaf solve with g and f."""
+    import oracle
+    oracle.set_threads(1)
+    g, f = patch_inputs(len(bounds), N, 1)
+    n, t0 = 0, time.perf_counter()
+    while n < sample and time.perf_counter() - t0 < seconds:
+        p = n % len(bounds)
+        kw = dict(coeffs=cells[p]) if cells is not None else {}
+        if op == "map":
+            oracle.leaf_map_d2n(N, bounds[p], np.zeros(4 * N), f[0, p], **kw)
+        else:
+            oracle.leaf_solve(N, bounds[p], g[0, p], f[0, p], **kw)
+        n += 1
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, patches=n, seconds=dt)
+
+
 def patch_bytes_flops(P, N, k):
     """Algorithmic HBM bytes and flops of one batched leaf-solve launch: read f (N^2) and
     g (4N), write u (N^2) per patch and right-hand side; four N x N x N products."""
@@ -278,7 +297,13 @@ def gpu_variable_run(steps, warmup, level=6, N=32):
     block = N * (N + 1) // 2 if pt.packed else N * N
     nbytes = P * (2 * N * block + 2 * N * N + 2 * N + 2 * N * N + 4 * N) * 8
     return dict(ms=ms, value=P / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9, patches=P,
-                packed=pt.packed, dtn_ms=dtn_ms, dtn_wall_ms=dtn_wall_ms)
+                packed=pt.packed, dtn_ms=dtn_ms, dtn_wall_ms=dtn_wall_ms, bounds=bounds,
+                cells=pt_cells_sample(bounds, N))
+
+
+def pt_cells_sample(bounds, N, n=512):
+    """Per-cell coefficients of the first n patches (the CPU baseline's sample)."""
+    return var_beta_cells(bounds[:n], N)
 
 
 def traffic_probe(bounds, N, k, copy_doubles=32 << 20, level=6, NV=32):
@@ -409,16 +434,31 @@ def main():
                                scope="leaf-solve stage only; the build stage is switched off in this library build")
         pp = gpu_particular_run(bounds, args.N, k, args.steps, args.warmup)
         log(f"particular Neumann data: {pp['ms']:.4f} ms per launch, {pp['gbs']:.0f} GB/s")
+        cpu_map = None
+        if not args.no_cpu_baseline:
+            cm = cpu_leaf_op_baseline("map", bounds, args.N, seconds=10.0, sample=20000)
+            cpu_map = dict(value=cm["value"], unit="leaf maps/s", cores=1, kind="port",
+                           sample="oracle particularNeumannData (banded LU of the 5-point system), %d patches of "
+                                  "the leaf set in %.1f s at 1 thread" % (cm["patches"], cm["seconds"]))
         out["components"].append(dict(
             name="upwards leaf stage (particularNeumannData on every leaf, ef_patches_apply_device MAP_D2N)",
+            cpu_baseline=cpu_map,
             kernel="patch_fd_kernel<1, true, 1>", value=pp["value"], unit="leaf maps/s", ms_per_launch=pp["ms"],
             patches=len(bounds), nrhs=k, achieved_gbs=pp["gbs"], algorithmic_bytes_per_launch=pp["bytes"],
             traffic=TRAFFIC_MAP_D2N if k == 16 else None, frac_of_peak=pp["gbs"] / 8000.0))
         pv = gpu_variable_run(args.steps, args.warmup)
         log(f"variable-beta leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s; "
             f"leaf DtN of the set {pv['dtn_ms']:.2f} ms")
+        cpu_var = None
+        if not args.no_cpu_baseline:
+            cv = cpu_leaf_op_baseline("solve", pv["bounds"], 32, cells=pv["cells"], seconds=10.0,
+                                      sample=len(pv["cells"]))
+            cpu_var = dict(value=cv["value"], unit="leaf solves/s", cores=1, kind="port",
+                           sample="oracle five-point leaf solve (banded LU), %d patches of the leaf set in %.1f s "
+                                  "at 1 thread" % (cv["patches"], cv["seconds"]))
         out["components"].append(dict(
             name="variable-beta leaf solves (five-point, [-1,1]^2 uniform level 6: 4,096 patches, N = 32, 1 right-hand side)",
+            cpu_baseline=cpu_var,
             kernel="bt_solve_sym_kernel<32>" if pv["packed"] else "bt_solve_wave_kernel<32>", value=pv["value"],
             unit="leaf solves/s", ms_per_launch=pv["ms"], patches=pv["patches"], nrhs=1,
             factor_storage="packed symmetric" if pv["packed"] else "full", achieved_gbs=pv["gbs"],
