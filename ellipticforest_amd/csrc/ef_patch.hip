@@ -178,8 +178,8 @@ void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double
 
 // Large patches: chunks of items through fold, four GEMMs, scaling and the output step.
 void launch_gemm_strided(const GemmStridedArgs& g, hipStream_t s) {
-    const long long tiles = (long long)(g.M / 16) * (g.N / 16);
-    hipLaunchKernelGGL(gemm_strided_f64_kernel, dim3((unsigned)((tiles + 3) / 4), (unsigned)g.batch), dim3(256), 0, s, g);
+    const long long blocks32 = (long long)((g.M + 31) / 32) * ((g.N + 31) / 32);  // one wave each
+    hipLaunchKernelGGL(gemm_strided_f64_kernel, dim3((unsigned)((blocks32 + 3) / 4), (unsigned)g.batch), dim3(256), 0, s, g);
     EF_LAUNCH_CHECK();
 }
 

@@ -7,9 +7,10 @@
 //   fold   B = f - 2 c g on the edge cells (or a unit boundary column for DtN chunks)
 //   GEMM   W2 = Q^T W1,  W1 = W2 Q,  scale W1 ./ D,  W2 = Q W1,  W1 = W2 Q^T
 //   out    u, or the Neumann data h / DtN column from the edge cells of u
-// The GEMM is register-resident like the small-patch kernel: one wave per 16 x 16 output
-// tile, operands read straight from memory in the v_mfma_f64_16x16x4_f64 layouts
-// (A[r][k0 + q], B[k0 + q][r] for lane 16q + r), no LDS and no scratch.
+// The GEMM is register-resident like the small-patch kernel: one wave per 32 x 32 block of
+// output (2 x 2 tiles of 16 x 16), operands read straight from memory in the
+// v_mfma_f64_16x16x4_f64 layouts (A[r][k0 + q], B[k0 + q][r] for lane 16q + r), no LDS and no
+// scratch.
 #include <hip/hip_runtime.h>
 #include "ef_device.hpp"
 #include "ef_kernels.hpp"
@@ -19,24 +20,41 @@ namespace efk {
 typedef double ld4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(256) gemm_strided_f64_kernel(GemmStridedArgs a) {
+    // one wave per 32 x 32 block of C: 2 x 2 tiles, so each A and B value loaded feeds two
+    // MFMAs.  On the last block row / column a missing second tile points at the first (loads
+    // in range, the duplicate is not stored).
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
-    const int tn = a.N / 16;
+    const int tn = (a.N + 31) / 32, tm = (a.M + 31) / 32;
     const long long w = (long long)blockIdx.x * 4 + wave;
-    if (w >= (long long)(a.M / 16) * tn) return;  // uniform per wave
+    if (w >= (long long)tm * tn) return;  // uniform per wave
     const long long b = blockIdx.y;
-    const int row0 = (int)(w / tn) * 16, col0 = (int)(w % tn) * 16;
-    const double* A = a.A + b * a.sa + (long long)(row0 + r) * a.ra;
-    const double* B = a.B + b * a.sb + (long long)(col0 + r) * a.cb;
-    ld4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int row0 = (int)(w / tn) * 32, col0 = (int)(w % tn) * 32;
+    const bool r1 = row0 + 16 < a.M, c1 = col0 + 16 < a.N;
+    const double* A0 = a.A + b * a.sa + (long long)(row0 + r) * a.ra;
+    const double* A1 = r1 ? A0 + 16LL * a.ra : A0;
+    const double* B0 = a.B + b * a.sb + (long long)(col0 + r) * a.cb;
+    const double* B1 = c1 ? B0 + 16LL * a.cb : B0;
+    ld4 c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
+    // K is a multiple of 16 (Np): four k-steps per iteration, their loads issued together
+#pragma unroll 4
     for (int k0 = 0; k0 < a.K; k0 += 4) {
-        const double av = A[(long long)(k0 + q) * a.ca];
-        const double bv = B[(long long)(k0 + q) * a.rb];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        const long long ka = (long long)(k0 + q) * a.ca, kb = (long long)(k0 + q) * a.rb;
+        const double a0 = A0[ka], a1 = A1[ka], b0 = B0[kb], b1 = B1[kb];
+        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
     }
-    double* C = a.C + b * a.sc + (long long)(col0 + r) * a.cc;
+    double* C = a.C + b * a.sc;
+    auto put = [&](const ld4& acc, int rb, int cb) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) C[(long long)(row0 + 4 * v + q) * a.rc] = acc[v];
+        for (int v = 0; v < 4; v++) C[(long long)(rb + 4 * v + q) * a.rc + (long long)(cb + r) * a.cc] = acc[v];
+    };
+    put(c00, row0, col0);
+    if (c1) put(c01, row0, col0 + 16);
+    if (r1) put(c10, row0 + 16, col0);
+    if (r1 && c1) put(c11, row0 + 16, col0 + 16);
 }
 
 // item t of the chunk -> (patch, rhs, DtN column or -1)
