@@ -10,8 +10,9 @@
 // where B is the load with the boundary data folded in (-2 c g at the adjacent cell).
 //
 // Work: one wave per (patch, right-hand side) item, four independent waves per
-// workgroup, a grid-stride loop over the items (the basis tables stay in registers) with
-// the next item's load and boundary data fetched while the current one is computed.
+// workgroup, a grid-stride loop over the items (the basis tables stay in registers); for
+// N <= 16 the next item's load and boundary data are fetched while the current one is
+// computed, for N > 16 a second wave per SIMD hides the loads instead.
 // No LDS.  Every product is done on v_mfma_f64_16x16x4_f64 in registers.  With
 // q = lane/16 and r = lane%16, the gfx950 operand layouts for a 16x16 block are
 //     A operand, step s: A[r][4s + q]        B operand, step s: B[4s + q][r]
@@ -111,11 +112,20 @@ __device__ inline double patch_g(const PatchItemIn<NB>& in, int t, int col) {
     return (t < 64) ? a : b;
 }
 
-// The N = 16 solve fits five waves per SIMD (at most 102 registers) and the N = 16 Neumann
-// data four, which overlaps one wave's loads with another's MFMA chain; the other variants
-// would spill there (NB = 2 needs about 330 registers), so they keep the compiler's choice.
+// Waves per SIMD: the N = 16 solve fits five (at most 102 registers) and the N = 16 Neumann
+// data four, which overlaps one wave's loads with another's MFMA chain.  NB = 2 fetches each
+// item when its turn comes instead of one item ahead (a prefetched item costs 64 registers),
+// which fits two waves per SIMD in 256 registers: a second wave hides the loads better than
+// the prefetch did (N = 32, 4,096 patches x 4 right-hand sides: 186 -> 145 us).  The padded
+// NB = 2 DtN would spill at two waves and keeps one.
 template <int NB, bool FULL, int OP>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 && FULL && OP == PATCH_OP_SOLVE ? 5 : (NB == 1 && FULL && OP == PATCH_OP_MAP_D2N ? 4 : 1), 8)))
+constexpr int patch_fd_waves() {
+    if (NB == 1) return !FULL ? 1 : (OP == PATCH_OP_SOLVE ? 5 : (OP == PATCH_OP_MAP_D2N ? 4 : 1));
+    return (!FULL && OP == PATCH_OP_DTN) ? 1 : 2;
+}
+
+template <int NB, bool FULL, int OP>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(patch_fd_waves<NB, FULL, OP>(), 8)))
 patch_fd_kernel(PatchFDArgs p) {
     // the wave index is uniform: keep it (and every item index and base address derived
     // from it) in scalar registers
@@ -148,8 +158,9 @@ patch_fd_kernel(PatchFDArgs p) {
     }
 
     int o = (int)(first / inner), in_i = (int)(first % inner);
+    constexpr bool PREFETCH = NB == 1;
     PatchItemIn<NB> cur{}, nxt{};
-    {
+    if (PREFETCH) {
         const int patch = (OP == PATCH_OP_DTN) ? o : in_i, rhs = (OP == PATCH_OP_DTN) ? 0 : o;
         patch_fetch<NB, FULL, OP>(p, patch, rhs, lane, cur);
     }
@@ -157,10 +168,11 @@ patch_fd_kernel(PatchFDArgs p) {
         const int patch = (OP == PATCH_OP_DTN) ? o : in_i;
         const int rhs = (OP == PATCH_OP_DTN) ? 0 : o;
         const int col = (OP == PATCH_OP_DTN) ? in_i : -1;
-        // next item, fetched while this one is computed
+        if (!PREFETCH) patch_fetch<NB, FULL, OP>(p, patch, rhs, lane, cur);
+        // next item, fetched while this one is computed (NB = 1)
         int o2 = o + step_o, i2 = in_i + step_i;
         if (i2 >= inner) { i2 -= inner; o2++; }
-        if (o2 < outer) {
+        if (PREFETCH && o2 < outer) {
             const int patch2 = (OP == PATCH_OP_DTN) ? o2 : i2, rhs2 = (OP == PATCH_OP_DTN) ? 0 : o2;
             patch_fetch<NB, FULL, OP>(p, patch2, rhs2, lane, nxt);
         }
@@ -311,7 +323,7 @@ patch_fd_kernel(PatchFDArgs p) {
     next_item:
         o = o2;
         in_i = i2;
-        cur = nxt;
+        if (PREFETCH) cur = nxt;
     }
 }
 

@@ -194,6 +194,18 @@ def test_gpu_adaptive_poisson_leaf_set_properties(patch_ctx):
     assert np.abs(hl - expect).max() < 1e-8
 
 
+@pytest.mark.parametrize("N", [24, 32])
+def test_emu_grid_stride_loop_two_by_two_blocks(emu_ctx, monkeypatch, N):
+    """One workgroup (four waves) for 2 patches x 3 right-hand sides and 2 x 4N DtN columns
+    with the two-by-two MFMA block grid: each wave fetches and solves many items in turn."""
+    from ellipticforest_amd import _lib
+    monkeypatch.setenv("EF_PATCH_MAX_BLOCKS", "1")
+    rng = np.random.default_rng(N + 5)
+    bounds = random_bounds(rng, 2)
+    pt = _lib.Patches(emu_ctx, bounds, N, lambda0=-0.75)
+    check_against_oracle(pt, bounds, N, rng, nrhs=3, lam=-0.75)
+
+
 def test_emu_grid_stride_loop(emu_ctx, monkeypatch):
     """Two workgroups for 3 patches x 5 right-hand sides and a DtN of 2 x 32 columns: every
     wave loops over several items."""
