@@ -54,7 +54,8 @@ void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mod
 inline long long bt_packed_entries(int Nn) { return (long long)Nn * (Nn + 1) / 2; }
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)
 template <int NB, bool FULL, int OP>
-__global__ void patch_fd_kernel(PatchFDArgs p);
+__global__ void patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __restrict__ coef,
+                                const long long* __restrict__ g_off, const long long* __restrict__ h_off);
 __global__ void gemm_strided_f64_kernel(GemmStridedArgs a);
 __global__ void patch_large_fold_kernel(PatchLargeArgs a);
 __global__ void patch_large_scale_kernel(PatchLargeArgs a);

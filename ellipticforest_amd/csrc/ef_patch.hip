@@ -136,9 +136,9 @@ void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStre
     const int op = a.op;
 #define EF_PATCH_LAUNCH(NB_, FULL_)                                                                         \
     do {                                                                                                     \
-        if (op == PATCH_OP_SOLVE) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE>), grid, block, 0, s, a); \
-        else if (op == PATCH_OP_MAP_D2N) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N>), grid, block, 0, s, a); \
-        else hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN>), grid, block, 0, s, a); \
+        if (op == PATCH_OP_SOLVE) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE>), grid, block, 0, s, a, a.cls, a.coef, a.g_off, a.h_off); \
+        else if (op == PATCH_OP_MAP_D2N) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N>), grid, block, 0, s, a, a.cls, a.coef, a.g_off, a.h_off); \
+        else hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN>), grid, block, 0, s, a, a.cls, a.coef, a.g_off, a.h_off); \
     } while (0)
     if (NB == 1) {
         if (full) EF_PATCH_LAUNCH(1, true);

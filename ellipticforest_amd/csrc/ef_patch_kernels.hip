@@ -58,16 +58,52 @@ template <int NB>
 struct PatchItemIn {
     double F[NB][NB][4];
     double G[NB];
-    PatchCoef cf;  // the item's coefficient entry (scalar loads issued with the prefetch)
-    int ce;
+    int ce;  // the item's coefficient entry (uniform)
+};
+
+// Read-only index arrays of PatchFDArgs, passed to the kernel as separate __restrict__
+// arguments: only then may the compiler read them with scalar loads inside the loop that
+// stores the results (a vector load there would have to be waited for like the data loads).
+struct PatchIdx {
+    const int* cls;
+    const PatchCoef* coef;
+    const long long* g_off;
+    const long long* h_off;
 };
 
 template <int NB, bool FULL, int OP>
-__device__ inline void patch_fetch(const PatchFDArgs& p, int patch, int rhs, int lane, PatchItemIn<NB>& in) {
+__device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int patch, int rhs, int lane,
+                                   PatchItemIn<NB>& in) {
     const int Nn = p.N, q = lane >> 4, r = lane & 15;
     const long long slot = (long long)rhs * p.npatches + patch;
-    in.ce = p.cls ? p.cls[patch] : patch;
-    in.cf = p.coef[in.ce];
+    in.ce = ix.cls ? ix.cls[patch] : patch;
+    if (FULL) {
+        // Every load is issued, with no branch around it and nothing that waits for it here:
+        // a load inside a branch leaves the compiler unsure how many loads are in flight
+        // after the branch, and it then waits for all of them -- the prefetched item
+        // included.  A missing f or g reads the basis table instead (as long as any load);
+        // the kernel replaces those values by zeros when it uses them (patch_zero_missing).
+        // The loads are global (not flat) loads: a flat load in flight makes the compiler
+        // wait for every load before any other wait.
+        const bool hf = p.f != nullptr, hg = p.g != nullptr;
+        const double* fb = hf ? p.f + slot * Nn * Nn : p.QB;
+#pragma unroll
+        for (int bi = 0; bi < NB; bi++)
+#pragma unroll
+            for (int bk = 0; bk < NB; bk++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) in.F[bi][bk][s] = gmem(fb)[(16 * bk + 4 * s + q) * Nn + 16 * bi + r];
+        if (OP == PATCH_OP_DTN) return;
+        const double* gb = p.QB;
+        long long gs = 0;
+        if (hg) {
+            if (ix.g_off) { gb = p.g + ix.g_off[patch] * p.vk + rhs; gs = p.vk; }
+            else { gb = p.g + slot * 4 * Nn; gs = 1; }
+        }
+#pragma unroll
+        for (int m = 0; m < NB; m++) in.G[m] = gmem(gb)[(long long)(64 * m + lane) * gs];  // 4N = 64 NB entries
+        return;
+    }
     const double* f = p.f ? p.f + slot * Nn * Nn : nullptr;
 #pragma unroll
     for (int bi = 0; bi < NB; bi++)
@@ -92,7 +128,7 @@ __device__ inline void patch_fetch(const PatchFDArgs& p, int patch, int rhs, int
     const double* g = nullptr;
     long long gs = 1;
     if (p.g) {
-        if (p.g_off) { g = p.g + p.g_off[patch] * p.vk + rhs; gs = p.vk; }
+        if (ix.g_off) { g = p.g + ix.g_off[patch] * p.vk + rhs; gs = p.vk; }
         else g = p.g + slot * 4 * Nn;
     }
 #pragma unroll
@@ -116,17 +152,19 @@ __device__ inline double patch_g(const PatchItemIn<NB>& in, int t, int col) {
 // data four, which overlaps one wave's loads with another's MFMA chain.  NB = 2 fetches each
 // item when its turn comes instead of one item ahead (a prefetched item costs 64 registers),
 // which fits two waves per SIMD in 256 registers: a second wave hides the loads better than
-// the prefetch did (N = 32, 4,096 patches x 4 right-hand sides: 186 -> 145 us).  The padded
-// NB = 2 DtN would spill at two waves and keeps one.
+// the prefetch did (N = 32, 4,096 patches x 4 right-hand sides: 186 -> 145 us).  The NB = 2
+// DtN would spill at two waves and keeps one.
 template <int NB, bool FULL, int OP>
 constexpr int patch_fd_waves() {
     if (NB == 1) return !FULL ? 1 : (OP == PATCH_OP_SOLVE ? 5 : (OP == PATCH_OP_MAP_D2N ? 4 : 1));
-    return (!FULL && OP == PATCH_OP_DTN) ? 1 : 2;
+    return OP == PATCH_OP_DTN ? 1 : 2;
 }
 
 template <int NB, bool FULL, int OP>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(patch_fd_waves<NB, FULL, OP>(), 8)))
-patch_fd_kernel(PatchFDArgs p) {
+patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __restrict__ coef,
+                const long long* __restrict__ g_off, const long long* __restrict__ h_off) {
+    const PatchIdx ix{cls, coef, g_off, h_off};
     // the wave index is uniform: keep it (and every item index and base address derived
     // from it) in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -162,23 +200,44 @@ patch_fd_kernel(PatchFDArgs p) {
     PatchItemIn<NB> cur{}, nxt{};
     if (PREFETCH) {
         const int patch = (OP == PATCH_OP_DTN) ? o : in_i, rhs = (OP == PATCH_OP_DTN) ? 0 : o;
-        patch_fetch<NB, FULL, OP>(p, patch, rhs, lane, cur);
+        patch_fetch<NB, FULL, OP>(p, ix, patch, rhs, lane, cur);
     }
-    while (o < outer) {
+    // One item: (o, in_i) from cur, with (o2, i2) fetched into nxt meanwhile (NB = 1).  The
+    // loop below alternates the two buffers, so no item is ever copied between registers (a
+    // copy would wait for the prefetch to arrive at the end of the item it overlaps).
+    auto item = [&](const int o, const int in_i, const int o2, const int i2, PatchItemIn<NB>& cur,
+                    PatchItemIn<NB>& nxt) {
         const int patch = (OP == PATCH_OP_DTN) ? o : in_i;
         const int rhs = (OP == PATCH_OP_DTN) ? 0 : o;
         const int col = (OP == PATCH_OP_DTN) ? in_i : -1;
-        if (!PREFETCH) patch_fetch<NB, FULL, OP>(p, patch, rhs, lane, cur);
-        // next item, fetched while this one is computed (NB = 1)
-        int o2 = o + step_o, i2 = in_i + step_i;
-        if (i2 >= inner) { i2 -= inner; o2++; }
-        if (PREFETCH && o2 < outer) {
-            const int patch2 = (OP == PATCH_OP_DTN) ? o2 : i2, rhs2 = (OP == PATCH_OP_DTN) ? 0 : o2;
-            patch_fetch<NB, FULL, OP>(p, patch2, rhs2, lane, nxt);
+        if (!PREFETCH) patch_fetch<NB, FULL, OP>(p, ix, patch, rhs, lane, cur);
+        if (PREFETCH) {
+            // past the last item the current one is fetched again: the fetch is never skipped
+            const bool more = o2 < outer;
+            const int oo = more ? o2 : o, ii = more ? i2 : in_i;
+            const int patch2 = (OP == PATCH_OP_DTN) ? oo : ii, rhs2 = (OP == PATCH_OP_DTN) ? 0 : oo;
+            patch_fetch<NB, FULL, OP>(p, ix, patch2, rhs2, lane, nxt);
+            // keep these loads ahead of the current item's work: issued later, they would
+            // shorten the time they have to arrive
+            __builtin_amdgcn_sched_barrier(0);
         }
-        const PatchCoef cf = cur.cf;
+        const PatchCoef cf = ix.coef[cur.ce];
         const double* dinv_row = p.dinv ? p.dinv + (long long)cur.ce * (16 * NB) * (16 * NB) : nullptr;
         const bool has_g = (OP == PATCH_OP_DTN) || p.g != nullptr;
+        if (FULL) {
+            // patch_zero_missing: the values read in place of a missing f or g become zeros
+            const bool hf = p.f != nullptr;
+#pragma unroll
+            for (int bi = 0; bi < NB; bi++)
+#pragma unroll
+                for (int bk = 0; bk < NB; bk++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) cur.F[bi][bk][s] = hf ? cur.F[bi][bk][s] : 0.0;
+            if (OP != PATCH_OP_DTN) {
+#pragma unroll
+                for (int m = 0; m < NB; m++) cur.G[m] = has_g ? cur.G[m] : 0.0;
+            }
+        }
 
         // fold the boundary data into the load: -2 (c_W g_W + c_E g_E) on the x-edge
         // cells, -2 (c_S g_S + c_N g_N) on the y-edge cells
@@ -192,10 +251,10 @@ patch_fd_kernel(PatchFDArgs p) {
         if (NB == 1 && FULL && OP != PATCH_OP_DTN) {
             const double ax = -2.0 * cf.cx, ay = (r == 0 || r == 15) ? -2.0 * cf.cy : 0.0;
             const int ybase = (r == 0) ? 32 + q : 48 + q;
-            if (has_g) xg = __shfl(cur.G[0], q == 0 ? r : 16 + r);
+            xg = __shfl(cur.G[0], q == 0 ? r : 16 + r);  // G is zero without g
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                if (has_g) yg[s] = __shfl(cur.G[0], ybase + 4 * s);
+                yg[s] = __shfl(cur.G[0], ybase + 4 * s);
                 double v = fma(ay, yg[s], cur.F[0][0][s]);
                 if ((s == 0 && q == 0) || (s == 3 && q == 3)) v = fma(ax, xg, v);
                 BT[0][0][s] = v;
@@ -233,6 +292,17 @@ patch_fd_kernel(PatchFDArgs p) {
         patch_mm<NB>([&](int bi, int bk, int s) { return Z[bk][bi][s]; },
                      [&](int bk, int bj, int s) { return QB[bk][bj][s]; }, Y);
         // Z3 = Z2 ./ D; padded modes (k or l >= N) are zero
+        if (NB == 1 && FULL) {
+            // computed here (no table load: every load of the loop is the prefetch's)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double d = cf.cx * muk[0][v] + cf.cy * mul[0] + cf.lam;
+                double rc = __builtin_amdgcn_rcp(d);
+                rc = fma(rc, fma(-d, rc, 1.0), rc);
+                rc = fma(rc, fma(-d, rc, 1.0), rc);
+                Y[0][0][v] *= rc;
+            }
+        } else
 #pragma unroll
         for (int bi = 0; bi < NB; bi++)
 #pragma unroll
@@ -280,8 +350,8 @@ patch_fd_kernel(PatchFDArgs p) {
             if (OP == PATCH_OP_DTN) {
                 h = p.out + (long long)patch * 16 * Nn * Nn + col;
                 ostride = 4 * Nn;
-            } else if (p.h_off) {
-                h = p.out + p.h_off[patch] * p.vk + rhs;
+            } else if (ix.h_off) {
+                h = p.out + ix.h_off[patch] * p.vk + rhs;
                 ostride = p.vk;
             } else {
                 h = p.out + ((long long)rhs * p.npatches + patch) * 4 * Nn;
@@ -289,16 +359,21 @@ patch_fd_kernel(PatchFDArgs p) {
             }
             const double sx = 2.0 / cf.dx, sy = 2.0 / cf.dy;
             if (NB == 1 && FULL && OP == PATCH_OP_MAP_D2N) {
-                // edge cells of lane (q, r): (4v + q, r)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const double uu = Y[0][0][v];
-                    if (v == 0 && q == 0) h[(long long)(W * 16 + r) * ostride] = sx * (uu - xg);
-                    if (v == 3 && q == 3) h[(long long)(E * 16 + r) * ostride] = -sx * (uu - xg);
-                    if (r == 0) h[(long long)(S * 16 + 4 * v + q) * ostride] = sy * (uu - yg[v]);
-                    if (r == 15) h[(long long)(N * 16 + 4 * v + q) * ostride] = -sy * (uu - yg[v]);
-                }
-                goto next_item;
+                // Lane t makes entry t of h (4N = 64 entries), so the item ends in one store
+                // issued by every lane.  u of lane (q, r), slot v is cell (4v + q, r); entry t
+                // needs: W (t < 16) cell (0, t): lane t, slot 0; E cell (15, t - 16): lane
+                // 48 + t - 16, slot 3; S cell (i, 0), i = t - 32: lane 16 (i % 4), slot i / 4;
+                // N cell (i, 15), i = t - 48: lane 16 (i % 4) + 15, slot i / 4.  Its boundary
+                // value g[t] is already in lane t.
+                const int side = lane >> 4, e = lane & 15;
+                const int src = side == W ? e : side == E ? 48 + e : 16 * (e & 3) + (side == N ? 15 : 0);
+                const int slot = side == W ? 0 : side == E ? 3 : e >> 2;
+                const double u0 = __shfl(Y[0][0][0], src), u1 = __shfl(Y[0][0][1], src);
+                const double u2 = __shfl(Y[0][0][2], src), u3 = __shfl(Y[0][0][3], src);
+                const double uu = slot == 0 ? u0 : slot == 1 ? u1 : slot == 2 ? u2 : u3;
+                const double sc = side == W ? sx : side == E ? -sx : side == S ? sy : -sy;
+                h[(long long)lane * ostride] = sc * (uu - cur.G[0]);
+                return;
             }
 #pragma unroll
             for (int bi = 0; bi < NB; bi++)
@@ -320,17 +395,30 @@ patch_fd_kernel(PatchFDArgs p) {
                         if (j == Nn - 1) h[(long long)(N * Nn + i) * ostride] = -sy * (uu - gnv);
                     }
         }
-    next_item:
+    };
+    auto advance = [&](int oo, int ii, int& o2, int& i2) {
+        o2 = oo + step_o;
+        i2 = ii + step_i;
+        if (i2 >= inner) { i2 -= inner; o2++; }
+    };
+    while (o < outer) {
+        int o2, i2;
+        advance(o, in_i, o2, i2);
+        item(o, in_i, o2, i2, cur, nxt);
         o = o2;
         in_i = i2;
-        if (PREFETCH) cur = nxt;
+        if (!PREFETCH || o >= outer) continue;
+        advance(o, in_i, o2, i2);
+        item(o, in_i, o2, i2, nxt, cur);
+        o = o2;
+        in_i = i2;
     }
 }
 
 #define EF_PATCH_INSTANCES(NB, FULL)                                              \
-    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_SOLVE>(PatchFDArgs); \
-    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_MAP_D2N>(PatchFDArgs); \
-    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_DTN>(PatchFDArgs);
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_SOLVE>(PatchFDArgs, const int*, const PatchCoef*, const long long*, const long long*); \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_MAP_D2N>(PatchFDArgs, const int*, const PatchCoef*, const long long*, const long long*); \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_DTN>(PatchFDArgs, const int*, const PatchCoef*, const long long*, const long long*);
 EF_PATCH_INSTANCES(1, true)
 EF_PATCH_INSTANCES(1, false)
 EF_PATCH_INSTANCES(2, true)

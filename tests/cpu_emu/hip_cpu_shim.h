@@ -124,6 +124,8 @@ inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 // memory fences: the emulated lanes share the host's coherent memory; ordering between lanes
 // comes from the wave and block barriers
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
+// instruction-scheduling fence of the GPU compiler: nothing to do on the CPU
+#define __builtin_amdgcn_sched_barrier(mask) ((void)(mask))
 #ifndef __ATOMIC_RELEASE
 #define __ATOMIC_RELEASE 3
 #define __ATOMIC_ACQUIRE 2

@@ -196,25 +196,27 @@ def test_gpu_adaptive_poisson_leaf_set_properties(patch_ctx):
 
 @pytest.mark.parametrize("N", [24, 32])
 def test_emu_grid_stride_loop_two_by_two_blocks(emu_ctx, monkeypatch, N):
-    """One workgroup (four waves) for 2 patches x 3 right-hand sides and 2 x 4N DtN columns
-    with the two-by-two MFMA block grid: each wave fetches and solves many items in turn."""
+    """One workgroup (four waves) for 2 patches x 3 right-hand sides (and, N = 24, 2 x 4N DtN
+    columns) with the two-by-two MFMA block grid: each wave fetches and solves several items in
+    turn."""
     from ellipticforest_amd import _lib
     monkeypatch.setenv("EF_PATCH_MAX_BLOCKS", "1")
     rng = np.random.default_rng(N + 5)
     bounds = random_bounds(rng, 2)
     pt = _lib.Patches(emu_ctx, bounds, N, lambda0=-0.75)
-    check_against_oracle(pt, bounds, N, rng, nrhs=3, lam=-0.75)
+    check_against_oracle(pt, bounds, N, rng, nrhs=3, lam=-0.75, dtn=(N == 24))
 
 
-def test_emu_grid_stride_loop(emu_ctx, monkeypatch):
-    """Two workgroups for 3 patches x 5 right-hand sides and a DtN of 2 x 32 columns: every
-    wave loops over several items."""
+@pytest.mark.parametrize("N", [8, 16])
+def test_emu_grid_stride_loop(emu_ctx, monkeypatch, N):
+    """Two workgroups for 3 patches x 5 right-hand sides and a DtN of 3 x 4N columns: every
+    wave loops over several items (N = 16: the unpadded single-block variant)."""
     from ellipticforest_amd import _lib
     monkeypatch.setenv("EF_PATCH_MAX_BLOCKS", "2")
     rng = np.random.default_rng(21)
     bounds = random_bounds(rng, 3)
-    pt = _lib.Patches(emu_ctx, bounds, 8, lambda0=-0.25)
-    check_against_oracle(pt, bounds, 8, rng, nrhs=5, lam=-0.25)
+    pt = _lib.Patches(emu_ctx, bounds, N, lambda0=-0.25)
+    check_against_oracle(pt, bounds, N, rng, nrhs=5, lam=-0.25)
 
 
 def test_emu_timer_and_stream(emu_ctx):
@@ -247,7 +249,7 @@ def test_emu_variable_patches_N32(emu_ctx):
     """Largest block-tridiagonal size (N = 32) on the emulation build."""
     from ellipticforest_amd import _lib
     rng = np.random.default_rng(33)
-    bounds = random_bounds(rng, 2)
+    bounds = random_bounds(rng, 1)
     cell = variable_cells(bounds, 32)
     pv = _lib.Patches(emu_ctx, bounds, 32, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
     assert pv.variable and pv.packed
