@@ -259,6 +259,27 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
                 if ((s == 0 && q == 0) || (s == 3 && q == 3)) v = fma(ax, xg, v);
                 BT[0][0][s] = v;
             }
+        } else if (NB == 2 && FULL && OP != PATCH_OP_DTN) {
+            // N = 32: element s of block (bi, bk) of lane (q, r) is cell (i, j) = (16 bk + 4s + q,
+            // 16 bi + r).  G[0] holds g_W (entries 0-31) and g_E (32-63), G[1] g_S and g_N.
+            // x-edges: (bk, s, q) = (0, 0, 0) takes g_W[j], (1, 3, 3) takes g_E[j];
+            // y-edges: (bi, r) = (0, 0) takes g_S[i], (1, 15) takes g_N[i].
+            const double ax = -2.0 * cf.cx, ay = -2.0 * cf.cy;
+            const int xsel = (q == 0) ? 0 : 32, ysel = (r == 0) ? 0 : 32;
+#pragma unroll
+            for (int bi = 0; bi < NB; bi++) {
+                const double xg2 = __shfl(cur.G[0], xsel + 16 * bi + r);
+                const bool yedge = (bi == 0) ? (r == 0) : (r == 15);
+#pragma unroll
+                for (int bk = 0; bk < NB; bk++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) {
+                        const double yg2 = __shfl(cur.G[NB - 1], ysel + 16 * bk + 4 * s + q);
+                        const double v = fma(yedge ? ay : 0.0, yg2, cur.F[bi][bk][s]);
+                        const bool xedge = (bk == 0 && s == 0 && q == 0) || (bk == 1 && s == 3 && q == 3);
+                        BT[bi][bk][s] = fma(xedge ? ax : 0.0, xg2, v);
+                    }
+            }
         } else {
 #pragma unroll
         for (int bi = 0; bi < NB; bi++) {
@@ -373,6 +394,41 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
                 const double uu = slot == 0 ? u0 : slot == 1 ? u1 : slot == 2 ? u2 : u3;
                 const double sc = side == W ? sx : side == E ? -sx : side == S ? sy : -sy;
                 h[(long long)lane * ostride] = sc * (uu - cur.G[0]);
+                return;
+            }
+            if (NB == 2 && FULL && OP == PATCH_OP_MAP_D2N) {
+                // Lane t makes entries t (W, E) and 64 + t (S, N) of h (4N = 128 entries): two
+                // stores issued by every lane.  u of lane (q, r), block (bi, bj), slot v is cell
+                // (16 bi + 4v + q, 16 bj + r).  W[j] = cell (0, j): lane j % 16, block (0, j / 16),
+                // slot 0; E[j] = cell (31, j): lane 48 + j % 16, block (1, j / 16), slot 3;
+                // S[i] = cell (i, 0): lane 16 (i % 4), block (i / 16, 0), slot (i / 4) % 4;
+                // N[i] = cell (i, 31): lane 16 (i % 4) + 15, block (i / 16, 1), slot (i / 4) % 4.
+                {
+                    const bool east = lane >= 32;
+                    const int j = lane & 31, bj = j >> 4;
+                    const int src = (east ? 48 : 0) + (j & 15);
+                    // W takes block (0, bj) slot 0, E block (1, bj) slot 3
+                    const double w0 = __shfl(Y[0][0][0], src), w1 = __shfl(Y[0][1][0], src);
+                    const double e0 = __shfl(Y[1][0][3], src), e1 = __shfl(Y[1][1][3], src);
+                    const double uu = east ? (bj ? e1 : e0) : (bj ? w1 : w0);
+                    h[(long long)lane * ostride] = (east ? -sx : sx) * (uu - cur.G[0]);
+                }
+                {
+                    const bool north = lane >= 32;
+                    const int i = lane & 31, bi = i >> 4, v = (i >> 2) & 3;
+                    const int src = 16 * (i & 3) + (north ? 15 : 0);
+                    double uu = 0.0;
+#pragma unroll
+                    for (int x = 0; x < 2; x++)
+#pragma unroll
+                        for (int y = 0; y < 2; y++)
+#pragma unroll
+                            for (int w = 0; w < 4; w++) {
+                                const double t = __shfl(Y[x][y][w], src);
+                                uu = (bi == x && (north ? 1 : 0) == y && v == w) ? t : uu;
+                            }
+                    h[(long long)(64 + lane) * ostride] = (north ? -sy : sy) * (uu - cur.G[1]);
+                }
                 return;
             }
 #pragma unroll
