@@ -215,6 +215,7 @@ struct ef_mesh_s {
     DevBuf<double> QA, QB, mu, dinv, leafT;  // fast-diagonalisation tables, leaf DtN
     // variable-coefficient leaves (block-tridiagonal path)
     bool variable = false;
+    bool dominant = false;                 // variable, every leaf diagonally dominant: no pivoting
     bool packed = false;                   // variable and symmetric: packed inverse pivot blocks
     DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork, vz;
     // operator arenas
@@ -361,6 +362,9 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
             L.dx = (nd.xu - nd.xl) / N;
             L.dy = (nd.yu - nd.yl) / N;
         }
+        M->dominant = true;
+        for (size_t li = 0; li < nl && M->dominant; li++)
+            M->dominant = efk::cells_dominant(c.cell + li * 6 * NN, N, ld[li].dx, ld[li].dy);
         M->leafdesc.upload(ld, M->stream);
         return;
     }
@@ -550,7 +554,7 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     if (M->variable) {
         // block-tridiagonal factors, then the DtN of every leaf (buildD2N) in chunks of leaves
         const int nl = (int)P.leaves.size();
-        launch_bt_factor(M->leafdesc.p, N, nl, s);
+        launch_bt_factor(M->leafdesc.p, N, nl, s, M->dominant);
         EF_LAUNCH_CHECK();
         if (M->packed) {
             hipLaunchKernelGGL(bt_pack_kernel, dim3((unsigned)std::min<long long>(blocks_for((long long)nl * N * bt_packed_entries(N)), 65535)),

@@ -32,6 +32,8 @@ long long patch_default_blocks();
 // True when the five-point matrix of a patch with per-cell coefficients c ([6][N*N]) is
 // symmetric to 1e-14 relative (then its inverse pivot blocks can be kept packed).
 bool cells_symmetric(const double* c, int N);
+// Every row of the patch's five-point matrix diagonally dominant (block LU without pivoting).
+bool cells_dominant(const double* c, int N, double dx, double dy);
 }
 
 // A context is one GPU.

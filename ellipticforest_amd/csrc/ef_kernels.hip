@@ -705,7 +705,10 @@ __device__ inline double wave_read_lane(double x, int src) {
 // run-time value; the row that pivots at step k holds row k of the inverse at the end, and the
 // inverse goes to LDS in row order (for the next block's S) and from there to Sinv.  The stencil
 // of each block row is computed one cell per lane (as bt_stencil) into LDS and written out.
-template <int NT>
+// PIVOT = false is for diagonally dominant matrices (the host checks every row), whose Schur
+// complements S_i stay diagonally dominant, so Gauss-Jordan needs no pivot search: step k is
+// unrolled, pivots on row k itself and leaves the inverse in natural row order.
+template <int NT, bool PIVOT>
 __global__ void __launch_bounds__(256) bt_factor_wave_kernel(const LeafDesc* leaves, int nleaves) {
     constexpr int NN = NT * NT;
     __shared__ double inv_all[4][NT][NT + 1];
@@ -769,8 +772,27 @@ __global__ void __launch_bounds__(256) bt_factor_wave_kernel(const LeafDesc* lea
             }
             col[r] = v;
         }
+        if (!PIVOT) {
+#pragma unroll
+            for (int k = 0; k < NT; k++) {
+                const double pk = wave_read_lane(col[k], k);
+                const double ip = (pk != 0.0) ? 1.0 / pk : 0.0;
+                const double rowk = col[k] * ip;
+#pragma unroll
+                for (int r = 0; r < NT; r++) {
+                    if (r == k) continue;
+                    const double l = wave_read_lane(col[r], k);
+                    col[r] = fma(-l, rowk, col[r]);
+                }
+                col[k] = rowk;
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < NT; r++)
+                if (lane >= NT && lane < 2 * NT) inv[r][c] = col[r];
+        }
         unsigned used = 0;
-        for (int k = 0; k < NT; k++) {
+        for (int k = 0; PIVOT && k < NT; k++) {
             // pivot: the largest unused |S(r, k)|, found by lane k on its own column
             double best = -1.0;
             int bi = 0;
@@ -797,10 +819,12 @@ __global__ void __launch_bounds__(256) bt_factor_wave_kernel(const LeafDesc* lea
             if (lane == 0) pinv[pr] = k;
         }
         // the inverse in row order: row pinv[r] of Sinv_i is row r of the right half
-        __builtin_amdgcn_wave_barrier();
+        if (PIVOT) {
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int r = 0; r < NT; r++)
-            if (lane >= NT && lane < 2 * NT) inv[pinv[r]][c] = col[r];
+            for (int r = 0; r < NT; r++)
+                if (lane >= NT && lane < 2 * NT) inv[pinv[r]][c] = col[r];
+        }
         __builtin_amdgcn_wave_barrier();
         auto* Si = gmem(L.Sinv) + (long long)i * NN;
         for (int e = lane; e < NN; e += 64) Si[e] = inv[e / NT][e % NT];
@@ -808,18 +832,24 @@ __global__ void __launch_bounds__(256) bt_factor_wave_kernel(const LeafDesc* lea
     }
 }
 
-template __global__ void bt_factor_wave_kernel<8>(const LeafDesc*, int);
-template __global__ void bt_factor_wave_kernel<16>(const LeafDesc*, int);
-template __global__ void bt_factor_wave_kernel<32>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<8, true>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<16, true>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<32, true>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<8, false>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<16, false>(const LeafDesc*, int);
+template __global__ void bt_factor_wave_kernel<32, false>(const LeafDesc*, int);
 
-void launch_bt_factor(const LeafDesc* leaves, int Nn, int nleaves, hipStream_t s) {
+void launch_bt_factor(const LeafDesc* leaves, int Nn, int nleaves, hipStream_t s, bool dominant) {
     const unsigned blocks = (unsigned)((nleaves + 3) / 4);
-    if (Nn == 8)
-        hipLaunchKernelGGL(bt_factor_wave_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves);
-    else if (Nn == 16)
-        hipLaunchKernelGGL(bt_factor_wave_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves);
-    else if (Nn == 32)
-        hipLaunchKernelGGL(bt_factor_wave_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves);
+#define EF_BT_FACTOR(NT_)                                                                                 \
+    do {                                                                                                   \
+        if (dominant) hipLaunchKernelGGL((bt_factor_wave_kernel<NT_, false>), dim3(blocks), dim3(256), 0, s, leaves, nleaves); \
+        else hipLaunchKernelGGL((bt_factor_wave_kernel<NT_, true>), dim3(blocks), dim3(256), 0, s, leaves, nleaves); \
+    } while (0)
+    if (Nn == 8) EF_BT_FACTOR(8);
+    else if (Nn == 16) EF_BT_FACTOR(16);
+    else if (Nn == 32) EF_BT_FACTOR(32);
+#undef EF_BT_FACTOR
     else
         hipLaunchKernelGGL(bt_factor_kernel, dim3(nleaves), dim3(256), 0, s, leaves, Nn);
 }

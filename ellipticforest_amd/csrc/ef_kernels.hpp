@@ -22,11 +22,13 @@ __global__ void root_dirichlet_kernel(const double* r, const double* a, double* 
 __global__ void root_robin_assemble_kernel(const double* T, const double* a, const double* b, const double* r,
                                            const double* h, double* A, double* rhs, int m, int k);
 __global__ void bt_factor_kernel(const LeafDesc* leaves, int Nn);
-template <int NT>
+template <int NT, bool PIVOT>
 __global__ void bt_factor_wave_kernel(const LeafDesc* leaves, int nleaves);
 // Stencil and block-LU factors (Sinv) of every leaf with variable coefficients: one wave per
-// leaf for N in {8, 16, 32}, the workgroup-per-leaf kernel otherwise.
-void launch_bt_factor(const LeafDesc* leaves, int Nn, int nleaves, hipStream_t s);
+// leaf for N in {8, 16, 32}, the workgroup-per-leaf kernel otherwise.  dominant: every leaf's
+// five-point matrix is diagonally dominant (cells_dominant), so the wave kernel may skip the
+// pivot search.
+void launch_bt_factor(const LeafDesc* leaves, int Nn, int nleaves, hipStream_t s, bool dominant = false);
 __global__ void bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f,
                                 int f_zero, const double* garena, double* harena, double* u);
 __global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work);

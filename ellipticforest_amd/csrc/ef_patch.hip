@@ -29,6 +29,7 @@ struct ef_patches_s {
     int N = 0, P = 0, NB = 1;
     long long max_blocks = 4096;
     bool variable = false;
+    bool dominant = false;         // variable, every patch diagonally dominant: no pivoting
     bool packed = false;           // variable and symmetric: packed inverse pivot blocks (N 8, 16, 32)
     bool large = false;            // N > 32: batched-GEMM fast diagonalisation
     int Np = 0;                    // padded side of the large path
@@ -76,6 +77,30 @@ bool cells_symmetric(const double* c, int N) {
     return true;
 }
 
+
+// True when every row of the five-point matrix of the patch (the stencil of
+// FiniteVolumeSolver.cpp:64-85, the ghost terms folded into the diagonal on the physical
+// sides) is diagonally dominant, |diagonal| >= sum of |off-diagonals| (to 1e-12 relative):
+// alpha, beta > 0 and lambda <= 0 give it.  Its block-LU then needs no pivoting.
+bool cells_dominant(const double* c, int N, double dx, double dy) {
+    const long long NN = (long long)N * N;
+    const double *a = c, *bW = c + NN, *bE = c + 2 * NN, *bS = c + 3 * NN, *bN = c + 4 * NN, *lam = c + 5 * NN;
+    const double dx2 = dx * dx, dy2 = dy * dy;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            const long long I = (long long)i * N + j;
+            const double cW = a[I] * bW[I] / dx2, cE = a[I] * bE[I] / dx2;
+            const double cS = a[I] * bS[I] / dy2, cN = a[I] * bN[I] / dy2;
+            double d = -a[I] * ((bE[I] + bW[I]) / dx2 + (bN[I] + bS[I]) / dy2) + lam[I];
+            double off = 0.0;
+            if (i == 0) d -= cW; else off += std::fabs(cW);
+            if (i == N - 1) d -= cE; else off += std::fabs(cE);
+            if (j == 0) d -= cS; else off += std::fabs(cS);
+            if (j == N - 1) d -= cN; else off += std::fabs(cN);
+            if (!(std::fabs(d) >= off * (1.0 - 1e-12))) return false;
+        }
+    return true;
+}
 
 // Workgroups of a patch launch: several rounds of every CU's resident waves (256 CUs x
 // 4 SIMDs x 5 waves / 4 waves per workgroup = 1,280 at a time); each wave
@@ -366,7 +391,11 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
             L.dy = p->coef_h[i].dy;
         }
         p->leafdesc.upload(p->ld_h, p->stream);
-        launch_bt_factor(p->leafdesc.p, N, p->P, p->stream);
+        bool dominant = true;
+        for (size_t i = 0; i < P && dominant; i++)
+            dominant = efk::cells_dominant(c->cell + i * 6 * NN, N, p->coef_h[i].dx, p->coef_h[i].dy);
+        p->dominant = dominant;
+        launch_bt_factor(p->leafdesc.p, N, p->P, p->stream, dominant);
         EF_LAUNCH_CHECK();
         if (p->packed) {
             hipLaunchKernelGGL(bt_pack_kernel, dim3((unsigned)std::min<long long>(blocks_for(P * N * NP), 65535)),

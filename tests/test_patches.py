@@ -239,10 +239,12 @@ def variable_cells(bounds, N, beta=BETA3, lam=-1.0):
                      for b in bounds])
 
 
-def alpha_cells(bounds, N):
-    """alpha = 1 + 0.3 sin(x + 2y) scales the rows of the five-point matrix: not symmetric."""
+def alpha_cells(bounds, N, lam=1.0):
+    """alpha = 1 + 0.3 sin(x + 2y) scales the rows of the five-point matrix: not symmetric.  With
+    lambda = +1 the rows are not diagonally dominant either, so the block LU pivots (lambda <= 0
+    takes the factorisation without pivoting)."""
     return np.stack([oracle.sample_coeffs(N, b, lambda x, y: 1.0 + 0.3 * np.sin(x + 2 * y), BETA3,
-                                          lambda x, y: -1.0 + 0 * x) for b in bounds])
+                                          lambda x, y: lam + 0 * x) for b in bounds])
 
 
 def test_emu_variable_patches_N32(emu_ctx):
