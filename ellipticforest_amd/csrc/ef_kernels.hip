@@ -1124,9 +1124,27 @@ __global__ void __launch_bounds__(256) bt_dtn_mfma_kernel(const LeafDesc* leaves
         return ((long long)i * NT + rt * 16 + 4 * v + q) * NB + c0 + 16 * t + r;
     };
     d4 Zp[RT][CT], Zc[RT][CT];
+    // Everything block i reads from memory (Sinv_i in the A layout, the cW or cE column scaling,
+    // and in the backward sweep Z_i) is loaded while block i -+ 1 is computed, and nothing else
+    // is loaded meanwhile: the compiler's waits count loads in issue order, so the block's own
+    // products never wait for memory.
+    double a_all[KS][RT], a_nxt[KS][RT], c_all[KS], c_nxt[KS];
+    auto load_a = [&](int i, int side, double (&A)[KS][RT], double (&C)[KS]) {
+        const auto* Si = gmem((const double*)L.Sinv) + (long long)i * NN;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) A[ks][rt] = Si[(rt * 16 + r) * NT + 4 * ks + q];
+            C[ks] = st[side * NN + i * NT + 4 * ks + q];  // cW (side 0) or cE (side 1) of block row i
+        }
+    };
+    load_a(0, 0, a_all, c_all);
     // forward
     for (int i = 0; i < NT; i++) {
-        const auto* Si = gmem((const double*)L.Sinv) + (long long)i * NN;
+        // next: block i + 1 with its cW, or for the backward sweep's first step block N - 2 with
+        // its cE
+        if (i + 1 < NT) load_a(i + 1, 0, a_nxt, c_nxt);
+        else load_a(NT - 2, 1, a_nxt, c_nxt);
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
@@ -1134,10 +1152,8 @@ __global__ void __launch_bounds__(256) bt_dtn_mfma_kernel(const LeafDesc* leaves
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = 4 * ks + q;
-            const double cw = (i > 0) ? st[i * NT + k] : 0.0;
-            double a[RT];
-#pragma unroll
-            for (int rt = 0; rt < RT; rt++) a[rt] = Si[(rt * 16 + r) * NT + 4 * ks + q];
+            const double cw = (i > 0) ? c_all[ks] : 0.0;
+            const double (&a)[RT] = a_all[ks];
 #pragma unroll
             for (int t = 0; t < CT; t++) {
                 double b = (i > 0) ? -cw * Zp[ks / 4][t][ks % 4] : 0.0;
@@ -1158,30 +1174,51 @@ __global__ void __launch_bounds__(256) bt_dtn_mfma_kernel(const LeafDesc* leaves
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
             for (int t = 0; t < CT; t++) Zp[rt][t] = Zc[rt][t];
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) a_all[ks][rt] = a_nxt[ks][rt];
+            c_all[ks] = c_nxt[ks];
+        }
     }
-    // backward: X_{N-1} = Z_{N-1}
-    emit(NT - 1, Zp);
-    for (int i = NT - 2; i >= 0; i--) {
-        const auto* Si = gmem((const double*)L.Sinv) + (long long)i * NN;
+    // backward: X_{N-1} = Z_{N-1}.  Z_i (the accumulator's start) is read one block ahead too;
+    // Z_{N-1} never went to memory, the forward sweep stored Z_0 .. Z_{N-2}.
+    d4 zn[RT][CT];
+    auto load_z = [&](int i) {
 #pragma unroll
         for (int rt = 0; rt < RT; rt++)
 #pragma unroll
             for (int t = 0; t < CT; t++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) Zc[rt][t][v] = Zw[zaddr(i, rt, t, v)];
+                for (int v = 0; v < 4; v++) zn[rt][t][v] = Zw[zaddr(i, rt, t, v)];
+    };
+    load_z(NT - 2);
+    emit(NT - 1, Zp);
+    for (int i = NT - 2; i >= 0; i--) {
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+            for (int t = 0; t < CT; t++) Zc[rt][t] = zn[rt][t];
+        const int in = i > 0 ? i - 1 : 0;
+        load_a(in, 1, a_nxt, c_nxt);
+        load_z(in);
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const int k = 4 * ks + q;
-            const double ce = st[NN + i * NT + k];
-            double a[RT];
-#pragma unroll
-            for (int rt = 0; rt < RT; rt++) a[rt] = Si[(rt * 16 + r) * NT + 4 * ks + q];
+            const double ce = c_all[ks];
+            const double (&a)[RT] = a_all[ks];
 #pragma unroll
             for (int t = 0; t < CT; t++) {
                 const double b = -ce * Zp[ks / 4][t][ks % 4];
 #pragma unroll
                 for (int rt = 0; rt < RT; rt++) Zc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b, Zc[rt][t], 0, 0, 0);
             }
+        }
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
+            for (int rt = 0; rt < RT; rt++) a_all[ks][rt] = a_nxt[ks][rt];
+            c_all[ks] = c_nxt[ks];
         }
         emit(i, Zc);
 #pragma unroll
