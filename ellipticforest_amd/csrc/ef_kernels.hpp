@@ -58,6 +58,7 @@ constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path
 template <int NB, bool FULL, int OP>
 __global__ void patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __restrict__ coef,
                                 const long long* __restrict__ g_off, const long long* __restrict__ h_off);
+template <bool AK, bool BK>
 __global__ void gemm_strided_f64_kernel(GemmStridedArgs a);
 __global__ void patch_large_fold_kernel(PatchLargeArgs a);
 __global__ void patch_large_scale_kernel(PatchLargeArgs a);

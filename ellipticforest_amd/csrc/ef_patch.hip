@@ -204,7 +204,18 @@ void launch_fd(ef_patches_s* p, int op, const double* g, const double* f, double
 // Large patches: chunks of items through fold, four GEMMs, scaling and the output step.
 void launch_gemm_strided(const GemmStridedArgs& g, hipStream_t s) {
     const long long blocks32 = (long long)((g.M + 31) / 32) * ((g.N + 31) / 32);  // one wave each
-    hipLaunchKernelGGL(gemm_strided_f64_kernel, dim3((unsigned)((blocks32 + 3) / 4), (unsigned)g.batch), dim3(256), 0, s, g);
+    const dim3 grid((unsigned)((blocks32 + 3) / 4), (unsigned)g.batch), block(256);
+    // an operand is read as 32-byte vectors along k when it is contiguous along k and every
+    // row / column it starts from is 32-byte aligned
+    auto aligned = [](const double* x, long long s0, long long s1) {
+        return ((uintptr_t)x % 32 == 0) && s0 % 4 == 0 && s1 % 4 == 0;
+    };
+    const bool ak = g.ca == 1 && aligned(g.A, g.sa, g.ra);
+    const bool bk = g.rb == 1 && aligned(g.B, g.sb, g.cb);
+    if (ak && bk) hipLaunchKernelGGL((gemm_strided_f64_kernel<true, true>), grid, block, 0, s, g);
+    else if (ak) hipLaunchKernelGGL((gemm_strided_f64_kernel<true, false>), grid, block, 0, s, g);
+    else if (bk) hipLaunchKernelGGL((gemm_strided_f64_kernel<false, true>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((gemm_strided_f64_kernel<false, false>), grid, block, 0, s, g);
     EF_LAUNCH_CHECK();
 }
 

@@ -19,10 +19,18 @@ namespace efk {
 
 typedef double ld4 __attribute__((ext_vector_type(4)));
 
+// AK: A is contiguous along k (ca == 1); BK: B is contiguous along k (rb == 1).
+template <bool AK, bool BK>
 __global__ void __launch_bounds__(256) gemm_strided_f64_kernel(GemmStridedArgs a) {
     // one wave per 32 x 32 block of C: 2 x 2 tiles, so each A and B value loaded feeds two
     // MFMAs.  On the last block row / column a missing second tile points at the first (loads
     // in range, the duplicate is not stored).
+    //
+    // Within each block of 16 k values, lane group q takes k = kb + 4q + s at step s (the
+    // reduction order is free as long as A and B agree): an operand contiguous along k is
+    // then read as 4 consecutive doubles per lane (two 16-byte loads, the four lane groups
+    // covering a row's 128 bytes), and one contiguous along the tile's rows or columns as 16
+    // consecutive doubles per lane group and step -- every cache line fetched is used whole.
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
     const int tn = (a.N + 31) / 32, tm = (a.M + 31) / 32;
@@ -36,15 +44,32 @@ __global__ void __launch_bounds__(256) gemm_strided_f64_kernel(GemmStridedArgs a
     const double* B0 = a.B + b * a.sb + (long long)(col0 + r) * a.cb;
     const double* B1 = c1 ? B0 + 16LL * a.cb : B0;
     ld4 c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
-    // K is a multiple of 16 (Np): four k-steps per iteration, their loads issued together
-#pragma unroll 4
-    for (int k0 = 0; k0 < a.K; k0 += 4) {
-        const long long ka = (long long)(k0 + q) * a.ca, kb = (long long)(k0 + q) * a.rb;
-        const double a0 = A0[ka], a1 = A1[ka], b0 = B0[kb], b1 = B1[kb];
-        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, c00, 0, 0, 0);
-        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, c01, 0, 0, 0);
-        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, c10, 0, 0, 0);
-        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, c11, 0, 0, 0);
+    // four consecutive k of operand X from k-index k (contiguous: one 32-byte aligned vector)
+    auto quad = [](const double* X, long long stride, int k, bool contiguous, double (&v)[4]) {
+        if (contiguous) {
+            const ld4 x = *(const ld4*)(X + k);
+            v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; v[3] = x[3];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) v[s] = X[(long long)(k + s) * stride];
+        }
+    };
+    // K is a multiple of 16 (Np)
+#pragma unroll 2
+    for (int kb = 0; kb < a.K; kb += 16) {
+        const int k = kb + 4 * q;
+        double a0[4], a1[4], b0[4], b1[4];
+        quad(A0, a.ca, k, AK, a0);
+        quad(A1, a.ca, k, AK, a1);
+        quad(B0, a.rb, k, BK, b0);
+        quad(B1, a.rb, k, BK, b1);
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s], b0[s], c00, 0, 0, 0);
+            c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s], b1[s], c01, 0, 0, 0);
+            c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[s], b0[s], c10, 0, 0, 0);
+            c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[s], b1[s], c11, 0, 0, 0);
+        }
     }
     double* C = a.C + b * a.sc;
     auto put = [&](const ld4& acc, int rb, int cb) {
@@ -56,6 +81,11 @@ __global__ void __launch_bounds__(256) gemm_strided_f64_kernel(GemmStridedArgs a
     if (r1) put(c10, row0 + 16, col0);
     if (r1 && c1) put(c11, row0 + 16, col0 + 16);
 }
+
+template __global__ void gemm_strided_f64_kernel<false, false>(GemmStridedArgs);
+template __global__ void gemm_strided_f64_kernel<true, false>(GemmStridedArgs);
+template __global__ void gemm_strided_f64_kernel<false, true>(GemmStridedArgs);
+template __global__ void gemm_strided_f64_kernel<true, true>(GemmStridedArgs);
 
 // item t of the chunk -> (patch, rhs, DtN column or -1)
 __device__ inline void large_item(const PatchLargeArgs& a, long long t, int& patch, int& rhs, int& col) {
