@@ -359,7 +359,8 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         const int i = 16 * bi + 4 * v + q, j = 16 * bj + r;
-                        if (FULL || (i < Nn && j < Nn)) u[i * Nn + j] = Y[bi][bj][v];
+                        // streaming stores: u is not read again by this launch
+                        if (FULL || (i < Nn && j < Nn)) __builtin_nontemporal_store(Y[bi][bj][v], gmem(u) + i * Nn + j);
                     }
         } else {
             // Neumann data (mapD2N, FiniteVolumeSolver.cpp:296-353):
