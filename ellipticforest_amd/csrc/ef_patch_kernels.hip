@@ -58,7 +58,7 @@ template <int NB>
 struct PatchItemIn {
     double F[NB][NB][4];
     double G[NB];
-    int ce;  // the item's coefficient entry (uniform)
+    int ce;       // the item's coefficient entry (uniform)
 };
 
 // Read-only index arrays of PatchFDArgs, passed to the kernel as separate __restrict__
@@ -74,7 +74,7 @@ struct PatchIdx {
 template <int NB, bool FULL, int OP>
 __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int patch, int rhs, int lane,
                                    PatchItemIn<NB>& in) {
-    const int Nn = p.N, q = lane >> 4, r = lane & 15;
+    const int Nn = FULL ? 16 * NB : p.N, q = lane >> 4, r = lane & 15;  // FULL: N = 16 NB
     const long long slot = (long long)rhs * p.npatches + patch;
     in.ce = ix.cls ? ix.cls[patch] : patch;
     if (FULL) {
@@ -169,7 +169,7 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
     // from it) in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
-    const int Nn = p.N;
+    const int Nn = FULL ? 16 * NB : p.N;  // FULL: N = 16 NB, a compile-time constant
     // items: (rhs, patch) rhs-major, or (patch, DtN column) patch-major
     const int inner = (OP == PATCH_OP_DTN) ? 4 * Nn : p.npatches;
     const int outer = (OP == PATCH_OP_DTN) ? p.npatches : p.nrhs;
