@@ -148,8 +148,9 @@ __device__ inline double patch_g(const PatchItemIn<NB>& in, int t, int col) {
     return (t < 64) ? a : b;
 }
 
-// Waves per SIMD: the N = 16 solve fits five (at most 102 registers) and the N = 16 Neumann
-// data four, which overlaps one wave's loads with another's MFMA chain.  NB = 2 fetches each
+// Waves per SIMD (minimum asked of the compiler): five for the N = 16 solve and four for the
+// N = 16 Neumann data, which overlaps one wave's loads with another's MFMA chain (both compile
+// to under 81 registers, six waves; seven would spill).  NB = 2 fetches each
 // item when its turn comes instead of one item ahead (a prefetched item costs 64 registers),
 // which fits two waves per SIMD in 256 registers: a second wave hides the loads better than
 // the prefetch did (N = 32, 4,096 patches x 4 right-hand sides: 186 -> 145 us).  The NB = 2
