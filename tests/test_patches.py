@@ -119,6 +119,17 @@ def test_emu_patch_argument_errors(emu_ctx):
     pt = _lib.Patches(emu_ctx, [[0, 1, 0, 1]], 8)
     with pytest.raises(_lib.EfError):
         pt.apply_device(7, None, None, None, 1)
+    # empty batch and no right-hand sides: argument errors, not empty results
+    with pytest.raises(_lib.EfError) as e:
+        _lib.Patches(emu_ctx, np.zeros((0, 4)), 16)
+    assert e.value.code == _lib.EF_ERR_ARG
+    with pytest.raises(_lib.EfError) as e:
+        pt.solve(np.zeros((0, 1, 32)), np.zeros((0, 1, 64)), 0)
+    assert e.value.code == _lib.EF_ERR_ARG
+    # no load and no boundary data (both NULL): the zero solution and zero Neumann data
+    p16 = _lib.Patches(emu_ctx, [[0, 1, 0, 1], [1, 2, 0, 1]], 16)
+    assert np.abs(p16.solve(None, None, 1)).max() == 0.0
+    assert np.abs(p16.map_d2n(None, None, 1)).max() == 0.0
 
 
 # ----- GPU: the product library through the C ABI -----------------------------------
