@@ -58,6 +58,7 @@ template <int NB>
 struct PatchItemIn {
     double F[NB][NB][4];
     double G[NB];
+    double T[4];  // N = 16 Neumann data / DtN: 1/D of this lane's modes from the entry's table
     int ce;       // the item's coefficient entry (uniform)
 };
 
@@ -93,6 +94,14 @@ __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int
             for (int bk = 0; bk < NB; bk++)
 #pragma unroll
                 for (int s = 0; s < 4; s++) in.F[bi][bk][s] = gmem(fb)[(16 * bk + 4 * s + q) * Nn + 16 * bi + r];
+        if (NB == 1 && OP != PATCH_OP_SOLVE) {
+            // the entry's 1/D table (the basis table when there is none: the kernel then
+            // computes 1/D and ignores these values).  Measured: it saves the Neumann-data
+            // variant 5 % and costs the solve 7 % (more registers, fewer waves), which computes.
+            const double* tb = p.dinv ? p.dinv + (long long)in.ce * 256 : p.QB;
+#pragma unroll
+            for (int v = 0; v < 4; v++) in.T[v] = gmem(tb)[(4 * v + q) * 16 + r];
+        }
         if (OP == PATCH_OP_DTN) return;
         const double* gb = p.QB;
         long long gs = 0;
@@ -315,14 +324,19 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
                      [&](int bk, int bj, int s) { return QB[bk][bj][s]; }, Y);
         // Z3 = Z2 ./ D; padded modes (k or l >= N) are zero
         if (NB == 1 && FULL) {
-            // computed here (no table load: every load of the loop is the prefetch's)
+            if (OP != PATCH_OP_SOLVE && p.dinv) {
+                // the table came with the item (a uniform branch, no memory access either side)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const double d = cf.cx * muk[0][v] + cf.cy * mul[0] + cf.lam;
-                double rc = __builtin_amdgcn_rcp(d);
-                rc = fma(rc, fma(-d, rc, 1.0), rc);
-                rc = fma(rc, fma(-d, rc, 1.0), rc);
-                Y[0][0][v] *= rc;
+                for (int v = 0; v < 4; v++) Y[0][0][v] *= cur.T[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const double d = cf.cx * muk[0][v] + cf.cy * mul[0] + cf.lam;
+                    double rc = __builtin_amdgcn_rcp(d);
+                    rc = fma(rc, fma(-d, rc, 1.0), rc);
+                    rc = fma(rc, fma(-d, rc, 1.0), rc);
+                    Y[0][0][v] *= rc;
+                }
             }
         } else
 #pragma unroll
