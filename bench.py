@@ -35,7 +35,7 @@ METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds
 # WRITE_SIZE x 1.0, factors from the copy kernel); profiles/patch_fd_kernel_adaptive_poisson_leafset.md and
 # profiles/bt_solve_sym_kernel_variable_beta_leafset.md have the commands and the raw values.
 TRAFFIC_PER_LAUNCH = 572.9e6          # leaf solves of this workload, 16 right-hand sides
-TRAFFIC_MAP_D2N = 318.9e6             # particular Neumann data, same leaf set and k
+TRAFFIC_MAP_D2N = 319.0e6             # particular Neumann data, same leaf set and k
 TRAFFIC_VAR_BETA = 1329.8e6           # variable-beta leaf solves, packed symmetric blocks
 
 
