@@ -55,7 +55,7 @@ void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mod
 // Packed storage of the symmetric inverse pivot blocks: entries per block.
 inline long long bt_packed_entries(int Nn) { return (long long)Nn * (Nn + 1) / 2; }
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)
-template <int NB, bool FULL, int OP>
+template <int NB, bool FULL, int OP, int NV = 0>
 __global__ void patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __restrict__ coef,
                                 const long long* __restrict__ g_off, const long long* __restrict__ h_off);
 template <bool AK, bool BK>

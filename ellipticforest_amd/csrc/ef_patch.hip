@@ -159,18 +159,26 @@ void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStre
     const bool full = (a.N == 16 * NB);
     const dim3 grid(blocks), block(256);
     const int op = a.op;
-#define EF_PATCH_LAUNCH(NB_, FULL_)                                                                         \
-    do {                                                                                                     \
-        if (op == PATCH_OP_SOLVE) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE>), grid, block, 0, s, a, a.cls, a.coef, a.g_off, a.h_off); \
-        else if (op == PATCH_OP_MAP_D2N) hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N>), grid, block, 0, s, a, a.cls, a.coef, a.g_off, a.h_off); \
-        else hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN>), grid, block, 0, s, a, a.cls, a.coef, a.g_off, a.h_off); \
+#define EF_PATCH_LAUNCH(NB_, FULL_, NV_)                                                                         \
+    do {                                                                                                          \
+        if (op == PATCH_OP_SOLVE)                                                                                 \
+            hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_SOLVE, NV_>), grid, block, 0, s, a, a.cls, a.coef, \
+                               a.g_off, a.h_off);                                                                 \
+        else if (op == PATCH_OP_MAP_D2N)                                                                          \
+            hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_MAP_D2N, NV_>), grid, block, 0, s, a, a.cls,   \
+                               a.coef, a.g_off, a.h_off);                                                         \
+        else                                                                                                      \
+            hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN, NV_>), grid, block, 0, s, a, a.cls, a.coef,   \
+                               a.g_off, a.h_off);                                                                 \
     } while (0)
+    // N = 8 (a common leaf size) has its own padded variant with the side known at compile time
     if (NB == 1) {
-        if (full) EF_PATCH_LAUNCH(1, true);
-        else EF_PATCH_LAUNCH(1, false);
+        if (full) EF_PATCH_LAUNCH(1, true, 0);
+        else if (a.N == 8) EF_PATCH_LAUNCH(1, false, 8);
+        else EF_PATCH_LAUNCH(1, false, 0);
     } else {
-        if (full) EF_PATCH_LAUNCH(2, true);
-        else EF_PATCH_LAUNCH(2, false);
+        if (full) EF_PATCH_LAUNCH(2, true, 0);
+        else EF_PATCH_LAUNCH(2, false, 0);
     }
 #undef EF_PATCH_LAUNCH
     EF_LAUNCH_CHECK();

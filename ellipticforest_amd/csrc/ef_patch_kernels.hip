@@ -75,10 +75,11 @@ struct PatchIdx {
 
 // HF, HG (unpadded variants): the launch has f, resp. g; without them nothing is loaded for them
 // and the values are zeros.
-template <int NB, bool FULL, int OP, bool HF = true, bool HG = true>
+template <int NB, bool FULL, int OP, bool HF = true, bool HG = true, int NV = 0>
 __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int patch, int rhs, int lane,
                                    PatchItemIn<NB>& in) {
-    const int Nn = FULL ? 16 * NB : p.N, q = lane >> 4, r = lane & 15;  // FULL: N = 16 NB
+    // FULL: N = 16 NB; NV: N fixed at compile time for a padded variant (0: p.N)
+    const int Nn = FULL ? 16 * NB : (NV ? NV : p.N), q = lane >> 4, r = lane & 15;
     const long long slot = (long long)rhs * p.npatches + patch;
     in.ce = ix.cls ? ix.cls[patch] : patch;
     if (FULL) {
@@ -174,7 +175,7 @@ constexpr int patch_fd_waves() {
     return OP == PATCH_OP_DTN ? 1 : 2;
 }
 
-template <int NB, bool FULL, int OP>
+template <int NB, bool FULL, int OP, int NV>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(patch_fd_waves<NB, FULL, OP>(), 8)))
 patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __restrict__ coef,
                 const long long* __restrict__ g_off, const long long* __restrict__ h_off) {
@@ -183,7 +184,7 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
     // from it) in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
-    const int Nn = FULL ? 16 * NB : p.N;  // FULL: N = 16 NB, a compile-time constant
+    const int Nn = FULL ? 16 * NB : (NV ? NV : p.N);  // compile-time for FULL (N = 16 NB) and NV
     // items: (rhs, patch) rhs-major, or (patch, DtN column) patch-major
     const int inner = (OP == PATCH_OP_DTN) ? 4 * Nn : p.npatches;
     const int outer = (OP == PATCH_OP_DTN) ? p.npatches : p.nrhs;
@@ -218,7 +219,7 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
         PatchItemIn<NB> cur{}, nxt{};
         if (PREFETCH) {
             const int patch = (OP == PATCH_OP_DTN) ? o : in_i, rhs = (OP == PATCH_OP_DTN) ? 0 : o;
-            patch_fetch<NB, FULL, OP, HF, HG>(p, ix, patch, rhs, lane, cur);
+            patch_fetch<NB, FULL, OP, HF, HG, NV>(p, ix, patch, rhs, lane, cur);
         }
         // One item: (o, in_i) from cur, with (o2, i2) fetched into nxt meanwhile (NB = 1).  The
         // loop below alternates the two buffers, so no item is ever copied between registers (a
@@ -228,13 +229,13 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
             const int patch = (OP == PATCH_OP_DTN) ? o : in_i;
             const int rhs = (OP == PATCH_OP_DTN) ? 0 : o;
             const int col = (OP == PATCH_OP_DTN) ? in_i : -1;
-            if (!PREFETCH) patch_fetch<NB, FULL, OP, HF, HG>(p, ix, patch, rhs, lane, cur);
+            if (!PREFETCH) patch_fetch<NB, FULL, OP, HF, HG, NV>(p, ix, patch, rhs, lane, cur);
             if (PREFETCH) {
                 // past the last item the current one is fetched again: the fetch is never skipped
                 const bool more = o2 < outer;
                 const int oo = more ? o2 : o, ii = more ? i2 : in_i;
                 const int patch2 = (OP == PATCH_OP_DTN) ? oo : ii, rhs2 = (OP == PATCH_OP_DTN) ? 0 : oo;
-                patch_fetch<NB, FULL, OP, HF, HG>(p, ix, patch2, rhs2, lane, nxt);
+                patch_fetch<NB, FULL, OP, HF, HG, NV>(p, ix, patch2, rhs2, lane, nxt);
                 // keep these loads ahead of the current item's work: issued later, they would
                 // shorten the time they have to arrive
                 __builtin_amdgcn_sched_barrier(0);
@@ -489,13 +490,18 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
     else run(T0{}, T0{});
 }
 
-#define EF_PATCH_INSTANCES(NB, FULL)                                              \
-    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_SOLVE>(PatchFDArgs, const int*, const PatchCoef*, const long long*, const long long*); \
-    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_MAP_D2N>(PatchFDArgs, const int*, const PatchCoef*, const long long*, const long long*); \
-    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_DTN>(PatchFDArgs, const int*, const PatchCoef*, const long long*, const long long*);
-EF_PATCH_INSTANCES(1, true)
-EF_PATCH_INSTANCES(1, false)
-EF_PATCH_INSTANCES(2, true)
-EF_PATCH_INSTANCES(2, false)
+#define EF_PATCH_INSTANCES(NB, FULL, NV)                                                                      \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_SOLVE, NV>(PatchFDArgs, const int*, const PatchCoef*, \
+                                                                      const long long*, const long long*);   \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_MAP_D2N, NV>(PatchFDArgs, const int*,              \
+                                                                        const PatchCoef*, const long long*, \
+                                                                        const long long*);                  \
+    template __global__ void patch_fd_kernel<NB, FULL, PATCH_OP_DTN, NV>(PatchFDArgs, const int*, const PatchCoef*, \
+                                                                    const long long*, const long long*);
+EF_PATCH_INSTANCES(1, true, 0)
+EF_PATCH_INSTANCES(1, false, 0)
+EF_PATCH_INSTANCES(1, false, 8)
+EF_PATCH_INSTANCES(2, true, 0)
+EF_PATCH_INSTANCES(2, false, 0)
 
 }  // namespace efk
