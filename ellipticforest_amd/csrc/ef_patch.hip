@@ -171,13 +171,14 @@ void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStre
             hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN, NV_>), grid, block, 0, s, a, a.cls, a.coef,   \
                                a.g_off, a.h_off);                                                                 \
     } while (0)
-    // N = 8 (a common leaf size) has its own padded variant with the side known at compile time
+    // N = 8 and 24 have their own padded variants with the side known at compile time
     if (NB == 1) {
         if (full) EF_PATCH_LAUNCH(1, true, 0);
         else if (a.N == 8) EF_PATCH_LAUNCH(1, false, 8);
         else EF_PATCH_LAUNCH(1, false, 0);
     } else {
         if (full) EF_PATCH_LAUNCH(2, true, 0);
+        else if (a.N == 24) EF_PATCH_LAUNCH(2, false, 24);
         else EF_PATCH_LAUNCH(2, false, 0);
     }
 #undef EF_PATCH_LAUNCH

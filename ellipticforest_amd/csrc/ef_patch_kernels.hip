@@ -503,5 +503,6 @@ EF_PATCH_INSTANCES(1, false, 0)
 EF_PATCH_INSTANCES(1, false, 8)
 EF_PATCH_INSTANCES(2, true, 0)
 EF_PATCH_INSTANCES(2, false, 0)
+EF_PATCH_INSTANCES(2, false, 24)
 
 }  // namespace efk
