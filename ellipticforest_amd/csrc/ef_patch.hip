@@ -171,10 +171,12 @@ void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStre
             hipLaunchKernelGGL((patch_fd_kernel<NB_, FULL_, PATCH_OP_DTN, NV_>), grid, block, 0, s, a, a.cls, a.coef,   \
                                a.g_off, a.h_off);                                                                 \
     } while (0)
-    // N = 8 and 24 have their own padded variants with the side known at compile time
+    // N = 4, 8, 12 and 24 have their own padded variants with the side known at compile time
     if (NB == 1) {
         if (full) EF_PATCH_LAUNCH(1, true, 0);
+        else if (a.N == 4) EF_PATCH_LAUNCH(1, false, 4);
         else if (a.N == 8) EF_PATCH_LAUNCH(1, false, 8);
+        else if (a.N == 12) EF_PATCH_LAUNCH(1, false, 12);
         else EF_PATCH_LAUNCH(1, false, 0);
     } else {
         if (full) EF_PATCH_LAUNCH(2, true, 0);

@@ -500,7 +500,9 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
                                                                     const long long*, const long long*);
 EF_PATCH_INSTANCES(1, true, 0)
 EF_PATCH_INSTANCES(1, false, 0)
+EF_PATCH_INSTANCES(1, false, 4)
 EF_PATCH_INSTANCES(1, false, 8)
+EF_PATCH_INSTANCES(1, false, 12)
 EF_PATCH_INSTANCES(2, true, 0)
 EF_PATCH_INSTANCES(2, false, 0)
 EF_PATCH_INSTANCES(2, false, 24)

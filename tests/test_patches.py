@@ -61,7 +61,7 @@ def check_against_oracle(pt, bounds, N, rng, nrhs=2, coeffs=None, lam=0.0, dtn=T
 
 
 # ----- CPU: the kernels through the emulation build ----------------------------------
-@pytest.mark.parametrize("N", [4, 8, 16])
+@pytest.mark.parametrize("N", [4, 8, 12, 16])
 @pytest.mark.parametrize("lam", [0.0, -1.0])
 def test_emu_fishpack_patches(emu_ctx, N, lam):
     from ellipticforest_amd import _lib
