@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libefhps.so")
 
-SOURCES = ["ef_kernels.hip", "ef_patch_kernels.hip", "ef_patch_large.hip", "ef_api.hip", "ef_patch.hip", "ef_mesh.cpp"]
+SOURCES = ["ef_kernels.hip", "ef_tree.hip", "ef_patch_kernels.hip", "ef_patch_large.hip", "ef_api.hip", "ef_patch.hip",
+           "ef_mesh.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
@@ -31,22 +32,35 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-# The device build path stays disabled until the cause of the GPU memory fault it raised is
-# shown; EF_DEVICE_BUILD=1 builds it enabled (for a run that tests a fix).
-DEVICE_BUILD_ENABLED = os.environ.get("EF_DEVICE_BUILD") == "1"
+def build(verbose=False, jobs=None):
+    """Compile every source to an object in parallel (hipcc, gfx950), then link the library."""
+    import concurrent.futures
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
 
+    def compile_one(name):
+        src = os.path.join(CSRC, name)
+        obj = os.path.join(objdir, name + ".o")
+        cmd = [hipcc()] + cflags + inc + ["-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {name}:\n{r.stdout}{r.stderr}")
+        return obj
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    defs = [] if DEVICE_BUILD_ENABLED else ["-DEF_DEVICE_BUILD_DISABLED"]
-    cmd = [hipcc()] + HIPCC_FLAGS + defs + ["-I", os.path.join(ROOT, "include"), "-I", CSRC] + srcs + [
-        "-o", LIB + ".tmp"]
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 4)))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libefhps.so")
+        raise RuntimeError("linking libefhps.so failed")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -4,7 +4,8 @@
 // The reference visits families one at a time through p4est callbacks
 // (Quadtree::merge/split, src/Quadtree.hpp:322-360) and keeps every operator in a
 // per-node std::vector.  Here the tree is planned once into level-ordered groups and
-// device arenas; each stage is a fixed sequence of batched launches per tree level.
+// device arenas; each stage is a fixed sequence of batched launches per tree level, and the
+// build reads the device back once (the zero-pivot flags of every merge) at its end.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,8 +20,9 @@
 #include <vector>
 
 #include "ef_hps.h"
-#include "ef_kernels.hpp"
 #include "ef_host.hpp"
+#include "ef_kernels.hpp"
+#include "ef_tree.hpp"
 
 using namespace efk;
 
@@ -40,17 +42,20 @@ struct NodeP {
     int s = 0;    // patch size after its own merge (grid nx): N for leaves
     int tag = 0;  // n_coarsens set by the parent's merge
     int n = 0;    // internal: child interface size
+    int group = -1, slot = -1;  // internal: merge group and position in it
     // operator arena offsets (elements)
     long long T_off = -1;               // internal: own DtN (8n x 8n)
     std::vector<long long> Tc_off;      // coarsened DtN after each coarsening step
-    long long S_off = -1, Xi_off = -1, H_off = -1;
+    long long S_off = -1, Xi_off = -1;
     // vector arena row offsets
     long long h_off = -1, g_off = -1, gc_off = -1, gt_off = -1, hd_off = -1, w_off = -1, ui_off = -1;
 };
 
-struct Group {  // merges at one depth with one interface size
+struct Group {  // merges at one depth with one interface size, contiguous in the arenas
     int depth = 0, n = 0;
     std::vector<int> nodes;
+    long long Xi0 = 0;  // Xi arena offset of the first merge (stride 16 n^2)
+    int info0 = 0;      // first zero-pivot flag of the group
 };
 
 struct Plan {
@@ -59,12 +64,13 @@ struct Plan {
     std::vector<NodeP> nodes;
     std::vector<int> leaves;                   // node ids, z-order
     std::vector<std::vector<int>> internal_at; // internal node ids per depth
-    std::vector<std::vector<Group>> groups;    // per depth
-    int maxdepth = 0;
-    long long T_elems = 0, S_elems = 0, Xi_elems = 0, H_elems = 0;
+    std::vector<Group> groups;                 // build order: deepest depth first
+    int maxdepth = 0, ninternal = 0;
+    long long T_elems = 0, S_elems = 0, Xi_elems = 0;
     long long vec_rows = 0;
-    long long work_X = 0, work_M = 0, work_U = 0;
-    int max_group = 0, max_n = 0;
+    long long work_X = 0;                      // max over groups of B m^2
+    long long gj_d = 0, gj_i = 0;              // GJ work (doubles, ints) of the largest group
+    int max_n = 0;
 };
 
 int build_node(Plan& P, int parent, int level, int ix, int iy, double xl, double xu, double yl, double yu,
@@ -104,20 +110,20 @@ int ilog2(int v) {
 }
 
 // Sizes and coarsening tags (tagPatchesForCoarsening_, HPSAlgorithm.hpp:676-696;
-// mergePatch_ :1004-1009 makes the parent 2n cells wide), then arena offsets.
+// mergePatch_ :1004-1009 makes the parent 2n cells wide), merge groups, arena offsets.
 void plan_tree(Plan& P, const ef_mesh_desc& d) {
     if (d.N < 4 || (d.N & 1)) throw EfError(EF_ERR_ARG, "N must be even and >= 4");
+    if (d.N > 32)
+        throw EfError(EF_ERR_ARG, "the mesh path takes leaf patches of up to 32 x 32 cells (single grids of any size "
+                                  "run on ef_patches_*)");
     if (d.nleaves < 1 || !d.level || !d.ix || !d.iy) throw EfError(EF_ERR_ARG, "empty leaf list");
     if (!(d.xu > d.xl) || !(d.yu > d.yl)) throw EfError(EF_ERR_ARG, "empty domain");
     P.N = d.N; P.xl = d.xl; P.xu = d.xu; P.yl = d.yl; P.yu = d.yu;
     int cursor = 0;
     build_node(P, -1, 0, 0, 0, d.xl, d.xu, d.yl, d.yu, d, cursor);
     if (cursor != d.nleaves) throw EfError(EF_ERR_ARG, "leaf list has entries beyond a complete quadtree");
-    // post-order sizes
-    std::vector<int> order;  // reverse pre-order is a valid children-first order
-    for (int i = (int)P.nodes.size() - 1; i >= 0; i--) order.push_back(i);
     P.maxdepth = 0;
-    for (int id : order) {
+    for (int id = (int)P.nodes.size() - 1; id >= 0; id--) {  // reverse pre-order: children first
         NodeP& nd = P.nodes[id];
         P.maxdepth = std::max(P.maxdepth, nd.level);
         if (nd.leaf) { nd.s = P.N; continue; }
@@ -131,42 +137,50 @@ void plan_tree(Plan& P, const ef_mesh_desc& d) {
         }
         P.max_n = std::max(P.max_n, nd.n);
     }
+    if (4LL * P.max_n > GJ_MAXM) throw EfError(EF_ERR_ARG, "interface systems larger than the inversion limit");
     P.internal_at.assign(P.maxdepth + 1, {});
     for (int id = 0; id < (int)P.nodes.size(); id++)
         if (!P.nodes[id].leaf) P.internal_at[P.nodes[id].level].push_back(id);
-    P.groups.assign(P.maxdepth + 1, {});
-    for (int dd = 0; dd <= P.maxdepth; dd++) {
+    // groups, deepest first; operator arenas in group order
+    long long T = 0, S = 0, Xi = 0;
+    int info = 0;
+    for (int dd = P.maxdepth; dd >= 0; dd--) {
         std::map<int, std::vector<int>> byn;
         for (int id : P.internal_at[dd]) byn[P.nodes[id].n].push_back(id);
         for (auto& kv : byn) {
             Group g;
             g.depth = dd; g.n = kv.first; g.nodes = kv.second;
-            P.groups[dd].push_back(g);
-            const long long nm = (long long)g.nodes.size(), n = g.n;
-            P.work_X = std::max(P.work_X, nm * 16 * n * n);
-            P.work_M = std::max(P.work_M, nm * 4 * n * GJ_NB);
-            P.work_U = std::max(P.work_U, nm * GJ_NB * 16 * n);
-            P.max_group = std::max(P.max_group, (int)nm);
+            g.Xi0 = Xi;
+            g.info0 = info;
+            const long long n = g.n, B = (long long)g.nodes.size();
+            for (size_t i = 0; i < g.nodes.size(); i++) {
+                NodeP& nd = P.nodes[g.nodes[i]];
+                nd.group = (int)P.groups.size();
+                nd.slot = (int)i;
+                nd.T_off = T; T += 64 * n * n;
+                nd.S_off = S; S += 32 * n * n;
+                nd.Xi_off = Xi; Xi += 16 * n * n;
+            }
+            info += (int)B;
+            P.work_X = std::max(P.work_X, B * 16 * n * n);
+            P.gj_d = std::max(P.gj_d, (long long)gj_work_doubles((int)(4 * n), (int)B));
+            P.gj_i = std::max(P.gj_i, (long long)gj_work_ints((int)(4 * n), (int)B));
+            P.groups.push_back(g);
         }
     }
-    // operator arenas
-    long long T = 0, S = 0, Xi = 0, H = 0, rows = 0;
-    for (auto& nd : P.nodes) {
-        if (!nd.leaf) {
-            const long long n = nd.n;
-            nd.T_off = T; T += 64 * n * n;
-            nd.S_off = S; S += 32 * n * n;
-            nd.Xi_off = Xi; Xi += 16 * n * n;
-            nd.H_off = H; H += 16 * n * n;
-            int m = nd.s;
-            nd.Tc_off.clear();
-            for (int t = 0; t < nd.tag; t++) {
-                m /= 2;
-                nd.Tc_off.push_back(T);
-                T += 16LL * m * m;
-            }
+    P.ninternal = info;
+    for (auto& nd : P.nodes) {  // coarsened copies of tagged nodes
+        nd.Tc_off.clear();
+        int m = nd.s;
+        for (int t = 0; t < nd.tag; t++) {
+            m /= 2;
+            nd.Tc_off.push_back(T);
+            T += 16LL * m * m;
         }
-        // vectors
+    }
+    // vector arena rows
+    long long rows = 0;
+    for (auto& nd : P.nodes) {
         nd.h_off = rows; rows += 4LL * nd.s;
         nd.g_off = rows; rows += 4LL * nd.s;
         if (nd.tag > 0) { nd.gc_off = rows; rows += 4LL * (nd.s >> nd.tag); }
@@ -178,19 +192,32 @@ void plan_tree(Plan& P, const ef_mesh_desc& d) {
             nd.ui_off = rows; rows += 4LL * nd.n;
         }
     }
-    P.T_elems = T; P.S_elems = S; P.Xi_elems = Xi; P.H_elems = H; P.vec_rows = rows;
+    P.T_elems = T; P.S_elems = S; P.Xi_elems = Xi; P.vec_rows = rows;
 }
 
 // ---------------------------------------------------------------------------
 // Context and mesh
 // ---------------------------------------------------------------------------
-struct SweepPlan {  // per right-hand-side count k
-    std::vector<std::unique_ptr<DevBuf<UpDesc>>> up_own;
-    std::vector<std::unique_ptr<DevBuf<GemmProb>>> up_w, up_h, down_u;
-    std::vector<std::unique_ptr<DevBuf<DownDesc>>> down;
-    std::vector<std::vector<std::unique_ptr<DevBuf<VecStepDesc>>>> unc;  // [depth][step]
-    std::vector<int> up_nmax, up_cnt, w_tiles, h_tiles, u_tiles, down_nmax, down_cnt;
-    std::vector<std::vector<int>> unc_cnt, unc_nf;
+struct GroupPlan {  // device descriptors of one merge group
+    std::vector<const CoarsenDesc*> csteps;
+    std::vector<int> csteps_cnt, csteps_maxm;
+    const AsmDesc* asmd = nullptr;
+    GemmLaunch gS, gT;
+};
+
+struct LevelSweep {  // device descriptors of one depth of the sweeps, k right-hand sides
+    int cnt = 0, nmax = 0;
+    const UpDesc* up = nullptr;
+    GemmLaunch gw, gh;
+    const DownDesc* down = nullptr;
+    GemmLaunch gu;
+    std::vector<const VecStepDesc*> unc;
+    std::vector<int> unc_cnt, unc_nf;
+};
+
+struct SweepPlan {
+    std::vector<LevelSweep> lv;
+    DevStore store;
 };
 
 }  // namespace
@@ -219,15 +246,12 @@ struct ef_mesh_s {
     bool packed = false;                   // variable and symmetric: packed inverse pivot blocks
     DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork, vz;
     // operator arenas
-    DevBuf<double> Tar, Sar, Xiar, Har;
-    // work
-    DevBuf<double> wX, wM, wU;
-    DevBuf<int> wpiv, winfo;
-    // build descriptors per group
-    std::vector<std::vector<std::unique_ptr<DevBuf<MergeDesc>>>> mdesc;
-    std::vector<std::vector<std::unique_ptr<DevBuf<GJDesc>>>> gjdesc;
-    std::vector<std::vector<std::unique_ptr<DevBuf<GemmProb>>>> tgemm;
-    std::vector<std::vector<std::vector<std::unique_ptr<DevBuf<CoarsenDesc>>>>> cdesc;  // [depth][group][step]
+    DevBuf<double> Tar, Sar, Xiar;
+    // build work: X of a group, Gauss-Jordan work, zero-pivot flags of every merge
+    DevBuf<double> wX, gjd;
+    DevBuf<int> gji, info;
+    std::vector<GroupPlan> gplans;
+    DevStore bstore;
     // vectors
     int kcap = 0;
     int k_up = 0;  // k of the last upwards call
@@ -236,10 +260,8 @@ struct ef_mesh_s {
     const double* f_last = nullptr;
     std::map<int, std::unique_ptr<SweepPlan>> sweeps;
     // root Robin work
-    DevBuf<double> rootA;
-    DevBuf<double> rootM, rootU;
-    DevBuf<int> rootpiv, rootinfo;
-    DevBuf<GJDesc> rootgj;
+    DevBuf<double> rootA, rootAi, rootgj, rootrhs;
+    DevBuf<int> rootgji, rootinfo;
     // stage timers: event pairs for build, upwards, solve
     hipEvent_t ev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     bool ev_recorded[3] = {false, false, false};
@@ -267,46 +289,19 @@ const double* full_T_ptr(ef_mesh_s* M, int id) {
     return M->Tar.p + nd.T_off;
 }
 
-void launch_gemm(const DevBuf<GemmProb>& probs, int nprob, int max_tiles, hipStream_t s) {
-    if (nprob == 0 || max_tiles == 0) return;
-    hipLaunchKernelGGL(gemm_f64_kernel, dim3(max_tiles, nprob), dim3(256), 0, s, probs.p);
-    EF_LAUNCH_CHECK();
-}
-
-int tiles_of(int M_, int N_) { return ((M_ + 63) / 64) * ((N_ + 63) / 64); }
-
-// gj_panel_lds_kernel keeps up to GJ_LDS_ROWS x (GJ_NB + 1) doubles of the panel in dynamic
-// LDS (118 KiB), above the 64 KiB a launch gets without raising the kernel's limit; gfx950
-// has 160 KiB per workgroup, static plus dynamic.  Raised once, on first need.
-void raise_panel_lds_limit(size_t dyn_bytes) {
-    static bool done = false;
-    if (done || dyn_bytes <= 64 * 1024) return;
-    hipFuncAttributes attr;
-    EF_HIP(hipFuncGetAttributes(&attr, (const void*)gj_panel_lds_kernel));
-    const int dyn = 160 * 1024 - (int)attr.sharedSizeBytes;
-    EF_HIP(hipFuncSetAttribute((const void*)gj_panel_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-    done = true;
-}
-
-// Gauss-Jordan over a batch of equally sized systems.
-void run_gj(const GJDesc* d_dev, int nsys, int m, int maxw, hipStream_t s) {
-    for (int k = 0; k < m; k += GJ_NB) {
-        const int nbb = std::min(GJ_NB, m - k);
-        if (m - k <= GJ_LDS_ROWS) {
-            const size_t lds = (size_t)(m - k) * (GJ_NB + 1) * sizeof(double);
-            raise_panel_lds_limit(lds);
-            hipLaunchKernelGGL(gj_panel_lds_kernel, dim3(nsys), dim3(256), lds, s, d_dev, k);
-        } else {
-            hipLaunchKernelGGL(gj_panel_global_kernel, dim3(nsys), dim3(256), 0, s, d_dev, k);
-        }
-        EF_LAUNCH_CHECK();
-        const int wtot = (m - k - nbb) + maxw;
-        hipLaunchKernelGGL(gj_swap_kernel, dim3(blocks_for(wtot), nsys), dim3(256), 0, s, d_dev, k);
-        EF_LAUNCH_CHECK();
-        const int wmax = std::max(m - k - nbb, maxw);
-        hipLaunchKernelGGL(gj_update_kernel, dim3(tiles_of(m, wmax), nsys * 3), dim3(256), 0, s, d_dev, k);
-        EF_LAUNCH_CHECK();
-    }
+GJWork gj_work(ef_mesh_s* M, int m, int B) {
+    GJWork w;
+    double* d = M->gjd.p;
+    w.wb = d; d += (size_t)B * m * m;
+    w.Pinv = d; d += (size_t)B * GJ_NB * GJ_NB;
+    w.Mm = d;
+    int* i = M->gji.p;
+    w.cand = i; i += (size_t)B * GJ_MAXCH * GJ_NB;
+    w.cfin = i; i += (size_t)B * GJ_NB;
+    w.src = i; i += (size_t)B * m;
+    w.ipiv = i; i += (size_t)B * m;
+    w.prow = i;
+    return w;
 }
 
 // ----- build ------------------------------------------------------------------
@@ -430,95 +425,99 @@ PatchFDArgs leaf_fd_args(ef_mesh_s* M, int op) {
     return a;
 }
 
-void plan_build_descriptors(ef_mesh_s* M) {
+// Descriptors of every merge group: coarsening steps, assembly, the S and T products.
+void plan_build(ef_mesh_s* M) {
     Plan& P = M->P;
-    M->mdesc.clear();
-    M->mdesc.resize(P.maxdepth + 1);
-    M->gjdesc.clear();
-    M->gjdesc.resize(P.maxdepth + 1);
-    M->tgemm.clear();
-    M->tgemm.resize(P.maxdepth + 1);
-    M->cdesc.clear();
-    M->cdesc.resize(P.maxdepth + 1);
-    for (int dd = P.maxdepth; dd >= 0; dd--) {
-        for (auto& g : P.groups[dd]) {
-            const int n = g.n;
-            const int nm = (int)g.nodes.size();
-            // coarsening steps of tagged children
-            std::vector<std::vector<CoarsenDesc>> steps;
-            for (int id : g.nodes)
-                for (int c = 0; c < 4; c++) {
-                    const NodeP& ch = P.nodes[P.nodes[id].child[c]];
-                    for (int t = 0; t < ch.tag; t++) {
-                        if ((int)steps.size() <= t) steps.resize(t + 1);
-                        CoarsenDesc cd;
-                        const int m = ch.s >> t;
-                        cd.m = m;
-                        cd.src = (t == 0) ? full_T_ptr(M, P.nodes[id].child[c]) : M->Tar.p + ch.Tc_off[t - 1];
-                        cd.dst = M->Tar.p + ch.Tc_off[t];
-                        steps[t].push_back(cd);
-                    }
-                }
-            std::vector<std::unique_ptr<DevBuf<CoarsenDesc>>> cv;
-            for (auto& st : steps) {
-                auto b = std::make_unique<DevBuf<CoarsenDesc>>();
-                b->upload(st, M->stream);
-                cv.push_back(std::move(b));
-            }
-            M->cdesc[dd].push_back(std::move(cv));
-            // merges
-            std::vector<MergeDesc> md(nm);
-            std::vector<GJDesc> gj(nm);
-            std::vector<GemmProb> tg;
-            for (int i = 0; i < nm; i++) {
-                const NodeP& nd = P.nodes[g.nodes[i]];
-                MergeDesc& m = md[i];
-                m.n = n;
-                for (int c = 0; c < 4; c++) m.Tc[c] = merge_T_ptr(M, nd.child[c]);
-                m.X = M->wX.p + (long long)i * 16 * n * n;
-                m.S = M->Sar.p + nd.S_off;
-                m.Xi = M->Xiar.p + nd.Xi_off;
-                m.H = M->Har.p + nd.H_off;
-                m.T = M->Tar.p + nd.T_off;
-                GJDesc& G = gj[i];
-                G.m = 4 * n;
-                G.X = m.X; G.ldx = 4 * n;
-                G.R1 = m.S; G.w1 = 8 * n; G.ld1 = 8 * n;
-                G.R2 = m.Xi; G.w2 = 4 * n; G.ld2 = 4 * n;
-                G.M = M->wM.p + (long long)i * 4 * n * GJ_NB;
-                G.ldu = 16 * n;
-                G.U = M->wU.p + (long long)i * GJ_NB * 16 * n;
-                G.piv = M->wpiv.p + (long long)i * GJ_NB;
-                G.info = M->winfo.p + i;
-                // T = blkdiag + H S, one problem per parent boundary block q (mergeT_, :940-968)
-                static const int QC[8] = {0, 2, 1, 3, 0, 1, 2, 3};
-                static const int TAUQ[4][2] = {{0, 4}, {2, 5}, {1, 6}, {3, 7}};
-                static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
-                for (int q = 0; q < 8; q++) {
-                    const int c = QC[q];
-                    const int hrow = (TAUQ[c][0] == q) ? 0 : 1;
-                    GemmProb p{};
-                    const double* Hc = m.H + (long long)c * 4 * n * n + (long long)hrow * n * 2 * n;
-                    p.A = Hc; p.lda = 2 * n; p.K = n;
-                    p.B = m.S + (long long)CHIF[c][0] * n * 8 * n; p.ldb = 8 * n;
-                    p.A2 = Hc + n; p.lda2 = 2 * n; p.K2 = n;
-                    p.B2 = m.S + (long long)CHIF[c][1] * n * 8 * n; p.ldb2 = 8 * n;
-                    p.D = m.T + (long long)q * n * 8 * n; p.ldd = 8 * n;
-                    p.Cin = p.D; p.ldcin = 8 * n;
-                    p.M = n; p.N = 8 * n; p.alpha = 1.0; p.beta = 1.0;
-                    tg.push_back(p);
+    hipStream_t st = M->stream;
+    M->gplans.clear();
+    M->bstore.clear();
+    for (const Group& g : P.groups) {
+        GroupPlan gp;
+        const int n = g.n, B = (int)g.nodes.size();
+        const long long n2 = (long long)n * n;
+        // coarsening steps of tagged children (coarsen_, HPSAlgorithm.hpp:707-741)
+        std::vector<std::vector<CoarsenDesc>> steps;
+        std::vector<int> maxm;
+        for (int id : g.nodes)
+            for (int c = 0; c < 4; c++) {
+                const NodeP& ch = P.nodes[P.nodes[id].child[c]];
+                for (int t = 0; t < ch.tag; t++) {
+                    if ((int)steps.size() <= t) { steps.resize(t + 1); maxm.resize(t + 1, 0); }
+                    CoarsenDesc cd;
+                    const int m = ch.s >> t;
+                    cd.m = m;
+                    cd.src = (t == 0) ? full_T_ptr(M, P.nodes[id].child[c]) : M->Tar.p + ch.Tc_off[t - 1];
+                    cd.dst = M->Tar.p + ch.Tc_off[t];
+                    check_range(cd.src, 16ULL * m * m * 8, "coarsen source");
+                    check_range(cd.dst, 4ULL * m * m * 8, "coarsen destination");
+                    steps[t].push_back(cd);
+                    maxm[t] = std::max(maxm[t], m);
                 }
             }
-            auto b1 = std::make_unique<DevBuf<MergeDesc>>();
-            b1->upload(md, M->stream);
-            M->mdesc[dd].push_back(std::move(b1));
-            auto b2 = std::make_unique<DevBuf<GJDesc>>();
-            b2->upload(gj, M->stream);
-            M->gjdesc[dd].push_back(std::move(b2));
-            auto b3 = std::make_unique<DevBuf<GemmProb>>();
-            b3->upload(tg, M->stream);
-            M->tgemm[dd].push_back(std::move(b3));
+        for (size_t t = 0; t < steps.size(); t++) {
+            gp.csteps.push_back(M->bstore.put(steps[t], st));
+            gp.csteps_cnt.push_back((int)steps[t].size());
+            gp.csteps_maxm.push_back(maxm[t]);
         }
+        // assembly into the group's X work (slot b at b * 16 n^2) and each parent's T
+        std::vector<AsmDesc> ad(B);
+        std::vector<TreeGemm> pS, pT;
+        for (int b = 0; b < B; b++) {
+            const NodeP& nd = P.nodes[g.nodes[b]];
+            AsmDesc& a = ad[b];
+            a.n = n;
+            a.pad = 0;
+            for (int c = 0; c < 4; c++) {
+                a.Tc[c] = merge_T_ptr(M, nd.child[c]);
+                check_range(a.Tc[c], 16ULL * n2 * 8, "child DtN of a merge");
+            }
+            a.X = M->wX.p + (long long)b * 16 * n2;
+            a.T = M->Tar.p + nd.T_off;
+            check_range(a.X, 16ULL * n2 * 8, "merge X work");
+            check_range(a.T, 64ULL * n2 * 8, "merge T");
+            const double* Xi = M->Xiar.p + nd.Xi_off;
+            double* Sm = M->Sar.p + nd.S_off;
+            double* Tm = M->Tar.p + nd.T_off;
+            // S (4n x 8n, WESN columns) = X^-1 S_RHS; column block q of S_RHS is nonzero in the
+            // two interface rows r of child c = q_child(q): -T_c(side on r, side q) when c is r's
+            // first child, +T_c(...) when its second (mergeS_, HPSAlgorithm.hpp:906-929)
+            for (int q = 0; q < 8; q++) {
+                const int c = q_child(q), sq = q_side(q);
+                const double* Tc = a.Tc[c];
+                TreeGemm p{};
+                const int r0 = child_if(c, 0), r1 = child_if(c, 1);
+                p.A0 = Xi + (long long)r0 * n; p.lda0 = 4 * n; p.K0 = n;
+                p.B0 = Tc + (long long)side_on_if(c, r0) * n * 4 * n + (long long)sq * n; p.ldb0 = 4 * n;
+                p.alpha0 = (if_c1(r0) == c) ? -1.0 : 1.0;
+                p.A1 = Xi + (long long)r1 * n; p.lda1 = 4 * n; p.K1 = n;
+                p.B1 = Tc + (long long)side_on_if(c, r1) * n * 4 * n + (long long)sq * n; p.ldb1 = 4 * n;
+                p.alpha1 = (if_c1(r1) == c) ? -1.0 : 1.0;
+                p.C = nullptr; p.beta = 0.0;
+                p.D = Sm + (long long)q * n; p.ldd = 8 * n;
+                p.M = 4 * n; p.N = n;
+                pS.push_back(p);
+            }
+            // T (8n x 8n) += H S: block row q is child c's DtN rows of side q against its two
+            // interface sides, times the matching rows of S (mergeT_, :940-968, WESN rows)
+            for (int q = 0; q < 8; q++) {
+                const int c = q_child(q), sq = q_side(q);
+                const double* Tc = a.Tc[c];
+                TreeGemm p{};
+                const int r0 = child_if(c, 0), r1 = child_if(c, 1);
+                p.A0 = Tc + (long long)sq * n * 4 * n + (long long)side_on_if(c, r0) * n; p.lda0 = 4 * n; p.K0 = n;
+                p.B0 = Sm + (long long)r0 * n * 8 * n; p.ldb0 = 8 * n; p.alpha0 = 1.0;
+                p.A1 = Tc + (long long)sq * n * 4 * n + (long long)side_on_if(c, r1) * n; p.lda1 = 4 * n; p.K1 = n;
+                p.B1 = Sm + (long long)r1 * n * 8 * n; p.ldb1 = 8 * n; p.alpha1 = 1.0;
+                p.D = Tm + (long long)q * n * 8 * n; p.ldd = 8 * n;
+                p.C = p.D; p.ldc = 8 * n; p.beta = 1.0;
+                p.M = n; p.N = 8 * n;
+                pT.push_back(p);
+            }
+        }
+        gp.asmd = M->bstore.put(ad, st);
+        gp.gS = plan_gemm(pS, n <= 16 ? 1 : 2, M->bstore, st);
+        gp.gT = plan_gemm(pT, 2, M->bstore, st);
+        M->gplans.push_back(std::move(gp));
     }
 }
 
@@ -534,23 +533,44 @@ void stage_end(ef_mesh_s* M, int st) {
     M->ev_recorded[st] = true;
 }
 
+void check_tree_kernels() {
+    static bool done = false;
+    if (done) return;
+    check_kernel_resources((const void*)tree_gemm_kernel<1>, "tree_gemm_kernel<1>");
+    check_kernel_resources((const void*)tree_gemm_kernel<2>, "tree_gemm_kernel<2>");
+    check_kernel_resources((const void*)gj_panel_leaf_kernel<0>, "gj_panel_leaf_kernel<0>");
+    check_kernel_resources((const void*)gj_panel_leaf_kernel<1>, "gj_panel_leaf_kernel<1>");
+    check_kernel_resources((const void*)gj_panel_final_kernel, "gj_panel_final_kernel");
+    check_kernel_resources((const void*)gj_mult_kernel, "gj_mult_kernel");
+    check_kernel_resources((const void*)gj_update_kernel, "gj_update_kernel");
+    check_kernel_resources((const void*)gj_perm_kernel, "gj_perm_kernel");
+    check_kernel_resources((const void*)gj_gather_kernel, "gj_gather_kernel");
+    check_kernel_resources((const void*)tree_assemble_kernel, "tree_assemble_kernel");
+    check_kernel_resources((const void*)tree_coarsen_kernel, "tree_coarsen_kernel");
+    check_kernel_resources((const void*)tree_up_gather_kernel, "tree_up_gather_kernel");
+    check_kernel_resources((const void*)tree_uncoarsen_kernel, "tree_uncoarsen_kernel");
+    check_kernel_resources((const void*)tree_down_scatter_kernel, "tree_down_scatter_kernel");
+    check_kernel_resources((const void*)tree_root_dirichlet_kernel, "tree_root_dirichlet_kernel");
+    check_kernel_resources((const void*)tree_root_robin_kernel, "tree_root_robin_kernel");
+    done = true;
+}
+
 void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     Plan& P = M->P;
     hipStream_t s = M->stream;
     const int N = P.N;
+    check_tree_kernels();
     M->solver_type = c.solver_type;
     setup_leaf_classes(M, c);
-    // arenas
+    // arenas and work
     M->Tar.alloc(std::max<long long>(P.T_elems, 1));
     M->Sar.alloc(std::max<long long>(P.S_elems, 1));
     M->Xiar.alloc(std::max<long long>(P.Xi_elems, 1));
-    M->Har.alloc(std::max<long long>(P.H_elems, 1));
     M->wX.alloc(std::max<long long>(P.work_X, 1));
-    M->wM.alloc(std::max<long long>(P.work_M, 1));
-    M->wU.alloc(std::max<long long>(P.work_U, 1));
-    M->wpiv.alloc(std::max<long long>((long long)P.max_group * GJ_NB, 1));
-    M->winfo.alloc(std::max(P.max_group, 1));
-    plan_build_descriptors(M);
+    M->gjd.alloc(std::max<long long>(P.gj_d, 1));
+    M->gji.alloc(std::max<long long>(P.gj_i, 1));
+    M->info.alloc(std::max(P.ninternal, 1));
+    plan_build(M);
     if (M->variable) {
         // block-tridiagonal factors, then the DtN of every leaf (buildD2N) in chunks of leaves
         const int nl = (int)P.leaves.size();
@@ -574,43 +594,34 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
         a.out = M->leafT.p;
         launch_patch_fd(a, N <= 16 ? 1 : 2, patch_default_blocks(), s);
     }
+    if (sync_check()) EF_HIP(hipStreamSynchronize(s));
     // merges, deepest level first (post-order dependencies)
-    EF_HIP(hipMemsetAsync(M->winfo.p, 0, sizeof(int) * std::max(P.max_group, 1), s));
-    std::vector<int> info_h;
-    for (int dd = P.maxdepth; dd >= 0; dd--) {
-        for (size_t gi = 0; gi < P.groups[dd].size(); gi++) {
-            const Group& g = P.groups[dd][gi];
-            const int n = g.n, nm = (int)g.nodes.size();
-            auto& steps = M->cdesc[dd][gi];
-            for (size_t t = 0; t < steps.size(); t++) {
-                // largest coarsened block of this step
-                int maxm = 0;
-                for (int id : g.nodes)
-                    for (int c = 0; c < 4; c++) {
-                        const NodeP& ch = P.nodes[P.nodes[id].child[c]];
-                        if ((int)t < ch.tag) maxm = std::max(maxm, ch.s >> t);
-                    }
-                const long long out = (long long)(2 * maxm) * (2 * maxm);
-                hipLaunchKernelGGL(coarsen_kernel, dim3(blocks_for(out), (unsigned)steps[t]->n), dim3(256), 0, s,
-                                   steps[t]->p);
-                EF_LAUNCH_CHECK();
-            }
-            hipLaunchKernelGGL(merge_assemble_kernel, dim3(blocks_for(144LL * n * n), nm), dim3(256), 0, s,
-                               M->mdesc[dd][gi]->p);
-            EF_LAUNCH_CHECK();
-            run_gj(M->gjdesc[dd][gi]->p, nm, 4 * n, 12 * n, s);
-            launch_gemm(*M->tgemm[dd][gi], 8 * nm, tiles_of(n, 8 * n), s);
-            // zero-pivot report of this group
-            info_h.assign(nm, 0);
-            efk::copy_d2h(info_h.data(), M->winfo.p, sizeof(int) * nm, s);
-            EF_HIP(hipStreamSynchronize(s));
-            for (int i = 0; i < nm; i++)
-                if (info_h[i] != 0)
-                    throw EfError(EF_ERR_SINGULAR, "zero pivot in the interface system of a merge at depth " +
-                                                       std::to_string(dd) + " (column " + std::to_string(info_h[i]) + ")");
+    EF_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int) * std::max(P.ninternal, 1), s));
+    for (size_t gi = 0; gi < P.groups.size(); gi++) {
+        const Group& g = P.groups[gi];
+        const GroupPlan& gp = M->gplans[gi];
+        const int n = g.n, B = (int)g.nodes.size(), m = 4 * n;
+        for (size_t t = 0; t < gp.csteps.size(); t++) {
+            const long long out = 4LL * gp.csteps_maxm[t] * gp.csteps_maxm[t];
+            EF_LAUNCH("tree_coarsen_kernel", tree_coarsen_kernel, dim3(blocks_for(out), gp.csteps_cnt[t]), dim3(256), s,
+                      gp.csteps[t]);
         }
+        EF_LAUNCH("tree_assemble_kernel", tree_assemble_kernel, dim3(blocks_for(80LL * n * n), B), dim3(256), s, gp.asmd);
+        run_gj_inverse(M->wX.p, M->Xiar.p + g.Xi0, m, B, gj_work(M, m, B), M->info.p + g.info0, s);
+        run_gemm(gp.gS, s);
+        run_gemm(gp.gT, s);
     }
-    EF_HIP(hipStreamSynchronize(s));
+    // zero pivots of any merge: one read at the end of the stage
+    std::vector<int> info(std::max(P.ninternal, 1), 0);
+    efk::copy_d2h(info.data(), M->info.p, sizeof(int) * info.size(), s);
+    for (size_t gi = 0; gi < P.groups.size(); gi++) {
+        const Group& g = P.groups[gi];
+        for (size_t b = 0; b < g.nodes.size(); b++)
+            if (info[g.info0 + b] != 0)
+                throw EfError(EF_ERR_SINGULAR, "zero pivot in the interface system of a merge at depth " +
+                                                   std::to_string(g.depth) + " (column " +
+                                                   std::to_string(info[g.info0 + b]) + ")");
+    }
     M->built = true;
 }
 
@@ -620,31 +631,27 @@ void ensure_vectors(ef_mesh_s* M, int k) {
     M->vec.alloc((size_t)M->P.vec_rows * k);
     EF_HIP(hipMemsetAsync(M->vec.p, 0, (size_t)M->P.vec_rows * k * sizeof(double), M->stream));
     M->kcap = k;
-    M->sweeps.clear();  // offsets scale with k only, but keep plans tied to the arena
+    M->sweeps.clear();  // descriptors point into the arena
 }
 
 SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
     auto it = M->sweeps.find(k);
     if (it != M->sweeps.end()) return *it->second;
     Plan& P = M->P;
+    hipStream_t st = M->stream;
     auto sp = std::make_unique<SweepPlan>();
     double* V = M->vec.p;
+    const long long vrows = P.vec_rows;
+    auto vcheck = [&](long long off, long long rows, const char* what) {
+        if (off < 0 || off + rows > vrows) throw EfError(EF_ERR_INTERNAL, std::string("vector arena range: ") + what);
+    };
     const int D = P.maxdepth + 1;
-    sp->up_own.resize(D);
-    sp->up_w.resize(D);
-    sp->up_h.resize(D);
-    sp->down.resize(D);
-    sp->down_u.resize(D);
-    sp->unc.resize(D);
-    sp->up_nmax.assign(D, 0); sp->up_cnt.assign(D, 0); sp->w_tiles.assign(D, 0); sp->h_tiles.assign(D, 0);
-    sp->u_tiles.assign(D, 0); sp->down_nmax.assign(D, 0); sp->down_cnt.assign(D, 0);
-    sp->unc_cnt.assign(D, {}); sp->unc_nf.assign(D, {});
-    static const int QC[8] = {0, 2, 1, 3, 0, 1, 2, 3};
-    static const int TAUQ[4][2] = {{0, 4}, {2, 5}, {1, 6}, {3, 7}};
-    static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
+    const int tn = k <= 16 ? 1 : 2;
+    sp->lv.resize(D);
     for (int dd = 0; dd < D; dd++) {
+        LevelSweep& L = sp->lv[dd];
         std::vector<UpDesc> ud;
-        std::vector<GemmProb> pw, ph, pu;
+        std::vector<TreeGemm> pw, ph, pu;
         std::vector<DownDesc> dn;
         std::vector<std::vector<VecStepDesc>> us;
         for (int id : P.internal_at[dd]) {
@@ -657,80 +664,81 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 u.tag[c] = ch.tag;
                 u.clen[c] = 4 * ch.s;
                 u.ch_off[c] = ch.h_off;
+                vcheck(ch.h_off, 4LL * ch.s, "child h");
             }
             u.hd_off = nd.hd_off; u.w_off = nd.w_off; u.h_off = nd.h_off;
-            u.Xi = M->Xiar.p + nd.Xi_off;
-            u.H = M->Har.p + nd.H_off;
+            vcheck(nd.hd_off, 4LL * n, "hDiff"); vcheck(nd.w_off, 4LL * n, "w"); vcheck(nd.h_off, 8LL * n, "h");
             ud.push_back(u);
-            sp->up_nmax[dd] = std::max(sp->up_nmax[dd], n);
-            // w = Xi hd   (mergeW_ with a stored inverse instead of a re-factorisation)
-            GemmProb p{};
-            p.A = u.Xi; p.lda = 4 * n; p.K = 4 * n;
-            p.B = V + nd.hd_off * k; p.ldb = k;
+            L.nmax = std::max(L.nmax, n);
+            const double* Xi = M->Xiar.p + nd.Xi_off;
+            // w = X^-1 hDiff   (mergeW_ with the stored inverse instead of a re-factorisation)
+            TreeGemm p{};
+            p.A0 = Xi; p.lda0 = 4 * n; p.K0 = 4 * n; p.alpha0 = 1.0;
+            p.B0 = V + nd.hd_off * k; p.ldb0 = k;
             p.D = V + nd.w_off * k; p.ldd = k;
-            p.M = 4 * n; p.N = k; p.alpha = 1.0; p.beta = 0.0; p.Cin = nullptr; p.ldcin = 0;
+            p.M = 4 * n; p.N = k;
             pw.push_back(p);
-            sp->w_tiles[dd] = std::max(sp->w_tiles[dd], tiles_of(4 * n, k));
-            // h[q] += H_c[rows of q] [w_if1; w_if2]   (mergeH_)
+            // h(q) += T_c(side q, interface sides) w(interfaces of c)   (mergeH_, the H block of child c)
             for (int q = 0; q < 8; q++) {
-                const int c = QC[q];
-                const int hrow = (TAUQ[c][0] == q) ? 0 : 1;
-                GemmProb h{};
-                const double* Hc = u.H + (long long)c * 4 * n * n + (long long)hrow * n * 2 * n;
-                h.A = Hc; h.lda = 2 * n; h.K = n;
-                h.B = V + (nd.w_off + (long long)CHIF[c][0] * n) * k; h.ldb = k;
-                h.A2 = Hc + n; h.lda2 = 2 * n; h.K2 = n;
-                h.B2 = V + (nd.w_off + (long long)CHIF[c][1] * n) * k; h.ldb2 = k;
+                const int c = q_child(q), sq = q_side(q);
+                const double* Tc = merge_T_ptr(M, nd.child[c]);
+                const int r0 = child_if(c, 0), r1 = child_if(c, 1);
+                TreeGemm h{};
+                h.A0 = Tc + (long long)sq * n * 4 * n + (long long)side_on_if(c, r0) * n; h.lda0 = 4 * n; h.K0 = n;
+                h.B0 = V + (nd.w_off + (long long)r0 * n) * k; h.ldb0 = k; h.alpha0 = 1.0;
+                h.A1 = Tc + (long long)sq * n * 4 * n + (long long)side_on_if(c, r1) * n; h.lda1 = 4 * n; h.K1 = n;
+                h.B1 = V + (nd.w_off + (long long)r1 * n) * k; h.ldb1 = k; h.alpha1 = 1.0;
                 h.D = V + (nd.h_off + (long long)q * n) * k; h.ldd = k;
-                h.Cin = h.D; h.ldcin = k;
-                h.M = n; h.N = k; h.alpha = 1.0; h.beta = 1.0;
+                h.C = h.D; h.ldc = k; h.beta = 1.0;
+                h.M = n; h.N = k;
                 ph.push_back(h);
             }
-            sp->h_tiles[dd] = std::max(sp->h_tiles[dd], tiles_of(n, k));
-            // downward
+            // downward: u_int = S g + w   (applyS_, HPSAlgorithm.hpp:1194-1230)
             DownDesc dsc{};
             dsc.n = n; dsc.tag = nd.tag;
             dsc.gc_off = nd.gc_off; dsc.g_off = nd.g_off; dsc.w_off = nd.w_off; dsc.ui_off = nd.ui_off;
-            for (int c = 0; c < 4; c++) dsc.cg_off[c] = P.nodes[nd.child[c]].gc_off;
+            for (int c = 0; c < 4; c++) {
+                dsc.cg_off[c] = P.nodes[nd.child[c]].gc_off;
+                vcheck(dsc.cg_off[c], 4LL * n, "child g");
+            }
+            vcheck(nd.g_off, 8LL * n, "g"); vcheck(nd.ui_off, 4LL * n, "interface potential");
             dsc.S = M->Sar.p + nd.S_off;
             dn.push_back(dsc);
-            sp->down_nmax[dd] = std::max(sp->down_nmax[dd], n);
-            GemmProb pd{};
-            pd.A = dsc.S; pd.lda = 8 * n; pd.K = 8 * n;
-            pd.B = V + nd.g_off * k; pd.ldb = k;
+            TreeGemm pd{};
+            pd.A0 = dsc.S; pd.lda0 = 8 * n; pd.K0 = 8 * n; pd.alpha0 = 1.0;
+            pd.B0 = V + nd.g_off * k; pd.ldb0 = k;
             pd.D = V + nd.ui_off * k; pd.ldd = k;
-            pd.Cin = V + nd.w_off * k; pd.ldcin = k;
-            pd.M = 4 * n; pd.N = k; pd.alpha = 1.0; pd.beta = M->homogeneous_rhs ? 0.0 : 1.0;
+            if (!M->homogeneous_rhs) { pd.C = V + nd.w_off * k; pd.ldc = k; pd.beta = 1.0; }
+            pd.M = 4 * n; pd.N = k;
             pu.push_back(pd);
-            sp->u_tiles[dd] = std::max(sp->u_tiles[dd], tiles_of(4 * n, k));
             // uncoarsen_ steps: coarse gc -> ... -> g, alternating through gt
-            for (int st = 1; st <= nd.tag; st++) {
-                if ((int)us.size() < st) us.resize(st);
+            for (int stp = 1; stp <= nd.tag; stp++) {
+                if ((int)us.size() < stp) us.resize(stp);
                 VecStepDesc v;
-                const int nfine = nd.s >> (nd.tag - st);
+                const int nfine = nd.s >> (nd.tag - stp);
                 v.nfine = nfine;
-                const bool dst_is_g = ((nd.tag - st) % 2) == 0;
+                const bool dst_is_g = ((nd.tag - stp) % 2) == 0;
                 v.dst_off = dst_is_g ? nd.g_off : nd.gt_off;
-                if (st == 1) v.src_off = nd.gc_off;
-                else v.src_off = (((nd.tag - (st - 1)) % 2) == 0) ? nd.g_off : nd.gt_off;
-                us[st - 1].push_back(v);
+                if (stp == 1) v.src_off = nd.gc_off;
+                else v.src_off = (((nd.tag - (stp - 1)) % 2) == 0) ? nd.g_off : nd.gt_off;
+                vcheck(v.dst_off, 4LL * nfine, "uncoarsen destination");
+                vcheck(v.src_off, 2LL * nfine, "uncoarsen source");
+                us[stp - 1].push_back(v);
             }
         }
-        sp->up_cnt[dd] = (int)ud.size();
-        sp->down_cnt[dd] = (int)dn.size();
-        auto a = std::make_unique<DevBuf<UpDesc>>(); a->upload(ud, M->stream); sp->up_own[dd] = std::move(a);
-        auto b = std::make_unique<DevBuf<GemmProb>>(); b->upload(pw, M->stream); sp->up_w[dd] = std::move(b);
-        auto c2 = std::make_unique<DevBuf<GemmProb>>(); c2->upload(ph, M->stream); sp->up_h[dd] = std::move(c2);
-        auto e = std::make_unique<DevBuf<DownDesc>>(); e->upload(dn, M->stream); sp->down[dd] = std::move(e);
-        auto f = std::make_unique<DevBuf<GemmProb>>(); f->upload(pu, M->stream); sp->down_u[dd] = std::move(f);
-        for (auto& st : us) {
-            auto bb = std::make_unique<DevBuf<VecStepDesc>>();
-            bb->upload(st, M->stream);
+        L.cnt = (int)ud.size();
+        check_range(V, (size_t)vrows * k * 8, "vector arena");
+        L.up = sp->store.put(ud, st);
+        L.down = sp->store.put(dn, st);
+        L.gw = plan_gemm(pw, tn, sp->store, st);
+        L.gh = plan_gemm(ph, tn, sp->store, st);
+        L.gu = plan_gemm(pu, tn, sp->store, st);
+        for (auto& v : us) {
             int nfmax = 0;
-            for (auto& v : st) nfmax = std::max(nfmax, v.nfine);
-            sp->unc_cnt[dd].push_back((int)st.size());
-            sp->unc_nf[dd].push_back(nfmax);
-            sp->unc[dd].push_back(std::move(bb));
+            for (auto& x : v) nfmax = std::max(nfmax, x.nfine);
+            L.unc.push_back(sp->store.put(v, st));
+            L.unc_cnt.push_back((int)v.size());
+            L.unc_nf.push_back(nfmax);
         }
     }
     SweepPlan& ref = *sp;
@@ -767,6 +775,7 @@ void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u,
         }
         launch_patch_fd(a, N <= 16 ? 1 : 2, patch_default_blocks(), M->stream);
     }
+    if (sync_check()) EF_HIP(hipStreamSynchronize(M->stream));
 }
 
 void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
@@ -776,19 +785,17 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     hipStream_t s = M->stream;
     ensure_vectors(M, k);
     SweepPlan& sp = sweep_plan(M, k);
-    const int N = P.N;
-    const int nl = (int)P.leaves.size();
+    check_range(f_dev, (size_t)k * P.leaves.size() * P.N * P.N * 8, "upwards load f");
     // leaf particular Neumann data (particularNeumannData, FiniteVolumeSolver.cpp:457-462)
     launch_leaf(M, 0, f_dev, 0, nullptr, k);
     if (!M->homogeneous_rhs) {  // upwards4to1 is skipped entirely for homogeneous-rhs (:532)
         for (int dd = P.maxdepth; dd >= 0; dd--) {
-            const int cnt = sp.up_cnt[dd];
-            if (cnt == 0) continue;
-            hipLaunchKernelGGL(up_gather_kernel, dim3(blocks_for(12LL * sp.up_nmax[dd] * k), cnt), dim3(256), 0, s,
-                               sp.up_own[dd]->p, M->vec.p, k);
-            EF_LAUNCH_CHECK();
-            launch_gemm(*sp.up_w[dd], cnt, sp.w_tiles[dd], s);
-            launch_gemm(*sp.up_h[dd], 8 * cnt, sp.h_tiles[dd], s);
+            const LevelSweep& L = sp.lv[dd];
+            if (L.cnt == 0) continue;
+            EF_LAUNCH("tree_up_gather_kernel", tree_up_gather_kernel, dim3(blocks_for(12LL * L.nmax * k), L.cnt),
+                      dim3(256), s, L.up, M->vec.p, k);
+            run_gemm(L.gw, s);
+            run_gemm(L.gh, s);
         }
     }
     M->k_up = k;
@@ -805,59 +812,67 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
     hipStream_t s = M->stream;
     ensure_vectors(M, k);
     SweepPlan& sp = sweep_plan(M, k);
-    const int N = P.N;
-    const int nl = (int)P.leaves.size();
     const NodeP& root = P.nodes[0];
     const int m = 4 * root.s;
     double* V = M->vec.p;
+    check_range(r_dev, (size_t)m * k * 8, "root data r");
+    check_range(a_dev, (size_t)m * 8, "root data a");
+    check_range(u_dev, (size_t)k * P.leaves.size() * P.N * P.N * 8, "solution u");
     if (!robin) {
-        hipLaunchKernelGGL(root_dirichlet_kernel, dim3(blocks_for((long long)m * k)), dim3(256), 0, s, r_dev, a_dev,
-                           V + root.g_off * k, m, k);
-        EF_LAUNCH_CHECK();
+        EF_LAUNCH("tree_root_dirichlet_kernel", tree_root_dirichlet_kernel, dim3(blocks_for((long long)m * k)), dim3(256),
+                  s, r_dev, a_dev, V + root.g_off * k, m, k);
     } else {
         // g = (diag(a) + diag(b) T)^-1 (r - b h)
+        check_range(b_dev, (size_t)m * 8, "root data b");
+        if (m > GJ_MAXM) throw EfError(EF_ERR_ARG, "root boundary system larger than the inversion limit");
         M->rootA.ensure((size_t)m * m);
-        M->rootM.ensure((size_t)m * GJ_NB);
-        M->rootU.ensure((size_t)GJ_NB * (m + k));
-        M->rootpiv.ensure(GJ_NB);
+        M->rootAi.ensure((size_t)m * m);
+        M->rootrhs.ensure((size_t)m * k);
+        M->rootgj.ensure(gj_work_doubles(m, 1));
+        M->rootgji.ensure(gj_work_ints(m, 1));
         M->rootinfo.ensure(1);
         const double* Troot = full_T_ptr(M, 0);
         const double* hroot = (root.leaf || M->homogeneous_rhs) ? nullptr : V + root.h_off * k;
-        hipLaunchKernelGGL(root_robin_assemble_kernel, dim3(blocks_for((long long)m * m + (long long)m * k)), dim3(256), 0,
-                           s, Troot, a_dev, b_dev, r_dev, hroot, M->rootA.p, V + root.g_off * k, m, k);
-        EF_LAUNCH_CHECK();
-        GJDesc G{};
-        G.m = m; G.X = M->rootA.p; G.ldx = m;
-        G.R1 = V + root.g_off * k; G.w1 = k; G.ld1 = k;
-        G.R2 = nullptr; G.w2 = 0; G.ld2 = 0;
-        G.M = M->rootM.p; G.U = M->rootU.p; G.ldu = m + k;
-        G.piv = M->rootpiv.p; G.info = M->rootinfo.p;
-        std::vector<GJDesc> gv = {G};
-        M->rootgj.upload(gv, M->stream);
+        EF_LAUNCH("tree_root_robin_kernel", tree_root_robin_kernel, dim3(blocks_for((long long)m * m + (long long)m * k)),
+                  dim3(256), s, Troot, a_dev, b_dev, r_dev, hroot, M->rootA.p, M->rootrhs.p, m, k);
+        GJWork w;
+        double* d = M->rootgj.p;
+        w.wb = d; d += (size_t)m * m;
+        w.Pinv = d; d += (size_t)GJ_NB * GJ_NB;
+        w.Mm = d;
+        int* ii = M->rootgji.p;
+        w.cand = ii; ii += (size_t)GJ_MAXCH * GJ_NB;
+        w.cfin = ii; ii += (size_t)GJ_NB;
+        w.src = ii; ii += (size_t)m;
+        w.ipiv = ii; ii += (size_t)m;
+        w.prow = ii;
         EF_HIP(hipMemsetAsync(M->rootinfo.p, 0, sizeof(int), s));
-        run_gj(M->rootgj.p, 1, m, k, s);
+        run_gj_inverse(M->rootA.p, M->rootAi.p, m, 1, w, M->rootinfo.p, s);
+        TreeGemm p{};
+        p.A0 = M->rootAi.p; p.lda0 = m; p.K0 = m; p.alpha0 = 1.0;
+        p.B0 = M->rootrhs.p; p.ldb0 = k;
+        p.D = V + root.g_off * k; p.ldd = k;
+        p.M = m; p.N = k;
+        DevStore tmp;
+        GemmLaunch g = plan_gemm(std::vector<TreeGemm>{p}, k <= 16 ? 1 : 2, tmp, s);
+        run_gemm(g, s);
         int info = 0;
         efk::copy_d2h(&info, M->rootinfo.p, sizeof(int), s);
-        EF_HIP(hipStreamSynchronize(s));
         if (info) throw EfError(EF_ERR_SINGULAR, "zero pivot in the root boundary system");
     }
     for (int dd = 0; dd <= P.maxdepth; dd++) {
-        const int cnt = sp.down_cnt[dd];
-        if (cnt == 0) continue;
-        for (size_t st = 0; st < sp.unc[dd].size(); st++) {
-            hipLaunchKernelGGL(uncoarsen_kernel, dim3(blocks_for(4LL * sp.unc_nf[dd][st] * k), sp.unc_cnt[dd][st]),
-                               dim3(256), 0, s, sp.unc[dd][st]->p, V, k);
-            EF_LAUNCH_CHECK();
-        }
-        launch_gemm(*sp.down_u[dd], cnt, sp.u_tiles[dd], s);
-        hipLaunchKernelGGL(down_scatter_kernel, dim3(blocks_for(16LL * sp.down_nmax[dd] * k), cnt), dim3(256), 0, s,
-                           sp.down[dd]->p, V, k);
-        EF_LAUNCH_CHECK();
+        const LevelSweep& L = sp.lv[dd];
+        if (L.cnt == 0) continue;
+        for (size_t stp = 0; stp < L.unc.size(); stp++)
+            EF_LAUNCH("tree_uncoarsen_kernel", tree_uncoarsen_kernel, dim3(blocks_for(4LL * L.unc_nf[stp] * k), L.unc_cnt[stp]),
+                      dim3(256), s, L.unc[stp], V, k);
+        run_gemm(L.gu, s);
+        EF_LAUNCH("tree_down_scatter_kernel", tree_down_scatter_kernel, dim3(blocks_for(16LL * L.nmax * k), L.cnt),
+                  dim3(256), s, L.down, V, k);
     }
     // leaf solves (leafSolve, HPSAlgorithm.hpp:584-596); homogeneous-rhs solves with f = 0
     launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, (M->homogeneous_rhs || !f_dev) ? 1 : 0, u_dev, k);
 }
-
 
 }  // namespace
 
@@ -960,8 +975,9 @@ int ef_mesh_footprint(const ef_mesh_desc* desc, int precision, int max_rhs, size
     Plan P;
     plan_tree(P, *desc);
     const long long N = P.N;
-    long long elems = P.T_elems + P.S_elems + P.Xi_elems + P.H_elems + P.work_X + P.work_M + P.work_U;
-    elems += (long long)P.leaves.size() * 16 * N * N;  // leaf DtN upper bound (one per leaf)
+    long long elems = P.T_elems + P.S_elems + P.Xi_elems + P.work_X + P.gj_d;
+    elems += (P.gj_i + P.ninternal + 1) / 2;                     // int work
+    elems += (long long)P.leaves.size() * 16 * N * N;           // leaf DtN upper bound (one per leaf)
     elems += P.vec_rows * std::max(max_rhs, 1);
     elems += 2LL * std::max(max_rhs, 1) * (long long)P.leaves.size() * N * N;  // f and u staging
     (void)precision;
@@ -1026,12 +1042,6 @@ int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs) 
 int ef_build(ef_mesh mesh, const ef_coeffs* coeffs, int precision) {
     EF_API_BEGIN
     if (!mesh || !coeffs) throw EfError(EF_ERR_ARG, "null argument");
-#ifdef EF_DEVICE_BUILD_DISABLED
-    // The device build faulted an MI355X twice with no cause found in the indexing (the same
-    // sources pass bounds-checked CPU emulation).  Until the cause is shown it stays off.
-    throw EfError(EF_ERR_STATE, "the device build path is disabled in this library build "
-                                "(GPU memory fault under investigation); build with EF_DEVICE_BUILD=1 to enable");
-#endif
     if (coeffs->solver_type != EF_SOLVER_FISHPACK && coeffs->solver_type != EF_SOLVER_FIVE_POINT)
         throw EfError(EF_ERR_ARG, "unknown solver type");
     if (precision != EF_PRECISION_FP64) throw EfError(EF_ERR_ARG, "only FP64 precision is available in this build");
@@ -1206,36 +1216,39 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
         case 1: need_built(); need_internal(); copy_mat(M->Sar.p + nd.S_off, 4 * n, 8 * n); break;
         case 2: {  // X: re-assembled from the children's stored DtN (mergeX_)
             need_built(); need_internal();
-            DevBuf<double> tX, tS, tXi, tH, tT;
-            tX.alloc(16LL * n * n); tS.alloc(32LL * n * n); tXi.alloc(16LL * n * n); tH.alloc(16LL * n * n);
+            DevBuf<double> tX, tT;
+            tX.alloc(16LL * n * n);
             tT.alloc(64LL * n * n);
-            MergeDesc md{};
-            md.n = n;
-            for (int c = 0; c < 4; c++) md.Tc[c] = merge_T_ptr(M, nd.child[c]);
-            md.X = tX.p; md.S = tS.p; md.Xi = tXi.p; md.H = tH.p; md.T = tT.p;
-            DevBuf<MergeDesc> dmd;
-            dmd.upload(std::vector<MergeDesc>{md}, M->stream);
-            hipLaunchKernelGGL(merge_assemble_kernel, dim3(blocks_for(144LL * n * n), 1), dim3(256), 0, M->stream, dmd.p);
-            EF_LAUNCH_CHECK();
+            AsmDesc ad{};
+            ad.n = n;
+            for (int c = 0; c < 4; c++) ad.Tc[c] = merge_T_ptr(M, nd.child[c]);
+            ad.X = tX.p; ad.T = tT.p;
+            DevBuf<AsmDesc> dad;
+            dad.upload(std::vector<AsmDesc>{ad}, M->stream);
+            EF_LAUNCH("tree_assemble_kernel", tree_assemble_kernel, dim3(blocks_for(80LL * n * n), 1), dim3(256), M->stream,
+                      dad.p);
             EF_HIP(hipStreamSynchronize(M->stream));
             copy_mat(tX.p, 4 * n, 4 * n);
             break;
         }
-        case 3: {  // H: dense 8n x 4n of mergeT_ from the four 2n x 2n child blocks
+        case 3: {  // H (8n x 4n of mergeT_): child c's tau-side rows against its interface sides
             need_built(); need_internal();
             *rows = 8 * n; *cols = 4 * n;
             if (out) {
-                std::vector<double> hc(16LL * n * n);
-                efk::copy_d2h(hc.data(), M->Har.p + nd.H_off, sizeof(double) * hc.size(), M->stream);
-                EF_HIP(hipStreamSynchronize(M->stream));
-                static const int CHIF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
                 std::fill(out, out + 32LL * n * n, 0.0);
-                for (int c = 0; c < 4; c++)
+                std::vector<double> Tc(16LL * n * n);
+                for (int c = 0; c < 4; c++) {
+                    efk::copy_d2h(Tc.data(), merge_T_ptr(M, nd.child[c]), sizeof(double) * Tc.size(), M->stream);
+                    // tau sides of child c in the reference's row order: (W or E), then (S or N)
+                    const int ts[2] = {(c & 1) ? (int)E : (int)W, (c >> 1) ? (int)N : (int)S};
                     for (int rr = 0; rr < 2 * n; rr++)
-                        for (int slot = 0; slot < 2; slot++)
+                        for (int slot = 0; slot < 2; slot++) {
+                            const int r = child_if(c, slot), side = side_on_if(c, r);
                             for (int j = 0; j < n; j++)
-                                out[(long long)(c * 2 * n + rr) * 4 * n + CHIF[c][slot] * n + j] =
-                                    hc[(long long)c * 4 * n * n + (long long)rr * 2 * n + slot * n + j];
+                                out[(long long)(c * 2 * n + rr) * 4 * n + (long long)r * n + j] =
+                                    Tc[(long long)(ts[rr / n] * n + rr % n) * 4 * n + (long long)side * n + j];
+                        }
+                }
             }
             break;
         }

@@ -36,54 +36,6 @@ __host__ __device__ inline EF_GLOBAL T* gmem(T* p) { return (EF_GLOBAL T*)p; }
 
 enum Side { W = 0, E = 1, S = 2, N = 3 };
 
-// Batched GEMM problem: D = alpha * (A B + A2 B2) + beta * Cin, row-major.
-// The optional second segment lets one problem read its K range from two places.
-struct GemmProb {
-    const double* A;  const double* B;  int lda, ldb, K;
-    const double* A2; const double* B2; int lda2, ldb2, K2;
-    const double* Cin; int ldcin;
-    double* D; int ldd;
-    int M, N;
-    double alpha, beta;
-};
-
-// FP32 variant used by the mixed-precision merge products.
-struct GemmProbF {
-    const float* A;  const float* B;  int lda, ldb, K;
-    const float* A2; const float* B2; int lda2, ldb2, K2;
-    const float* Cin; int ldcin;
-    float* D; int ldd;
-    int M, N;
-    float alpha, beta;
-};
-
-// Blocked Gauss-Jordan elimination with partial pivoting of [X | R1 | R2]:
-// on exit X is reduced (not kept), R1 <- X^-1 R1 and R2 <- X^-1 R2.
-// The merge uses R1 = S_RHS (-> S) and R2 = I (-> X^-1); the root boundary solve
-// uses R1 = right-hand sides and no R2.
-struct GJDesc {
-    int m;                 // order of X
-    double* X;  int ldx;
-    double* R1; int w1, ld1;
-    double* R2; int w2, ld2;
-    double* M;             // m x nb multipliers (work)
-    double* U;             // nb x (m + w1 + w2) pivot rows (work), ld = ldu
-    int ldu;
-    int* piv;              // nb pivot rows of the current panel (work)
-    int* info;             // first zero pivot (1-based column) or 0
-};
-
-// One 4-to-1 merge (HPSAlgorithm::merge4to1, HPSAlgorithm.hpp:497-518).
-struct MergeDesc {
-    int n;                 // child interface size after coarsening
-    const double* Tc[4];   // children's DtN used by the merge, each 4n x 4n (coarsened)
-    double* X;             // 4n x 4n work: Schur complement, reduced in place
-    double* S;             // 4n x 8n
-    double* Xi;            // 4n x 4n
-    double* H;             // 4 x (2n x 2n)
-    double* T;             // 8n x 8n
-};
-
 // coarsen_ step (HPSAlgorithm.hpp:707-741): T (4m x 4m) -> L21 T L12 (2m x 2m) per side block.
 struct CoarsenDesc {
     const double* src;  // 4m x 4m
@@ -115,8 +67,8 @@ struct UpDesc {
     long long hd_off;       // hDiff work (4n rows)
     long long w_off;        // w (4n rows)
     long long h_off;        // tau h (8n rows)
-    const double* Xi;       // 4n x 4n
-    const double* H;        // 4 x (2n x 2n)
+    const double* Xi;       // 4n x 4n (unused by the gather)
+    const double* H;        // unused by the gather
 };
 
 // Downward family step (split1to4, HPSAlgorithm.hpp:562-577) for one merge.
@@ -192,20 +144,6 @@ struct VecStepDesc {
 
 // The planner fills these on the host and the kernels read them on the device: both
 // compiler passes must agree on every size and offset (x86-64 and amdgcn are both LP64).
-static_assert(sizeof(GemmProb) == 120, "GemmProb");
-static_assert(offsetof(GemmProb, A2) == 32, "GemmProb.A2");
-static_assert(offsetof(GemmProb, Cin) == 64, "GemmProb.Cin");
-static_assert(offsetof(GemmProb, D) == 80, "GemmProb.D");
-static_assert(offsetof(GemmProb, alpha) == 104, "GemmProb.alpha");
-static_assert(sizeof(GemmProbF) == 112, "GemmProbF");
-static_assert(sizeof(GJDesc) == 96, "GJDesc");
-static_assert(offsetof(GJDesc, X) == 8, "GJDesc.X");
-static_assert(offsetof(GJDesc, R1) == 24, "GJDesc.R1");
-static_assert(offsetof(GJDesc, R2) == 40, "GJDesc.R2");
-static_assert(offsetof(GJDesc, M) == 56, "GJDesc.M");
-static_assert(offsetof(GJDesc, piv) == 80, "GJDesc.piv");
-static_assert(sizeof(MergeDesc) == 80, "MergeDesc");
-static_assert(offsetof(MergeDesc, Tc) == 8, "MergeDesc.Tc");
 static_assert(sizeof(CoarsenDesc) == 24, "CoarsenDesc");
 static_assert(sizeof(LeafDesc) == 88, "LeafDesc");
 static_assert(offsetof(LeafDesc, h_off) == 8, "LeafDesc.h_off");

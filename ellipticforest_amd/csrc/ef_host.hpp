@@ -1,9 +1,14 @@
-// ef_host.hpp -- host-side helpers shared by the C ABI sources of libefhps.so: the
-// error type that every entry point turns into an EF_ERR_* code, HIP status checks and
-// an owning device buffer.
+// ef_host.hpp -- host-side helpers shared by the C ABI sources of libefhps.so: the error type
+// that every entry point turns into an EF_ERR_* code, HIP status checks, an owning device
+// buffer, and the launch checks every kernel launch of the tree path goes through.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -34,35 +39,92 @@ long long patch_default_blocks();
 bool cells_symmetric(const double* c, int N);
 // Every row of the patch's five-point matrix diagonally dominant (block LU without pivoting).
 bool cells_dominant(const double* c, int N, double dx, double dy);
-}
-
-// A context is one GPU.
-struct ef_ctx_s {
-    int device = 0;
-    hipStream_t stream = nullptr;
-};
-
-namespace {
 
 struct EfError : std::runtime_error {
     int code;
     EfError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// EF_SYNC_CHECK=1: every launch of the tree path prints its name before it starts and is
+// waited for and checked right after, so a failing kernel is named by the last line printed.
+inline bool sync_check() {
+    static const bool on = [] {
+        const char* e = std::getenv("EF_SYNC_CHECK");
+        return e && std::strcmp(e, "0") != 0;
+    }();
+    return on;
+}
+}  // namespace efk
+
 #define EF_HIP(x)                                                                                    \
     do {                                                                                             \
         hipError_t e_ = (x);                                                                         \
-        if (e_ != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess)                                                                        \
+            throw efk::EfError(EF_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
 #define EF_LAUNCH_CHECK_NAMED(name)                                                                  \
     do {                                                                                             \
         hipError_t e_ = hipGetLastError();                                                           \
-        if (e_ != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string("launch of ") + name + ": " + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess)                                                                        \
+            throw efk::EfError(EF_ERR_DEVICE, std::string("launch of ") + name + ": " + hipGetErrorString(e_)); \
     } while (0)
 #define EF_LAUNCH_CHECK() EF_LAUNCH_CHECK_NAMED(__func__ + std::string(":") + std::to_string(__LINE__))
 
+namespace efk {
+
 inline unsigned blocks_for(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// Grid and block of a launch against the limits of a gfx950 dispatch; the name is printed
+// first in EF_SYNC_CHECK mode.
+inline void pre_launch(const char* name, dim3 grid, dim3 block) {
+    if (grid.x == 0 || grid.y == 0 || grid.z == 0 || grid.y > 65535 || grid.z > 65535 || block.x == 0 ||
+        block.x * block.y * block.z > 1024 || (unsigned long long)grid.x * block.x > 0xffffffffULL)
+        throw EfError(EF_ERR_INTERNAL, std::string("launch of ") + name + ": grid/block outside the dispatch limits");
+    if (sync_check()) {
+        std::fprintf(stderr, "[ef-launch] %s grid=(%u,%u,%u) block=%u\n", name, grid.x, grid.y, grid.z, block.x);
+        std::fflush(stderr);
+    }
+}
+inline void post_launch(const char* name, hipStream_t s) {
+    EF_LAUNCH_CHECK_NAMED(name);
+    if (sync_check()) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess)
+            throw EfError(EF_ERR_DEVICE, std::string("kernel ") + name + " failed: " + hipGetErrorString(e));
+    }
+}
+
+// Launch with the checks above (no dynamic LDS in the tree path).
+#define EF_LAUNCH(name, kern, grid, block, stream, ...)                     \
+    do {                                                                    \
+        const dim3 g_ = (grid), b_ = (block);                              \
+        efk::pre_launch(name, g_, b_);                                      \
+        hipLaunchKernelGGL(kern, g_, b_, 0, stream, __VA_ARGS__);           \
+        efk::post_launch(name, stream);                                     \
+    } while (0)
+
+// [p, p + bytes) must lie inside one device allocation (the runtime's record of it).
+inline void check_range(const void* p, size_t bytes, const char* what) {
+    if (bytes == 0) return;
+    if (p == nullptr) throw EfError(EF_ERR_INTERNAL, std::string("null device pointer: ") + what);
+    void* base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) {
+        (void)hipGetLastError();
+        throw EfError(EF_ERR_INTERNAL, std::string("pointer outside every device allocation: ") + what);
+    }
+    const uintptr_t b = (uintptr_t)base, q = (uintptr_t)p;
+    if (q < b || q + bytes > b + size)
+        throw EfError(EF_ERR_INTERNAL, std::string("device range past the end of its allocation: ") + what);
+}
+// A row-major matrix of `rows` rows, `cols` used columns, leading dimension ld (elements of T).
+template <class T>
+inline void check_matrix(const T* p, long long rows, long long cols, long long ld, const char* what) {
+    if (rows <= 0 || cols <= 0) return;
+    if (ld < cols) throw EfError(EF_ERR_INTERNAL, std::string("leading dimension below the column count: ") + what);
+    check_range(p, (size_t)(((rows - 1) * ld + cols) * (long long)sizeof(T)), what);
+}
 
 // Owning device buffer.
 template <class T>
@@ -91,20 +153,50 @@ struct DevBuf {
     void upload(const std::vector<T>& v, hipStream_t s) {
         alloc(v.size());
         if (v.empty()) return;
-        efk::copy_h2d(p, v.data(), v.size() * sizeof(T), s);
+        copy_h2d(p, v.data(), v.size() * sizeof(T), s);
     }
 };
 
+// Device allocations kept together (descriptor tables of one plan).
+struct DevStore {
+    std::vector<std::unique_ptr<DevBuf<char>>> bufs;
+    template <class T>
+    T* put(const std::vector<T>& v, hipStream_t s) {
+        if (v.empty()) return nullptr;
+        auto b = std::make_unique<DevBuf<char>>();
+        b->alloc(v.size() * sizeof(T));
+        copy_h2d(b->p, v.data(), v.size() * sizeof(T), s);
+        T* out = (T*)b->p;
+        bufs.push_back(std::move(b));
+        return out;
+    }
+    void clear() { bufs.clear(); }
+};
+
+// Static resources of a kernel against the invariants of the tree path: no scratch, static
+// LDS below 64 KiB.
+inline void check_kernel_resources(const void* fn, const char* name) {
+    hipFuncAttributes a;
+    EF_HIP(hipFuncGetAttributes(&a, fn));
+    if (a.localSizeBytes != 0)
+        throw EfError(EF_ERR_INTERNAL, std::string("kernel ") + name + " uses scratch memory (" +
+                                           std::to_string(a.localSizeBytes) + " B per lane)");
+    if (a.sharedSizeBytes > 64 * 1024)
+        throw EfError(EF_ERR_INTERNAL, std::string("kernel ") + name + " uses more than 64 KiB of LDS");
+}
+
 inline int fail(const EfError& e) {
-    efk::set_last_error(e.what());
+    set_last_error(e.what());
     return e.code;
 }
+
+}  // namespace efk
 
 // Every C entry point body sits between these: no exception crosses the ABI.
 #define EF_API_BEGIN try {
 #define EF_API_END                                              \
     }                                                           \
-    catch (const EfError& e) { return fail(e); }                \
+    catch (const efk::EfError& e) { return efk::fail(e); }      \
     catch (const std::bad_alloc&) {                             \
         efk::set_last_error("host out of memory");              \
         return EF_ERR_INTERNAL;                                 \
@@ -115,4 +207,8 @@ inline int fail(const EfError& e) {
     }                                                           \
     return EF_OK;
 
-}  // namespace
+// A context is one GPU.
+struct ef_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};

@@ -12,434 +12,6 @@ namespace efk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// Element c (0..3) of a four-entry descriptor array by selects: a runtime index into a
-// struct copied from memory would place the struct in scratch or LDS.
-template <class T>
-__device__ inline T pick4(const T (&a)[4], int c) {
-    return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
-}
-
-// ---------------------------------------------------------------------------
-// Batched FP64 GEMM on MFMA (v_mfma_f64_16x16x4_f64).
-// Workgroup = 4 waves computing a 64x64 tile of D; each wave owns 32x32 = 2x2 MFMA
-// blocks.  K is staged through LDS 16 at a time.  Operand layout of the f64 MFMA on
-// gfx950: lane l supplies A[l%16][l/16] and B[l/16][l%16]; accumulator slot v of lane
-// l is D[4v + l/16][l%16].
-// ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
-
-__device__ inline void gemm_tile_f64(const GemmProb& p, int tm, int tn, double* sA, double* sB) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int row0 = tm * GBM, col0 = tn * GBN;
-    d4 acc[2][2];
-    for (int i = 0; i < 2; i++)
-        for (int j = 0; j < 2; j++) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-
-    for (int seg = 0; seg < 2; seg++) {
-        const double* A = seg ? p.A2 : p.A;
-        const double* B = seg ? p.B2 : p.B;
-        const int lda = seg ? p.lda2 : p.lda;
-        const int ldb = seg ? p.ldb2 : p.ldb;
-        const int K = seg ? p.K2 : p.K;
-        if (K <= 0 || A == nullptr) continue;
-        for (int k0 = 0; k0 < K; k0 += GBK) {
-            // A tile: GBM x GBK, stored k-major in LDS
-            for (int e = tid; e < GBM * GBK; e += 256) {
-                const int r = e / GBK, c = e % GBK;
-                const int gr = row0 + r, gc = k0 + c;
-                double v = 0.0;
-                if (gr < p.M && gc < K) v = A[(long long)gr * lda + gc];
-                sA[c * (GBM + GPAD) + r] = v;
-            }
-            // B tile: GBK x GBN
-            for (int e = tid; e < GBK * GBN; e += 256) {
-                const int r = e / GBN, c = e % GBN;
-                const int gr = k0 + r, gc = col0 + c;
-                double v = 0.0;
-                if (gr < K && gc < p.N) v = B[(long long)gr * ldb + gc];
-                sB[r * (GBN + GPAD) + c] = v;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < GBK; kk += 4) {
-                const int kr = kk + (lane >> 4);
-                double a[2], b[2];
-#pragma unroll
-                for (int i = 0; i < 2; i++) a[i] = sA[kr * (GBM + GPAD) + wm * 32 + i * 16 + (lane & 15)];
-#pragma unroll
-                for (int j = 0; j < 2; j++) b[j] = sB[kr * (GBN + GPAD) + wn * 32 + j * 16 + (lane & 15)];
-#pragma unroll
-                for (int i = 0; i < 2; i++)
-#pragma unroll
-                    for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-            }
-            __syncthreads();
-        }
-    }
-    // epilogue
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int r = row0 + wm * 32 + i * 16 + 4 * v + (lane >> 4);
-                const int c = col0 + wn * 32 + j * 16 + (lane & 15);
-                if (r < p.M && c < p.N) {
-                    double out = p.alpha * acc[i][j][v];
-                    if (p.beta != 0.0) out += p.beta * p.Cin[(long long)r * p.ldcin + c];
-                    p.D[(long long)r * p.ldd + c] = out;
-                }
-            }
-}
-
-__global__ void __launch_bounds__(256) gemm_f64_kernel(const GemmProb* probs) {
-    __shared__ double sA[GBK * (GBM + GPAD)];
-    __shared__ double sB[GBK * (GBN + GPAD)];
-    const GemmProb p = probs[blockIdx.y];
-    const int tilesN = (p.N + GBN - 1) / GBN;
-    const int tilesM = (p.M + GBM - 1) / GBM;
-    if ((int)blockIdx.x >= tilesM * tilesN) return;
-    gemm_tile_f64(p, blockIdx.x / tilesN, blockIdx.x % tilesN, sA, sB);
-}
-
-// ---------------------------------------------------------------------------
-// Blocked Gauss-Jordan with partial pivoting on [X | R1 | R2], GJ_NB columns a step.
-// Step k: (1) panel: unblocked LU with partial pivoting of X[k:m, k:k+nb]
-// (pivot = first row of largest magnitude, as LAPACK's idamax), inverse P^-1 of the
-// pivot block, multipliers for every other row; (2) the panel's row interchanges on
-// the columns right of the panel and a copy of the pivot rows into U; (3) one GEMM
-// [X_right | R1 | R2] += Mfull * U, which normalises the pivot rows
-// (Mfull = P^-1 - I there) and eliminates the panel columns from all other rows.
-// ---------------------------------------------------------------------------
-
-template <bool LDS>
-__device__ inline void gj_panel_body(const GJDesc& d, int k, double* pan, int ldp) {
-    __shared__ double red_v[256];
-    __shared__ int red_i[256];
-    __shared__ double Li[GJ_NB][GJ_NB + 1], Ui[GJ_NB][GJ_NB + 1], Pinv[GJ_NB][GJ_NB + 1];
-    __shared__ double prow[GJ_NB];
-    const int tid = threadIdx.x;
-    const int m = d.m;
-    const int nbb = min(GJ_NB, m - k);
-    const int rows = m - k;
-    auto P = [&](int r, int c) -> double& {  // panel-local row r in [0, m-k), col c in [0, nbb)
-        if (LDS) return pan[r * ldp + c];
-        return d.X[(long long)(k + r) * d.ldx + k + c];
-    };
-    if (LDS) {
-        for (int e = tid; e < rows * nbb; e += 256) {
-            const int r = e / nbb, c = e % nbb;
-            pan[r * ldp + c] = d.X[(long long)(k + r) * d.ldx + k + c];
-        }
-        __syncthreads();
-    }
-    for (int j = 0; j < nbb; j++) {
-        // pivot search over panel rows [j, rows)
-        double best = -1.0;
-        int bi = 0x7fffffff;
-        for (int r = j + tid; r < rows; r += 256) {
-            const double v = fabs(P(r, j));
-            if (v > best) { best = v; bi = r; }
-        }
-        red_v[tid] = best;
-        red_i[tid] = bi;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) {
-                const double v2 = red_v[tid + s];
-                const int i2 = red_i[tid + s];
-                if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
-            }
-            __syncthreads();
-        }
-        int p = red_i[0];
-        double pv = red_v[0];
-        if (p < j || p >= rows) { p = j; pv = 0.0; }  // column of NaNs: keep the diagonal, flag it
-        __syncthreads();
-        if (tid == 0) {
-            d.piv[j] = k + p;
-            if (pv == 0.0 && *d.info == 0) *d.info = k + j + 1;
-        }
-        if (p != j) {
-            for (int c = tid; c < nbb; c += 256) {
-                const double t = P(j, c);
-                P(j, c) = P(p, c);
-                P(p, c) = t;
-            }
-        }
-        __syncthreads();
-        for (int c = tid; c < nbb; c += 256) prow[c] = P(j, c);
-        __syncthreads();
-        const double piv = prow[j];
-        const double inv = (piv != 0.0) ? 1.0 / piv : 0.0;
-        for (int r = j + 1 + tid; r < rows; r += 256) {
-            const double l = P(r, j) * inv;
-            P(r, j) = l;
-            for (int c = j + 1; c < nbb; c++) P(r, c) -= l * prow[c];
-        }
-        __syncthreads();
-    }
-    // inverses of L11 (unit lower, below the diagonal of the pivot block) and U11
-    // (upper, with the diagonal): column c of each, one thread per column
-    if (tid < nbb) {
-        const int c = tid;
-        for (int i = 0; i < nbb; i++) Li[i][c] = 0.0, Ui[i][c] = 0.0;
-        Li[c][c] = 1.0;
-        for (int i = c + 1; i < nbb; i++) {
-            double s = 0.0;
-            for (int t = c; t < i; t++) s += P(i, t) * Li[t][c];
-            Li[i][c] = -s;
-        }
-        const double ucc = P(c, c);
-        Ui[c][c] = (ucc != 0.0) ? 1.0 / ucc : 0.0;
-        for (int i = c - 1; i >= 0; i--) {
-            double s = 0.0;
-            for (int t = i + 1; t <= c; t++) s += P(i, t) * Ui[t][c];
-            const double uii = P(i, i);
-            Ui[i][c] = (uii != 0.0) ? -s / uii : 0.0;
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < nbb * nbb; e += 256) {
-        const int i = e / nbb, c = e % nbb;
-        double s = 0.0;
-        for (int t = max(i, c); t < nbb; t++) s += Ui[i][t] * Li[t][c];
-        Pinv[i][c] = s;
-    }
-    __syncthreads();
-    // multipliers (ld GJ_NB): rows above -X[r,panel] P^-1, pivot rows P^-1 - I, rows below -L21 L11^-1
-    for (int r = tid; r < m; r += 256) {
-        double* Mr = d.M + (long long)r * GJ_NB;
-        if (r < k) {
-            const double* xr = d.X + (long long)r * d.ldx + k;
-            for (int c = 0; c < nbb; c++) {
-                double s = 0.0;
-                for (int t = 0; t < nbb; t++) s += xr[t] * Pinv[t][c];
-                Mr[c] = -s;
-            }
-        } else if (r < k + nbb) {
-            const int i = r - k;
-            for (int c = 0; c < nbb; c++) Mr[c] = Pinv[i][c] - (i == c ? 1.0 : 0.0);
-        } else {
-            const int rr = r - k;
-            for (int c = 0; c < nbb; c++) {
-                double s = 0.0;
-                for (int t = c; t < nbb; t++) s += P(rr, t) * Li[t][c];
-                Mr[c] = -s;
-            }
-        }
-        for (int c = nbb; c < GJ_NB; c++) Mr[c] = 0.0;
-    }
-}
-
-__global__ void __launch_bounds__(256) gj_panel_lds_kernel(const GJDesc* descs, int k) {
-    extern __shared__ double pan[];
-    const GJDesc d = descs[blockIdx.x];
-    if (k >= d.m) return;
-    gj_panel_body<true>(d, k, pan, GJ_NB + 1);
-}
-__global__ void __launch_bounds__(256) gj_panel_global_kernel(const GJDesc* descs, int k) {
-    const GJDesc d = descs[blockIdx.x];
-    if (k >= d.m) return;
-    gj_panel_body<false>(d, k, nullptr, 0);
-}
-
-// Column c of the right part: [X cols k+nb .. m) ++ R1 ++ R2
-__device__ inline double* gj_col(const GJDesc& d, int k, int nbb, int c, int& ld) {
-    const int wx = d.m - k - nbb;
-    if (c < wx) { ld = d.ldx; return d.X + k + nbb + c; }
-    c -= wx;
-    if (c < d.w1) { ld = d.ld1; return d.R1 + c; }
-    c -= d.w1;
-    ld = d.ld2;
-    return d.R2 + c;
-}
-
-__global__ void __launch_bounds__(256) gj_swap_kernel(const GJDesc* descs, int k) {
-    const GJDesc d = descs[blockIdx.y];
-    if (k >= d.m) return;
-    const int nbb = min(GJ_NB, d.m - k);
-    const int wtot = (d.m - k - nbb) + d.w1 + d.w2;
-    const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= wtot) return;
-    int ld;
-    double* col = gj_col(d, k, nbb, c, ld);
-    for (int j = 0; j < nbb; j++) {
-        const int p = d.piv[j];
-        const int r = k + j;
-        if (p != r) {
-            const double t = col[(long long)r * ld];
-            col[(long long)r * ld] = col[(long long)p * ld];
-            col[(long long)p * ld] = t;
-        }
-        d.U[(long long)j * d.ldu + c] = col[(long long)r * ld];
-    }
-}
-
-__global__ void __launch_bounds__(256) gj_update_kernel(const GJDesc* descs, int k) {
-    __shared__ double sA[GBK * (GBM + GPAD)];
-    __shared__ double sB[GBK * (GBN + GPAD)];
-    const int which = blockIdx.y % 3;
-    const GJDesc d = descs[blockIdx.y / 3];
-    if (k >= d.m) return;
-    const int nbb = min(GJ_NB, d.m - k);
-    const int wx = d.m - k - nbb;
-    GemmProb p;
-    p.A = d.M; p.lda = GJ_NB; p.K = nbb;
-    p.A2 = nullptr; p.B2 = nullptr; p.lda2 = p.ldb2 = p.K2 = 0;
-    p.ldb = d.ldu;
-    p.M = d.m;
-    p.alpha = 1.0; p.beta = 1.0;
-    if (which == 0) { p.D = d.X + k + nbb; p.ldd = d.ldx; p.N = wx; p.B = d.U; }
-    else if (which == 1) { p.D = d.R1; p.ldd = d.ld1; p.N = d.w1; p.B = d.U + wx; }
-    else { p.D = d.R2; p.ldd = d.ld2; p.N = d.w2; p.B = d.U + wx + d.w1; }
-    if (p.N <= 0 || p.D == nullptr) return;
-    p.Cin = p.D; p.ldcin = p.ldd;
-    const int tilesN = (p.N + GBN - 1) / GBN;
-    const int tilesM = (p.M + GBM - 1) / GBM;
-    if ((int)blockIdx.x >= tilesM * tilesN) return;
-    gemm_tile_f64(p, blockIdx.x / tilesN, blockIdx.x % tilesN, sA, sB);
-}
-
-// ---------------------------------------------------------------------------
-// Merge assembly (createMatrixBlocks_ + mergeX_ + mergeS_ right-hand side +
-// mergeT_'s block diagonal and H, HPSAlgorithm.hpp:799-968), with the WESN column
-// permutation of reorderOperators_ (:979-993) folded into where S_RHS and T land.
-//
-// Interfaces r = 0..3 are [alpha-gamma, beta-omega, alpha-beta, gamma-omega]; each
-// has a first child (its equation enters with -) and a second child (+):
-//   r: (c1, side at c1, c2, side at c2)
-// Parent boundary blocks q = 0..7 in WESN order: (child, side).
-// ---------------------------------------------------------------------------
-__constant__ int IF_C1[4] = {0, 1, 0, 2};
-__constant__ int IF_S1[4] = {N, N, E, E};
-__constant__ int IF_C2[4] = {2, 3, 1, 3};
-__constant__ int IF_S2[4] = {S, S, W, W};
-__constant__ int Q_CHILD[8] = {0, 2, 1, 3, 0, 1, 2, 3};
-__constant__ int Q_SIDE[8] = {W, W, E, E, S, S, N, N};
-// child c's two tau sides (natural order of its H rows) and their WESN block q
-__constant__ int TAU_SIDE[4][2] = {{W, S}, {E, S}, {W, N}, {E, N}};
-__constant__ int TAU_Q[4][2] = {{0, 4}, {2, 5}, {1, 6}, {3, 7}};
-// child c's two interfaces (H column slots) and the side of c on each
-__constant__ int CH_IF[4][2] = {{0, 2}, {1, 2}, {0, 3}, {1, 3}};
-__constant__ int CH_IF_SIDE[4][2] = {{N, E}, {N, W}, {S, E}, {S, W}};
-
-__device__ inline double tblk(const double* T, int n, int sa, int sb, int i, int j) {
-    return T[(long long)(sa * n + i) * (4 * n) + sb * n + j];
-}
-
-// side of child c on interface r, or -1
-__device__ inline int side_on_if(int c, int r) {
-    if (IF_C1[r] == c) return IF_S1[r];
-    if (IF_C2[r] == c) return IF_S2[r];
-    return -1;
-}
-
-__global__ void __launch_bounds__(256) merge_assemble_kernel(const MergeDesc* descs) {
-    const MergeDesc d = descs[blockIdx.y];
-    const int n = d.n;
-    const long long n2 = (long long)n * n;
-    const long long e0 = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long total = 144 * n2;
-    if (e0 >= total) return;
-    long long e = e0;
-    if (e < 16 * n2) {  // X (4n x 4n)
-        const int r = (int)(e / (4 * n)), c = (int)(e % (4 * n));
-        const int ri = r / n, i = r % n, ci = c / n, j = c % n;
-        const int c1 = IF_C1[ri], s1 = IF_S1[ri], c2 = IF_C2[ri], s2 = IF_S2[ri];
-        double v;
-        if (ri == ci) {
-            v = tblk(pick4(d.Tc, c1), n, s1, s1, i, j) - tblk(pick4(d.Tc, c2), n, s2, s2, i, j);
-        } else {
-            v = 0.0;
-            const int sc1 = side_on_if(c1, ci);
-            const int sc2 = side_on_if(c2, ci);
-            if (sc1 >= 0) v = tblk(pick4(d.Tc, c1), n, s1, sc1, i, j);
-            else if (sc2 >= 0) v = -tblk(pick4(d.Tc, c2), n, s2, sc2, i, j);
-        }
-        d.X[e] = v;
-        return;
-    }
-    e -= 16 * n2;
-    if (e < 32 * n2) {  // S_RHS (4n x 8n), columns in WESN order
-        const int r = (int)(e / (8 * n)), c = (int)(e % (8 * n));
-        const int ri = r / n, i = r % n, q = c / n, j = c % n;
-        const int cq = Q_CHILD[q], sq = Q_SIDE[q];
-        double v = 0.0;
-        if (cq == IF_C1[ri]) v = -tblk(pick4(d.Tc, cq), n, IF_S1[ri], sq, i, j);
-        else if (cq == IF_C2[ri]) v = tblk(pick4(d.Tc, cq), n, IF_S2[ri], sq, i, j);
-        d.S[e] = v;
-        return;
-    }
-    e -= 32 * n2;
-    if (e < 16 * n2) {  // Xi <- I
-        const int r = (int)(e / (4 * n)), c = (int)(e % (4 * n));
-        d.Xi[e] = (r == c) ? 1.0 : 0.0;
-        return;
-    }
-    e -= 16 * n2;
-    if (e < 64 * n2) {  // T <- block diagonal of children's tau-tau blocks (WESN rows and columns)
-        const int r = (int)(e / (8 * n)), c = (int)(e % (8 * n));
-        const int q = r / n, i = r % n, q2 = c / n, j = c % n;
-        double v = 0.0;
-        if (Q_CHILD[q] == Q_CHILD[q2]) v = tblk(pick4(d.Tc, Q_CHILD[q]), n, Q_SIDE[q], Q_SIDE[q2], i, j);
-        d.T[e] = v;
-        return;
-    }
-    e -= 64 * n2;
-    {  // H_c (2n x 2n) = [T_c(tau, side@if1) | T_c(tau, side@if2)]
-        const int c = (int)(e / (4 * n2));
-        const long long ee = e % (4 * n2);
-        const int rr = (int)(ee / (2 * n)), cc = (int)(ee % (2 * n));
-        const int ts = TAU_SIDE[c][rr / n];
-        const int is = CH_IF_SIDE[c][cc / n];
-        d.H[e] = tblk(pick4(d.Tc, c), n, ts, is, rr % n, cc % n);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// coarsen_ (HPSAlgorithm.hpp:707-741): dst = blkdiag4(L21) * src * blkdiag4(L12).
-// L21 averages pairs; column b of L12(nfine) is nonzero in rows
-// {0 if b<=2} u {2b-1, 2b if b>=1} u {2b+1, 2b+2 if b<=nc-2} u {nf-1 if b>=nc-3}
-// (SpecialMatrices.hpp:136-173).
-// ---------------------------------------------------------------------------
-__device__ inline int l12_col(int b, int nf, int* rows, double* w) {
-    const int nc = nf / 2;
-    int cnt = 0;
-    if (b <= 2) { rows[cnt] = 0; w[cnt] = (b == 0) ? 1.40625 : (b == 1 ? -0.5625 : 0.15625); cnt++; }
-    if (b >= 1) { rows[cnt] = 2 * b - 1; w[cnt] = 0.25; cnt++; rows[cnt] = 2 * b; w[cnt] = 0.75; cnt++; }
-    if (b <= nc - 2) { rows[cnt] = 2 * b + 1; w[cnt] = 0.75; cnt++; rows[cnt] = 2 * b + 2; w[cnt] = 0.25; cnt++; }
-    if (b >= nc - 3) {
-        rows[cnt] = nf - 1;
-        w[cnt] = (b == nc - 1) ? 1.40625 : (b == nc - 2 ? -0.5625 : 0.15625);
-        cnt++;
-    }
-    return cnt;
-}
-
-__global__ void __launch_bounds__(256) coarsen_kernel(const CoarsenDesc* descs) {
-    const CoarsenDesc d = descs[blockIdx.y];
-    const int m = d.m, mc = m / 2;
-    const long long total = (long long)(4 * mc) * (4 * mc);
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= total) return;
-    const int R = (int)(e / (4 * mc)), C = (int)(e % (4 * mc));
-    const int sa = R / mc, a = R % mc, sb = C / mc, b = C % mc;
-    int rows[6];
-    double w[6];
-    const int cnt = l12_col(b, m, rows, w);
-    const double* r0 = d.src + (long long)(sa * m + 2 * a) * (4 * m) + sb * m;
-    const double* r1 = r0 + 4 * m;
-    double s = 0.0;
-    for (int t = 0; t < cnt; t++) {
-        const double v = 0.5 * r0[rows[t]] + 0.5 * r1[rows[t]];
-        s += v * w[t];
-    }
-    d.dst[e] = s;
-}
-
 // boundary cell of WESN position b (0..4N): (i, j)
 __device__ inline void bcell(int b, int Nn, int& i, int& j) {
     const int s = b / Nn, t = b % Nn;
@@ -447,123 +19,6 @@ __device__ inline void bcell(int b, int Nn, int& i, int& j) {
     else if (s == E) { i = Nn - 1; j = t; }
     else if (s == S) { i = t; j = 0; }
     else { i = t; j = Nn - 1; }
-}
-
-// ---------------------------------------------------------------------------
-// Upward sweep gather (coarsenUpwards_ + hDiff of mergeW_ + boundary part of mergeH_,
-// HPSAlgorithm.hpp:1024-1117):
-//   hd[r]  = h_c2(s2_r) - h_c1(s1_r)                      (4n x k)
-//   h[q]   = h_{child q}(side q)        (WESN order)      (8n x k)
-// child h is coarsened tag times by pairwise averaging within each side.
-// ---------------------------------------------------------------------------
-__device__ inline double coarse_h(const double* harena, long long off, int clen, int tag, int side, int i, int k, int rhs) {
-    const int sl = clen / 4;  // fine side length
-    const long long base = off + (long long)side * sl;
-    if (tag == 0) return harena[(base + i) * k + rhs];
-    const int w = 1 << tag;
-    auto x = [&](int t) { return harena[(base + (long long)i * w + t) * k + rhs]; };
-    if (tag == 1) return 0.5 * x(0) + 0.5 * x(1);
-    if (tag == 2) {
-        const double a = 0.5 * x(0) + 0.5 * x(1), b = 0.5 * x(2) + 0.5 * x(3);
-        return 0.5 * a + 0.5 * b;
-    }
-    // deeper coarsening: pairwise averages of blocks of four (the tag-2 order), then the
-    // mean of those; equal to the nested pairwise order up to rounding
-    double s = 0.0;
-    for (int t = 0; t < w; t += 4) {
-        const double a = 0.5 * x(t) + 0.5 * x(t + 1), b = 0.5 * x(t + 2) + 0.5 * x(t + 3);
-        s += 0.5 * a + 0.5 * b;
-    }
-    return s * (4.0 / (double)w);
-}
-
-__global__ void __launch_bounds__(256) up_gather_kernel(const UpDesc* descs, double* harena, int k) {
-    const UpDesc d = descs[blockIdx.y];
-    const int n = d.n;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long total = 12LL * n * k;
-    if (e >= total) return;
-    const int rhs = (int)(e % k);
-    const int row = (int)(e / k);
-    if (row < 4 * n) {
-        const int r = row / n, i = row % n;
-        const int c1 = IF_C1[r], c2 = IF_C2[r];
-        const double a = coarse_h(harena, pick4(d.ch_off, c2), pick4(d.clen, c2), pick4(d.tag, c2), IF_S2[r], i, k, rhs);
-        const double b = coarse_h(harena, pick4(d.ch_off, c1), pick4(d.clen, c1), pick4(d.tag, c1), IF_S1[r], i, k, rhs);
-        harena[(d.hd_off + row) * k + rhs] = a - b;
-    } else {
-        const int rr = row - 4 * n;
-        const int q = rr / n, i = rr % n;
-        const int c = Q_CHILD[q];
-        harena[(d.h_off + rr) * k + rhs] = coarse_h(harena, pick4(d.ch_off, c), pick4(d.clen, c), pick4(d.tag, c), Q_SIDE[q], i, k, rhs);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Downward sweep: uncoarsen_ step (g <- blkdiag4(L12(nf)) g, HPSAlgorithm.hpp:1165-1183)
-// and the child scatter of applyS_ (:1194-1230).
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) uncoarsen_kernel(const VecStepDesc* descs, double* garena, int k) {
-    const VecStepDesc d = descs[blockIdx.y];
-    const int nf = d.nfine, nc = nf / 2;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 4LL * nf * k) return;
-    const int rhs = (int)(e % k);
-    const int row = (int)(e / k);
-    const int side = row / nf, i = row % nf;
-    const double* src = garena + (d.src_off + (long long)side * nc) * k + rhs;
-    double s;
-    if (i == 0) s = 1.40625 * src[0] + (-0.5625) * src[(long long)k] + 0.15625 * src[2LL * k];
-    else if (i == nf - 1) s = 0.15625 * src[(long long)(nc - 3) * k] + (-0.5625) * src[(long long)(nc - 2) * k] + 1.40625 * src[(long long)(nc - 1) * k];
-    else if (i & 1) { const int j = (i - 1) / 2; s = 0.75 * src[(long long)j * k] + 0.25 * src[(long long)(j + 1) * k]; }
-    else { const int j = (i - 2) / 2; s = 0.25 * src[(long long)j * k] + 0.75 * src[(long long)(j + 1) * k]; }
-    garena[(d.dst_off + row) * k + rhs] = s;
-}
-
-// children g: alpha = [aW, ab, aS, ag], beta = [ab, bE, bS, bo], gamma = [gW, go, ag, gN],
-// omega = [go, oE, bo, oN]  -- tau side -> parent WESN block q, interface -> u_int block r
-__constant__ int CH_SRC_Q[4][4] = {{0, -1, 4, -1}, {-1, 2, 5, -1}, {1, -1, -1, 6}, {-1, 3, -1, 7}};
-__constant__ int CH_SRC_R[4][4] = {{-1, 2, -1, 0}, {2, -1, -1, 1}, {-1, 3, 0, -1}, {3, -1, 1, -1}};
-
-__global__ void __launch_bounds__(256) down_scatter_kernel(const DownDesc* descs, double* garena, int k) {
-    const DownDesc d = descs[blockIdx.y];
-    const int n = d.n;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 16LL * n * k) return;
-    const int rhs = (int)(e % k);
-    const int row = (int)(e / k);
-    const int c = row / (4 * n), rr = row % (4 * n);
-    const int s = rr / n, i = rr % n;
-    const int q = CH_SRC_Q[c][s];
-    double v;
-    if (q >= 0) v = garena[(d.g_off + (long long)q * n + i) * k + rhs];
-    else v = garena[(d.ui_off + (long long)CH_SRC_R[c][s] * n + i) * k + rhs];
-    garena[(pick4(d.cg_off, c) + rr) * k + rhs] = v;
-}
-
-// Root Dirichlet data: g = r / a, the LU solve of diag(a) in solveStage
-// (HPSAlgorithm.hpp:402-420).  r is [rhs][m]; g is written in arena layout [row][rhs].
-__global__ void root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (long long)m * k) return;
-    const int i = (int)(e / k), c = (int)(e % k);
-    g[e] = r[(long long)c * m + i] / a[i];
-}
-
-// General root condition a u + b du/dn = r (HPSAlgorithm.hpp:402-420):
-// A = diag(a) + diag(b) T and rhs = r - b .* h, rhs in arena layout [row][rhs].
-__global__ void root_robin_assemble_kernel(const double* T, const double* a, const double* b, const double* r,
-                                           const double* h, double* A, double* rhs, int m, int k) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long nA = (long long)m * m;
-    if (e < nA) {
-        const int i = (int)(e / m), j = (int)(e % m);
-        A[e] = b[i] * T[e] + (i == j ? a[i] : 0.0);
-    } else if (e < nA + (long long)m * k) {
-        const long long t = e - nA;
-        const int i = (int)(t / k), c = (int)(t % k);
-        rhs[t] = r[(long long)c * m + i] - b[i] * (h ? h[t] : 0.0);
-    }
 }
 
 __global__ void fill_kernel(double* p, double v, long long n) {

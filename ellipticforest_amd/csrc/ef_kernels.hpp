@@ -5,22 +5,6 @@
 
 namespace efk {
 
-constexpr int GJ_NB = 32;          // Gauss-Jordan panel width
-constexpr int GJ_LDS_ROWS = 448;   // panel rows kept in LDS (448 x 33 x 8 B)
-
-__global__ void gemm_f64_kernel(const GemmProb* probs);
-__global__ void gj_panel_lds_kernel(const GJDesc* descs, int k);
-__global__ void gj_panel_global_kernel(const GJDesc* descs, int k);
-__global__ void gj_swap_kernel(const GJDesc* descs, int k);
-__global__ void gj_update_kernel(const GJDesc* descs, int k);
-__global__ void merge_assemble_kernel(const MergeDesc* descs);
-__global__ void coarsen_kernel(const CoarsenDesc* descs);
-__global__ void up_gather_kernel(const UpDesc* descs, double* harena, int k);
-__global__ void uncoarsen_kernel(const VecStepDesc* descs, double* garena, int k);
-__global__ void down_scatter_kernel(const DownDesc* descs, double* garena, int k);
-__global__ void root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k);
-__global__ void root_robin_assemble_kernel(const double* T, const double* a, const double* b, const double* r,
-                                           const double* h, double* A, double* rhs, int m, int k);
 __global__ void bt_factor_kernel(const LeafDesc* leaves, int Nn);
 template <int NT, bool PIVOT>
 __global__ void bt_factor_wave_kernel(const LeafDesc* leaves, int nleaves);

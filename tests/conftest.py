@@ -22,14 +22,6 @@ def gpu_present():
 
 EMULATOR = os.environ.get("EF_TEST_EMULATOR") == "1"
 
-# The device build path of libefhps.so is switched off in this build: on an MI355X it raised
-# a GPU memory fault twice, and the cause has not been shown yet (the same kernels pass the
-# bounds-checked CPU emulation in tests/cpu_emu).  GPU parity tests are skipped meanwhile.
-DEVICE_BUILD_DISABLED_REASON = (
-    "device build path disabled: it faulted the GPU twice and the cause is not shown yet; "
-    "the same kernels are checked on the CPU emulator (tests/test_emulated_kernels.py)")
-
-
 def emulator_library():
     """The CPU-emulation build of the HIP sources (built on first use)."""
     from cpu_emu import build_emu
@@ -48,12 +40,10 @@ def emu_ctx():
 @pytest.fixture(scope="session")
 def ctx():
     import ellipticforest_amd as ef
-    from ellipticforest_amd import _lib, build as efbuild
+    from ellipticforest_amd import _lib
     if EMULATOR:
         print("[gpu] context on the CPU emulator", flush=True)
         return ef.Context(0, library=emulator_library())
-    if not efbuild.DEVICE_BUILD_ENABLED:
-        pytest.skip(DEVICE_BUILD_DISABLED_REASON)
     if not gpu_present():
         pytest.fail("GPU test selected but no GPU device is visible")
     print(f"[gpu] creating context on device 0 ({_lib.LIB_PATH})", flush=True)
@@ -62,8 +52,7 @@ def ctx():
 
 @pytest.fixture(scope="session")
 def patch_ctx():
-    """Context for the batched patch entry points (ef_patches_*).  They share no kernel
-    with the build stage, so they run while that path is switched off."""
+    """Context for the batched patch entry points (ef_patches_*)."""
     import ellipticforest_amd as ef
     from ellipticforest_amd import _lib
     if EMULATOR:

@@ -18,8 +18,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "ellipticforest_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libefhps_emu.so")
-SOURCES = ["ef_kernels.hip", "ef_patch_kernels.hip", "ef_patch_large.hip", "ef_api.hip", "ef_patch.hip", "ef_mesh.cpp"]
-HEADERS = ["ef_kernels.hpp", "ef_device.hpp", "ef_host.hpp"]
+SOURCES = ["ef_kernels.hip", "ef_tree.hip", "ef_patch_kernels.hip", "ef_patch_large.hip", "ef_api.hip", "ef_patch.hip",
+           "ef_mesh.cpp"]
+HEADERS = ["ef_kernels.hpp", "ef_device.hpp", "ef_host.hpp", "ef_tree.hpp"]
 
 
 def transform(text):

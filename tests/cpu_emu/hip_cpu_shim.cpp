@@ -13,6 +13,7 @@ dim3 blockDim, gridDim;
 
 static const size_t PAGE = 4096;
 static std::map<void*, std::pair<void*, size_t>> g_maps;
+static std::map<void*, size_t> g_sizes;  // user bytes of each allocation
 static std::mutex g_mu;
 
 // Each buffer sits inside its own reservation with 1 GiB of PROT_NONE on both sides, so
@@ -56,8 +57,23 @@ hipError_t hipMalloc(void** p, size_t bytes) {
     if (!q) return 2;
     std::lock_guard<std::mutex> lk(g_mu);
     g_maps[q] = {base, len};
+    g_sizes[q] = bytes ? bytes : 8;
     *p = q;
     return 0;
+}
+
+hipError_t hipMemGetAddressRange(void** base, size_t* size, void* ptr) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const uintptr_t w = (uintptr_t)ptr;
+    for (auto& m : g_sizes) {
+        const uintptr_t b = (uintptr_t)m.first;
+        if (w >= b && w < b + m.second) {
+            *base = m.first;
+            *size = m.second;
+            return 0;
+        }
+    }
+    return 1;
 }
 bool emu_ptrcheck_on() {
     static const bool on = getenv("EF_EMU_PTRCHECK") && std::strcmp(getenv("EF_EMU_PTRCHECK"), "0") != 0;
@@ -96,11 +112,13 @@ hipError_t hipFree(void* p) {
     if (it == g_maps.end()) { fprintf(stderr, "hipFree of unknown pointer\n"); abort(); }
     munmap(it->second.first, it->second.second);
     g_maps.erase(it);
+    g_sizes.erase(p);
     return 0;
 }
 
 // ----- workgroup emulation -------------------------------------------------------
-static const int NT = 256;
+static const int NT = 1024;  // largest workgroup; a launch uses blockDim.x of them
+static int g_nt = 256;
 static std::barrier<>* g_block_bar = nullptr;
 static std::barrier<>* g_wave_bar[NT / 64];
 // Operand slots of the wave-collective operations, double-buffered by call parity: a lane
@@ -164,10 +182,12 @@ static void pool_init() {
             for (;;) {
                 g_start->arrive_and_wait();
                 if (g_quit) return;
-                g_par = 0;  // every block starts its wave collectives on slot 0
-                (*g_body)();
-                g_wave_bar[t / 64]->arrive_and_drop();
-                g_block_bar->arrive_and_drop();
+                if (t < g_nt) {
+                    g_par = 0;  // every block starts its wave collectives on slot 0
+                    (*g_body)();
+                    g_wave_bar[t / 64]->arrive_and_drop();
+                    g_block_bar->arrive_and_drop();
+                }
                 g_end->arrive_and_wait();
             }
         });
@@ -175,8 +195,12 @@ static void pool_init() {
 }
 
 void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body) {
-    if (b.x != NT || b.y != 1 || b.z != 1) { fprintf(stderr, "emulator supports 256-thread blocks only\n"); abort(); }
+    if (b.x % 64 != 0 || b.x > (unsigned)NT || b.y != 1 || b.z != 1) {
+        fprintf(stderr, "emulator supports 1-D blocks of 64..1024 threads (multiples of 64) only\n");
+        abort();
+    }
     pool_init();
+    g_nt = (int)b.x;
     gridDim = g;
     blockDim = b;
     if (shm) g_dyn = guarded_alloc(shm, &g_dyn_base, &g_dyn_len);
@@ -199,9 +223,9 @@ void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body)
     }
     for (const emu_idx& bi : order) {
         blockIdx = bi;
-        std::barrier<> bb(NT);
+        std::barrier<> bb(g_nt);
         g_block_bar = &bb;
-        for (int i = 0; i < NT / 64; i++) g_wave_bar[i] = new std::barrier<>(64);
+        for (int i = 0; i < g_nt / 64; i++) g_wave_bar[i] = new std::barrier<>(64);
         if (shm) {
             // fresh dynamic shared memory per block, filled as EF_EMU_FILL says (see guarded_alloc)
             const char* fill = getenv("EF_EMU_FILL");
@@ -219,7 +243,7 @@ void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body)
         }
         g_start->arrive_and_wait();
         g_end->arrive_and_wait();
-        for (int i = 0; i < NT / 64; i++) delete g_wave_bar[i];
+        for (int i = 0; i < g_nt / 64; i++) delete g_wave_bar[i];
     }
     if (shm) { munmap(g_dyn_base, g_dyn_len); g_dyn = nullptr; }
 }

@@ -48,11 +48,13 @@ typedef void* hipStream_t;
 enum { hipSuccess = 0 };
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
 enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize };
-struct hipFuncAttributes { size_t sharedSizeBytes = 0; };
+struct hipFuncAttributes { size_t sharedSizeBytes = 0; size_t localSizeBytes = 0; int numRegs = 0; int maxThreadsPerBlock = 1024; };
 inline const char* hipGetErrorString(hipError_t) { return "emulated error"; }
 hipError_t hipMalloc(void** p, size_t bytes);
 template <class T> hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc((void**)p, bytes); }
 hipError_t hipFree(void* p);
+// allocation containing ptr (base and size in bytes), as the HIP runtime reports it
+hipError_t hipMemGetAddressRange(void** base, size_t* size, void* ptr);
 // EF_EMU_PTRCHECK=1: report every aligned 8-byte word of a kernel argument or of a host-to-device
 // copy that points into mapped memory outside every device allocation.  Such a word is a host
 // address handed to the device, which works here (one address space) and faults a GPU.
@@ -76,7 +78,7 @@ inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
 inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { *p = std::malloc(n ? n : 1); return *p ? 0 : 2; }
 inline hipError_t hipHostFree(void* p) { std::free(p); return 0; }
 inline hipError_t hipDeviceSynchronize() { return 0; }
-inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { a->sharedSizeBytes = 0; return 0; }
+inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { *a = hipFuncAttributes{}; return 0; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
 
 // events: host wall clock (launches run synchronously in the emulation)
