@@ -5,12 +5,17 @@ u = sin x + sin y, refineByFunction(|f| > 1.2, min level 0, max level 7), 16x16 
 patch (7,750 leaves, root side 512), FISHPACK semantics, FP64.
 
 A step is one upwards + solve pass of a batch of --nrhs right-hand sides whose loads and
-boundary data are already resident in HBM; value = solves per second over the K timed
-steps.  buildStage seconds and per-request latency percentiles are reported beside it.
+boundary data are already resident in HBM (HPSAlgorithm::upwardsStage + solveStage over the
+whole tree); value = solves per second over the K timed steps, queued back to back on the
+mesh's stream between one pair of HIP events.  Reported beside it: buildStage seconds (HIP
+events around ef_build, steady state and first call), per-request latency percentiles (one
+right-hand side per request, host clock around a synchronised upwards + solve), the relative
+error of the result against the FP64 oracle, the sweep's HBM roofline (algorithmic operator
+and vector bytes per step over the step time) and the build's matrix-core rate.
 
-The CPU baseline is the oracle restatement of the reference algorithm (oracle/), timed
-on this host: one buildStage with all host threads, then single right-hand-side
-upwards + solve passes.
+The CPU baseline is the oracle restatement of the reference algorithm (oracle/), timed on
+this host: one buildStage with all host threads, upwards + solve passes at one thread and at
+all threads.
 
 Prints one JSON line.
 """
@@ -50,7 +55,8 @@ def workload(max_level, N):
 
 
 def cpu_baseline(leaves, N, nsolve, threads):
-    """Oracle (reference algorithm restated in C++) on this host."""
+    """Oracle (reference algorithm restated in C++) on this host: buildStage with all threads,
+    upwards + solve at one thread (nsolve passes) and at all threads (one pass)."""
     import oracle
     oracle.set_threads(threads)
     O = oracle.Forest(leaves, N, BOUNDS, solver=oracle.FISHPACK)
@@ -59,6 +65,10 @@ def cpu_baseline(leaves, N, nsolve, threads):
     t_build = time.perf_counter() - t0
     f = O.sample_rhs(F)
     r, a, b = O.boundary_data(lambda s, x, y: (U(x, y), 1.0, 0.0))
+    t0 = time.perf_counter()
+    O.upwards(f)
+    O.solve(r, a, b)
+    t_all = time.perf_counter() - t0
     oracle.set_threads(1)
     times = []
     u = None
@@ -72,19 +82,58 @@ def cpu_baseline(leaves, N, nsolve, threads):
         X, Y, _, _ = oracle.cell_centres(N, bd)
         err = max(err, float(np.abs(u[k] - U(X, Y)).max()))
     return dict(build_s=t_build, solve_s=float(np.mean(times)), solves_per_s=1.0 / float(np.mean(times)),
-                build_threads=threads, solve_threads=1, disc_err=err, u=u, f=f, r=r, a=a)
+                solves_per_s_all=1.0 / t_all, build_threads=threads, solve_threads=1, disc_err=err, u=u, f=f, r=r,
+                a=a)
 
 
-def gpu_run(leaves, N, k, steps, warmup, ref):
+def tree_counts(D):
+    """Interface sizes n of every merge and the coarsening steps (fine side m) of the mesh."""
+    info = D.node_info
+    ns, coarsen = [], []
+    for i in range(D.nnodes):
+        level, is_leaf, tag, s = info[i, 0], info[i, 3], info[i, 5], info[i, 6]
+        if not is_leaf:
+            ns.append(s // 2)
+        m = s
+        for _ in range(tag):
+            coarsen.append(m)
+            m //= 2
+    return np.array(ns, np.float64), np.array(coarsen, np.float64)
+
+
+def sweep_bytes(D, N, k):
+    """Algorithmic HBM bytes of one upwards + solve step with k right-hand sides: every merge's
+    X^-1 (16 n^2), S (32 n^2) and the 16 n^2 nonzeros of H (its children's DtN blocks) read
+    once, and per right-hand side each leaf's load f read and solution u written (FP64).  The
+    constant-coefficient leaves share their sine-transform tables, so leaf operators add nothing."""
+    ns, _ = tree_counts(D)
+    return 8.0 * (64.0 * float(np.sum(ns ** 2)) + k * D.nleaves * 2.0 * N * N)
+
+
+def build_flops(D, N, nclasses):
+    """Matrix-core work of buildStage.  dense: the reference's count (dgesv of X against the dense
+    S_RHS, dense H S, L21 T L12, leaf DtN by 32 N^4), sum 810.67 n^3 + 48 m^3 + 32 N^4 per leaf
+    class.  executed: what the kernels do (X^-1 by Gauss-Jordan 2 (4n)^3, S from the nonzero
+    half of S_RHS 128 n^3, H S from the nonzero half of H 256 n^3; coarsening as sparse reads)."""
+    ns, cm = tree_counts(D)
+    dense = float(np.sum(810.67 * ns ** 3) + np.sum(48.0 * cm ** 3) + 32.0 * N ** 4 * nclasses)
+    executed = float(np.sum(512.0 * ns ** 3))
+    return dense, executed
+
+
+def gpu_run(leaves, N, k, steps, warmup, ref, nlat):
     """Device path through the C ABI only (device memory, streams and HIP-event timers of
-    libefhps.so; no second HIP runtime in the process).  Returns a dict or raises EfError."""
+    libefhps.so; no second HIP runtime in the process)."""
     import ellipticforest_amd as ef
     ctx = ef.Context(0)
     D = ef.DeviceMesh(ctx, leaves, N, BOUNDS)
-    t0 = time.perf_counter()
-    D.build()
-    D.sync()
-    t_build = time.perf_counter() - t0
+    builds = []
+    for _ in range(3):  # the first call includes allocations and kernel loading
+        t0 = time.perf_counter()
+        D.build()
+        D.sync()
+        builds.append(dict(wall_s=time.perf_counter() - t0, event_s=D.stage_ms()["build-stage"] / 1e3))
+    log("buildStage: " + ", ".join(f"{b['event_s'] * 1e3:.2f} ms" for b in builds))
     nl = len(leaves)
     f = ef.DeviceArray(ctx, (k, nl, N * N))
     r = ef.DeviceArray(ctx, (k, 4 * D.M))
@@ -101,29 +150,40 @@ def gpu_run(leaves, N, k, steps, warmup, ref):
     for _ in range(warmup):
         step(k, u)
     D.sync()
-    up_ms, solve_ms = 0.0, 0.0
+    # timed region: K steps queued back to back between two HIP events on the mesh's stream
+    timer = ef.Timer(ctx)
     t0 = time.perf_counter()
+    timer.start(D.stream)
     for _ in range(steps):
         step(k, u)
-        st = D.stage_ms()  # event pair per stage on the mesh stream (waits for the step)
-        up_ms += st["upwards-stage"]
-        solve_ms += st["solve-stage"]
-    wall = time.perf_counter() - t0
-    ms_step = (up_ms + solve_ms) / steps
-    u0 = u.download(nl * N * N).reshape(nl, N * N)
-    rel = float(np.abs(u0 - ref["u"]).max() / np.abs(ref["u"]).max())
+    timer.stop(D.stream)
+    ms_step = timer.elapsed_ms() / steps
+    D.sync()
+    wall_step = (time.perf_counter() - t0) / steps * 1e3
+    st = D.stage_ms()  # the last step's stages
+    uu = u.download().reshape(k, nl, N * N)
+    rel = float(max(np.abs(uu[j] - ref["u"]).max() for j in range(k)) / np.abs(ref["u"]).max())
     # per-request latency: one right-hand side at a time, host clock around a synchronised step
     u1 = ef.DeviceArray(ctx, (1, nl, N * N))
+    for _ in range(3):
+        step(1, u1)
+    D.sync()
     lat = []
-    for _ in range(max(20, steps)):
-        D.sync()
+    for _ in range(nlat):
         t1 = time.perf_counter()
         step(1, u1)
         D.sync()
         lat.append((time.perf_counter() - t1) * 1e3)
-    return dict(build_s=t_build, build_ms_events=D.stage_ms()["build-stage"], ms_per_step=ms_step, wall_s=wall,
-                value=k / (ms_step / 1e3), rel_err=rel, upwards_ms=up_ms / steps, solve_ms=solve_ms / steps,
-                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
+    rel1 = float(np.abs(u1.download().reshape(nl, N * N) - ref["u"]).max() / np.abs(ref["u"]).max())
+    nbytes = sweep_bytes(D, N, k)
+    dense, executed = build_flops(D, N, len({lv for lv, _, _ in leaves}))  # FISHPACK leaf classes: one per leaf level
+    b_s = builds[-1]["event_s"]
+    return dict(builds=builds, build_s=b_s, ms_per_step=ms_step, wall_ms_per_step=wall_step, value=k / (ms_step / 1e3),
+                rel_err=max(rel, rel1), upwards_ms=st["upwards-stage"], solve_ms=st["solve-stage"],
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)), nlat=nlat,
+                sweep_bytes=nbytes, sweep_gbs=nbytes / (ms_step / 1e3) / 1e9,
+                build_tflops_dense=dense / b_s / 1e12, build_tflops_executed=executed / b_s / 1e12,
+                dense_flops=dense, executed_flops=executed, nnodes=D.nnodes, M=D.M)
 
 
 def leaf_bounds(leaves, N):
@@ -360,6 +420,32 @@ def copy_bandwidth(doubles=64 << 20, reps=5):
     return 2 * doubles * 8 / (best / 1e3) / 1e9
 
 
+def leaf_components(out, leaves, args, cpu_patch):
+    """Leaf-patch kernels on their own (the leaf stage of the path as one batch)."""
+    bounds = leaf_bounds(leaves, args.N)
+    k = args.patch_nrhs
+    pr = gpu_patch_run(bounds, args.N, k, args.steps, args.warmup, cpu_patch["check"])
+    log(f"leaf solves: {pr['ms']:.4f} ms per launch of {len(bounds)} x {k}, {pr['gbs']:.0f} GB/s")
+    comps = [dict(
+        name="leaf solve stage (FiniteVolumeSolver::solve on every leaf, ef_patches_apply_device)",
+        kernel="patch_fd_kernel<1, true, 0>", value=pr["value"], unit="leaf solves/s", ms_per_launch=pr["ms"],
+        patches=len(bounds), nrhs=k, rel_err_vs_fp64_oracle=pr["rel_err"], achieved_gbs=pr["gbs"],
+        algorithmic_bytes_per_launch=pr["bytes"],
+        traffic_from_profiles=TRAFFIC_PER_LAUNCH if k == 16 else None)]
+    pp = gpu_particular_run(bounds, args.N, k, args.steps, args.warmup)
+    comps.append(dict(
+        name="upwards leaf stage (particularNeumannData on every leaf, ef_patches_apply_device MAP_D2N)",
+        kernel="patch_fd_kernel<1, true, 1>", value=pp["value"], unit="leaf maps/s", ms_per_launch=pp["ms"],
+        patches=len(bounds), nrhs=k, achieved_gbs=pp["gbs"], algorithmic_bytes_per_launch=pp["bytes"]))
+    pv = gpu_variable_run(args.steps, args.warmup)
+    comps.append(dict(
+        name="variable-beta leaf solves (five-point, [-1,1]^2 uniform level 6: 4,096 patches, N = 32, 1 right-hand side)",
+        kernel="bt_solve_sym_kernel<32>" if pv["packed"] else "bt_solve_wave_kernel<32>", value=pv["value"],
+        unit="leaf solves/s", ms_per_launch=pv["ms"], patches=pv["patches"], nrhs=1, achieved_gbs=pv["gbs"],
+        algorithmic_bytes_per_launch=pv["bytes"], leaf_dtn_ms=pv["dtn_ms"]))
+    out["components"] = comps
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,7 +455,9 @@ def main():
     ap.add_argument("--max-level", type=int, default=7)
     ap.add_argument("--N", type=int, default=16)
     ap.add_argument("--cpu-solves", type=int, default=3)
+    ap.add_argument("--latency-requests", type=int, default=200)
     ap.add_argument("--patch-nrhs", type=int, default=16)
+    ap.add_argument("--leaf-components", action="store_true", help="also time the leaf-patch kernels on their own")
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only 5 leaf-solve, 5 particular-Neumann-data and 5 variable-beta leaf-solve launches and 3 "
@@ -385,96 +473,63 @@ def main():
     if args.traffic_probe:
         traffic_probe(leaf_bounds(leaves, args.N), args.N, args.patch_nrhs)
         return
-    threads = min(os.cpu_count() or 1, 64)
+    threads = os.cpu_count() or 1
     if args.no_cpu_baseline:
-        ref = dict(solves_per_s=None, build_s=None, disc_err=None)
-        cpu_patch = dict(value=None, patches=0, seconds=0.0, check=None)
+        import oracle
+        O = oracle.Forest(leaves, args.N, BOUNDS, solver=oracle.FISHPACK)
+        f = O.sample_rhs(F)
+        r, a, b = O.boundary_data(lambda s, x, y: (U(x, y), 1.0, 0.0))
+        ref = dict(solves_per_s=None, build_s=None, disc_err=None, u=None, f=f, r=r, a=a)
     else:
         log(f"cpu baseline: oracle build with {threads} threads, {args.cpu_solves} single-thread solves")
         ref = cpu_baseline(leaves, args.N, args.cpu_solves, threads)
-        log(f"cpu baseline: build {ref['build_s']:.2f}s, solve {ref['solve_s']:.2f}s, disc err {ref['disc_err']:.3e}")
-        cpu_patch = cpu_patch_baseline(leaf_bounds(leaves, args.N), args.N, args.patch_nrhs, 20000)
-        log(f"cpu baseline: {cpu_patch['value']:.0f} leaf solves/s (oracle, 1 thread)")
+        log(f"cpu baseline: build {ref['build_s']:.2f}s, solve {ref['solve_s']:.2f}s (1 thread), "
+            f"{1.0 / ref['solves_per_s_all']:.2f}s ({threads} threads), disc err {ref['disc_err']:.3e}")
     out = dict(metric=METRIC, value=None, unit="solves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=None, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
                data="synthetic (analytic f and Dirichlet data sampled at cell centres / root face points)",
                config=dict(workload="adaptive Poisson [-10,10]^2, |f|>1.2 refinement, levels 0..%d, %dx%d cells per "
-                           "patch, FISHPACK semantics" % (args.max_level, args.N, args.N),
+                           "patch, FISHPACK semantics, %d right-hand sides per step" % (
+                               args.max_level, args.N, args.N, args.nrhs),
                            leaves=len(leaves), nrhs_per_step=args.nrhs, parallelism="single GPU"),
-               build_s=None, p50_ms=None, p99_ms=None, rel_err_vs_fp64_oracle=None,
-               roofline=None,
+               build_s=None, p50_ms=None, p99_ms=None, rel_err_vs_fp64_oracle=None, roofline=None,
                cpu_baseline=None if args.no_cpu_baseline else dict(
                    value=ref["solves_per_s"], unit="solves/s", cores=1, kind="port",
-                   sample="oracle (reference algorithm restated in C++): upwards+solve of 1 RHS on the full "
-                          "workload, mean of %d passes at 1 thread; its buildStage took %.2f s on %d threads"
-                          % (args.cpu_solves, ref["build_s"], threads),
+                   sample="oracle (reference algorithm restated in C++, -O3 -march=x86-64-v3): upwards + solve of "
+                          "1 right-hand side on the full workload, mean of %d passes at 1 thread" % args.cpu_solves,
+                   value_all_cores=ref["solves_per_s_all"], host_threads=threads,
                    build_s=ref["build_s"], build_threads=threads, discretization_error=ref["disc_err"]))
-    # leaf-solve stage on its own: the whole leaf set of the workload as one batch
+    if ref["u"] is None:
+        ref["u"] = np.zeros((len(leaves), args.N * args.N))
     try:
-        bounds = leaf_bounds(leaves, args.N)
-        k = args.patch_nrhs
-        pr = gpu_patch_run(bounds, args.N, k, args.steps, args.warmup, cpu_patch["check"])
-        log(f"leaf solves: {pr['ms']:.4f} ms per launch of {len(bounds)} x {k}, {pr['gbs']:.0f} GB/s, "
-            f"{pr['tflops']:.2f} TFLOP/s, rel err {pr['rel_err']}")
-        out["components"] = [dict(
-            name="leaf solve stage (FiniteVolumeSolver::solve on every leaf, ef_patches_apply_device)",
-            kernel="patch_fd_kernel<1, true, 0>", value=pr["value"], unit="leaf solves/s", ms_per_launch=pr["ms"],
-            ms_min_single=pr["ms_min"], ms_single_mean=pr["ms_single_mean"], wall_ms_per_step=pr["wall_ms"], patches=len(bounds), nrhs=k,
-            rel_err_vs_fp64_oracle=pr["rel_err"], tflops=pr["tflops"], algorithmic_bytes_per_launch=pr["bytes"],
-            traffic=TRAFFIC_PER_LAUNCH if k == 16 else None,
-            cpu_baseline=dict(value=cpu_patch["value"], unit="leaf solves/s", cores=1, kind="port",
-                              sample="oracle leaf solve (banded LU of the 5-point system), %d patches of the "
-                                     "workload in %.1f s at 1 thread" % (cpu_patch["patches"], cpu_patch["seconds"])))]
-        copy_gbs = copy_bandwidth()
-        log(f"plain copy on this GPU: {copy_gbs:.0f} GB/s")
-        out["roofline"] = dict(bound="hbm", achieved=pr["gbs"], peak=8000.0, unit="GB/s", frac=pr["gbs"] / 8000.0,
-                               traffic=TRAFFIC_PER_LAUNCH if k == 16 else None, kernel="patch_fd_kernel<1, true, 0>",
-                               copy_kernel_gbs=copy_gbs, frac_of_copy=pr["gbs"] / copy_gbs,
-                               algorithmic_bytes_per_launch=pr["bytes"],
-                               scope="leaf-solve stage only; the build stage is switched off in this library build")
-        pp = gpu_particular_run(bounds, args.N, k, args.steps, args.warmup)
-        log(f"particular Neumann data: {pp['ms']:.4f} ms per launch, {pp['gbs']:.0f} GB/s")
-        cpu_map = None
-        if not args.no_cpu_baseline:
-            cm = cpu_leaf_op_baseline("map", bounds, args.N, seconds=10.0, sample=20000)
-            cpu_map = dict(value=cm["value"], unit="leaf maps/s", cores=1, kind="port",
-                           sample="oracle particularNeumannData (banded LU of the 5-point system), %d patches of "
-                                  "the leaf set in %.1f s at 1 thread" % (cm["patches"], cm["seconds"]))
-        out["components"].append(dict(
-            name="upwards leaf stage (particularNeumannData on every leaf, ef_patches_apply_device MAP_D2N)",
-            cpu_baseline=cpu_map,
-            kernel="patch_fd_kernel<1, true, 1>", value=pp["value"], unit="leaf maps/s", ms_per_launch=pp["ms"],
-            patches=len(bounds), nrhs=k, achieved_gbs=pp["gbs"], algorithmic_bytes_per_launch=pp["bytes"],
-            traffic=TRAFFIC_MAP_D2N if k == 16 else None, frac_of_peak=pp["gbs"] / 8000.0))
-        pv = gpu_variable_run(args.steps, args.warmup)
-        log(f"variable-beta leaf solves: {pv['ms']:.4f} ms per launch, {pv['gbs']:.0f} GB/s; "
-            f"leaf DtN of the set {pv['dtn_ms']:.2f} ms")
-        cpu_var = None
-        if not args.no_cpu_baseline:
-            cv = cpu_leaf_op_baseline("solve", pv["bounds"], 32, cells=pv["cells"], seconds=10.0,
-                                      sample=len(pv["cells"]))
-            cpu_var = dict(value=cv["value"], unit="leaf solves/s", cores=1, kind="port",
-                           sample="oracle five-point leaf solve (banded LU), %d patches of the leaf set in %.1f s "
-                                  "at 1 thread" % (cv["patches"], cv["seconds"]))
-        out["components"].append(dict(
-            name="variable-beta leaf solves (five-point, [-1,1]^2 uniform level 6: 4,096 patches, N = 32, 1 right-hand side)",
-            cpu_baseline=cpu_var,
-            kernel="bt_solve_sym_kernel<32>" if pv["packed"] else "bt_solve_wave_kernel<32>", value=pv["value"],
-            unit="leaf solves/s", ms_per_launch=pv["ms"], patches=pv["patches"], nrhs=1,
-            factor_storage="packed symmetric" if pv["packed"] else "full", achieved_gbs=pv["gbs"],
-            algorithmic_bytes_per_launch=pv["bytes"], traffic=TRAFFIC_VAR_BETA if pv["packed"] else None,
-            frac_of_peak=pv["gbs"] / 8000.0, leaf_dtn_ms=pv["dtn_ms"],
-            leaf_dtn_wall_ms_incl_copy_to_host=pv["dtn_wall_ms"]))
-    except Exception as e:
-        out["components_error"] = f"{type(e).__name__}: {e}"
-        log(out["components_error"])
-    try:
-        g = gpu_run(leaves, args.N, args.nrhs, args.steps, args.warmup, ref)
+        g = gpu_run(leaves, args.N, args.nrhs, args.steps, args.warmup, ref, args.latency_requests)
+        log(f"step {g['ms_per_step']:.4f} ms ({g['value']:.0f} solves/s), sweep {g['sweep_gbs']:.0f} GB/s, "
+            f"build {g['build_s'] * 1e3:.2f} ms, p50 {g['p50_ms']:.3f} ms p99 {g['p99_ms']:.3f} ms, rel err {g['rel_err']:.2e}")
         out.update(value=g["value"], ms_per_step=g["ms_per_step"], build_s=g["build_s"], p50_ms=g["p50_ms"],
-                   p99_ms=g["p99_ms"], rel_err_vs_fp64_oracle=g["rel_err"])
+                   p99_ms=g["p99_ms"], rel_err_vs_fp64_oracle=None if args.no_cpu_baseline else g["rel_err"],
+                   wall_ms_per_step=g["wall_ms_per_step"], upwards_ms=g["upwards_ms"], solve_ms=g["solve_ms"],
+                   build_calls=g["builds"], latency_requests=g["nlat"])
+        out["roofline"] = dict(
+            bound="hbm", achieved=g["sweep_gbs"], peak=8000.0, unit="GB/s", frac=g["sweep_gbs"] / 8000.0,
+            traffic=None, kernel="HPS sweep step (upwards + solve: leaf kernels, tree_gemm_kernel, gather/scatter)",
+            algorithmic_bytes_per_step=g["sweep_bytes"],
+            note="achieved = algorithmic operator + vector bytes of one step / HIP-event step time; traffic "
+                 "(HBM bytes by counters) is in profiles/, not measured in this run")
+        out["build_roofline"] = dict(
+            bound="mfma", peak=78.6, unit="TFLOP/s", achieved_dense_count=g["build_tflops_dense"],
+            achieved_executed=g["build_tflops_executed"], frac_executed=g["build_tflops_executed"] / 78.6,
+            dense_flops=g["dense_flops"], executed_flops=g["executed_flops"],
+            speedup_vs_cpu_build=None if args.no_cpu_baseline else ref["build_s"] / g["build_s"])
     except Exception as e:  # the device path reports why it did not run
         out["error"] = f"{type(e).__name__}: {e}"
         log(out["error"])
+    if args.leaf_components:
+        try:
+            cpu_patch = dict(check=None) if args.no_cpu_baseline else cpu_patch_baseline(
+                leaf_bounds(leaves, args.N), args.N, args.patch_nrhs, 20000)
+            leaf_components(out, leaves, args, cpu_patch)
+        except Exception as e:
+            out["components_error"] = f"{type(e).__name__}: {e}"
     print(json.dumps(out), flush=True)
 
 

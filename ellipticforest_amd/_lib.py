@@ -29,7 +29,7 @@ EF_PRECISION_MIXED = 1
 EXPORTS = [
     "ef_ctx_create", "ef_ctx_destroy", "ef_last_error", "ef_version", "ef_refine_by_function",
     "ef_mesh_create", "ef_mesh_destroy", "ef_mesh_footprint", "ef_mesh_info", "ef_mesh_nodes",
-    "ef_mesh_set_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
+    "ef_mesh_set_stream", "ef_mesh_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
     "ef_memcpy", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
     "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_dtn", "ef_patches_sync",
@@ -85,6 +85,7 @@ def load_library(path):
     L.ef_mesh_info.argtypes = [_vp, _ip]
     L.ef_mesh_nodes.argtypes = [_vp, _ip, _dp]
     L.ef_mesh_set_stream.argtypes = [_vp, _vp]
+    L.ef_mesh_stream.argtypes = [_vp, ctypes.POINTER(_vp)]
     L.ef_mesh_set_options.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     L.ef_build.argtypes = [_vp, ctypes.POINTER(Coeffs), ctypes.c_int]
     L.ef_upwards.argtypes = [_vp, _dp, ctypes.c_int]
@@ -255,6 +256,13 @@ class DeviceMesh:
 
     def set_stream(self, stream_ptr):
         self._check(self.L.ef_mesh_set_stream(self.h, stream_ptr))
+
+    @property
+    def stream(self):
+        """The hipStream_t (as an integer) the mesh's stages run on."""
+        s = _vp()
+        self._check(self.L.ef_mesh_stream(self.h, ctypes.byref(s)))
+        return s.value
 
     def build(self, solver_type=EF_SOLVER_FISHPACK, lambda0=0.0, alpha0=1.0, beta0=1.0, cell=None,
               precision=EF_PRECISION_FP64):

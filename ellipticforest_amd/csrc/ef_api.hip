@@ -1028,6 +1028,13 @@ int ef_mesh_set_stream(ef_mesh mesh, void* hip_stream) {
     EF_API_END
 }
 
+int ef_mesh_stream(ef_mesh mesh, void** hip_stream) {
+    EF_API_BEGIN
+    if (!mesh || !hip_stream) throw EfError(EF_ERR_ARG, "null argument");
+    *hip_stream = (void*)mesh->stream;
+    EF_API_END
+}
+
 int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs) {
     EF_API_BEGIN
     if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");

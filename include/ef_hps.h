@@ -91,6 +91,10 @@ int ef_mesh_info(ef_mesh mesh, int* info);
 int ef_mesh_nodes(ef_mesh mesh, int* info, double* bounds);
 /* Use a caller-owned hipStream_t (passed as void*); NULL restores the mesh's own stream. */
 int ef_mesh_set_stream(ef_mesh mesh, void* hip_stream);
+/* The stream the mesh's stages run on (its own, or the one given to ef_mesh_set_stream), for
+ * timing queued stages with ef_timer_* (no counterpart in the reference, whose stages run on
+ * the host). */
+int ef_mesh_stream(ef_mesh mesh, void** hip_stream);
 
 /* Stage options of EllipticForestApp (HPSAlgorithm.hpp:134, 532, 587, 1202). */
 int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs);

@@ -48,7 +48,9 @@ def solve_both(ctx, leaves, N, bounds, lam=0.0, check_ops=False, nodes=None):
             if not D.node_info[node, 3]:
                 for w in ("S", "X", "H"):
                     assert rel(D.get(node, w), O.get(node, w)) < TOL_OP, (w, node)
-                assert rel(D.get(node, "w"), O.get(node, "w")) < TOL_OP, ("w", node)
+                # w is part of the solve (X^-1 hDiff): the library applies the stored inverse, the
+                # oracle re-solves with LU (mergeW_), so it carries cond(X) eps like the solution
+                assert rel(D.get(node, "w"), O.get(node, "w")) < TOL_U, ("w", node)
             assert rel(D.get(node, "g"), O.get(node, "g")) < TOL_U, ("g", node)
     return O, D, uo, ug
 
