@@ -32,7 +32,8 @@ EXPORTS = [
     "ef_mesh_set_stream", "ef_mesh_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
     "ef_memcpy", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
-    "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_dtn", "ef_patches_sync",
+    "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_apply_device_noalloc",
+    "ef_patches_dtn", "ef_patches_sync",
     "ef_patches_last_ms", "ef_patches_stream", "ef_timer_create", "ef_timer_destroy", "ef_timer_record",
     "ef_timer_elapsed_ms",
 ]
@@ -104,6 +105,7 @@ def load_library(path):
     L.ef_patches_info.argtypes = [_vp, _ip]
     L.ef_patches_apply.argtypes = [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
     L.ef_patches_apply_device.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int]
+    L.ef_patches_apply_device_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int]
     L.ef_patches_dtn.argtypes = [_vp, _dp]
     L.ef_patches_sync.argtypes = [_vp]
     L.ef_patches_last_ms.argtypes = [_vp, _dp]

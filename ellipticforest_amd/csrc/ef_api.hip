@@ -930,6 +930,7 @@ int ef_ctx_create(int device, ef_ctx* out) {
     EF_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw EfError(EF_ERR_ARG, "no such device");
     EF_HIP(hipSetDevice(device));
+    efk::noalloc_setup(device);
     auto* c = new ef_ctx_s();
     c->device = device;
     *out = c;

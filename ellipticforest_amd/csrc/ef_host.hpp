@@ -39,6 +39,8 @@ long long patch_default_blocks();
 bool cells_symmetric(const double* c, int N);
 // Every row of the patch's five-point matrix diagonally dominant (block LU without pivoting).
 bool cells_dominant(const double* c, int N, double dx, double dy);
+// Device address of the no-allocation scratch array on `device` (current), taken once per device.
+void noalloc_setup(int device);
 
 struct EfError : std::runtime_error {
     int code;

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -187,6 +188,41 @@ void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStre
     EF_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------------------------------
+// Work space of ef_patches_apply_device_noalloc: a file-scope device array, one copy per GPU,
+// held until the process ends.  In host code its name is only a host-side stand-in; the
+// device address comes from hipGetSymbolAddress, taken once per device at context creation
+// (efk::noalloc_setup) with that device current, and only that address is used.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 22;  // 32 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch;  // device -> address of its copy of `scratch`
+
+double* scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    return it == g_scratch.end() ? nullptr : it->second;
+}
+
+// Doubles of work space an apply of this batch needs beyond its inputs and outputs.
+size_t apply_scratch_doubles(const ef_patches_s* p, int op) {
+    if (p->variable) return (p->packed && op == PATCH_OP_MAP_D2N) ? (size_t)p->P * p->N * p->N : 0;
+    if (p->large) return 2 * (size_t)p->Np * p->Np;  // one item in each of the two GEMM buffers
+    return 0;
+}
+
+// Device address of the no-allocation entry point's scratch array on the current device,
+// taken once per device (the library's context creation calls this with the device current).
+void noalloc_setup(int device) {
+    if (scratch_address(device)) return;
+    double* ptr = nullptr;
+    EF_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw EfError(EF_ERR_DEVICE, "no device address for the no-allocation scratch array");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    g_scratch[device] = ptr;
+}
+
 }  // namespace efk
 
 namespace {
@@ -230,13 +266,23 @@ void launch_gemm_strided(const GemmStridedArgs& g, hipStream_t s) {
     EF_LAUNCH_CHECK();
 }
 
-void launch_large(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs, int patch0 = 0) {
+// W1, W2: work buffers of at least wcap doubles each (null: the batch's own, grown as needed).
+// Items are independent, so the chunking does not change any result.
+void launch_large(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs, int patch0 = 0,
+                  double* W1 = nullptr, double* W2 = nullptr, long long wcap = 0) {
     const int N = p->N, Np = p->Np;
     const long long per = (long long)Np * Np;
     const long long total = (op == PATCH_OP_DTN) ? 4LL * N : (long long)p->P * nrhs;
-    const long long chunk = std::max<long long>(1, std::min<long long>({65535LL, total, (16LL << 20) / per}));
-    p->W1.ensure((size_t)(chunk * per));
-    p->W2.ensure((size_t)(chunk * per));
+    long long chunk = std::max<long long>(1, std::min<long long>({65535LL, total, (16LL << 20) / per}));
+    if (W1) {
+        chunk = std::min(chunk, wcap / per);
+        if (chunk < 1) throw EfError(EF_ERR_ARG, "work buffers too small for one large patch");
+    } else {
+        p->W1.ensure((size_t)(chunk * per));
+        p->W2.ensure((size_t)(chunk * per));
+        W1 = p->W1.p;
+        W2 = p->W2.p;
+    }
     for (long long first = 0; first < total; first += chunk) {
         const int items = (int)std::min(chunk, total - first);
         PatchLargeArgs a{};
@@ -251,7 +297,7 @@ void launch_large(ef_patches_s* p, int op, const double* g, const double* f, dou
         a.mu = p->mu.p;
         a.f = f;
         a.g = g;
-        a.work = p->W1.p;
+        a.work = W1;
         a.out = out;
         a.item0 = (op == PATCH_OP_DTN) ? 0 : first;
         hipLaunchKernelGGL(patch_large_fold_kernel, dim3(blocks_for(items * per)), dim3(256), 0, p->stream, a);
@@ -261,25 +307,25 @@ void launch_large(ef_patches_s* p, int op, const double* g, const double* f, dou
         m.batch = items;
         // W2 = Q^T W1
         m.A = p->Qp.p; m.sa = 0; m.ra = 1; m.ca = Np;
-        m.B = p->W1.p; m.sb = per; m.rb = Np; m.cb = 1;
-        m.C = p->W2.p; m.sc = per; m.rc = Np; m.cc = 1;
+        m.B = W1; m.sb = per; m.rb = Np; m.cb = 1;
+        m.C = W2; m.sc = per; m.rc = Np; m.cc = 1;
         launch_gemm_strided(m, p->stream);
         // W1 = W2 Q
-        m.A = p->W2.p; m.sa = per; m.ra = Np; m.ca = 1;
+        m.A = W2; m.sa = per; m.ra = Np; m.ca = 1;
         m.B = p->Qp.p; m.sb = 0; m.rb = Np; m.cb = 1;
-        m.C = p->W1.p;
+        m.C = W1;
         launch_gemm_strided(m, p->stream);
         hipLaunchKernelGGL(patch_large_scale_kernel, dim3(blocks_for(items * per)), dim3(256), 0, p->stream, a);
         EF_LAUNCH_CHECK();
         // W2 = Q W1
         m.A = p->Qp.p; m.sa = 0; m.ra = Np; m.ca = 1;
-        m.B = p->W1.p; m.sb = per; m.rb = Np; m.cb = 1;
-        m.C = p->W2.p;
+        m.B = W1; m.sb = per; m.rb = Np; m.cb = 1;
+        m.C = W2;
         launch_gemm_strided(m, p->stream);
         // W1 = W2 Q^T
-        m.A = p->W2.p; m.sa = per; m.ra = Np; m.ca = 1;
+        m.A = W2; m.sa = per; m.ra = Np; m.ca = 1;
         m.B = p->Qp.p; m.sb = 0; m.rb = 1; m.cb = Np;
-        m.C = p->W1.p;
+        m.C = W1;
         launch_gemm_strided(m, p->stream);
         const long long nout = (op == PATCH_OP_SOLVE) ? (long long)items * N * N : (long long)items * 4 * N;
         hipLaunchKernelGGL(patch_large_out_kernel, dim3(blocks_for(nout)), dim3(256), 0, p->stream, a);
@@ -310,14 +356,18 @@ void record_end(ef_patches_s* p) {
 }
 
 // Block-tridiagonal path, one right-hand side per launch ([patch][...] layouts with k = 1).
-void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs) {
+// vz: work space of the packed Neumann maps, P N^2 doubles (null: the batch's own, grown as needed).
+void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double* out, int nrhs, double* vz = nullptr) {
     const long long NN = (long long)p->N * p->N, B4 = 4LL * p->N;
-    if (p->packed && op == PATCH_OP_MAP_D2N) p->vz.ensure((size_t)p->P * NN);
+    if (p->packed && op == PATCH_OP_MAP_D2N && !vz) {
+        p->vz.ensure((size_t)p->P * NN);
+        vz = p->vz.p;
+    }
     if (op == PATCH_OP_MAP_D2N && g == nullptr) {
         for (int c = 0; c < nrhs; c++) {
             launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, 0, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
                             (const double*)nullptr, out + c * p->P * B4, (double*)nullptr, p->stream, p->packed,
-                            p->vz.p);
+                            vz);
             EF_LAUNCH_CHECK();
         }
         return;
@@ -327,7 +377,7 @@ void launch_bt(ef_patches_s* p, int op, const double* g, const double* f, double
     for (int c = 0; c < nrhs; c++) {
         launch_bt_solve(p->leafdesc.p, p->N, p->P, 1, mode, f ? f + c * p->P * NN : nullptr, f ? 0 : 1,
                         g + c * p->P * B4, mode == 2 ? out + c * p->P * B4 : (double*)nullptr,
-                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream, p->packed, p->vz.p);
+                        mode == 1 ? out + c * p->P * NN : (double*)nullptr, p->stream, p->packed, vz);
         EF_LAUNCH_CHECK();
     }
 }
@@ -509,6 +559,28 @@ int ef_patches_apply_device(ef_patches p, int op, const double* g_dev, const dou
     record_begin(p);
     apply(p, op, g_dev, f_dev, out_dev, nrhs);
     record_end(p);
+    EF_API_END
+}
+
+int ef_patches_apply_device_noalloc(ef_patches p, int op, const double* g_dev, const double* f_dev, double* out_dev,
+                                    int nrhs) {
+    EF_API_BEGIN
+    if (!p) throw EfError(EF_ERR_ARG, "null handle");
+    if (op != PATCH_OP_SOLVE && op != PATCH_OP_MAP_D2N) throw EfError(EF_ERR_ARG, "op must be EF_PATCH_SOLVE or EF_PATCH_MAP_D2N");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    if (!out_dev) throw EfError(EF_ERR_ARG, "null output");
+    EF_HIP(hipSetDevice(p->ctx->device));
+    double* work = scratch_address(p->ctx->device);
+    if (!work) throw EfError(EF_ERR_STATE, "no scratch address for this device (the context did not set it up)");
+    const size_t need = apply_scratch_doubles(p, op);
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        throw EfError(EF_ERR_ARG, "this batch needs " + std::to_string(need) + " doubles of work space; the "
+                                  "no-allocation entry point has " + std::to_string(NOALLOC_SCRATCH_DOUBLES));
+    if (p->variable) launch_bt(p, op, g_dev, f_dev, out_dev, nrhs, work);
+    else if (p->large) {
+        const long long half = (long long)(NOALLOC_SCRATCH_DOUBLES / 2);
+        launch_large(p, op, g_dev, f_dev, out_dev, nrhs, 0, work, work + half, half);
+    } else launch_fd(p, op, g_dev, f_dev, out_dev, nrhs);
     EF_API_END
 }
 

@@ -184,6 +184,16 @@ int ef_patches_info(ef_patches p, int* info);
 int ef_patches_apply(ef_patches p, int op, const double* g, const double* f, double* out, int nrhs);
 int ef_patches_apply_device(ef_patches p, int op, const double* g_dev, const double* f_dev, double* out_dev,
                             int nrhs);
+/* ef_patches_apply_device without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Work space (the forward rows of packed variable-coefficient
+ * Neumann maps, the GEMM buffers of patches with N > 32) comes from one fixed array of 2^22
+ * doubles (32 MiB) per GPU, kept until the process ends; a batch needing more is refused with
+ * EF_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (different streams, different threads) race.  Nothing in the array needs to start at zero.
+ * The call records no timer events (ef_patches_last_ms does not see it). */
+int ef_patches_apply_device_noalloc(ef_patches p, int op, const double* g_dev, const double* f_dev,
+                                    double* out_dev, int nrhs);
 /* FiniteVolumeSolver::buildD2N (:355-455) of every patch: T is [npatches][4N][4N]. */
 int ef_patches_dtn(ef_patches p, double* T);
 int ef_patches_sync(ef_patches p);

@@ -28,6 +28,7 @@
 #define __host__
 #define __constant__
 #define __launch_bounds__(x)
+#define __align__(n) __attribute__((aligned(n)))
 using std::max;
 using std::min;
 
@@ -80,6 +81,9 @@ inline hipError_t hipHostFree(void* p) { std::free(p); return 0; }
 inline hipError_t hipDeviceSynchronize() { return 0; }
 inline hipError_t hipFuncGetAttributes(hipFuncAttributes* a, const void*) { *a = hipFuncAttributes{}; return 0; }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return 0; }
+// device symbols are host objects here: their address is the symbol's own
+#define HIP_SYMBOL(X) ((const void*)&(X))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) { *p = const_cast<void*>(sym); return 0; }
 
 // events: host wall clock (launches run synchronously in the emulation)
 typedef double* hipEvent_t;
