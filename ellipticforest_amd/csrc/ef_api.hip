@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -265,9 +266,20 @@ struct ef_mesh_s {
     // stage timers: event pairs for build, upwards, solve
     hipEvent_t ev[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     bool ev_recorded[3] = {false, false, false};
+    // captured sweeps: one executable graph per (stage, k, device pointers, flags)
+    struct SweepGraph {
+        int kind = 0, k = 0, flags = 0;
+        std::array<const void*, 4> ptr{};
+        hipGraphExec_t exec = nullptr;
+        unsigned long long used = 0;
+    };
+    std::vector<SweepGraph> graphs;
+    unsigned long long graph_clock = 0;
 };
 
 namespace {
+
+void clear_graphs(ef_mesh_s* M);
 
 const double* leaf_T_ptr(ef_mesh_s* M, int leaf) {
     const long long per = 16LL * M->P.N * M->P.N;
@@ -536,11 +548,14 @@ void stage_end(ef_mesh_s* M, int st) {
 void check_tree_kernels() {
     static bool done = false;
     if (done) return;
-    check_kernel_resources((const void*)tree_gemm_kernel<1>, "tree_gemm_kernel<1>");
-    check_kernel_resources((const void*)tree_gemm_kernel<2>, "tree_gemm_kernel<2>");
-    check_kernel_resources((const void*)gj_panel_leaf_kernel<0>, "gj_panel_leaf_kernel<0>");
-    check_kernel_resources((const void*)gj_panel_leaf_kernel<1>, "gj_panel_leaf_kernel<1>");
-    check_kernel_resources((const void*)gj_panel_final_kernel, "gj_panel_final_kernel");
+    check_kernel_resources((const void*)tree_gemm_kernel<1, 1>, "tree_gemm_kernel<1,1>");
+    check_kernel_resources((const void*)tree_gemm_kernel<2, 2>, "tree_gemm_kernel<2,2>");
+    check_kernel_resources((const void*)tree_gemm_reduce_kernel, "tree_gemm_reduce_kernel");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 2, true>, "gj_panel_kernel<0,2,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 4, true>, "gj_panel_kernel<0,4,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 16, true>, "gj_panel_kernel<0,16,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 16, false>, "gj_panel_kernel<0,16>");
+    check_kernel_resources((const void*)gj_panel_kernel<1, 16, true>, "gj_panel_kernel<1,16,fin>");
     check_kernel_resources((const void*)gj_mult_kernel, "gj_mult_kernel");
     check_kernel_resources((const void*)gj_update_kernel, "gj_update_kernel");
     check_kernel_resources((const void*)gj_perm_kernel, "gj_perm_kernel");
@@ -560,6 +575,7 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     hipStream_t s = M->stream;
     const int N = P.N;
     check_tree_kernels();
+    clear_graphs(M);  // captured sweeps read the operator arenas reallocated below
     M->solver_type = c.solver_type;
     setup_leaf_classes(M, c);
     // arenas and work
@@ -626,8 +642,67 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
 }
 
 // ----- sweeps ----------------------------------------------------------------
+// The upwards and the Dirichlet solve sweeps are captured once per (k, device pointers) into
+// a HIP graph and replayed with one hipGraphLaunch (EF_GRAPH=0 launches them one by one).
+// Captured work is kernels only: every allocation, descriptor upload and host check happens
+// before capture; graphs are dropped whenever the arenas or descriptor tables they point
+// into are reallocated.
+constexpr size_t MAX_GRAPHS = 16;
+
+bool graphs_on() {  // read on every sweep, so a process can compare both ways
+    const char* e = std::getenv("EF_GRAPH");
+    return !(e && std::strcmp(e, "0") == 0) && !sync_check();
+}
+
+void clear_graphs(ef_mesh_s* M) {
+    for (auto& g : M->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    M->graphs.clear();
+}
+
+template <class Body>
+void run_sweep(ef_mesh_s* M, int kind, int k, const std::array<const void*, 4>& ptr, int flags, Body&& body) {
+    if (!graphs_on()) {
+        body();
+        return;
+    }
+    hipStream_t s = M->stream;
+    for (auto& g : M->graphs)
+        if (g.kind == kind && g.k == k && g.flags == flags && g.ptr == ptr) {
+            g.used = ++M->graph_clock;
+            EF_HIP(hipGraphLaunch(g.exec, s));
+            return;
+        }
+    EF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        body();
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        throw;
+    }
+    EF_HIP(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    if (M->graphs.size() >= MAX_GRAPHS) {
+        auto lru = std::min_element(M->graphs.begin(), M->graphs.end(),
+                                    [](const auto& a, const auto& b) { return a.used < b.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        M->graphs.erase(lru);
+    }
+    ef_mesh_s::SweepGraph g;
+    g.kind = kind; g.k = k; g.flags = flags; g.ptr = ptr; g.exec = exec; g.used = ++M->graph_clock;
+    M->graphs.push_back(g);
+    EF_HIP(hipGraphLaunch(exec, s));
+}
+
 void ensure_vectors(ef_mesh_s* M, int k) {
     if (k <= M->kcap) return;
+    clear_graphs(M);
     M->vec.alloc((size_t)M->P.vec_rows * k);
     EF_HIP(hipMemsetAsync(M->vec.p, 0, (size_t)M->P.vec_rows * k * sizeof(double), M->stream));
     M->kcap = k;
@@ -642,11 +717,19 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
     auto sp = std::make_unique<SweepPlan>();
     double* V = M->vec.p;
     const long long vrows = P.vec_rows;
+    if (vrows > 0x7fffffffLL) throw EfError(EF_ERR_ARG, "vector arena too large");
     auto vcheck = [&](long long off, long long rows, const char* what) {
         if (off < 0 || off + rows > vrows) throw EfError(EF_ERR_INTERNAL, std::string("vector arena range: ") + what);
     };
     const int D = P.maxdepth + 1;
     const int tn = k <= 16 ? 1 : 2;
+    // split-K for the long products of the upper levels (few merges, large operators): chunks
+    // of 128 once the concatenated k range reaches 512; the choice depends on the operator only,
+    // never on k, so a column's result is the same in any batch
+    auto split_kc = [](int K0, int K1) {
+        if (K0 + K1 < 512 || K0 % 128 != 0 || (K1 > 0 && K1 % 128 != 0)) return 0;
+        return 128;
+    };
     sp->lv.resize(D);
     for (int dd = 0; dd < D; dd++) {
         LevelSweep& L = sp->lv[dd];
@@ -674,9 +757,10 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
             // w = X^-1 hDiff   (mergeW_ with the stored inverse instead of a re-factorisation)
             TreeGemm p{};
             p.A0 = Xi; p.lda0 = 4 * n; p.K0 = 4 * n; p.alpha0 = 1.0;
-            p.B0 = V + nd.hd_off * k; p.ldb0 = k;
-            p.D = V + nd.w_off * k; p.ldd = k;
+            p.B0 = V + nd.hd_off; p.ldb0 = 1; p.bcs = (int)vrows;
+            p.D = V + nd.w_off; p.ldd = 1; p.dcs = (int)vrows;
             p.M = 4 * n; p.N = k;
+            p.kc = split_kc(p.K0, 0);
             pw.push_back(p);
             // h(q) += T_c(side q, interface sides) w(interfaces of c)   (mergeH_, the H block of child c)
             for (int q = 0; q < 8; q++) {
@@ -685,12 +769,14 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 const int r0 = child_if(c, 0), r1 = child_if(c, 1);
                 TreeGemm h{};
                 h.A0 = Tc + (long long)sq * n * 4 * n + (long long)side_on_if(c, r0) * n; h.lda0 = 4 * n; h.K0 = n;
-                h.B0 = V + (nd.w_off + (long long)r0 * n) * k; h.ldb0 = k; h.alpha0 = 1.0;
+                h.B0 = V + nd.w_off + (long long)r0 * n; h.ldb0 = 1; h.alpha0 = 1.0;
                 h.A1 = Tc + (long long)sq * n * 4 * n + (long long)side_on_if(c, r1) * n; h.lda1 = 4 * n; h.K1 = n;
-                h.B1 = V + (nd.w_off + (long long)r1 * n) * k; h.ldb1 = k; h.alpha1 = 1.0;
-                h.D = V + (nd.h_off + (long long)q * n) * k; h.ldd = k;
-                h.C = h.D; h.ldc = k; h.beta = 1.0;
+                h.B1 = V + nd.w_off + (long long)r1 * n; h.ldb1 = 1; h.alpha1 = 1.0;
+                h.bcs = (int)vrows;
+                h.D = V + nd.h_off + (long long)q * n; h.ldd = 1; h.dcs = (int)vrows;
+                h.C = h.D; h.ldc = 1; h.ccs = (int)vrows; h.beta = 1.0;
                 h.M = n; h.N = k;
+                h.kc = split_kc(h.K0, h.K1);
                 ph.push_back(h);
             }
             // downward: u_int = S g + w   (applyS_, HPSAlgorithm.hpp:1194-1230)
@@ -706,10 +792,11 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
             dn.push_back(dsc);
             TreeGemm pd{};
             pd.A0 = dsc.S; pd.lda0 = 8 * n; pd.K0 = 8 * n; pd.alpha0 = 1.0;
-            pd.B0 = V + nd.g_off * k; pd.ldb0 = k;
-            pd.D = V + nd.ui_off * k; pd.ldd = k;
-            if (!M->homogeneous_rhs) { pd.C = V + nd.w_off * k; pd.ldc = k; pd.beta = 1.0; }
+            pd.B0 = V + nd.g_off; pd.ldb0 = 1; pd.bcs = (int)vrows;
+            pd.D = V + nd.ui_off; pd.ldd = 1; pd.dcs = (int)vrows;
+            if (!M->homogeneous_rhs) { pd.C = V + nd.w_off; pd.ldc = 1; pd.ccs = (int)vrows; pd.beta = 1.0; }
             pd.M = 4 * n; pd.N = k;
+            pd.kc = split_kc(pd.K0, 0);
             pu.push_back(pd);
             // uncoarsen_ steps: coarse gc -> ... -> g, alternating through gt
             for (int stp = 1; stp <= nd.tag; stp++) {
@@ -751,9 +838,10 @@ void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u,
     const int N = M->P.N;
     const int nl = (int)M->P.leaves.size();
     if (M->variable) {
-        if (M->packed && mode != 1) M->vz.ensure((size_t)nl * k * N * N);
+        if (M->packed && mode != 1 && M->vz.n < (size_t)nl * k * N * N)
+            throw EfError(EF_ERR_INTERNAL, "leaf work buffer not sized before the sweep");
         launch_bt_solve(M->leafdesc.p, N, nl, k, mode, f, f_zero, (const double*)M->vec.p, M->vec.p, u, M->stream,
-                        M->packed, M->vz.p);
+                        M->packed, M->vz.p, M->P.vec_rows);
         EF_LAUNCH_CHECK();
     } else {
         // mode 0: particular Neumann data (g = 0) into the h rows of every leaf;
@@ -763,7 +851,7 @@ void launch_leaf(ef_mesh_s* M, int mode, const double* f, int f_zero, double* u,
         a.nrhs = k;
         a.cls = M->leaf_cls.p;
         a.f = f_zero ? nullptr : f;
-        a.vk = k;
+        a.vk = (int)M->P.vec_rows;  // arena column stride
         if (mode == 0) {
             a.g = nullptr;
             a.out = M->vec.p;
@@ -786,18 +874,21 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     ensure_vectors(M, k);
     SweepPlan& sp = sweep_plan(M, k);
     check_range(f_dev, (size_t)k * P.leaves.size() * P.N * P.N * 8, "upwards load f");
-    // leaf particular Neumann data (particularNeumannData, FiniteVolumeSolver.cpp:457-462)
-    launch_leaf(M, 0, f_dev, 0, nullptr, k);
-    if (!M->homogeneous_rhs) {  // upwards4to1 is skipped entirely for homogeneous-rhs (:532)
-        for (int dd = P.maxdepth; dd >= 0; dd--) {
-            const LevelSweep& L = sp.lv[dd];
-            if (L.cnt == 0) continue;
-            EF_LAUNCH("tree_up_gather_kernel", tree_up_gather_kernel, dim3(blocks_for(12LL * L.nmax * k), L.cnt),
-                      dim3(256), s, L.up, M->vec.p, k);
-            run_gemm(L.gw, s);
-            run_gemm(L.gh, s);
+    if (M->variable && M->packed) M->vz.ensure((size_t)P.leaves.size() * k * P.N * P.N);
+    run_sweep(M, 0, k, {f_dev, nullptr, nullptr, nullptr}, M->homogeneous_rhs ? 1 : 0, [&] {
+        // leaf particular Neumann data (particularNeumannData, FiniteVolumeSolver.cpp:457-462)
+        launch_leaf(M, 0, f_dev, 0, nullptr, k);
+        if (!M->homogeneous_rhs) {  // upwards4to1 is skipped entirely for homogeneous-rhs (:532)
+            for (int dd = P.maxdepth; dd >= 0; dd--) {
+                const LevelSweep& L = sp.lv[dd];
+                if (L.cnt == 0) continue;
+                EF_LAUNCH("tree_up_gather_kernel", tree_up_gather_kernel, dim3(blocks_for(12LL * L.nmax * k), L.cnt),
+                          dim3(256), s, L.up, M->vec.p, k, P.vec_rows);
+                run_gemm(L.gw, s);
+                run_gemm(L.gh, s);
+            }
         }
-    }
+    });
     M->k_up = k;
     M->f_last = f_dev;
 }
@@ -818,9 +909,30 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
     check_range(r_dev, (size_t)m * k * 8, "root data r");
     check_range(a_dev, (size_t)m * 8, "root data a");
     check_range(u_dev, (size_t)k * P.leaves.size() * P.N * P.N * 8, "solution u");
+    // downward sweep (solve4to1, HPSAlgorithm.hpp:562-577) and leaf solves (leafSolve, :584-596);
+    // homogeneous-rhs solves with f = 0
+    const bool f_zero = M->homogeneous_rhs || !f_dev;
+    auto downward = [&] {
+        for (int dd = 0; dd <= P.maxdepth; dd++) {
+            const LevelSweep& L = sp.lv[dd];
+            if (L.cnt == 0) continue;
+            for (size_t stp = 0; stp < L.unc.size(); stp++)
+                EF_LAUNCH("tree_uncoarsen_kernel", tree_uncoarsen_kernel,
+                          dim3(blocks_for(4LL * L.unc_nf[stp] * k), L.unc_cnt[stp]), dim3(256), s, L.unc[stp], V, k,
+                          P.vec_rows);
+            run_gemm(L.gu, s);
+            EF_LAUNCH("tree_down_scatter_kernel", tree_down_scatter_kernel, dim3(blocks_for(16LL * L.nmax * k), L.cnt),
+                      dim3(256), s, L.down, V, k, P.vec_rows);
+        }
+        launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, f_zero ? 1 : 0, u_dev, k);
+    };
     if (!robin) {
-        EF_LAUNCH("tree_root_dirichlet_kernel", tree_root_dirichlet_kernel, dim3(blocks_for((long long)m * k)), dim3(256),
-                  s, r_dev, a_dev, V + root.g_off * k, m, k);
+        run_sweep(M, 1, k, {f_dev, r_dev, a_dev, u_dev}, (M->homogeneous_rhs ? 1 : 0) | (f_zero ? 2 : 0), [&] {
+            EF_LAUNCH("tree_root_dirichlet_kernel", tree_root_dirichlet_kernel, dim3(blocks_for((long long)m * k)),
+                      dim3(256), s, r_dev, a_dev, V + root.g_off, m, k, P.vec_rows);
+            downward();
+        });
+        return;
     } else {
         // g = (diag(a) + diag(b) T)^-1 (r - b h)
         check_range(b_dev, (size_t)m * 8, "root data b");
@@ -832,9 +944,9 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         M->rootgji.ensure(gj_work_ints(m, 1));
         M->rootinfo.ensure(1);
         const double* Troot = full_T_ptr(M, 0);
-        const double* hroot = (root.leaf || M->homogeneous_rhs) ? nullptr : V + root.h_off * k;
+        const double* hroot = (root.leaf || M->homogeneous_rhs) ? nullptr : V + root.h_off;
         EF_LAUNCH("tree_root_robin_kernel", tree_root_robin_kernel, dim3(blocks_for((long long)m * m + (long long)m * k)),
-                  dim3(256), s, Troot, a_dev, b_dev, r_dev, hroot, M->rootA.p, M->rootrhs.p, m, k);
+                  dim3(256), s, Troot, a_dev, b_dev, r_dev, hroot, M->rootA.p, M->rootrhs.p, m, k, P.vec_rows);
         GJWork w;
         double* d = M->rootgj.p;
         w.wb = d; d += (size_t)m * m;
@@ -850,8 +962,8 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         run_gj_inverse(M->rootA.p, M->rootAi.p, m, 1, w, M->rootinfo.p, s);
         TreeGemm p{};
         p.A0 = M->rootAi.p; p.lda0 = m; p.K0 = m; p.alpha0 = 1.0;
-        p.B0 = M->rootrhs.p; p.ldb0 = k;
-        p.D = V + root.g_off * k; p.ldd = k;
+        p.B0 = M->rootrhs.p; p.ldb0 = 1; p.bcs = m;
+        p.D = V + root.g_off; p.ldd = 1; p.dcs = (int)P.vec_rows;
         p.M = m; p.N = k;
         DevStore tmp;
         GemmLaunch g = plan_gemm(std::vector<TreeGemm>{p}, k <= 16 ? 1 : 2, tmp, s);
@@ -860,18 +972,7 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         efk::copy_d2h(&info, M->rootinfo.p, sizeof(int), s);
         if (info) throw EfError(EF_ERR_SINGULAR, "zero pivot in the root boundary system");
     }
-    for (int dd = 0; dd <= P.maxdepth; dd++) {
-        const LevelSweep& L = sp.lv[dd];
-        if (L.cnt == 0) continue;
-        for (size_t stp = 0; stp < L.unc.size(); stp++)
-            EF_LAUNCH("tree_uncoarsen_kernel", tree_uncoarsen_kernel, dim3(blocks_for(4LL * L.unc_nf[stp] * k), L.unc_cnt[stp]),
-                      dim3(256), s, L.unc[stp], V, k);
-        run_gemm(L.gu, s);
-        EF_LAUNCH("tree_down_scatter_kernel", tree_down_scatter_kernel, dim3(blocks_for(16LL * L.nmax * k), L.cnt),
-                  dim3(256), s, L.down, V, k);
-    }
-    // leaf solves (leafSolve, HPSAlgorithm.hpp:584-596); homogeneous-rhs solves with f = 0
-    launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, (M->homogeneous_rhs || !f_dev) ? 1 : 0, u_dev, k);
+    downward();
 }
 
 }  // namespace
@@ -961,6 +1062,7 @@ int ef_mesh_destroy(ef_mesh mesh) {
     if (!mesh) return EF_OK;
     (void)hipSetDevice(mesh->ctx->device);
     if (mesh->stream) (void)hipStreamSynchronize(mesh->stream);
+    clear_graphs(mesh);
     for (auto& pr : mesh->ev)
         for (auto& e : pr)
             if (e) (void)hipEventDestroy(e);
@@ -1203,16 +1305,12 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
     };
     auto need_built = [&]() { if (!M->built) throw EfError(EF_ERR_STATE, "operators exist after the build stage"); };
     auto need_internal = [&]() { if (nd.leaf) throw EfError(EF_ERR_ARG, "leaf nodes carry no S, X, H or w"); };
-    const int k = std::max(M->kcap, 1);
     auto copy_vec = [&](long long off, int len) {
         *rows = len; *cols = 1;
         if (!out) return;
         if (M->kcap == 0) throw EfError(EF_ERR_STATE, "no vectors yet");
         if (rhs < 0 || rhs >= M->kcap) throw EfError(EF_ERR_ARG, "rhs out of range");
-        std::vector<double> tmp((size_t)len * k);
-        efk::copy_d2h(tmp.data(), M->vec.p + off * k, sizeof(double) * len * k, M->stream);
-        EF_HIP(hipStreamSynchronize(M->stream));
-        for (int i = 0; i < len; i++) out[i] = tmp[(size_t)i * k + rhs];
+        efk::copy_d2h(out, M->vec.p + (long long)rhs * P.vec_rows + off, sizeof(double) * len, M->stream);
     };
     switch (which) {
         case 0: {  // T (coarsened if the node was tagged, as coarsen_ leaves it)

@@ -11,8 +11,9 @@
 //      Xi   4n x 4n   inverse of the interface Schur complement X (mergeX_, :870)
 //      H    4 blocks of 2n x 2n, child c's tau-side rows x its two interface columns
 //      T    8n x 8n   DtN of tau in WESN order
-//  * multi-RHS vectors (h, g, w, u_int) live in per-batch arenas: node vector of
-//    length L starts at off*k and element (i, rhs) is at off*k + i*k + rhs;
+//  * multi-RHS vectors (h, g, w, u_int) live in per-batch arenas of R rows and k columns,
+//    right-hand side major: element (row, rhs) is at rhs * R + row, so one right-hand side's
+//    node vector is contiguous;
 //  * leaf f / u for k right-hand sides: [rhs][leaf][cell].
 #pragma once
 #include <cstddef>
@@ -103,7 +104,7 @@ struct PatchFDArgs {
     const double* g;        // boundary data, or null (zero); see g_off
     double* out;            // u [nrhs][npatches][N*N] | h (see h_off) | T [npatches][4N][4N]
     // Vector addressing.  Null offsets: g and h are [nrhs][npatches][4N].  Otherwise the
-    // multi-RHS arena layout: element t of patch p, rhs c at (off[p] + t) * vk + c.
+    // multi-RHS arena layout: element t of patch p, rhs c at c * vk + off[p] + t.
     const long long* g_off;
     const long long* h_off;
     int vk;

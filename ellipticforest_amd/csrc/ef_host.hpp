@@ -128,6 +128,14 @@ inline void check_matrix(const T* p, long long rows, long long cols, long long l
     check_range(p, (size_t)(((rows - 1) * ld + cols) * (long long)sizeof(T)), what);
 }
 
+// Element (i, j) of a rows x cols array at p[i * rs + j * cs] (elements of T).
+template <class T>
+inline void check_strided(const T* p, long long rows, long long cols, long long rs, long long cs, const char* what) {
+    if (rows <= 0 || cols <= 0) return;
+    if (rs < 0 || cs < 0) throw EfError(EF_ERR_INTERNAL, std::string("negative stride: ") + what);
+    check_range(p, (size_t)(((rows - 1) * rs + (cols - 1) * cs + 1) * (long long)sizeof(T)), what);
+}
+
 // Owning device buffer.
 template <class T>
 struct DevBuf {
@@ -169,6 +177,13 @@ struct DevStore {
         b->alloc(v.size() * sizeof(T));
         copy_h2d(b->p, v.data(), v.size() * sizeof(T), s);
         T* out = (T*)b->p;
+        bufs.push_back(std::move(b));
+        return out;
+    }
+    double* alloc_doubles(size_t n) {
+        auto b = std::make_unique<DevBuf<char>>();
+        b->alloc(n * sizeof(double));
+        double* out = (double*)b->p;
         bufs.push_back(std::move(b));
         return out;
     }

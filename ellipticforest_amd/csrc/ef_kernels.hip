@@ -347,7 +347,7 @@ __device__ inline void bt_substitute(const LeafDesc& L, int Nn, double* b, doubl
 // Neumann data (g = 0, h out), 1 solve with g (u out), 2 mapD2N with g (h out).
 __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode,
                                                        const double* f, int f_zero, const double* garena,
-                                                       double* harena, double* u) {
+                                                       double* harena, double* u, long long vcs) {
     __shared__ double b[BT_MAXN * BT_MAXN];
     __shared__ double tmp[BT_MAXN];
     const int leaf = blockIdx.x, rhs = blockIdx.y;
@@ -355,16 +355,16 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
     const int tid = threadIdx.x;
     const int NN = Nn * Nn;
     const double* fl = f_zero ? nullptr : f + ((long long)rhs * nleaves + leaf) * NN;
-    const double* g = (mode != 0) ? garena + L.g_off * k + rhs : nullptr;
+    const double* g = (mode != 0) ? garena + (long long)rhs * vcs + L.g_off : nullptr;
     for (int e = tid; e < NN; e += 256) {
         const int i = e / Nn, j = e % Nn;
         double v = fl ? fl[e] : 0.0;
         if (g && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
             double add = 0.0;
-            if (i == 0) add += L.stencil[e] * g[(long long)(0 * Nn + j) * k];
-            if (i == Nn - 1) add += L.stencil[NN + e] * g[(long long)(1 * Nn + j) * k];
-            if (j == 0) add += L.stencil[2 * NN + e] * g[(long long)(2 * Nn + i) * k];
-            if (j == Nn - 1) add += L.stencil[3 * NN + e] * g[(long long)(3 * Nn + i) * k];
+            if (i == 0) add += L.stencil[e] * g[0 * Nn + j];
+            if (i == Nn - 1) add += L.stencil[NN + e] * g[1 * Nn + j];
+            if (j == 0) add += L.stencil[2 * NN + e] * g[2 * Nn + i];
+            if (j == Nn - 1) add += L.stencil[3 * NN + e] * g[3 * Nn + i];
             v += -2.0 * add;
         }
         b[e] = v;
@@ -375,14 +375,14 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
         double* ul = u + ((long long)rhs * nleaves + leaf) * NN;
         for (int e = tid; e < NN; e += 256) ul[e] = b[e];
     } else {
-        double* h = harena + L.h_off * k + rhs;
+        double* h = harena + (long long)rhs * vcs + L.h_off;
         for (int t = tid; t < 4 * Nn; t += 256) {
             int i, j;
             bcell(t, Nn, i, j);
             const int sd = t / Nn;
             const double dd = (sd < 2) ? L.dx : L.dy;
             const double sg = (sd == W || sd == S) ? 2.0 / dd : -2.0 / dd;
-            h[(long long)t * k] = sg * (b[i * Nn + j] - (g ? g[(long long)t * k] : 0.0));
+            h[t] = sg * (b[i * Nn + j] - (g ? g[t] : 0.0));
         }
     }
 }
@@ -416,7 +416,7 @@ __device__ inline void bt_wave_load(const double* Si, int j, int h, double (&Sro
 template <int NT>
 __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leaves, int nleaves, int k, int mode,
                                                             const double* f, int f_zero, const double* garena,
-                                                            double* harena, double* u) {
+                                                            double* harena, double* u, long long vcs) {
     constexpr int NN = NT * NT, C = NT * NT / 64;
     __shared__ double zs_all[4][NN];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
     const int j = lane % NT, h = lane / NT;
     const LeafDesc L = leaves[leaf];
     const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
-    const auto* g = (mode != 0) ? gmem(garena) + L.g_off * k + rhs : nullptr;
+    const auto* g = (mode != 0) ? gmem(garena) + (long long)rhs * vcs + L.g_off : nullptr;
     const auto* cWv = gmem(L.stencil);
     const auto* cEv = cWv + NN;
     const auto* cSv = cWv + 2 * NN;
@@ -437,10 +437,10 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
         double b = fl ? fl[I] : 0.0;
         if (g) {
             double add = 0.0;
-            if (i == 0) add += cWv[I] * g[(long long)(0 * NT + j) * k];
-            if (i == NT - 1) add += cEv[I] * g[(long long)(1 * NT + j) * k];
-            if (j == 0) add += cSv[I] * g[(long long)(2 * NT + i) * k];
-            if (j == NT - 1) add += cNv[I] * g[(long long)(3 * NT + i) * k];
+            if (i == 0) add += cWv[I] * g[0 * NT + j];
+            if (i == NT - 1) add += cEv[I] * g[1 * NT + j];
+            if (j == 0) add += cSv[I] * g[2 * NT + i];
+            if (j == NT - 1) add += cNv[I] * g[3 * NT + i];
             b += -2.0 * add;
         }
         return b;
@@ -505,9 +505,9 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
     }
     // Neumann data of the edge cells: h_W = (2/dx)(u_W - g_W), h_E = -(2/dx)(u_E - g_E),
     // h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N)
-    auto* hv = gmem(harena) + L.h_off * k + rhs;
+    auto* hv = gmem(harena) + (long long)rhs * vcs + L.h_off;
     const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
-    auto gv = [&](int t) { return g ? g[(long long)t * k] : 0.0; };
+    auto gv = [&](int t) { return g ? g[t] : 0.0; };
     for (int t = lane; t < 4 * NT; t += 64) {
         const int sd = t / NT, m = t % NT;
         double v;
@@ -515,16 +515,16 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
         else if (sd == E) v = -sx * (zs[(NT - 1) * NT + m] - gv(t));
         else if (sd == S) v = sy * (zs[m * NT] - gv(t));
         else v = -sy * (zs[m * NT + NT - 1] - gv(t));
-        hv[(long long)t * k] = v;
+        hv[t] = v;
     }
 }
 
 template __global__ void bt_solve_wave_kernel<8>(const LeafDesc*, int, int, int, const double*, int, const double*,
-                                                 double*, double*);
+                                                 double*, double*, long long);
 template __global__ void bt_solve_wave_kernel<16>(const LeafDesc*, int, int, int, const double*, int, const double*,
-                                                  double*, double*);
+                                                  double*, double*, long long);
 template __global__ void bt_solve_wave_kernel<32>(const LeafDesc*, int, int, int, const double*, int, const double*,
-                                                  double*, double*);
+                                                  double*, double*, long long);
 
 // Leaf DtN with variable coefficients on the matrix cores, N = NT in {16, 32}: the same
 // substitution as bt_dtn_kernel for all 4N boundary columns at once, as block products
@@ -767,7 +767,7 @@ __device__ inline double wave_half_sum(double x) {
 template <int NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f, int f_zero,
-                    const double* garena, double* harena, double* zbuf) {
+                    const double* garena, double* harena, double* zbuf, long long vcs) {
     using PK = BtPack<NT>;
     constexpr int NN = NT * NT, G = 64 / NT, C = NT / G, R = PK::R, STEPS = 2 * NT - 1;
     __shared__ double sp_all[4][2][PK::LSIZE];
@@ -780,7 +780,7 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
     const int j = lane % NT, h = lane / NT;
     const LeafDesc L = leaves[leaf];
     const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
-    const auto* g = (mode != 0) ? gmem(garena) + L.g_off * k + rhs : nullptr;
+    const auto* g = (mode != 0) ? gmem(garena) + (long long)rhs * vcs + L.g_off : nullptr;
     auto* zl = gmem(zbuf) + ((long long)rhs * nleaves + leaf) * NN;
     const auto* cWv = gmem(L.stencil);
     const auto* cEv = cWv + NN;
@@ -792,10 +792,10 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
         double b = fl ? fl[I] : 0.0;
         if (g) {
             double add = 0.0;
-            if (i == 0) add += cWv[I] * g[(long long)(0 * NT + j) * k];
-            if (i == NT - 1) add += cEv[I] * g[(long long)(1 * NT + j) * k];
-            if (j == 0) add += cSv[I] * g[(long long)(2 * NT + i) * k];
-            if (j == NT - 1) add += cNv[I] * g[(long long)(3 * NT + i) * k];
+            if (i == 0) add += cWv[I] * g[0 * NT + j];
+            if (i == NT - 1) add += cEv[I] * g[1 * NT + j];
+            if (j == 0) add += cSv[I] * g[2 * NT + i];
+            if (j == NT - 1) add += cNv[I] * g[3 * NT + i];
             b += -2.0 * add;
         }
         return b;
@@ -894,9 +894,9 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    auto* hv = gmem(harena) + L.h_off * k + rhs;
+    auto* hv = gmem(harena) + (long long)rhs * vcs + L.h_off;
     const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
-    auto gv = [&](int t) { return g ? g[(long long)t * k] : 0.0; };
+    auto gv = [&](int t) { return g ? g[t] : 0.0; };
     for (int t = lane; t < 4 * NT; t += 64) {
         const int sd = t / NT, mm = t % NT;
         double v;
@@ -904,43 +904,44 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
         else if (sd == E) v = -sx * (zl[(NT - 1) * NT + mm] - gv(t));
         else if (sd == S) v = sy * (zl[mm * NT] - gv(t));
         else v = -sy * (zl[mm * NT + NT - 1] - gv(t));
-        hv[(long long)t * k] = v;
+        hv[t] = v;
     }
 }
 
 template __global__ void bt_solve_sym_kernel<8>(const LeafDesc*, int, int, int, const double*, int, const double*,
-                                                double*, double*);
+                                                double*, double*, long long);
 template __global__ void bt_solve_sym_kernel<16>(const LeafDesc*, int, int, int, const double*, int, const double*,
-                                                 double*, double*);
+                                                 double*, double*, long long);
 template __global__ void bt_solve_sym_kernel<32>(const LeafDesc*, int, int, int, const double*, int, const double*,
-                                                 double*, double*);
+                                                 double*, double*, long long);
 
 void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
-                     const double* garena, double* harena, double* u, hipStream_t s, bool packed, double* zwork) {
+                     const double* garena, double* harena, double* u, hipStream_t s, bool packed, double* zwork,
+                     long long vcs) {
     const unsigned blocks = (unsigned)(((long long)nleaves * k + 3) / 4);
     // the full blocks (Sinv) are always kept: sides without a packed kernel use them
     double* zbuf = (mode == 1) ? u : zwork;
     if (packed && Nn == 8)
         hipLaunchKernelGGL(bt_solve_sym_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
-                           garena, harena, zbuf);
+                           garena, harena, zbuf, vcs);
     else if (packed && Nn == 16)
         hipLaunchKernelGGL(bt_solve_sym_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
-                           f_zero, garena, harena, zbuf);
+                           f_zero, garena, harena, zbuf, vcs);
     else if (packed && Nn == 32)
         hipLaunchKernelGGL(bt_solve_sym_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
-                           f_zero, garena, harena, zbuf);
+                           f_zero, garena, harena, zbuf, vcs);
     else if (Nn == 8)
         hipLaunchKernelGGL(bt_solve_wave_kernel<8>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f, f_zero,
-                           garena, harena, u);
+                           garena, harena, u, vcs);
     else if (Nn == 16)
         hipLaunchKernelGGL(bt_solve_wave_kernel<16>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
-                           f_zero, garena, harena, u);
+                           f_zero, garena, harena, u, vcs);
     else if (Nn == 32)
         hipLaunchKernelGGL(bt_solve_wave_kernel<32>, dim3(blocks), dim3(256), 0, s, leaves, nleaves, k, mode, f,
-                           f_zero, garena, harena, u);
+                           f_zero, garena, harena, u, vcs);
     else
         hipLaunchKernelGGL(bt_solve_kernel, dim3(nleaves, k), dim3(256), 0, s, leaves, Nn, nleaves, k, mode, f, f_zero,
-                           garena, harena, u);
+                           garena, harena, u, vcs);
 }
 
 // Leaf DtN (buildD2N, FiniteVolumeSolver.cpp:355-455) with variable coefficients: column b is

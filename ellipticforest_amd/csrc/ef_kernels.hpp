@@ -14,7 +14,7 @@ __global__ void bt_factor_wave_kernel(const LeafDesc* leaves, int nleaves);
 // pivot search.
 void launch_bt_factor(const LeafDesc* leaves, int Nn, int nleaves, hipStream_t s, bool dominant = false);
 __global__ void bt_solve_kernel(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f,
-                                int f_zero, const double* garena, double* harena, double* u);
+                                int f_zero, const double* garena, double* harena, double* u, long long vcs);
 __global__ void bt_dtn_kernel(const LeafDesc* leaves, int Nn, int first, double* work);
 template <int NT>
 __global__ void bt_dtn_mfma_kernel(const LeafDesc* leaves, int first, double* work);
@@ -23,19 +23,20 @@ __global__ void bt_dtn_mfma_kernel(const LeafDesc* leaves, int first, double* wo
 void launch_bt_dtn(const LeafDesc* leaves, int Nn, int nleaves, double* work, long long chunk, hipStream_t s);
 template <int NT>
 __global__ void bt_solve_wave_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f,
-                                     int f_zero, const double* garena, double* harena, double* u);
+                                     int f_zero, const double* garena, double* harena, double* u, long long vcs);
 template <int NT>
 __global__ void bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const double* f,
-                                    int f_zero, const double* garena, double* harena, double* zbuf);
+                                    int f_zero, const double* garena, double* harena, double* zbuf, long long vcs);
 __global__ void bt_pack_kernel(const LeafDesc* leaves, int nleaves, int Nn);
 // Leaf solves with variable coefficients: one wave per (leaf, rhs) for N in {8, 16, 32},
 // the workgroup-per-leaf kernel otherwise.  Modes as bt_solve_kernel.  With `packed`, every
 // leaf's SinvP holds its packed symmetric inverse pivot blocks (bt_pack_kernel), which the
 // kernel for N in {8, 16, 32} reads instead of Sinv; in modes 0 and 2 it needs zwork,
-// nleaves * k * N * N doubles of work space.
+// nleaves * k * N * N doubles of work space.  Boundary data and Neumann data of leaf L and
+// right-hand side c: garena / harena + c * vcs + L.g_off / L.h_off, 4N consecutive doubles.
 void launch_bt_solve(const LeafDesc* leaves, int Nn, int nleaves, int k, int mode, const double* f, int f_zero,
                      const double* garena, double* harena, double* u, hipStream_t s, bool packed = false,
-                     double* zwork = nullptr);
+                     double* zwork = nullptr, long long vcs = 0);
 // Packed storage of the symmetric inverse pivot blocks: entries per block.
 inline long long bt_packed_entries(int Nn) { return (long long)Nn * (Nn + 1) / 2; }
 constexpr int BT_MAX_N = 32;  // largest leaf side of the block-tridiagonal path (LDS tiles)

@@ -112,7 +112,7 @@ __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int
         }
         const double* gb;
         long long gs;
-        if (ix.g_off) { gb = p.g + ix.g_off[patch] * p.vk + rhs; gs = p.vk; }
+        if (ix.g_off) { gb = p.g + (long long)rhs * p.vk + ix.g_off[patch]; gs = 1; }
         else { gb = p.g + slot * 4 * Nn; gs = 1; }
 #pragma unroll
         for (int m = 0; m < NB; m++) in.G[m] = gmem(gb)[(long long)(64 * m + lane) * gs];  // 4N = 64 NB entries
@@ -142,7 +142,7 @@ __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int
     const double* g = nullptr;
     long long gs = 1;
     if (p.g) {
-        if (ix.g_off) { g = p.g + ix.g_off[patch] * p.vk + rhs; gs = p.vk; }
+        if (ix.g_off) g = p.g + (long long)rhs * p.vk + ix.g_off[patch];
         else g = p.g + slot * 4 * Nn;
     }
 #pragma unroll
@@ -383,8 +383,8 @@ patch_fd_kernel(PatchFDArgs p, const int* __restrict__ cls, const PatchCoef* __r
                     h = p.out + (long long)patch * 16 * Nn * Nn + col;
                     ostride = 4 * Nn;
                 } else if (ix.h_off) {
-                    h = p.out + ix.h_off[patch] * p.vk + rhs;
-                    ostride = p.vk;
+                    h = p.out + (long long)rhs * p.vk + ix.h_off[patch];
+                    ostride = 1;
                 } else {
                     h = p.out + ((long long)rhs * p.npatches + patch) * 4 * Nn;
                     ostride = 1;

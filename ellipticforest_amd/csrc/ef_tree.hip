@@ -33,6 +33,13 @@ __device__ inline td4 mfma_f64(double a, double b, td4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// lane `src`'s value of x in every lane (src uniform)
+__device__ inline double readlane_f64(double x, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+
 template <class T>
 __device__ inline T pick4(const T (&a)[4], int c) {
     return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
@@ -46,30 +53,63 @@ __device__ inline T pick4(const T (&a)[4], int c) {
 // Within each block of 16 k values lane group q takes k = kb + 4q + s at MFMA step s: A is
 // read as two 16-byte vectors per lane and row (a 128-byte row segment per lane group), B as
 // one row segment of 16 consecutive columns per lane group and step.
-template <int TN>
+template <int RT, int TN>
 __device__ inline void gemm_segment(const double* __restrict__ a0p, const double* __restrict__ a1p,
                                     const double* const (&bp)[TN], long long ldb, int K, double alpha,
                                     td4 (&acc)[2][TN]) {
     const int q = (int)(threadIdx.x & 63) >> 4;
-    int kb = 0;
-    for (; kb + 16 <= K; kb += 16) {
-        const int k = kb + 4 * q;
-        const td2 x0 = *(const td2*)(a0p + k), x1 = *(const td2*)(a0p + k + 2);
-        const td2 y0 = *(const td2*)(a1p + k), y1 = *(const td2*)(a1p + k + 2);
-        const double a0[4] = {alpha * x0[0], alpha * x0[1], alpha * x1[0], alpha * x1[1]};
-        const double a1[4] = {alpha * y0[0], alpha * y0[1], alpha * y1[0], alpha * y1[1]};
+    // the operands of block kb: A rows (two 16-byte vectors per row) and B columns
+    struct Blk {
+        td2 x0, x1, y0, y1;
         double b[TN][4];
+    };
+    auto load = [&](int kb, Blk& o) {
+        const int k = kb + 4 * q;
+        o.x0 = *(const td2*)(a0p + k);
+        o.x1 = *(const td2*)(a0p + k + 2);
+        if (RT == 2) {
+            o.y0 = *(const td2*)(a1p + k);
+            o.y1 = *(const td2*)(a1p + k + 2);
+        }
+        if (ldb == 1) {  // B contiguous along k: two 16-byte vectors per column
 #pragma unroll
-        for (int j = 0; j < TN; j++)
+            for (int j = 0; j < TN; j++) {
+                const td2 u0 = *(const td2*)(bp[j] + k), u1 = *(const td2*)(bp[j] + k + 2);
+                o.b[j][0] = u0[0]; o.b[j][1] = u0[1]; o.b[j][2] = u1[0]; o.b[j][3] = u1[1];
+            }
+        } else {
 #pragma unroll
-            for (int s = 0; s < 4; s++) b[j][s] = bp[j][(long long)(k + s) * ldb];
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) o.b[j][s] = bp[j][(long long)(k + s) * ldb];
+        }
+    };
+    auto mma = [&](const Blk& o) {
+        const double a0[4] = {alpha * o.x0[0], alpha * o.x0[1], alpha * o.x1[0], alpha * o.x1[1]};
+        double a1[4] = {0.0, 0.0, 0.0, 0.0};
+        if (RT == 2) {
+            a1[0] = alpha * o.y0[0]; a1[1] = alpha * o.y0[1]; a1[2] = alpha * o.y1[0]; a1[3] = alpha * o.y1[1];
+        }
 #pragma unroll
         for (int s = 0; s < 4; s++)
 #pragma unroll
             for (int j = 0; j < TN; j++) {
-                acc[0][j] = mfma_f64(a0[s], b[j][s], acc[0][j]);
-                acc[1][j] = mfma_f64(a1[s], b[j][s], acc[1][j]);
+                acc[0][j] = mfma_f64(a0[s], o.b[j][s], acc[0][j]);
+                if (RT == 2) acc[1][j] = mfma_f64(a1[s], o.b[j][s], acc[1][j]);
             }
+    };
+    const int nfull = K / 16;
+    int kb = 0;
+    if (nfull > 0) {
+        // software pipeline: the next block's loads are in flight while this block's MFMAs run
+        Blk cur, nxt;
+        load(0, cur);
+        for (int i = 0; i < nfull; i++) {
+            if (i + 1 < nfull) load(16 * (i + 1), nxt);
+            mma(cur);
+            cur = nxt;
+        }
+        kb = 16 * nfull;
     }
     if (kb < K) {  // partial last block: k >= K contributes zero (addresses clamped to K - 1)
         const int k = kb + 4 * q;
@@ -78,9 +118,12 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         for (int s = 0; s < 4; s++) {
             const bool ok = k + s < K;
             const int kk = ok ? k + s : K - 1;
-            const double x = a0p[kk], y = a1p[kk];
+            const double x = a0p[kk];
             a0[s] = ok ? alpha * x : 0.0;
-            a1[s] = ok ? alpha * y : 0.0;
+            if (RT == 2) {
+                const double y = a1p[kk];
+                a1[s] = ok ? alpha * y : 0.0;
+            }
 #pragma unroll
             for (int j = 0; j < TN; j++) b[j][s] = bp[j][(long long)kk * ldb];
         }
@@ -89,12 +132,12 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
 #pragma unroll
             for (int j = 0; j < TN; j++) {
                 acc[0][j] = mfma_f64(a0[s], b[j][s], acc[0][j]);
-                acc[1][j] = mfma_f64(a1[s], b[j][s], acc[1][j]);
+                if (RT == 2) acc[1][j] = mfma_f64(a1[s], b[j][s], acc[1][j]);
             }
     }
 }
 
-template <int TN>
+template <int RT, int TN>
 __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restrict__ probs,
                                                          const int* __restrict__ start, int nprob) {
     const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -107,11 +150,15 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
     }
     const TreeGemm p = probs[lo];
     const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
-    constexpr int CW = 16 * TN;
-    const int tn = (p.N + CW - 1) / CW;
-    const int t = w - start[lo];
-    const int row0 = (t / tn) * 32, col0 = (t % tn) * CW;
-    const int ra = min(row0 + r, p.M - 1), rb = min(row0 + 16 + r, p.M - 1);
+    constexpr int RW = 16 * RT, CW = 16 * TN;
+    const int tm = (p.M + RW - 1) / RW, tn = (p.N + CW - 1) / CW;
+    int t = w - start[lo];
+    const int chunk = t / (tm * tn);
+    t -= chunk * tm * tn;
+    const int row0 = (t / tn) * RW, col0 = (t % tn) * CW;
+    int rows[RT];
+#pragma unroll
+    for (int i = 0; i < RT; i++) rows[i] = min(row0 + 16 * i + r, p.M - 1);
     int cj[TN];
 #pragma unroll
     for (int j = 0; j < TN; j++) cj[j] = min(col0 + 16 * j + r, p.N - 1);
@@ -120,69 +167,110 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < TN; j++) acc[i][j] = td4{0.0, 0.0, 0.0, 0.0};
-    if (p.K0 > 0) {
+    // k range of this wave: all of both segments, or one chunk of one segment
+    int kb0 = 0, ke0 = p.K0, kb1 = 0, ke1 = p.K1;
+    if (p.kc > 0) {
+        const int kbeg = chunk * p.kc, kend = kbeg + p.kc;
+        if (kbeg < p.K0) { kb0 = kbeg; ke0 = min(kend, p.K0); kb1 = ke1 = 0; }
+        else { kb0 = ke0 = 0; kb1 = kbeg - p.K0; ke1 = min(kend - p.K0, p.K1); }
+    }
+    if (ke0 > kb0) {
         const double* bp[TN];
 #pragma unroll
-        for (int j = 0; j < TN; j++) bp[j] = p.B0 + cj[j];
-        gemm_segment<TN>(p.A0 + (long long)ra * p.lda0, p.A0 + (long long)rb * p.lda0, bp, p.ldb0, p.K0, p.alpha0, acc);
+        for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj[j] * p.bcs;
+        gemm_segment<RT, TN>(p.A0 + (long long)rows[0] * p.lda0 + kb0, p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp,
+                         p.ldb0, ke0 - kb0, p.alpha0, acc);
     }
-    if (p.K1 > 0) {
+    if (ke1 > kb1) {
         const double* bp[TN];
 #pragma unroll
-        for (int j = 0; j < TN; j++) bp[j] = p.B1 + cj[j];
-        gemm_segment<TN>(p.A1 + (long long)ra * p.lda1, p.A1 + (long long)rb * p.lda1, bp, p.ldb1, p.K1, p.alpha1, acc);
+        for (int j = 0; j < TN; j++) bp[j] = p.B1 + (long long)kb1 * p.ldb1 + (long long)cj[j] * p.bcs;
+        gemm_segment<RT, TN>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
+                         p.ldb1, ke1 - kb1, p.alpha1, acc);
     }
+    // with RT = 1 both accumulator rows hold the same 16 rows; only the first is stored
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < RT; i++)
 #pragma unroll
         for (int j = 0; j < TN; j++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = row0 + 16 * i + 4 * v + q, col = col0 + 16 * j + r;
                 if (row < p.M && col < p.N) {
-                    double o = acc[i][j][v];
-                    if (p.C) o += p.beta * p.C[(long long)row * p.ldc + col];
-                    p.D[(long long)row * p.ldd + col] = o;
+                    if (p.kc > 0) {
+                        p.P[((long long)chunk * p.M + row) * p.N + col] = acc[i][j][v];
+                    } else {
+                        double o = acc[i][j][v];
+                        if (p.C) o += p.beta * p.C[(long long)row * p.ldc + (long long)col * p.ccs];
+                        p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
+                    }
                 }
             }
 }
 
-template __global__ void tree_gemm_kernel<1>(const TreeGemm*, const int*, int);
-template __global__ void tree_gemm_kernel<2>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 1>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<2, 1>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<2, 2>(const TreeGemm*, const int*, int);
+
+// D = sum over chunks of the partials (in chunk order) + beta C, one thread per element of the
+// split problems of a launch.
+__global__ void __launch_bounds__(256) tree_gemm_reduce_kernel(const TreeGemm* __restrict__ probs,
+                                                                const long long* __restrict__ estart, int nprob) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= estart[nprob]) return;
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (estart[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const TreeGemm p = probs[lo];
+    const long long i = e - estart[lo];
+    const int row = (int)(i / p.N), col = (int)(i % p.N);
+    const long long mn = (long long)p.M * p.N;
+    double o = 0.0;
+    for (int c = 0; c < p.nks; c++) o += p.P[c * mn + i];
+    if (p.C) o += p.beta * p.C[(long long)row * p.ldc + (long long)col * p.ccs];
+    p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
+}
 
 // ---------------------------------------------------------------------------------------
 // Gauss-Jordan inversion
 // ---------------------------------------------------------------------------------------
-// Tournament LU of the panel (columns k .. k+nb) with partial pivoting.  One workgroup of 1024
-// threads handles up to GJ_CH rows; each row is held by two adjacent lanes, 16 panel columns
-// each, in registers.  Stage 0: the rows of chunk `blockIdx.x` (positions k + ch GJ_CH + i);
-// stage 1: the candidates of every chunk (a.cand).  The nb rows pivoted on, in order, are the
-// chunk's candidates, or with one chunk / in stage 1 the pivots of the step (a.cfin).  Ties go
-// to the lower position, as idamax takes the first maximum.  Register indices are compile-time
-// constants (one instantiation of panel_column per column), so the rows never leave registers.
+// Tournament LU of the panel (columns k .. k+nb) with partial pivoting, fused with the step's
+// pivot bookkeeping.  One workgroup of 64 WAVES threads holds up to 32 WAVES panel rows, each row
+// in two adjacent lanes (16 columns each) in registers.  STAGE 0: the rows of chunk blockIdx.x
+// (positions k + ch * rows + i); STAGE 1: the candidates of every chunk (a.cand).  Ties go to
+// the lower position, as idamax takes the first maximum.  FIN = false: the nb rows pivoted on,
+// in order, are the chunk's candidates.  FIN = true (one chunk, or stage 1): they are the pivots
+// of the step, and the same workgroup forms P^-1 of the pivot block from the rows it published
+// (the multiplier of each eliminated entry is kept in place, so the pivot row of column J holds
+// row J of [L11 \ U11]), the LAPACK-order interchanges (ipiv) and the position -> row map (src).
+// Register indices are compile-time constants (one instantiation of panel_column per column).
 constexpr int GJ_HALF = GJ_NB / 2;
+constexpr int GJ_MAXWAVES = 16;  // 1024 threads: GJ_CH rows
 struct PanelRows {
-    double v[GJ_HALF];  // columns 16 h .. 16 h + 15 of the row
+    double v[GJ_HALF];  // columns 16 h .. 16 h + 15 of the row (multipliers left of the pivot column)
     int pos;            // position of the row in the matrix
     int h;              // which half of the row
     bool act;           // row present and not yet pivoted on
 };
-constexpr int GJ_LEAF_THREADS = 2 * GJ_CH;
-constexpr int GJ_LEAF_WAVES = GJ_LEAF_THREADS / 64;
 struct PanelShared {
-    double red_v[GJ_LEAF_WAVES];
-    int red_p[GJ_LEAF_WAVES];
+    double red_v[GJ_MAXWAVES];
+    int red_p[GJ_MAXWAVES];
     double prow[GJ_NB];
-    int piv;
+    int piv[GJ_NB];                 // pivot position chosen for each column (-1: none)
+    int top_row[GJ_NB], opos[GJ_NB], orow[GJ_NB];
+    int nout;
 };
 constexpr int GJ_NONE = 0x7fffffff;
 
-template <int J>
-__device__ __attribute__((always_inline)) inline void panel_column(PanelRows& R, PanelShared& sh, int nbb, int* out) {
+template <int J, int WAVES, bool FIN>
+__device__ __attribute__((always_inline)) inline void panel_column(PanelRows& R, PanelShared& sh, int nbb) {
     constexpr int HJ = J / GJ_HALF, JC = J % GJ_HALF;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (J >= nbb) {  // uniform
-        if (tid == 0) out[J] = -1;
+        if (tid == 0) sh.piv[J] = -1;
         return;
     }
     const bool own = R.h == HJ;  // this lane holds column J of its row
@@ -197,27 +285,29 @@ __device__ __attribute__((always_inline)) inline void panel_column(PanelRows& R,
     }
     if (lane == 0) { sh.red_v[wave] = bv; sh.red_p[wave] = bp; }
     __syncthreads();
-    if (tid == 0) {
-        double best = sh.red_v[0];
-        int bpos = sh.red_p[0];
-        for (int w2 = 1; w2 < GJ_LEAF_WAVES; w2++)
-            if (sh.red_v[w2] > best || (sh.red_v[w2] == best && sh.red_p[w2] < bpos)) { best = sh.red_v[w2]; bpos = sh.red_p[w2]; }
-        sh.piv = bpos;
-        out[J] = (bpos == GJ_NONE) ? -1 : bpos;
+    // every thread finishes the reduction itself (no second barrier)
+    double best = sh.red_v[0];
+    int pp = sh.red_p[0];
+#pragma unroll
+    for (int w2 = 1; w2 < WAVES; w2++) {
+        const double v2 = sh.red_v[w2];
+        const int p2 = sh.red_p[w2];
+        if (v2 > best || (v2 == best && p2 < pp)) { best = v2; pp = p2; }
     }
-    __syncthreads();
-    const int pp = sh.piv;
     if (R.act && R.pos == pp) {  // both lanes of the pivot row publish their halves
 #pragma unroll
         for (int c = 0; c < GJ_HALF; c++) sh.prow[GJ_HALF * R.h + c] = R.v[c];
         R.act = false;
     }
+    if (tid == 0) sh.piv[J] = (pp == GJ_NONE) ? -1 : pp;
     __syncthreads();
-    // multiplier of the row: computed by the lane holding column J, passed to its partner
+    // multiplier of the row: computed by the lane holding column J (and kept there), passed to
+    // its partner lane
     double lown = 0.0;
     if (pp != GJ_NONE && R.act && own) {
         const double piv = sh.prow[J];
         lown = R.v[JC] * ((piv != 0.0) ? 1.0 / piv : 0.0);
+        R.v[JC] = lown;
     }
     const double lpart = __shfl_xor(lown, 1);
     const double l = own ? lown : lpart;
@@ -228,35 +318,34 @@ __device__ __attribute__((always_inline)) inline void panel_column(PanelRows& R,
             if (col > J) R.v[c] -= l * sh.prow[col];
         }
     }
-    __syncthreads();  // prow, red_* and piv are rewritten by the next column
+    // the next column writes red_* only after its barrier, which every thread reaches after
+    // this update, and prow only after that barrier
 }
 
-template <int... J>
+template <int WAVES, bool FIN, int... J>
 __device__ __attribute__((always_inline)) inline void panel_columns(std::integer_sequence<int, J...>, PanelRows& R,
-                                                                    PanelShared& sh, int nbb, int* out) {
-    (panel_column<J>(R, sh, nbb, out), ...);
+                                                                    PanelShared& sh, int nbb) {
+    (panel_column<J, WAVES, FIN>(R, sh, nbb), ...);
 }
 
-template <int STAGE>
-__global__ void __launch_bounds__(GJ_LEAF_THREADS) gj_panel_leaf_kernel(GJArgs a) {
+template <int STAGE, int WAVES, bool FIN>
+__global__ void __launch_bounds__(64 * WAVES) gj_panel_kernel(GJArgs a) {
     __shared__ PanelShared sh;
+    constexpr int NT = 64 * WAVES, ROWS = NT / 2;
     const int b = blockIdx.y, ch = blockIdx.x;
     const int m = a.m, k = a.k, nbb = min(GJ_NB, m - k);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double* W = a.Wa + (long long)b * m * m;
     PanelRows R;
     R.h = tid & 1;
     const int i = tid >> 1;  // row of the chunk
-    int* out;
     if (STAGE == 0) {
-        R.pos = k + ch * GJ_CH + i;
+        R.pos = k + ch * ROWS + i;
         R.act = R.pos < m;
-        out = (a.nch == 1) ? a.cfin + (long long)b * GJ_NB : a.cand + ((long long)b * GJ_MAXCH + ch) * GJ_NB;
     } else {
         const int p = (i < a.nch * GJ_NB) ? a.cand[(long long)b * GJ_MAXCH * GJ_NB + i] : -1;
         R.act = p >= k && p < m;
         R.pos = R.act ? p : GJ_NONE;
-        out = a.cfin + (long long)b * GJ_NB;
     }
     const double* rowp = W + (long long)(R.act ? R.pos : k) * m + k + GJ_HALF * R.h;
 #pragma unroll
@@ -264,124 +353,106 @@ __global__ void __launch_bounds__(GJ_LEAF_THREADS) gj_panel_leaf_kernel(GJArgs a
         R.v[c] = 0.0;
         if (R.act && GJ_HALF * R.h + c < nbb) R.v[c] = rowp[c];  // columns k + 16h + c < m of a row of the matrix
     }
-    panel_columns(std::make_integer_sequence<int, GJ_NB>{}, R, sh, nbb, out);
-}
-
-template __global__ void gj_panel_leaf_kernel<0>(GJArgs);
-template __global__ void gj_panel_leaf_kernel<1>(GJArgs);
-
-// LU (in the order chosen) of the pivot block, P^-1, interchanges and the position -> row map
-// of the step.  One workgroup per matrix.
-__global__ void __launch_bounds__(256) gj_panel_final_kernel(GJArgs a) {
-    __shared__ double pb[GJ_NB][GJ_NB + 1];
-    __shared__ int piv[GJ_NB];
-    __shared__ double Li[GJ_NB][GJ_NB + 1], Ui[GJ_NB][GJ_NB + 1];
-    __shared__ int top_row[GJ_NB], opos[GJ_NB], orow[GJ_NB];
-    __shared__ int nout_s;
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const int m = a.m, k = a.k, nbb = min(GJ_NB, m - k);
-    const double* W = a.Wa + (long long)b * m * m;
-    if (tid < GJ_NB) {
-        const int p = (tid < nbb) ? a.cfin[(long long)b * GJ_NB + tid] : -1;
-        piv[tid] = (p >= k && p < m) ? p : -1;
-    }
+    panel_columns<WAVES, FIN>(std::make_integer_sequence<int, GJ_NB>{}, R, sh, nbb);
     __syncthreads();
-    for (int e = tid; e < GJ_NB * GJ_NB; e += 256) {
-        const int i = e / GJ_NB, c = e % GJ_NB;
-        double x = 0.0;
-        if (i < nbb && c < nbb && piv[i] >= 0) x = W[(long long)piv[i] * m + k + c];
-        pb[i][c] = x;
+    if (!FIN) {
+        if (tid < GJ_NB) a.cand[((long long)b * GJ_MAXCH + ch) * GJ_NB + tid] = (tid < nbb) ? sh.piv[tid] : -1;
+        return;
     }
-    __syncthreads();
-    // LU of the pivot rows in pivot order (the order partial pivoting chose them in)
-    for (int j = 0; j < nbb; j++) {
-        const double pj = pb[j][j];
-        const double inv = (pj != 0.0) ? 1.0 / pj : 0.0;
-        if (tid > j && tid < nbb) pb[tid][j] *= inv;
-        __syncthreads();
-        for (int e = tid; e < GJ_NB * GJ_NB; e += 256) {
-            const int i = e / GJ_NB, c = e % GJ_NB;
-            if (i > j && i < nbb && c > j && c < nbb) pb[i][c] -= pb[i][j] * pb[j][c];
+    if (wave == 0) {
+        // P^-1: Gauss-Jordan on [A11 | I] by one wave, lane c holding column c of A11 (its rows
+        // in pivot order: the pivot rows' panel values at the start of the step) and lane 32 + c
+        // column c of I, in registers; pivots and multipliers broadcast by readlane.  The order
+        // is the one partial pivoting chose, so the multipliers are bounded by 1 and no further
+        // pivoting is needed.  Beyond nbb the block is the identity.
+        const int c = lane & (GJ_NB - 1);
+        const bool ident = lane >= GJ_NB;
+        double v[GJ_NB];
+#pragma unroll
+        for (int r = 0; r < GJ_NB; r++) {
+            double x = (r == c) ? 1.0 : 0.0;
+            const int pr = sh.piv[r];
+            if (!ident && r < nbb && c < nbb && pr >= 0) x = W[(long long)pr * m + k + c];
+            v[r] = x;
         }
-        __syncthreads();
-    }
-    // inverses of L11 (unit lower) and U11 (upper, with the diagonal), column c per thread
-    if (tid < GJ_NB) {
-        const int c = tid;
-        for (int i = 0; i < GJ_NB; i++) { Li[i][c] = 0.0; Ui[i][c] = 0.0; }
-        if (c < nbb) {
-            Li[c][c] = 1.0;
-            for (int i = c + 1; i < nbb; i++) {
-                double s = 0.0;
-                for (int t = c; t < i; t++) s += pb[i][t] * Li[t][c];
-                Li[i][c] = -s;
-            }
-            const double ucc = pb[c][c];
-            Ui[c][c] = (ucc != 0.0) ? 1.0 / ucc : 0.0;
-            for (int i = c - 1; i >= 0; i--) {
-                double s = 0.0;
-                for (int t = i + 1; t <= c; t++) s += pb[i][t] * Ui[t][c];
-                const double uii = pb[i][i];
-                Ui[i][c] = (uii != 0.0) ? -s / uii : 0.0;
-            }
-        }
-    }
-    __syncthreads();
-    double* Pinv = a.Pinv + (long long)b * GJ_NB * GJ_NB;
-    for (int e = tid; e < GJ_NB * GJ_NB; e += 256) {
-        const int i = e / GJ_NB, c = e % GJ_NB;
-        double s = 0.0;
-        if (i < nbb && c < nbb)
-            for (int t = max(i, c); t < nbb; t++) s += Ui[i][t] * Li[t][c];
-        Pinv[e] = s;
-    }
-    // interchanges (LAPACK order: position k+j with the chosen row's current position)
-    if (tid == 0) {
-        int* prow = a.prow + (long long)b * GJ_NB;
-        int* ipiv = a.ipiv + (long long)b * m;
-        int nout = 0, zero = 0;
-        for (int j = 0; j < GJ_NB; j++) top_row[j] = k + j;
+        int zero = 0;
+#pragma unroll
         for (int j = 0; j < GJ_NB; j++) {
-            const int target = (j < nbb) ? piv[j] : -1;
-            prow[j] = (target >= 0) ? target : min(k + j, m - 1);
-            if (j >= nbb) continue;
-            if (zero == 0 && (target < 0 || pb[j][j] == 0.0)) zero = j + 1;
+            const double pv = readlane_f64(v[j], j);
+            if (zero == 0 && j < nbb && (pv == 0.0 || sh.piv[j] < 0)) zero = j + 1;
+            const double rowj = v[j] * ((pv != 0.0) ? 1.0 / pv : 0.0);
+#pragma unroll
+            for (int r = 0; r < GJ_NB; r++)
+                if (r != j) v[r] -= readlane_f64(v[r], j) * rowj;
+            v[j] = rowj;
+        }
+        if (ident) {
+            double* Pinv = a.Pinv + (long long)b * GJ_NB * GJ_NB;
+#pragma unroll
+            for (int r = 0; r < GJ_NB; r++) Pinv[r * GJ_NB + c] = (r < nbb && c < nbb) ? v[r] : 0.0;
+        }
+        // interchanges (LAPACK order: position k+j with the chosen row's current position)
+        int top = k + lane;      // row now at position k + lane (lanes < nbb)
+        int op = -1, orw = -1;   // entry `lane` of the positions beyond the panel that changed row
+        int nout = 0, mine = k + lane;
+        for (int j = 0; j < nbb; j++) {
+            const int target = sh.piv[j];
             const int apos = k + j;
-            if (target < 0) { ipiv[apos] = apos; continue; }
-            int t = -1;
-            for (int jj = j; jj < nbb && t < 0; jj++)
-                if (top_row[jj] == target) t = k + jj;
-            for (int o = 0; o < nout && t < 0; o++)
-                if (orow[o] == target) t = opos[o];
-            if (t < 0) t = target;
-            const int row_a = top_row[j];
-            if (t != apos) {
-                top_row[j] = target;
-                if (t < k + nbb) top_row[t - k] = row_a;
-                else {
-                    int o = 0;
-                    while (o < nout && opos[o] != t) o++;
-                    if (o == nout && nout < GJ_NB) { opos[nout] = t; nout++; }
-                    if (o < GJ_NB) orow[o] = row_a;
+            int t = apos;
+            if (target >= 0) {
+                const unsigned long long m1 = __ballot(lane >= j && lane < nbb && top == target);
+                if (m1) {
+                    t = k + __ffsll(m1) - 1;
+                } else {
+                    const unsigned long long m2 = __ballot(lane < nout && orw == target);
+                    t = m2 ? __shfl(op, __ffsll(m2) - 1) : target;
+                }
+                const int row_a = __shfl(top, j);
+                if (t != apos) {
+                    if (lane == j) top = target;
+                    if (t < k + nbb) {
+                        if (lane == t - k) top = row_a;
+                    } else {
+                        const unsigned long long m3 = __ballot(lane < nout && op == t);
+                        const int o = m3 ? __ffsll(m3) - 1 : nout;
+                        if (lane == o) { op = t; orw = row_a; }
+                        if (!m3) nout++;
+                    }
                 }
             }
-            ipiv[apos] = t;
+            if (lane == j) mine = t;
         }
-        nout_s = nout;
-        if (zero && a.info[b] == 0) a.info[b] = k + zero;
+        if (lane < GJ_NB) {
+            sh.top_row[lane] = top;
+            sh.opos[lane] = op;
+            sh.orow[lane] = orw;
+            const int target = (lane < nbb) ? sh.piv[lane] : -1;
+            a.prow[(long long)b * GJ_NB + lane] = (target >= 0) ? target : min(k + lane, m - 1);
+            if (lane < nbb) a.ipiv[(long long)b * m + k + lane] = mine;
+        }
+        if (lane == 0) {
+            sh.nout = nout;
+            if (zero && a.info[b] == 0) a.info[b] = k + zero;
+        }
     }
     __syncthreads();
     int* src = a.src + (long long)b * m;
-    const int nout = nout_s;
-    for (int i = tid; i < m; i += 256) {
-        int s = i;
-        if (i >= k && i < k + nbb) s = top_row[i - k];
+    const int nout = sh.nout;
+    for (int r = tid; r < m; r += NT) {
+        int s = r;
+        if (r >= k && r < k + nbb) s = sh.top_row[r - k];
         else
             for (int o = 0; o < nout; o++)
-                if (opos[o] == i) s = orow[o];
-        src[i] = s;
+                if (sh.opos[o] == r) s = sh.orow[o];
+        src[r] = s;
     }
 }
+
+template __global__ void gj_panel_kernel<0, 2, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 4, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 16, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 16, false>(GJArgs);
+template __global__ void gj_panel_kernel<1, 16, true>(GJArgs);
 
 // Multipliers of the step: M(i, :) = P^-1(j, :) - e_j on pivot position k + j, -A(src(i), panel) P^-1 elsewhere.
 __global__ void __launch_bounds__(256) gj_mult_kernel(GJArgs a) {
@@ -573,13 +644,14 @@ __global__ void __launch_bounds__(256) tree_coarsen_kernel(const CoarsenDesc* __
 // Sweeps
 // ---------------------------------------------------------------------------------------
 // entry i of side `side` of a child's h (side length clen / 4), coarsened `tag` times by L21
-// (pairwise averages, nested as coarsenUpwards_ applies them one step at a time)
-__device__ inline double coarse_h(const double* V, long long off, int clen, int tag, int side, int i, int k, int rhs) {
+// (pairwise averages, nested as coarsenUpwards_ applies them one step at a time); Vr is the
+// right-hand side's column of the arena
+__device__ inline double coarse_h(const double* Vr, long long off, int clen, int tag, int side, int i) {
     const int sl = clen / 4;
     const long long base = off + (long long)side * sl;
-    if (tag == 0) return V[(base + i) * k + rhs];
+    if (tag == 0) return Vr[base + i];
     const int w = 1 << tag;
-    auto x = [&](int t) { return V[(base + (long long)i * w + t) * k + rhs]; };
+    auto x = [&](int t) { return Vr[base + (long long)i * w + t]; };
     if (tag == 1) return 0.5 * x(0) + 0.5 * x(1);
     double s = 0.0;  // tag >= 2: blocks of four in the nested order, then their mean
     for (int t = 0; t < w; t += 4) {
@@ -589,40 +661,42 @@ __device__ inline double coarse_h(const double* V, long long off, int clen, int 
     return s * (4.0 / (double)w);
 }
 
-__global__ void __launch_bounds__(256) tree_up_gather_kernel(const UpDesc* __restrict__ descs, double* V, int k) {
+__global__ void __launch_bounds__(256) tree_up_gather_kernel(const UpDesc* __restrict__ descs, double* V, int k,
+                                                              long long vrows) {
     const UpDesc d = descs[blockIdx.y];
     const int n = d.n;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= 12LL * n * k) return;
-    const int rhs = (int)(e % k);
-    const int row = (int)(e / k);
+    const int rhs = (int)(e / (12 * n)), row = (int)(e % (12 * n));
+    double* Vr = V + (long long)rhs * vrows;
     if (row < 4 * n) {  // hDiff of mergeW_ (HPSAlgorithm.hpp:1059-1087)
         const int r = row / n, i = row % n;
         const int c1 = if_c1(r), c2 = if_c2(r);
-        const double h2 = coarse_h(V, pick4(d.ch_off, c2), pick4(d.clen, c2), pick4(d.tag, c2), if_s2(r), i, k, rhs);
-        const double h1 = coarse_h(V, pick4(d.ch_off, c1), pick4(d.clen, c1), pick4(d.tag, c1), if_s1(r), i, k, rhs);
-        V[(d.hd_off + row) * k + rhs] = h2 - h1;
+        const double h2 = coarse_h(Vr, pick4(d.ch_off, c2), pick4(d.clen, c2), pick4(d.tag, c2), if_s2(r), i);
+        const double h1 = coarse_h(Vr, pick4(d.ch_off, c1), pick4(d.clen, c1), pick4(d.tag, c1), if_s1(r), i);
+        Vr[d.hd_off + row] = h2 - h1;
     } else {  // boundary part of mergeH_ (:1098-1117), WESN order (:1128-1150)
         const int rr = row - 4 * n, q = rr / n, i = rr % n, c = q_child(q);
-        V[(d.h_off + rr) * k + rhs] = coarse_h(V, pick4(d.ch_off, c), pick4(d.clen, c), pick4(d.tag, c), q_side(q), i, k, rhs);
+        Vr[d.h_off + rr] = coarse_h(Vr, pick4(d.ch_off, c), pick4(d.clen, c), pick4(d.tag, c), q_side(q), i);
     }
 }
 
-__global__ void __launch_bounds__(256) tree_uncoarsen_kernel(const VecStepDesc* __restrict__ descs, double* V, int k) {
+__global__ void __launch_bounds__(256) tree_uncoarsen_kernel(const VecStepDesc* __restrict__ descs, double* V, int k,
+                                                              long long vrows) {
     const VecStepDesc d = descs[blockIdx.y];
     const int nf = d.nfine, nc = nf / 2;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= 4LL * nf * k) return;
-    const int rhs = (int)(e % k), row = (int)(e / k);
+    const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
     const int side = row / nf, i = row % nf;
-    const double* src = V + (d.src_off + (long long)side * nc) * k + rhs;
-    auto g = [&](int j) { return src[(long long)j * k]; };
+    double* Vr = V + (long long)rhs * vrows;
+    const double* src = Vr + d.src_off + (long long)side * nc;
     double s;
-    if (i == 0) s = 1.40625 * g(0) + (-0.5625) * g(1) + 0.15625 * g(2);
-    else if (i == nf - 1) s = 0.15625 * g(nc - 3) + (-0.5625) * g(nc - 2) + 1.40625 * g(nc - 1);
-    else if (i & 1) { const int j = (i - 1) / 2; s = 0.75 * g(j) + 0.25 * g(j + 1); }
-    else { const int j = (i - 2) / 2; s = 0.25 * g(j) + 0.75 * g(j + 1); }
-    V[(d.dst_off + row) * k + rhs] = s;
+    if (i == 0) s = 1.40625 * src[0] + (-0.5625) * src[1] + 0.15625 * src[2];
+    else if (i == nf - 1) s = 0.15625 * src[nc - 3] + (-0.5625) * src[nc - 2] + 1.40625 * src[nc - 1];
+    else if (i & 1) { const int j = (i - 1) / 2; s = 0.75 * src[j] + 0.25 * src[j + 1]; }
+    else { const int j = (i - 2) / 2; s = 0.25 * src[j] + 0.75 * src[j + 1]; }
+    Vr[d.dst_off + row] = s;
 }
 
 // children's g (applyS_, HPSAlgorithm.hpp:1194-1230): alpha = [aW, ab, aS, ag], beta = [ab, bE,
@@ -643,33 +717,36 @@ __device__ inline int child_src_r(int c, int s) {
     return 0;
 }
 
-__global__ void __launch_bounds__(256) tree_down_scatter_kernel(const DownDesc* __restrict__ descs, double* V, int k) {
+__global__ void __launch_bounds__(256) tree_down_scatter_kernel(const DownDesc* __restrict__ descs, double* V, int k,
+                                                                 long long vrows) {
     const DownDesc d = descs[blockIdx.y];
     const int n = d.n;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= 16LL * n * k) return;
-    const int rhs = (int)(e % k), row = (int)(e / k);
+    const int rhs = (int)(e / (16 * n)), row = (int)(e % (16 * n));
     const int c = row / (4 * n), rr = row % (4 * n);
     const int s = rr / n, i = rr % n;
     const int q = child_src_q(c, s);
-    double v;
-    if (q >= 0) v = V[(d.g_off + (long long)q * n + i) * k + rhs];
-    else v = V[(d.ui_off + (long long)child_src_r(c, s) * n + i) * k + rhs];
-    V[(pick4(d.cg_off, c) + rr) * k + rhs] = v;
+    double* Vr = V + (long long)rhs * vrows;
+    const double v = (q >= 0) ? Vr[d.g_off + (long long)q * n + i] : Vr[d.ui_off + (long long)child_src_r(c, s) * n + i];
+    Vr[pick4(d.cg_off, c) + rr] = v;
 }
 
 // g = r / a: the LU solve of diag(a) in solveStage (HPSAlgorithm.hpp:402-420); r is [rhs][m],
-// g is in arena layout [row][rhs].
-__global__ void __launch_bounds__(256) tree_root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k) {
+// g is column rhs of the arena (g + rhs * vrows).
+__global__ void __launch_bounds__(256) tree_root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k,
+                                                                   long long vrows) {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long long)m * k) return;
-    const int i = (int)(e / k), c = (int)(e % k);
-    g[e] = r[(long long)c * m + i] / a[i];
+    const int c = (int)(e / m), i = (int)(e % m);
+    g[(long long)c * vrows + i] = r[e] / a[i];
 }
 
-// A = diag(a) + diag(b) T and rhs = r - b h (HPSAlgorithm.hpp:402-420), rhs in arena layout.
+// A = diag(a) + diag(b) T and rhs = r - b h (HPSAlgorithm.hpp:402-420); rhs is [rhs][m], h a
+// column of the arena (h + rhs * vrows) or null.
 __global__ void __launch_bounds__(256) tree_root_robin_kernel(const double* T, const double* a, const double* b, const double* r,
-                                                              const double* h, double* A, double* rhs, int m, int k) {
+                                                              const double* h, double* A, double* rhs, int m, int k,
+                                                              long long vrows) {
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     const long long nA = (long long)m * m;
     if (e < nA) {
@@ -677,54 +754,85 @@ __global__ void __launch_bounds__(256) tree_root_robin_kernel(const double* T, c
         A[e] = b[i] * T[e] + (i == j ? a[i] : 0.0);
     } else if (e < nA + (long long)m * k) {
         const long long t = e - nA;
-        const int i = (int)(t / k), c = (int)(t % k);
-        rhs[t] = r[(long long)c * m + i] - b[i] * (h ? h[t] : 0.0);
+        const int c = (int)(t / m), i = (int)(t % m);
+        rhs[t] = r[t] - b[i] * (h ? h[(long long)c * vrows + i] : 0.0);
     }
 }
 
 // ---------------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------------
-GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s) {
+GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& store, hipStream_t s) {
     GemmLaunch g;
     g.tn = tn;
-    if (probs.empty()) return g;
+    g.rt = (tn == 1) ? 1 : 2;
+    if (probs_in.empty()) return g;
+    std::vector<TreeGemm> probs = probs_in;
     std::vector<int> start(probs.size() + 1, 0);
-    const int cw = 16 * tn;
-    long long total = 0;
+    std::vector<long long> estart(probs.size() + 1, 0);
+    const int rw = 16 * g.rt, cw = 16 * tn;
+    long long total = 0, elems = 0;
+    bool any_split = false;
     for (size_t i = 0; i < probs.size(); i++) {
-        const TreeGemm& p = probs[i];
+        TreeGemm& p = probs[i];
         start[i] = (int)total;
-        if (p.M <= 0 || p.N <= 0) continue;
+        estart[i] = elems;
+        p.nks = 1;
+        p.P = nullptr;
+        if (p.bcs == 0) p.bcs = 1;
+        if (p.ccs == 0) p.ccs = 1;
+        if (p.dcs == 0) p.dcs = 1;
+        if (p.M <= 0 || p.N <= 0) { p.kc = 0; continue; }
         // extents: the kernel clamps rows to M-1, columns to N-1 and k to K-1
-        check_matrix(p.D, p.M, p.N, p.ldd, "GEMM D");
-        if (p.C) check_matrix(p.C, p.M, p.N, p.ldc, "GEMM C");
+        check_strided(p.D, p.M, p.N, p.ldd, p.dcs, "GEMM D");
+        if (p.C) check_strided(p.C, p.M, p.N, p.ldc, p.ccs, "GEMM C");
         const double* As[2] = {p.A0, p.A1};
         const double* Bs[2] = {p.B0, p.B1};
         const int lda[2] = {p.lda0, p.lda1}, ldb[2] = {p.ldb0, p.ldb1}, K[2] = {p.K0, p.K1};
         for (int sg = 0; sg < 2; sg++) {
             if (K[sg] <= 0) continue;
             check_matrix(As[sg], p.M, K[sg], lda[sg], "GEMM A");
-            check_matrix(Bs[sg], K[sg], p.N, ldb[sg], "GEMM B");
+            check_strided(Bs[sg], K[sg], p.N, ldb[sg], p.bcs, "GEMM B");
             if (((uintptr_t)As[sg] & 15) != 0 || (lda[sg] & 1) != 0)
                 throw EfError(EF_ERR_INTERNAL, "GEMM A operand not 16-byte aligned with an even leading dimension");
+            if (ldb[sg] == 1 && (((uintptr_t)Bs[sg] & 15) != 0 || (p.bcs & 1) != 0))
+                throw EfError(EF_ERR_INTERNAL, "GEMM B operand contiguous along k but not 16-byte aligned");
         }
-        total += (long long)((p.M + 31) / 32) * ((p.N + cw - 1) / cw);
+        if (p.kc > 0) {
+            if (p.kc % 16 != 0 || (p.K1 > 0 && p.K0 % p.kc != 0))
+                throw EfError(EF_ERR_INTERNAL, "split-K chunk must be a multiple of 16 dividing the first segment");
+            p.nks = (p.K0 + p.K1 + p.kc - 1) / p.kc;
+            if (p.nks <= 1) { p.kc = 0; p.nks = 1; }
+        }
+        if (p.kc > 0) {
+            p.P = store.alloc_doubles((size_t)p.nks * p.M * p.N);
+            elems += (long long)p.M * p.N;
+            any_split = true;
+        }
+        total += (long long)((p.M + rw - 1) / rw) * ((p.N + cw - 1) / cw) * p.nks;
         if (total > 0x7fffffffLL) throw EfError(EF_ERR_INTERNAL, "GEMM launch too large");
     }
     start[probs.size()] = (int)total;
+    estart[probs.size()] = elems;
     g.nprob = (int)probs.size();
     g.nwaves = (int)total;
     g.probs = store.put(probs, s);
     g.start = store.put(start, s);
+    if (any_split) {
+        g.estart = store.put(estart, s);
+        g.nelem = elems;
+    }
     return g;
 }
 
 void run_gemm(const GemmLaunch& g, hipStream_t s) {
     if (g.nwaves == 0) return;
     const dim3 grid(blocks_for(g.nwaves, 4)), block(256);
-    if (g.tn == 1) EF_LAUNCH("tree_gemm_kernel<1>", tree_gemm_kernel<1>, grid, block, s, g.probs, g.start, g.nprob);
-    else EF_LAUNCH("tree_gemm_kernel<2>", tree_gemm_kernel<2>, grid, block, s, g.probs, g.start, g.nprob);
+    if (g.tn == 1) EF_LAUNCH("tree_gemm_kernel<1,1>", (tree_gemm_kernel<1, 1>), grid, block, s, g.probs, g.start, g.nprob);
+    else EF_LAUNCH("tree_gemm_kernel<2,2>", (tree_gemm_kernel<2, 2>), grid, block, s, g.probs, g.start, g.nprob);
+    if (g.nelem > 0)
+        EF_LAUNCH("tree_gemm_reduce_kernel", tree_gemm_reduce_kernel, dim3(blocks_for(g.nelem)), block, s, g.probs,
+                  g.estart, g.nprob);
 }
 
 size_t gj_work_doubles(int m, int B) {
@@ -761,10 +869,17 @@ void run_gj_inverse(double* wa, double* xi, int m, int B, const GJWork& w, int* 
     for (int k = 0; k < m; k += GJ_NB) {
         a.Wa = cur; a.Wb = nxt; a.k = k;
         a.nch = (m - k + GJ_CH - 1) / GJ_CH;
-        EF_LAUNCH("gj_panel_leaf_kernel<0>", gj_panel_leaf_kernel<0>, dim3(a.nch, B), dim3(GJ_LEAF_THREADS), s, a);
-        if (a.nch > 1)
-            EF_LAUNCH("gj_panel_leaf_kernel<1>", gj_panel_leaf_kernel<1>, dim3(1, B), dim3(GJ_LEAF_THREADS), s, a);
-        EF_LAUNCH("gj_panel_final_kernel", gj_panel_final_kernel, dim3(B), dim3(256), s, a);
+        const int rows = m - k;
+        if (rows <= 64)
+            EF_LAUNCH("gj_panel_kernel<0,2,fin>", (gj_panel_kernel<0, 2, true>), dim3(1, B), dim3(128), s, a);
+        else if (rows <= 128)
+            EF_LAUNCH("gj_panel_kernel<0,4,fin>", (gj_panel_kernel<0, 4, true>), dim3(1, B), dim3(256), s, a);
+        else if (a.nch == 1)
+            EF_LAUNCH("gj_panel_kernel<0,16,fin>", (gj_panel_kernel<0, 16, true>), dim3(1, B), dim3(1024), s, a);
+        else {
+            EF_LAUNCH("gj_panel_kernel<0,16>", (gj_panel_kernel<0, 16, false>), dim3(a.nch, B), dim3(1024), s, a);
+            EF_LAUNCH("gj_panel_kernel<1,16,fin>", (gj_panel_kernel<1, 16, true>), dim3(1, B), dim3(1024), s, a);
+        }
         EF_LAUNCH("gj_mult_kernel", gj_mult_kernel, dim3(blocks_for((long long)m * GJ_NB), B), dim3(256), s, a);
         EF_LAUNCH("gj_update_kernel", gj_update_kernel, dim3(blocks_for((long long)tpr * tpr, 4), B), dim3(256), s, a);
         std::swap(cur, nxt);

@@ -47,8 +47,12 @@ __host__ __device__ inline int child_if(int c, int slot) {
 //   D = alpha0 A0 B0 + alpha1 A1 B1 + beta C        (segment 1 absent when K1 == 0,
 //                                                    C absent when null)
 // A0 is M x K0 (lda0), B0 is K0 x N (ldb0), ...  C may alias D (read, then written, by the
-// same lane).  One wave computes a 32 x (16 TN) block; the blocks of all problems of a
+// same lane).  One wave computes a (16 RT) x (16 TN) block; the blocks of all problems of a
 // launch form one flat list (problem p owns waves start[p] .. start[p+1]-1).
+// Split-K (kc > 0): the concatenated k range of the two segments is cut into nks chunks of kc
+// (a multiple of 16 that divides K0 when K1 > 0); each block of each chunk writes its partial
+// product to P[chunk][M][N], and tree_gemm_reduce_kernel sums the chunks in order (fixed order,
+// so a column's result does not depend on the other columns of the batch).
 // ---------------------------------------------------------------------------------------
 struct TreeGemm {
     const double* A0;
@@ -57,24 +61,28 @@ struct TreeGemm {
     const double* B1;
     const double* C;
     double* D;
-    int lda0, ldb0, K0, lda1, ldb1, K1;
-    int ldc, ldd, M, N;
+    int lda0, ldb0, K0, lda1, ldb1, K1;  // element (i, j) of B_s at B_s[i * ldb_s + j * bcs]
+    int ldc, ldd, M, N;                  // element (i, j) of C at C[i * ldc + j * ccs], of D likewise
+    int bcs, ccs, dcs, kc;               // column strides (0 means 1); split-K chunk length (0: none)
     double alpha0, alpha1, beta;
+    int nks, pad;                        // split-K chunk count (set by plan_gemm)
+    double* P;                           // partials, nks x M x N (split problems)
 };
-static_assert(sizeof(TreeGemm) == 112, "TreeGemm layout");
+static_assert(sizeof(TreeGemm) == 144, "TreeGemm layout");
 
-template <int TN>
+template <int RT, int TN>
 __global__ void tree_gemm_kernel(const TreeGemm* probs, const int* start, int nprob);
+__global__ void tree_gemm_reduce_kernel(const TreeGemm* probs, const long long* estart, int nprob);
 
 // ---------------------------------------------------------------------------------------
 // In-place blocked Gauss-Jordan inversion with partial pivoting of a batch of B equally
 // sized m x m matrices (m <= GJ_MAXM), GJ_NB columns per step, ping-pong buffers Wa -> Wb.
 // Step k (columns k .. k+nb):
-//   panel_leaf   LU with partial pivoting of each chunk of up to GJ_CH panel rows (one
+//   panel        LU with partial pivoting of each chunk of up to GJ_CH panel rows (one
 //                workgroup per chunk, rows in registers): nb candidate pivot rows per chunk;
-//                with several chunks a second pass over all candidates picks the pivots
-//   panel_final  LU of the pivot block in pivot order -> P^-1, the row interchanges (ipiv,
-//                LAPACK convention) and the position -> source-row map src of this step
+//                with several chunks a second pass over all candidates picks the pivots.  The
+//                pass that picks the pivots also forms P^-1 of the pivot block, the row
+//                interchanges (ipiv, LAPACK convention) and the position -> row map src
 //   mult         multipliers M (m x nb): P^-1 - I on the pivot positions, -A(i, panel) P^-1
 //                elsewhere
 //   update       Wb(i, :) = Wa(src(i), :) + M(i, :) Wa(pivot rows, :) outside the panel,
@@ -104,9 +112,8 @@ struct GJArgs {
     int m, B, k, nch;
 };
 
-template <int STAGE>
-__global__ void gj_panel_leaf_kernel(GJArgs a);
-__global__ void gj_panel_final_kernel(GJArgs a);
+template <int STAGE, int WAVES, bool FIN>
+__global__ void gj_panel_kernel(GJArgs a);
 __global__ void gj_mult_kernel(GJArgs a);
 __global__ void gj_update_kernel(GJArgs a);
 __global__ void gj_perm_kernel(GJArgs a);
@@ -131,20 +138,21 @@ __global__ void tree_assemble_kernel(const AsmDesc* d);
 __global__ void tree_coarsen_kernel(const CoarsenDesc* d);
 
 // ---------------------------------------------------------------------------------------
-// Sweeps over the vector arena V (element (row, rhs) at row * k + rhs).
+// Sweeps over the vector arena V: vrows rows by k right-hand sides, right-hand side major
+// (element (row, rhs) at rhs * vrows + row).
 // ---------------------------------------------------------------------------------------
 // upwards4to1 gather (coarsenUpwards_ + hDiff of mergeW_ + boundary part of mergeH_):
 //   hd(r) = h_c2(side) - h_c1(side) on the four interfaces, h_tau(q) = h_child(q)(side q)
-__global__ void tree_up_gather_kernel(const UpDesc* d, double* V, int k);
+__global__ void tree_up_gather_kernel(const UpDesc* d, double* V, int k, long long vrows);
 // uncoarsen_ step: g <- blkdiag4(L12(nfine)) g
-__global__ void tree_uncoarsen_kernel(const VecStepDesc* d, double* V, int k);
+__global__ void tree_uncoarsen_kernel(const VecStepDesc* d, double* V, int k, long long vrows);
 // applyS_ scatter: children's g from tau's g (WESN) and the interface potential
-__global__ void tree_down_scatter_kernel(const DownDesc* d, double* V, int k);
+__global__ void tree_down_scatter_kernel(const DownDesc* d, double* V, int k, long long vrows);
 // root Dirichlet data g = r / a (solveStage with b = 0)
-__global__ void tree_root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k);
+__global__ void tree_root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k, long long vrows);
 // root Robin system A = diag(a) + diag(b) T, rhs = r - b h
 __global__ void tree_root_robin_kernel(const double* T, const double* a, const double* b, const double* r,
-                                       const double* h, double* A, double* rhs, int m, int k);
+                                       const double* h, double* A, double* rhs, int m, int k, long long vrows);
 
 // ---------------------------------------------------------------------------------------
 // Host side (ef_tree.hip)
@@ -153,11 +161,13 @@ __global__ void tree_root_robin_kernel(const double* T, const double* a, const d
 struct GemmLaunch {
     TreeGemm* probs = nullptr;
     int* start = nullptr;
-    int nprob = 0, nwaves = 0, tn = 2;
+    long long* estart = nullptr;  // reduce: first element of each problem (split problems only)
+    int nprob = 0, nwaves = 0, rt = 2, tn = 2;
+    long long nelem = 0;
 };
-// Checks every problem (extents inside their allocations, alignment), uploads problems and
-// wave offsets into `store` (device memory the caller keeps alive) and returns the launch.
-// tn = 1 (16-column blocks) suits N <= 16, tn = 2 otherwise.
+// Checks every problem (extents inside their allocations, alignment), allocates the partials of
+// split problems, uploads problems and wave offsets into `store` (device memory the caller keeps
+// alive) and returns the launch.  tn = 1 (16 x 16 blocks) suits N <= 16, tn = 2 (32 x 32) the rest.
 struct DevStore;
 GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s);
 void run_gemm(const GemmLaunch& g, hipStream_t s);

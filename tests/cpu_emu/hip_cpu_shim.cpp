@@ -152,6 +152,18 @@ emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int) {
     return out;
 }
 
+unsigned long long __ballot(int pred) {
+    const int t = threadIdx.x, w = t / 64 * 64;
+    const int p = g_par;
+    g_par ^= 1;
+    g_mfma_a[p][t] = pred != 0 ? 1.0 : 0.0;
+    emu_wave_sync();
+    unsigned long long m = 0;
+    for (int l = 0; l < 64; l++)
+        if (g_mfma_a[p][w + l] != 0.0) m |= 1ULL << l;
+    return m;
+}
+
 double __shfl(double v, int src, int) {
     const int t = threadIdx.x, w = t / 64 * 64;
     const int p = g_par;
@@ -161,46 +173,76 @@ double __shfl(double v, int src, int) {
     return g_mfma_a[p][w + (src & 63)];
 }
 
-// Persistent pool of 256 workers (one per lane of a block).  Per block the main thread
-// publishes the body, releases the workers through g_start and waits on g_end.  The
-// block and wave barriers are rebuilt per block because a finishing thread drops out
-// of them, as an exited wave stops counting at s_barrier.
-static std::barrier<>* g_start = nullptr;
-static std::barrier<>* g_end = nullptr;
+// Persistent pools of workers, one pool per block size (one worker per lane of a block).  Per
+// block the main thread publishes the body, releases the pool's workers through its start
+// barrier and waits on its end barrier.  The block and wave barriers are rebuilt per block
+// because a finishing thread drops out of them, as an exited wave stops counting at s_barrier.
 static const std::function<void()>* g_body = nullptr;
-static bool g_quit = false;
-static std::vector<std::thread>* g_pool = nullptr;
+struct EmuPool {
+    std::barrier<>* start;
+    std::barrier<>* end;
+};
+static std::map<int, EmuPool> g_pools;
 
-static void pool_init() {
-    if (g_pool) return;
-    g_start = new std::barrier<>(NT + 1);
-    g_end = new std::barrier<>(NT + 1);
-    g_pool = new std::vector<std::thread>();
-    for (int t = 0; t < NT; t++)
-        g_pool->emplace_back([t]() {
+static EmuPool& pool_for(int nt) {
+    auto it = g_pools.find(nt);
+    if (it != g_pools.end()) return it->second;
+    EmuPool P{new std::barrier<>(nt + 1), new std::barrier<>(nt + 1)};
+    g_pools[nt] = P;
+    for (int t = 0; t < nt; t++)
+        std::thread([t, P]() {
             threadIdx = {(unsigned)t, 0, 0};
             for (;;) {
-                g_start->arrive_and_wait();
-                if (g_quit) return;
-                if (t < g_nt) {
-                    g_par = 0;  // every block starts its wave collectives on slot 0
-                    (*g_body)();
-                    g_wave_bar[t / 64]->arrive_and_drop();
-                    g_block_bar->arrive_and_drop();
-                }
-                g_end->arrive_and_wait();
+                P.start->arrive_and_wait();
+                g_par = 0;  // every block starts its wave collectives on slot 0
+                (*g_body)();
+                g_wave_bar[t / 64]->arrive_and_drop();
+                g_block_bar->arrive_and_drop();
+                P.end->arrive_and_wait();
             }
-        });
-    for (auto& th : *g_pool) th.detach();
+        }).detach();
+    return g_pools[nt];
 }
 
+struct EmuGraph {
+    std::vector<std::function<void()>> nodes;
+};
+static thread_local EmuGraph* g_capture = nullptr;
+bool emu_capturing() { return g_capture != nullptr; }
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    if (g_capture) return hipErrorStreamCaptureUnsupported;
+    g_capture = new EmuGraph();
+    return 0;
+}
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    *g = g_capture;
+    g_capture = nullptr;
+    return *g ? 0 : hipErrorStreamCaptureUnsupported;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t) {
+    *e = new EmuGraph(*g);
+    return 0;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    if (g_capture) return hipErrorStreamCaptureUnsupported;
+    for (auto& n : e->nodes) n();
+    return 0;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return 0; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return 0; }
+
 void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body) {
+    if (g_capture) {  // recorded with its arguments, run at hipGraphLaunch
+        std::function<void()> f = body;
+        g_capture->nodes.push_back([=]() { emu_run_grid(g, b, shm, f); });
+        return;
+    }
     if (b.x % 64 != 0 || b.x > (unsigned)NT || b.y != 1 || b.z != 1) {
         fprintf(stderr, "emulator supports 1-D blocks of 64..1024 threads (multiples of 64) only\n");
         abort();
     }
-    pool_init();
     g_nt = (int)b.x;
+    EmuPool& pool = pool_for(g_nt);
     gridDim = g;
     blockDim = b;
     if (shm) g_dyn = guarded_alloc(shm, &g_dyn_base, &g_dyn_len);
@@ -241,8 +283,8 @@ void emu_run_grid(dim3 g, dim3 b, size_t shm, const std::function<void()>& body)
                 std::memset(g_dyn, 0xff, shm);
             }
         }
-        g_start->arrive_and_wait();
-        g_end->arrive_and_wait();
+        pool.start->arrive_and_wait();
+        pool.end->arrive_and_wait();
         for (int i = 0; i < g_nt / 64; i++) delete g_wave_bar[i];
     }
     if (shm) { munmap(g_dyn_base, g_dyn_len); g_dyn = nullptr; }

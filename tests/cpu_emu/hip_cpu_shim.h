@@ -61,20 +61,40 @@ hipError_t hipMemGetAddressRange(void** base, size_t* size, void* ptr);
 // address handed to the device, which works here (one address space) and faults a GPU.
 bool emu_ptrcheck_on();
 void emu_ptrcheck(const void* p, size_t n, const char* what, const void* kernel, int index);
+// Stream capture: kernels launched while a capture is open are recorded, not run; copies,
+// fills and synchronisations are errors then (the product must capture kernels only).
+bool emu_capturing();
+enum { hipErrorStreamCaptureUnsupported = 900 };
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+struct EmuGraph;
+typedef EmuGraph* hipGraph_t;
+typedef EmuGraph* hipGraphExec_t;
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode);
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g);
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t);
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t);
+hipError_t hipGraphDestroy(hipGraph_t g);
+hipError_t hipGraphExecDestroy(hipGraphExec_t e);
 inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k) {
+    if (emu_capturing()) return hipErrorStreamCaptureUnsupported;
     if (n) std::memcpy(d, s, n);
     if (n && k == hipMemcpyHostToDevice && emu_ptrcheck_on()) emu_ptrcheck(d, n, "host-to-device copy", nullptr, -1);
     return 0;
 }
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
-inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return 0; }
-inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return 0; }
+inline hipError_t hipMemset(void* d, int v, size_t n) {
+    if (emu_capturing()) return hipErrorStreamCaptureUnsupported;
+    std::memset(d, v, n);
+    return 0;
+}
+inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { return hipMemset(d, v, n); }
 inline hipError_t hipGetLastError() { return 0; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return 0; }
 inline hipError_t hipSetDevice(int) { return 0; }
 inline hipError_t hipStreamCreate(hipStream_t* s) { *s = (hipStream_t)0x1; return 0; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return 0; }
-inline hipError_t hipStreamSynchronize(hipStream_t) { return 0; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return emu_capturing() ? hipErrorStreamCaptureUnsupported : 0; }
 #define hipHostMallocDefault 0u
 inline hipError_t hipHostMalloc(void** p, size_t n, unsigned) { *p = std::malloc(n ? n : 1); return *p ? 0 : 2; }
 inline hipError_t hipHostFree(void* p) { std::free(p); return 0; }
@@ -110,6 +130,9 @@ inline double __hiloint2double(int hi, int lo) {
     std::memcpy(&x, &u, 8);
     return x;
 }
+// wave vote: bit l set when lane l's predicate holds
+unsigned long long __ballot(int pred);
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 // v_readlane_b32: lane src's value in every lane (src uniform); 32-bit values are exact in a double
 inline int __builtin_amdgcn_readlane(int x, int src) { return (int)__shfl((double)x, src); }
 // v_permlane32_swap: lanes 32..63 of the first operand trade places with lanes 0..31 of the

@@ -1,5 +1,6 @@
 """Writes the golden fixtures under tests/golden/ (run from the repo root:
-``python tests/golden/make_golden.py``).
+``python tests/golden/make_golden.py``; ``... make_golden.py varbeta`` writes only the
+variable-beta global solutions).
 
 Three kinds of data, none produced by running the reference (it cannot be built here:
 no MPI, p4est, PETSc, FISHPACK90 or LAPACK):
@@ -84,6 +85,44 @@ def global_5pt(M, bounds, f, g, lam=0.0):
     return u.reshape(M, M)
 
 
+def global_5pt_var(M, bounds, alpha, beta, lam, f, g):
+    """Cell-centred 5-point solution of alpha div(beta grad u) + lambda u = f on an M x M grid:
+    alpha and lambda at cell centres, beta at each face centre (sampled once per face and
+    shared by the two cells on it), ghost = 2g - u on the physical boundary.  At an interior
+    face the HPS interface condition (equal and opposite normal derivatives of the two leaf
+    solutions, h_L + h_R = 0) gives g = (u_L + u_R)/2, so the uniform-mesh HPS solution of
+    the variable-beta operator is this global solution."""
+    xl, xu, yl, yu = bounds
+    dx, dy = (xu - xl) / M, (yu - yl) / M
+    xs = (xl + dx / 2) + np.arange(M) * dx
+    ys = (yl + dy / 2) + np.arange(M) * dy
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    xf = xl + np.arange(M + 1) * dx  # x faces
+    yf = yl + np.arange(M + 1) * dy
+    bx = beta(*np.meshgrid(xf, ys, indexing="ij"))  # (M+1) x M: face between cells i-1 and i
+    by = beta(*np.meshgrid(xs, yf, indexing="ij"))  # M x (M+1)
+    a = alpha(X, Y)
+    n = M * M
+    idx = np.arange(n).reshape(M, M)
+    cW, cE = a * bx[:-1, :] / dx**2, a * bx[1:, :] / dx**2
+    cS, cN = a * by[:, :-1] / dy**2, a * by[:, 1:] / dy**2
+    diag = -(cW + cE + cS + cN) + lam(X, Y)
+    rhs = f(X, Y).copy()
+    rows, cols, vals = [], [], []
+    rows.append(idx[1:, :].ravel()); cols.append(idx[:-1, :].ravel()); vals.append(cW[1:, :].ravel())
+    rows.append(idx[:-1, :].ravel()); cols.append(idx[1:, :].ravel()); vals.append(cE[:-1, :].ravel())
+    rows.append(idx[:, 1:].ravel()); cols.append(idx[:, :-1].ravel()); vals.append(cS[:, 1:].ravel())
+    rows.append(idx[:, :-1].ravel()); cols.append(idx[:, 1:].ravel()); vals.append(cN[:, :-1].ravel())
+    diag[0, :] -= cW[0, :]; rhs[0, :] -= 2 * cW[0, :] * g(np.full(M, xl), ys)
+    diag[-1, :] -= cE[-1, :]; rhs[-1, :] -= 2 * cE[-1, :] * g(np.full(M, xu), ys)
+    diag[:, 0] -= cS[:, 0]; rhs[:, 0] -= 2 * cS[:, 0] * g(xs, np.full(M, yl))
+    diag[:, -1] -= cN[:, -1]; rhs[:, -1] -= 2 * cN[:, -1] * g(xs, np.full(M, yu))
+    rows.append(idx.ravel()); cols.append(idx.ravel()); vals.append(diag.ravel())
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    u = spla.spsolve(A.tocsc(), rhs.ravel())
+    return u.reshape(M, M)
+
+
 def leaf_blocks(U, level, N):
     """Global grid -> per-leaf cell vectors in z-order (index j + i*N)."""
     from oracle import mesh
@@ -93,7 +132,24 @@ def leaf_blocks(U, level, N):
     return np.array(out)
 
 
+def write_varbeta():
+    """global_5pt_varbeta_*.npz: the variable-beta operator of the GPU parity tests (alpha = 1,
+    beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1, Dirichlet data and f of the manufactured
+    u = sin(3x) cosh(y)) solved globally on uniform meshes."""
+    sys.path.insert(0, os.path.dirname(HERE))
+    import test_gpu_parity as gp
+    for name, level, N in [("global_5pt_varbeta_L2N8", 2, 8), ("global_5pt_varbeta_L3N16", 3, 16)]:
+        bounds = (-1.0, 1.0, -1.0, 1.0)
+        M = N << level
+        U = global_5pt_var(M, bounds, gp.ALPHA, gp.BETA, gp.LAMBDA, gp.F3, gp.U3)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), level=level, N=N, bounds=np.array(bounds),
+                            u=leaf_blocks(U, level, N))
+
+
 def main():
+    if sys.argv[1:] == ["varbeta"]:
+        write_varbeta()
+        return
     with open(os.path.join(HERE, "reference_known_answers.json"), "w") as f:
         json.dump(KNOWN, f, indent=1)
     u_ex = lambda x, y: np.sin(x) + np.sin(y)
@@ -124,6 +180,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "oracle_fixtures.npz"), T4=T4, T8_helmholtz=T8,
                         merge_X=merge["X"], merge_S=merge["S"], merge_T=merge["T"], merge_H=merge["H"],
                         adaptive_leaves=np.array(leaves), adaptive_u=u)
+    write_varbeta()
     print("fixtures written to", HERE)
 
 

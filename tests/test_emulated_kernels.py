@@ -83,3 +83,10 @@ def test_emu_device_memory_and_stage_timers(emu_ctx):
     st = D.stage_ms()
     assert set(st) == {"build-stage", "upwards-stage", "solve-stage"}
     assert all(v >= 0.0 for v in st.values())
+
+
+def test_emu_graph_replay(emu_ctx):
+    """Captured sweeps (emulated capture: kernels recorded, copies and fills refused while a
+    capture is open) replay bitwise like the eager launches."""
+    leaves = omesh.refine_by_function(lambda x, y: (x < 0.2) & (y < 0.2), 4, 4, 0, 1, 0, 1, 0, 3)
+    gp.graph_replay_case(emu_ctx, leaves, 4, (0, 1, 0, 1), k=2)

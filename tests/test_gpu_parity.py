@@ -237,9 +237,81 @@ def test_variable_coefficients_uniform(ctx):
     assert rel(ug, uo) < TOL_U
 
 
+@pytest.mark.parametrize("name", ["global_5pt_varbeta_L2N8", "global_5pt_varbeta_L3N16"])
+def test_variable_equals_global_solve(ctx, name):
+    """The variable-beta operator on a uniform mesh against the independent global 5-point
+    solve with face-sampled beta (tests/golden/make_golden.py)."""
+    import ellipticforest_amd as ef
+    say(f"golden {name}")
+    d = np.load(os.path.join(GOLD, name + ".npz"))
+    level, N, bounds = int(d["level"]), int(d["N"]), tuple(d["bounds"])
+    leaves = omesh.uniform(level)
+    D = ef.DeviceMesh(ctx, leaves, N, bounds)
+    cell = np.concatenate([oracle.sample_coeffs(N, tuple(D.node_bounds[n]), ALPHA, BETA, LAMBDA).ravel()
+                           for n in D.leaf_nodes])
+    f = np.stack([F3(*oracle.cell_centres(N, tuple(D.node_bounds[n]))[:2]) for n in D.leaf_nodes])
+    side, px, py = oracle.root_face_points(D.M, bounds)
+    D.build(solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    D.upwards(f)
+    u = D.solve(U3(px, py), np.ones_like(px))
+    assert rel(u, d["u"]) < 1e-11
+
+
 def test_variable_coefficients_N32_shape(ctx):
     """The variable-beta operator on N = 32 patches at level 3: parity and second-order error."""
     say("variable beta: uniform level 3, N=32")
     uo, ug, err = variable_both(ctx, omesh.uniform(3), 32)
     assert rel(ug, uo) < TOL_U
     assert err < 2e-4
+
+
+def graph_replay_case(ctx, leaves, N, bounds, k=3):
+    """Upwards + Dirichlet solve through device pointers with the captured sweep against the
+    same calls launched kernel by kernel (EF_GRAPH=0): bitwise equal on every replay, with
+    other work on the mesh between replays, when the inputs change under the same pointers,
+    and after the vector arena has been reallocated for a larger k."""
+    from ellipticforest_amd import _lib
+    D = _lib.DeviceMesh(ctx, leaves, N, bounds)
+    D.build()
+    nl, m = len(leaves), 4 * D.M
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((2, k, nl, N * N))
+    r = rng.standard_normal((2, k, m))
+    a = np.ones(m)
+    fd, rd, ad, ud = (_lib.DeviceArray(ctx, s) for s in (f[0].shape, r[0].shape, a.shape, f[0].shape))
+    ad.upload(a)
+
+    def sweep(i):
+        fd.upload(f[i])
+        rd.upload(r[i])
+        D.upwards_device(fd.ptr, k)
+        D.solve_device(fd.ptr, rd.ptr, ad.ptr, None, ud.ptr, k)
+        D.sync()
+        return ud.download().reshape(f[i].shape)
+
+    os.environ["EF_GRAPH"] = "0"
+    try:
+        eager = [sweep(0), sweep(1)]
+    finally:
+        del os.environ["EF_GRAPH"]
+    for rep in range(3):
+        for i in (0, 1):
+            say(f"graph replay {rep} input {i}")
+            assert np.array_equal(sweep(i), eager[i]), (rep, i)
+        D.upwards(f[1][:1])  # other work on the mesh between replays (k = 1, host buffers)
+        D.solve(r[1][:1], a)
+    big, ub = _lib.DeviceArray(ctx, (k + 2, nl, N * N)), _lib.DeviceArray(ctx, (k + 2, nl, N * N))
+    rb = _lib.DeviceArray(ctx, (k + 2, m))
+    big.upload(np.zeros((k + 2, nl, N * N)))
+    rb.upload(np.zeros((k + 2, m)))
+    D.upwards_device(big.ptr, k + 2)  # grows the arena: every captured sweep is dropped
+    D.solve_device(big.ptr, rb.ptr, ad.ptr, None, ub.ptr, k + 2)
+    D.sync()
+    assert np.array_equal(sweep(0), eager[0])
+    return eager
+
+
+def test_graph_replay_matches_eager(ctx):
+    say("graph replay, adaptive N=8 mesh with coarsening")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 8, 8, -10, 10, -10, 10, 0, 5)
+    graph_replay_case(ctx, leaves, 8, (-10, 10, -10, 10))

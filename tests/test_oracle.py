@@ -203,3 +203,26 @@ def test_variable_coefficient_second_order():
                         for k, bd in enumerate(Fo.leaf_bounds)))
     orders = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert orders.min() > 1.8, (errs, orders)
+
+
+@pytest.mark.parametrize("name", ["global_5pt_varbeta_L2N8", "global_5pt_varbeta_L3N16"])
+def test_variable_hps_equals_global_solve(name):
+    """Variable beta (sampled at faces), alpha = 1, lambda = -1: the uniform-mesh HPS solution is
+    the global 5-point solution with face-sampled beta (scipy.sparse, independent of the oracle).
+    The same solve with beta(y, x) (a transposed coefficient) must miss the fixture by far."""
+    import test_gpu_parity as gp
+    d = np.load(os.path.join(GOLD, name + ".npz"))
+    level, N, bounds = int(d["level"]), int(d["N"]), tuple(d["bounds"])
+
+    def run(beta):
+        Fo = oracle.Forest(mesh.uniform(level), N, bounds, solver=oracle.FIVE_POINT,
+                           coeff_fns=(gp.ALPHA, beta, gp.LAMBDA))
+        Fo.build()
+        Fo.upwards(Fo.sample_rhs(gp.F3))
+        r, a, b = Fo.boundary_data(lambda s, x, y: (gp.U3(x, y), 1.0, 0.0))
+        return Fo.solve(r, a, b)
+
+    scale = np.abs(d["u"]).max()
+    assert np.abs(run(gp.BETA) - d["u"]).max() / scale < 1e-12
+    if level == 2:
+        assert np.abs(run(lambda x, y: gp.BETA(y, x)) - d["u"]).max() / scale > 1e-4
