@@ -253,6 +253,7 @@ struct ef_mesh_s {
     DevBuf<int> gji, info;
     std::vector<GroupPlan> gplans;
     DevStore bstore;
+    std::vector<const void*> plan_sig;  // arena and leaf-DtN addresses gplans was made for
     // vectors
     int kcap = 0;
     int k_up = 0;  // k of the last upwards call
@@ -343,15 +344,15 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     M->variable = coefficients_vary(P, c);
     if (M->variable) {
         if (N > BT_MAX_N) throw EfError(EF_ERR_ARG, "variable coefficients need N <= " + std::to_string(BT_MAX_N));
-        M->vcoef.alloc(nl * 6 * NN);
+        M->vcoef.ensure(nl * 6 * NN);
         efk::copy_h2d(M->vcoef.p, c.cell, sizeof(double) * nl * 6 * NN, M->stream);
-        M->vstencil.alloc(nl * 5 * NN);
-        M->vSinv.alloc(nl * (size_t)N * NN);
+        M->vstencil.ensure(nl * 5 * NN);
+        M->vSinv.ensure(nl * (size_t)N * NN);
         M->packed = (N == 8 || N == 16 || N == 32);
         for (size_t li = 0; li < nl && M->packed; li++) M->packed = efk::cells_symmetric(c.cell + li * 6 * NN, N);
         const long long NP = bt_packed_entries(N);
-        if (M->packed) M->vSinvP.alloc(nl * (size_t)N * NP);
-        M->leafT.alloc(nl * 16 * NN);
+        if (M->packed) M->vSinvP.ensure(nl * (size_t)N * NP);
+        M->leafT.ensure(nl * 16 * NN);
         std::vector<LeafDesc> ld(nl);
         for (size_t li = 0; li < nl; li++) {
             const NodeP& nd = P.nodes[P.leaves[li]];
@@ -404,7 +405,7 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     }
     M->tleaf_class0 = M->leaf_class[0];
     const int ncls = (int)M->classes_h.size();
-    M->leafT.alloc((size_t)ncls * 16 * NN);
+    M->leafT.ensure((size_t)ncls * 16 * NN);
     M->classes.upload(M->classes_h, M->stream);
     M->leaf_cls.upload(M->leaf_class, M->stream);
     std::vector<long long> goff(nl), hoff(nl);
@@ -467,7 +468,7 @@ void plan_build(ef_mesh_s* M) {
                 }
             }
         for (size_t t = 0; t < steps.size(); t++) {
-            gp.csteps.push_back(M->bstore.put(steps[t], st));
+            gp.csteps.push_back(M->bstore.stage(steps[t]));
             gp.csteps_cnt.push_back((int)steps[t].size());
             gp.csteps_maxm.push_back(maxm[t]);
         }
@@ -526,11 +527,12 @@ void plan_build(ef_mesh_s* M) {
                 pT.push_back(p);
             }
         }
-        gp.asmd = M->bstore.put(ad, st);
+        gp.asmd = M->bstore.stage(ad);
         gp.gS = plan_gemm(pS, n <= 16 ? 1 : 2, M->bstore, st);
         gp.gT = plan_gemm(pT, 2, M->bstore, st);
         M->gplans.push_back(std::move(gp));
     }
+    M->bstore.flush(st);
 }
 
 void stage_begin(ef_mesh_s* M, int st) {
@@ -551,11 +553,12 @@ void check_tree_kernels() {
     check_kernel_resources((const void*)tree_gemm_kernel<1, 1>, "tree_gemm_kernel<1,1>");
     check_kernel_resources((const void*)tree_gemm_kernel<2, 2>, "tree_gemm_kernel<2,2>");
     check_kernel_resources((const void*)tree_gemm_reduce_kernel, "tree_gemm_reduce_kernel");
-    check_kernel_resources((const void*)gj_panel_kernel<0, 2, true>, "gj_panel_kernel<0,2,fin>");
-    check_kernel_resources((const void*)gj_panel_kernel<0, 4, true>, "gj_panel_kernel<0,4,fin>");
-    check_kernel_resources((const void*)gj_panel_kernel<0, 16, true>, "gj_panel_kernel<0,16,fin>");
-    check_kernel_resources((const void*)gj_panel_kernel<0, 16, false>, "gj_panel_kernel<0,16>");
-    check_kernel_resources((const void*)gj_panel_kernel<1, 16, true>, "gj_panel_kernel<1,16,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 1, 1, true>, "gj_panel_kernel<0,1,1,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 1, 2, true>, "gj_panel_kernel<0,1,2,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 2, 2, true>, "gj_panel_kernel<0,2,2,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 4, 2, true>, "gj_panel_kernel<0,4,2,fin>");
+    check_kernel_resources((const void*)gj_panel_kernel<0, 4, 2, false>, "gj_panel_kernel<0,4,2>");
+    check_kernel_resources((const void*)gj_panel_kernel<1, 4, 1, true>, "gj_panel_kernel<1,4,1,fin>");
     check_kernel_resources((const void*)gj_mult_kernel, "gj_mult_kernel");
     check_kernel_resources((const void*)gj_update_kernel, "gj_update_kernel");
     check_kernel_resources((const void*)gj_perm_kernel, "gj_perm_kernel");
@@ -578,15 +581,23 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     clear_graphs(M);  // captured sweeps read the operator arenas reallocated below
     M->solver_type = c.solver_type;
     setup_leaf_classes(M, c);
-    // arenas and work
-    M->Tar.alloc(std::max<long long>(P.T_elems, 1));
-    M->Sar.alloc(std::max<long long>(P.S_elems, 1));
-    M->Xiar.alloc(std::max<long long>(P.Xi_elems, 1));
-    M->wX.alloc(std::max<long long>(P.work_X, 1));
-    M->gjd.alloc(std::max<long long>(P.gj_d, 1));
-    M->gji.alloc(std::max<long long>(P.gj_i, 1));
-    M->info.alloc(std::max(P.ninternal, 1));
-    plan_build(M);
+    // arenas and work (kept across builds of the mesh: their sizes depend on the tree only)
+    M->Tar.ensure(std::max<long long>(P.T_elems, 1));
+    M->Sar.ensure(std::max<long long>(P.S_elems, 1));
+    M->Xiar.ensure(std::max<long long>(P.Xi_elems, 1));
+    M->wX.ensure(std::max<long long>(P.work_X, 1));
+    M->gjd.ensure(std::max<long long>(P.gj_d, 1));
+    M->gji.ensure(std::max<long long>(P.gj_i, 1));
+    M->info.ensure(std::max(P.ninternal, 1));
+    // the merge descriptors depend only on these addresses and on the leaf DtN each leaf
+    // reads: planned again only when one of them moved
+    std::vector<const void*> sig = {M->Tar.p, M->Sar.p, M->Xiar.p, M->wX.p, M->gjd.p, M->gji.p, M->info.p};
+    for (size_t li = 0; li < P.leaves.size(); li++) sig.push_back(leaf_T_ptr(M, (int)li));
+    if (sig != M->plan_sig) {
+        M->plan_sig.clear();
+        plan_build(M);
+        M->plan_sig = sig;
+    }
     if (M->variable) {
         // block-tridiagonal factors, then the DtN of every leaf (buildD2N) in chunks of leaves
         const int nl = (int)P.leaves.size();
@@ -611,6 +622,7 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
         launch_patch_fd(a, N <= 16 ? 1 : 2, patch_default_blocks(), s);
     }
     if (sync_check()) EF_HIP(hipStreamSynchronize(s));
+    if (M->bstore.pending()) throw EfError(EF_ERR_INTERNAL, "merge descriptors staged but not uploaded");
     // merges, deepest level first (post-order dependencies)
     EF_HIP(hipMemsetAsync(M->info.p, 0, sizeof(int) * std::max(P.ninternal, 1), s));
     for (size_t gi = 0; gi < P.groups.size(); gi++) {
@@ -815,19 +827,20 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
         }
         L.cnt = (int)ud.size();
         check_range(V, (size_t)vrows * k * 8, "vector arena");
-        L.up = sp->store.put(ud, st);
-        L.down = sp->store.put(dn, st);
+        L.up = sp->store.stage(ud);
+        L.down = sp->store.stage(dn);
         L.gw = plan_gemm(pw, tn, sp->store, st);
         L.gh = plan_gemm(ph, tn, sp->store, st);
         L.gu = plan_gemm(pu, tn, sp->store, st);
         for (auto& v : us) {
             int nfmax = 0;
             for (auto& x : v) nfmax = std::max(nfmax, x.nfine);
-            L.unc.push_back(sp->store.put(v, st));
+            L.unc.push_back(sp->store.stage(v));
             L.unc_cnt.push_back((int)v.size());
             L.unc_nf.push_back(nfmax);
         }
     }
+    sp->store.flush(st);
     SweepPlan& ref = *sp;
     M->sweeps[k] = std::move(sp);
     return ref;
@@ -873,6 +886,7 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     hipStream_t s = M->stream;
     ensure_vectors(M, k);
     SweepPlan& sp = sweep_plan(M, k);
+    if (sp.store.pending()) throw EfError(EF_ERR_INTERNAL, "sweep descriptors staged but not uploaded");
     check_range(f_dev, (size_t)k * P.leaves.size() * P.N * P.N * 8, "upwards load f");
     if (M->variable && M->packed) M->vz.ensure((size_t)P.leaves.size() * k * P.N * P.N);
     run_sweep(M, 0, k, {f_dev, nullptr, nullptr, nullptr}, M->homogeneous_rhs ? 1 : 0, [&] {
@@ -903,6 +917,7 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
     hipStream_t s = M->stream;
     ensure_vectors(M, k);
     SweepPlan& sp = sweep_plan(M, k);
+    if (sp.store.pending()) throw EfError(EF_ERR_INTERNAL, "sweep descriptors staged but not uploaded");
     const NodeP& root = P.nodes[0];
     const int m = 4 * root.s;
     double* V = M->vec.p;
@@ -967,6 +982,7 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         p.M = m; p.N = k;
         DevStore tmp;
         GemmLaunch g = plan_gemm(std::vector<TreeGemm>{p}, k <= 16 ? 1 : 2, tmp, s);
+        tmp.flush(s);
         run_gemm(g, s);
         int info = 0;
         efk::copy_d2h(&info, M->rootinfo.p, sizeof(int), s);
@@ -1269,6 +1285,44 @@ int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind) {
         EF_LAUNCH_CHECK();
         EF_HIP(hipStreamSynchronize(nullptr));
     } else if (bytes) EF_HIP(hipMemcpy(dst, src, bytes, k));
+    EF_API_END
+}
+
+int ef_debug_invert(ef_ctx ctx, int m, int B, const double* A, double* Ainv, int* ipiv) {
+    EF_API_BEGIN
+    if (!ctx || !A || !Ainv) throw EfError(EF_ERR_ARG, "null argument");
+    if (m < 1 || B < 1) throw EfError(EF_ERR_ARG, "m and B must be >= 1");
+    EF_HIP(hipSetDevice(ctx->device));
+    check_tree_kernels();
+    const size_t mm = (size_t)m * m * B;
+    DevBuf<double> wa, xi, d;
+    DevBuf<int> ii, info;
+    wa.alloc(mm);
+    xi.alloc(mm);
+    d.alloc(gj_work_doubles(m, B));
+    ii.alloc(gj_work_ints(m, B));
+    info.alloc(B);
+    GJWork w;
+    double* dp = d.p;
+    w.wb = dp; dp += mm;
+    w.Pinv = dp; dp += (size_t)B * GJ_NB * GJ_NB;
+    w.Mm = dp;
+    int* ip = ii.p;
+    w.cand = ip; ip += (size_t)B * GJ_MAXCH * GJ_NB;
+    w.cfin = ip; ip += (size_t)B * GJ_NB;
+    w.src = ip; ip += (size_t)B * m;
+    w.ipiv = ip; ip += (size_t)B * m;
+    w.prow = ip;
+    hipStream_t s = ctx->stream;
+    efk::copy_h2d(wa.p, A, mm * sizeof(double), s);
+    EF_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * B, s));
+    run_gj_inverse(wa.p, xi.p, m, B, w, info.p, s);
+    efk::copy_d2h(Ainv, xi.p, mm * sizeof(double), s);
+    if (ipiv) efk::copy_d2h(ipiv, w.ipiv, sizeof(int) * B * m, s);
+    std::vector<int> inf(B);
+    efk::copy_d2h(inf.data(), info.p, sizeof(int) * B, s);
+    for (int b = 0; b < B; b++)
+        if (inf[b]) throw EfError(EF_ERR_SINGULAR, "zero pivot in matrix " + std::to_string(b));
     EF_API_END
 }
 

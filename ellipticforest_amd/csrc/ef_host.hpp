@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -159,26 +160,58 @@ struct DevBuf {
         if (count > n) alloc(count);
     }
     // Host -> device copy ordered on the stream the kernels that read it run on, and
-    // complete before the host vector can go away.
+    // complete before the host vector can go away.  The allocation is kept when it is
+    // large enough (n is the capacity).
     void upload(const std::vector<T>& v, hipStream_t s) {
-        alloc(v.size());
+        ensure(v.size());
         if (v.empty()) return;
         copy_h2d(p, v.data(), v.size() * sizeof(T), s);
     }
 };
 
-// Device allocations kept together (descriptor tables of one plan).
+// Descriptor tables of one plan, packed into a few device blocks.  stage() copies a table
+// into the host mirror of the current block and returns its device address at once;
+// flush() uploads everything staged since the last flush (one copy per block).  Nothing a
+// kernel reads from here is valid on the device before flush().
 struct DevStore {
-    std::vector<std::unique_ptr<DevBuf<char>>> bufs;
+    struct Block {
+        DevBuf<char> dev;
+        std::vector<char> host;
+        size_t used = 0, flushed = 0;
+    };
+    static constexpr size_t BLOCK = size_t(1) << 20, ALIGN = 256;
+    std::vector<std::unique_ptr<Block>> blocks;
+    std::vector<std::unique_ptr<DevBuf<char>>> bufs;  // device-only work (alloc_doubles)
     template <class T>
-    T* put(const std::vector<T>& v, hipStream_t s) {
+    T* stage(const std::vector<T>& v) {
         if (v.empty()) return nullptr;
-        auto b = std::make_unique<DevBuf<char>>();
-        b->alloc(v.size() * sizeof(T));
-        copy_h2d(b->p, v.data(), v.size() * sizeof(T), s);
-        T* out = (T*)b->p;
-        bufs.push_back(std::move(b));
-        return out;
+        const size_t bytes = v.size() * sizeof(T);
+        Block* b = blocks.empty() ? nullptr : blocks.back().get();
+        size_t off = b ? (b->used + ALIGN - 1) / ALIGN * ALIGN : 0;
+        if (!b || off + bytes > b->host.size()) {
+            auto nb = std::make_unique<Block>();
+            const size_t cap = std::max(BLOCK, (bytes + ALIGN - 1) / ALIGN * ALIGN);
+            nb->dev.alloc(cap);
+            nb->host.resize(cap);
+            b = nb.get();
+            blocks.push_back(std::move(nb));
+            off = 0;
+        }
+        std::memcpy(b->host.data() + off, v.data(), bytes);
+        b->used = off + bytes;
+        return (T*)(b->dev.p + off);
+    }
+    bool pending() const {
+        for (auto& b : blocks)
+            if (b->used > b->flushed) return true;
+        return false;
+    }
+    void flush(hipStream_t s) {
+        for (auto& b : blocks)
+            if (b->used > b->flushed) {
+                copy_h2d(b->dev.p + b->flushed, b->host.data() + b->flushed, b->used - b->flushed, s);
+                b->flushed = b->used;
+            }
     }
     double* alloc_doubles(size_t n) {
         auto b = std::make_unique<DevBuf<char>>();
@@ -187,7 +220,10 @@ struct DevStore {
         bufs.push_back(std::move(b));
         return out;
     }
-    void clear() { bufs.clear(); }
+    void clear() {
+        blocks.clear();
+        bufs.clear();
+    }
 };
 
 // Static resources of a kernel against the invariants of the tree path: no scratch, static

@@ -246,151 +246,257 @@ __global__ void __launch_bounds__(256) tree_gemm_reduce_kernel(const TreeGemm* _
 // of the step, and the same workgroup forms P^-1 of the pivot block from the rows it published
 // (the multiplier of each eliminated entry is kept in place, so the pivot row of column J holds
 // row J of [L11 \ U11]), the LAPACK-order interchanges (ipiv) and the position -> row map (src).
-// Register indices are compile-time constants (one instantiation of panel_column per column).
-constexpr int GJ_HALF = GJ_NB / 2;
-constexpr int GJ_MAXWAVES = 16;  // 1024 threads: GJ_CH rows
+// Tournament LU of the panel (columns k .. k+nb) with partial pivoting, fused with the step's
+// pivot bookkeeping.  One workgroup of 64 WAVES threads holds up to 64 WAVES RPL panel rows,
+// RPL whole rows (32 columns) per lane in registers, row q NT + tid of the chunk in slot q.
+// STAGE 0: the rows of chunk blockIdx.x (positions k + ch * rows + i); STAGE 1: the candidates
+// of every chunk (a.cand).  Ties go to the lower position, as idamax takes the first maximum.
+// FIN = false: the nb rows pivoted on, in order, are the chunk's candidates.  FIN = true (one
+// chunk, or stage 1): they are the pivots of the step, and the same workgroup forms P^-1 of
+// the pivot block, the LAPACK-order interchanges (ipiv) and the position -> row map (src).
+// Per column (all 32 unrolled, so register indices are compile-time constants): each lane's
+// best row, one DPP argmax per wave, the wave's best row published to LDS with its value, one
+// barrier, then every thread combines the waves' candidates and reads the pivot row from the
+// winner's slot (slots alternate by column parity, so column J + 1 never overwrites what a
+// slower wave still reads for column J).
+constexpr int GJ_NONE = 0x7fffffff;
+template <int RPL>
 struct PanelRows {
-    double v[GJ_HALF];  // columns 16 h .. 16 h + 15 of the row (multipliers left of the pivot column)
-    int pos;            // position of the row in the matrix
-    int h;              // which half of the row
-    bool act;           // row present and not yet pivoted on
+    double v[RPL][GJ_NB];  // the rows (multipliers left of the pivot column)
+    int pos[RPL];          // position of each row at the start of the step (its name)
+    int cur[RPL];          // its position now, after the step's interchanges so far
+    bool act[RPL];         // row present and not yet pivoted on
 };
+// Pivot order key: current position, then name (both < GJ_MAXM = 2^12).
+__device__ inline int piv_key(int cur, int pos) { return (cur << 12) | pos; }
+template <int WAVES, bool FIN>
 struct PanelShared {
-    double red_v[GJ_MAXWAVES];
-    int red_p[GJ_MAXWAVES];
-    double prow[GJ_NB];
-    int piv[GJ_NB];                 // pivot position chosen for each column (-1: none)
+    union {
+        double cand[2][WAVES][GJ_NB];   // each wave's best row, by column parity
+        double A[2][GJ_NB][GJ_NB];      // FIN: in-place Gauss-Jordan on P, ping-pong
+    } u;
+    double red_v[2][WAVES];
+    int red_p[2][WAVES];
+    int piv[GJ_NB];  // pivot position chosen for each column (-1: none)
     int top_row[GJ_NB], opos[GJ_NB], orow[GJ_NB];
     int nout;
 };
-constexpr int GJ_NONE = 0x7fffffff;
 
-template <int J, int WAVES, bool FIN>
-__device__ __attribute__((always_inline)) inline void panel_column(PanelRows& R, PanelShared& sh, int nbb) {
-    constexpr int HJ = J / GJ_HALF, JC = J % GJ_HALF;
+// Wave reductions into lane 63 through DPP lane shifts: row shifts 1, 2, 4, 8 leave each
+// 16-lane row's result in its lane 15; row_bcast:15 folds rows 0 -> 1 and 2 -> 3, row_bcast:31
+// folds lane 31 into rows 2 and 3.  Lanes without a source lane (or in rows outside the row
+// mask) see the identity.
+template <int CTRL, int ROWMASK>
+__device__ inline double max_dpp(double v) {  // identity -1 (every candidate is >= 0)
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(-1.0), __double2hiint(v), CTRL, ROWMASK, 0xf, false);
+    return fmax(v, __hiloint2double(hi, lo));
+}
+template <int CTRL, int ROWMASK>
+__device__ inline int min_dpp(int p) {  // identity GJ_NONE
+    return min(p, __builtin_amdgcn_update_dpp(GJ_NONE, p, CTRL, ROWMASK, 0xf, false));
+}
+__device__ inline double wave_max(double v) {
+    v = max_dpp<0x111, 0xf>(v);
+    v = max_dpp<0x112, 0xf>(v);
+    v = max_dpp<0x114, 0xf>(v);
+    v = max_dpp<0x118, 0xf>(v);
+    v = max_dpp<0x142, 0xa>(v);
+    v = max_dpp<0x143, 0xc>(v);
+    return v;
+}
+__device__ inline int wave_min(int p) {
+    p = min_dpp<0x111, 0xf>(p);
+    p = min_dpp<0x112, 0xf>(p);
+    p = min_dpp<0x114, 0xf>(p);
+    p = min_dpp<0x118, 0xf>(p);
+    p = min_dpp<0x142, 0xa>(p);
+    p = min_dpp<0x143, 0xc>(p);
+    return p;
+}
+__device__ inline double readlane_dyn(double x, int src) { return readlane_f64(x, src); }
+
+// Column J: the pivot is the largest |entry| among the active rows, the lowest current position
+// among equal ones (idamax takes the first maximum of the interchanged column): a wave max of
+// the values, then a wave min of the (position, name) keys of the lanes that hold that maximum.
+// After the choice the row at position k + J takes the pivot row's position (the interchange).
+// One wave reads the pivot row straight from its owner lane (readlane); several waves publish
+// their candidate rows in LDS.
+template <int J, int WAVES, int RPL, int STAGE, bool FIN>
+__device__ __attribute__((always_inline)) inline void panel_column(PanelRows<RPL>& R, PanelShared<WAVES, FIN>& sh,
+                                                                   int base, int top0) {
+    constexpr int BUF = J & 1;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (J >= nbb) {  // uniform
-        if (tid == 0) sh.piv[J] = -1;
-        return;
-    }
-    const bool own = R.h == HJ;  // this lane holds column J of its row
-    double bv = -1.0;
-    int bp = GJ_NONE;
-    if (R.act && own) { bv = fabs(R.v[JC]); bp = R.pos; }
+    double lv = -1.0;  // the lane's best row
+    int lk = GJ_NONE;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int op = __shfl_xor(bp, off);
-        if (ov > bv || (ov == bv && op < bp)) { bv = ov; bp = op; }
+    for (int q = 0; q < RPL; q++) {
+        const double x = R.act[q] ? fabs(R.v[q][J]) : -1.0;
+        const int key = piv_key(R.cur[q], R.pos[q]);
+        const bool take = x > lv || (x == lv && R.act[q] && key < lk);
+        lk = take ? key : lk;
+        lv = take ? x : lv;
     }
-    if (lane == 0) { sh.red_v[wave] = bv; sh.red_p[wave] = bp; }
-    __syncthreads();
-    // every thread finishes the reduction itself (no second barrier)
-    double best = sh.red_v[0];
-    int pp = sh.red_p[0];
+    const double wmax = readlane_f64(wave_max(lv), 63);
+    const int wkey = __builtin_amdgcn_readlane(wave_min((lv == wmax) ? lk : GJ_NONE), 63);
+    const int wbp = (wkey == GJ_NONE) ? GJ_NONE : (wkey & 4095);
+    int pkey = wkey;
+    double pr[GJ_NB];  // pivot row, columns J..
+    int pp = wbp;
+    if (WAVES == 1) {
+        if (tid == 0) sh.piv[J] = (pp == GJ_NONE) ? -1 : pp;
+        if (pp == GJ_NONE) return;  // uniform: no row left in the panel
+        // STAGE 0: row named base + q * 64 + lane; STAGE 1 (several waves) never gets here
+        const int off = pp - base, ol = off & 63, oq = off >> 6;
 #pragma unroll
-    for (int w2 = 1; w2 < WAVES; w2++) {
-        const double v2 = sh.red_v[w2];
-        const int p2 = sh.red_p[w2];
-        if (v2 > best || (v2 == best && p2 < pp)) { best = v2; pp = p2; }
-    }
-    if (R.act && R.pos == pp) {  // both lanes of the pivot row publish their halves
+        for (int c = J; c < GJ_NB; c++) {
+            double x = R.v[0][c];
 #pragma unroll
-        for (int c = 0; c < GJ_HALF; c++) sh.prow[GJ_HALF * R.h + c] = R.v[c];
-        R.act = false;
-    }
-    if (tid == 0) sh.piv[J] = (pp == GJ_NONE) ? -1 : pp;
-    __syncthreads();
-    // multiplier of the row: computed by the lane holding column J (and kept there), passed to
-    // its partner lane
-    double lown = 0.0;
-    if (pp != GJ_NONE && R.act && own) {
-        const double piv = sh.prow[J];
-        lown = R.v[JC] * ((piv != 0.0) ? 1.0 / piv : 0.0);
-        R.v[JC] = lown;
-    }
-    const double lpart = __shfl_xor(lown, 1);
-    const double l = own ? lown : lpart;
-    if (pp != GJ_NONE && R.act) {
-#pragma unroll
-        for (int c = 0; c < GJ_HALF; c++) {
-            const int col = GJ_HALF * R.h + c;
-            if (col > J) R.v[c] -= l * sh.prow[col];
+            for (int q = 1; q < RPL; q++) x = (oq == q) ? R.v[q][c] : x;
+            pr[c] = readlane_dyn(x, ol);
         }
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL; q++)
+            if (R.act[q] && R.pos[q] == wbp) {  // the wave's best row: columns J.. are what the pivot row needs
+#pragma unroll
+                for (int c = J; c < GJ_NB; c++) sh.u.cand[BUF][wave][c] = R.v[q][c];
+            }
+        if (lane == 63) { sh.red_v[BUF][wave] = wmax; sh.red_p[BUF][wave] = wkey; }
+        __syncthreads();
+        double best = sh.red_v[BUF][0];
+        int ww = 0;
+        pkey = sh.red_p[BUF][0];
+#pragma unroll
+        for (int w2 = 1; w2 < WAVES; w2++) {
+            const double v2 = sh.red_v[BUF][w2];
+            const int p2 = sh.red_p[BUF][w2];
+            const bool take = v2 > best || (v2 == best && p2 < pkey);
+            best = take ? v2 : best;
+            pkey = take ? p2 : pkey;
+            ww = take ? w2 : ww;
+        }
+        pp = (pkey == GJ_NONE) ? GJ_NONE : (pkey & 4095);
+        if (tid == 0) sh.piv[J] = (pp == GJ_NONE) ? -1 : pp;
+        if (pp == GJ_NONE) return;  // uniform: no row left in the panel
+        const double* prow = sh.u.cand[BUF][ww];
+#pragma unroll
+        for (int c = J; c < GJ_NB; c++) pr[c] = prow[c];
     }
-    // the next column writes red_* only after its barrier, which every thread reaches after
-    // this update, and prow only after that barrier
+    const int pcur = pkey >> 12, top = top0 + J;
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        if (R.act[q] && R.pos[q] == pp) R.act[q] = false;
+        else if (R.act[q] && R.cur[q] == top) R.cur[q] = pcur;  // the interchange
+    }
+    const double piv = pr[J];
+    const double rp = (piv != 0.0) ? 1.0 / piv : 0.0;
+    // branch-free: rows that are not active (pivoted on, absent) take l = 0 and keep their values
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const double l = R.act[q] ? R.v[q][J] * rp : 0.0;  // multiplier, kept in place
+        R.v[q][J] = R.act[q] ? l : R.v[q][J];
+#pragma unroll
+        for (int c = J + 1; c < GJ_NB; c++) R.v[q][c] -= l * pr[c];
+    }
 }
 
-template <int WAVES, bool FIN, int... J>
-__device__ __attribute__((always_inline)) inline void panel_columns(std::integer_sequence<int, J...>, PanelRows& R,
-                                                                    PanelShared& sh, int nbb) {
-    (panel_column<J, WAVES, FIN>(R, sh, nbb), ...);
+template <int WAVES, int RPL, int STAGE, bool FIN, int... J>
+__device__ __attribute__((always_inline)) inline void panel_columns(std::integer_sequence<int, J...>, PanelRows<RPL>& R,
+                                                                    PanelShared<WAVES, FIN>& sh, int nbb, int base,
+                                                                    int top0) {
+    // columns past nbb (a last step narrower than GJ_NB) are skipped; nbb is uniform
+    ((J < nbb ? panel_column<J, WAVES, RPL, STAGE, FIN>(R, sh, base, top0) : void()), ...);
 }
 
-template <int STAGE, int WAVES, bool FIN>
+template <int STAGE, int WAVES, int RPL, bool FIN>
 __global__ void __launch_bounds__(64 * WAVES) gj_panel_kernel(GJArgs a) {
-    __shared__ PanelShared sh;
-    constexpr int NT = 64 * WAVES, ROWS = NT / 2;
+    __shared__ PanelShared<WAVES, FIN> sh;
+    constexpr int NT = 64 * WAVES, ROWS = NT * RPL;
     const int b = blockIdx.y, ch = blockIdx.x;
     const int m = a.m, k = a.k, nbb = min(GJ_NB, m - k);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double* W = a.Wa + (long long)b * m * m;
-    PanelRows R;
-    R.h = tid & 1;
-    const int i = tid >> 1;  // row of the chunk
-    if (STAGE == 0) {
-        R.pos = k + ch * ROWS + i;
-        R.act = R.pos < m;
-    } else {
-        const int p = (i < a.nch * GJ_NB) ? a.cand[(long long)b * GJ_MAXCH * GJ_NB + i] : -1;
-        R.act = p >= k && p < m;
-        R.pos = R.act ? p : GJ_NONE;
-    }
-    const double* rowp = W + (long long)(R.act ? R.pos : k) * m + k + GJ_HALF * R.h;
+    PanelRows<RPL> R;
 #pragma unroll
-    for (int c = 0; c < GJ_HALF; c++) {
-        R.v[c] = 0.0;
-        if (R.act && GJ_HALF * R.h + c < nbb) R.v[c] = rowp[c];  // columns k + 16h + c < m of a row of the matrix
+    for (int q = 0; q < RPL; q++) {
+        const int i = q * NT + tid;  // row of the chunk
+        if (STAGE == 0) {
+            R.pos[q] = k + ch * ROWS + i;
+            R.act[q] = R.pos[q] < m;
+        } else {
+            const int p = (i < a.nch * GJ_NB) ? a.cand[(long long)b * GJ_MAXCH * GJ_NB + i] : -1;
+            R.act[q] = p >= k && p < m;
+            R.pos[q] = R.act[q] ? p : GJ_NONE;
+        }
+        R.cur[q] = R.pos[q];
+        const double* rowp = W + (long long)(R.act[q] ? R.pos[q] : k) * m + k;
+#pragma unroll
+        for (int c = 0; c < GJ_NB; c++)  // columns k + c < m of a row of the matrix
+            R.v[q][c] = (R.act[q] && c < nbb) ? rowp[c] : 0.0;
     }
-    panel_columns<WAVES, FIN>(std::make_integer_sequence<int, GJ_NB>{}, R, sh, nbb);
+    if (tid < GJ_NB) sh.piv[tid] = -1;
+    static_assert(WAVES > 1 || STAGE == 0, "one-wave panels hold chunk rows in position order");
+    // position k + J is the top of column J (a chunk's own first rows in the tournament stage)
+    panel_columns<WAVES, RPL, STAGE, FIN>(std::make_integer_sequence<int, GJ_NB>{}, R, sh, nbb, k + ch * ROWS,
+                                          STAGE == 0 ? k + ch * ROWS : k);
     __syncthreads();
     if (!FIN) {
-        if (tid < GJ_NB) a.cand[((long long)b * GJ_MAXCH + ch) * GJ_NB + tid] = (tid < nbb) ? sh.piv[tid] : -1;
+        if (tid < GJ_NB) a.cand[((long long)b * GJ_MAXCH + ch) * GJ_NB + tid] = sh.piv[tid];
         return;
     }
+    // P^-1: in-place Gauss-Jordan on P in LDS by the whole workgroup, P = the pivot rows (in
+    // pivot order) as they were at the start of the step.  Step j leaves column j holding column j
+    // of the inverse: rp = 1/p_jj on the diagonal, -a_rj rp elsewhere (what [P | I] elimination
+    // puts in the identity's column j), and row j scaled by rp.  The order is the one partial
+    // pivoting chose, so the multipliers below the diagonal are bounded by 1 and no further
+    // pivoting is needed.  Beyond nbb the block is the identity.
+    for (int e = tid; e < GJ_NB * GJ_NB; e += NT) {
+        const int r = e / GJ_NB, c = e % GJ_NB;
+        double x = (r == c) ? 1.0 : 0.0;
+        const int pr = sh.piv[r];
+        if (r < nbb && c < nbb && pr >= 0) x = W[(long long)pr * m + k + c];
+        sh.u.A[0][r][c] = x;
+    }
+    __syncthreads();
+    int zero = 0;
+    for (int j = 0; j < nbb; j++) {
+        const int cur = j & 1, nxt = cur ^ 1;
+        const double pv = sh.u.A[cur][j][j];
+        if (zero == 0 && (pv == 0.0 || sh.piv[j] < 0)) zero = j + 1;
+        const double rp = (pv != 0.0) ? 1.0 / pv : 0.0;
+        // NT is a multiple of GJ_NB: each thread keeps one column c and rows r0 + i NT / GJ_NB;
+        // every read of the step is issued before its writes
+        constexpr int RR = GJ_NB * GJ_NB / NT;
+        const int c = tid % GJ_NB, r0 = tid / GJ_NB;
+        const double rowj = (c == j) ? rp : sh.u.A[cur][j][c] * rp;
+        double arj[RR], arc[RR];
+#pragma unroll
+        for (int i = 0; i < RR; i++) {
+            const int r = r0 + i * (NT / GJ_NB);
+            arj[i] = sh.u.A[cur][r][j];
+            arc[i] = sh.u.A[cur][r][c];
+        }
+#pragma unroll
+        for (int i = 0; i < RR; i++) {
+            const int r = r0 + i * (NT / GJ_NB);
+            double out;
+            if (r == j) out = rowj;
+            else if (c == j) out = -arj[i] * rp;
+            else out = arc[i] - arj[i] * rowj;
+            sh.u.A[nxt][r][c] = out;
+        }
+        __syncthreads();
+    }
+    {
+        const int fin = nbb & 1;
+        double* Pinv = a.Pinv + (long long)b * GJ_NB * GJ_NB;
+        for (int e = tid; e < GJ_NB * GJ_NB; e += NT) {
+            const int r = e / GJ_NB, c = e % GJ_NB;
+            Pinv[e] = (r < nbb && c < nbb) ? sh.u.A[fin][r][c] : 0.0;
+        }
+    }
     if (wave == 0) {
-        // P^-1: Gauss-Jordan on [A11 | I] by one wave, lane c holding column c of A11 (its rows
-        // in pivot order: the pivot rows' panel values at the start of the step) and lane 32 + c
-        // column c of I, in registers; pivots and multipliers broadcast by readlane.  The order
-        // is the one partial pivoting chose, so the multipliers are bounded by 1 and no further
-        // pivoting is needed.  Beyond nbb the block is the identity.
-        const int c = lane & (GJ_NB - 1);
-        const bool ident = lane >= GJ_NB;
-        double v[GJ_NB];
-#pragma unroll
-        for (int r = 0; r < GJ_NB; r++) {
-            double x = (r == c) ? 1.0 : 0.0;
-            const int pr = sh.piv[r];
-            if (!ident && r < nbb && c < nbb && pr >= 0) x = W[(long long)pr * m + k + c];
-            v[r] = x;
-        }
-        int zero = 0;
-#pragma unroll
-        for (int j = 0; j < GJ_NB; j++) {
-            const double pv = readlane_f64(v[j], j);
-            if (zero == 0 && j < nbb && (pv == 0.0 || sh.piv[j] < 0)) zero = j + 1;
-            const double rowj = v[j] * ((pv != 0.0) ? 1.0 / pv : 0.0);
-#pragma unroll
-            for (int r = 0; r < GJ_NB; r++)
-                if (r != j) v[r] -= readlane_f64(v[r], j) * rowj;
-            v[j] = rowj;
-        }
-        if (ident) {
-            double* Pinv = a.Pinv + (long long)b * GJ_NB * GJ_NB;
-#pragma unroll
-            for (int r = 0; r < GJ_NB; r++) Pinv[r * GJ_NB + c] = (r < nbb && c < nbb) ? v[r] : 0.0;
-        }
         // interchanges (LAPACK order: position k+j with the chosen row's current position)
         int top = k + lane;      // row now at position k + lane (lanes < nbb)
         int op = -1, orw = -1;   // entry `lane` of the positions beyond the panel that changed row
@@ -448,11 +554,12 @@ __global__ void __launch_bounds__(64 * WAVES) gj_panel_kernel(GJArgs a) {
     }
 }
 
-template __global__ void gj_panel_kernel<0, 2, true>(GJArgs);
-template __global__ void gj_panel_kernel<0, 4, true>(GJArgs);
-template __global__ void gj_panel_kernel<0, 16, true>(GJArgs);
-template __global__ void gj_panel_kernel<0, 16, false>(GJArgs);
-template __global__ void gj_panel_kernel<1, 16, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 1, 1, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 1, 2, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 2, 2, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 4, 2, true>(GJArgs);
+template __global__ void gj_panel_kernel<0, 4, 2, false>(GJArgs);
+template __global__ void gj_panel_kernel<1, 4, 1, true>(GJArgs);
 
 // Multipliers of the step: M(i, :) = P^-1(j, :) - e_j on pivot position k + j, -A(src(i), panel) P^-1 elsewhere.
 __global__ void __launch_bounds__(256) gj_mult_kernel(GJArgs a) {
@@ -816,10 +923,10 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
     estart[probs.size()] = elems;
     g.nprob = (int)probs.size();
     g.nwaves = (int)total;
-    g.probs = store.put(probs, s);
-    g.start = store.put(start, s);
+    g.probs = store.stage(probs);
+    g.start = store.stage(start);
     if (any_split) {
-        g.estart = store.put(estart, s);
+        g.estart = store.stage(estart);
         g.nelem = elems;
     }
     return g;
@@ -870,15 +977,18 @@ void run_gj_inverse(double* wa, double* xi, int m, int B, const GJWork& w, int* 
         a.Wa = cur; a.Wb = nxt; a.k = k;
         a.nch = (m - k + GJ_CH - 1) / GJ_CH;
         const int rows = m - k;
+        // rows per workgroup: 64 WAVES RPL (chunks of GJ_CH = 512 rows; up to 256 candidates)
         if (rows <= 64)
-            EF_LAUNCH("gj_panel_kernel<0,2,fin>", (gj_panel_kernel<0, 2, true>), dim3(1, B), dim3(128), s, a);
+            EF_LAUNCH("gj_panel_kernel<0,1,1,fin>", (gj_panel_kernel<0, 1, 1, true>), dim3(1, B), dim3(64), s, a);
         else if (rows <= 128)
-            EF_LAUNCH("gj_panel_kernel<0,4,fin>", (gj_panel_kernel<0, 4, true>), dim3(1, B), dim3(256), s, a);
+            EF_LAUNCH("gj_panel_kernel<0,1,2,fin>", (gj_panel_kernel<0, 1, 2, true>), dim3(1, B), dim3(64), s, a);
+        else if (rows <= 256)
+            EF_LAUNCH("gj_panel_kernel<0,2,2,fin>", (gj_panel_kernel<0, 2, 2, true>), dim3(1, B), dim3(128), s, a);
         else if (a.nch == 1)
-            EF_LAUNCH("gj_panel_kernel<0,16,fin>", (gj_panel_kernel<0, 16, true>), dim3(1, B), dim3(1024), s, a);
+            EF_LAUNCH("gj_panel_kernel<0,4,2,fin>", (gj_panel_kernel<0, 4, 2, true>), dim3(1, B), dim3(256), s, a);
         else {
-            EF_LAUNCH("gj_panel_kernel<0,16>", (gj_panel_kernel<0, 16, false>), dim3(a.nch, B), dim3(1024), s, a);
-            EF_LAUNCH("gj_panel_kernel<1,16,fin>", (gj_panel_kernel<1, 16, true>), dim3(1, B), dim3(1024), s, a);
+            EF_LAUNCH("gj_panel_kernel<0,4,2>", (gj_panel_kernel<0, 4, 2, false>), dim3(a.nch, B), dim3(256), s, a);
+            EF_LAUNCH("gj_panel_kernel<1,4,1,fin>", (gj_panel_kernel<1, 4, 1, true>), dim3(1, B), dim3(256), s, a);
         }
         EF_LAUNCH("gj_mult_kernel", gj_mult_kernel, dim3(blocks_for((long long)m * GJ_NB), B), dim3(256), s, a);
         EF_LAUNCH("gj_update_kernel", gj_update_kernel, dim3(blocks_for((long long)tpr * tpr, 4), B), dim3(256), s, a);

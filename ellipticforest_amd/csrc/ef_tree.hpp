@@ -112,7 +112,7 @@ struct GJArgs {
     int m, B, k, nch;
 };
 
-template <int STAGE, int WAVES, bool FIN>
+template <int STAGE, int WAVES, int RPL, bool FIN>
 __global__ void gj_panel_kernel(GJArgs a);
 __global__ void gj_mult_kernel(GJArgs a);
 __global__ void gj_update_kernel(GJArgs a);
@@ -166,8 +166,8 @@ struct GemmLaunch {
     long long nelem = 0;
 };
 // Checks every problem (extents inside their allocations, alignment), allocates the partials of
-// split problems, uploads problems and wave offsets into `store` (device memory the caller keeps
-// alive) and returns the launch.  tn = 1 (16 x 16 blocks) suits N <= 16, tn = 2 (32 x 32) the rest.
+// split problems, stages problems and wave offsets in `store` (device memory the caller keeps
+// alive; valid on the device after store.flush) and returns the launch.  tn = 1 (16 x 16 blocks) suits N <= 16, tn = 2 (32 x 32) the rest.
 struct DevStore;
 GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s);
 void run_gemm(const GemmLaunch& g, hipStream_t s);

@@ -144,6 +144,15 @@ int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_device_free(ef_ctx ctx, void* ptr);
 int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
 
+/* Diagnostic: the batched inversion every merge uses (blocked Gauss-Jordan with partial
+ * pivoting, tournament pivoting over 512-row chunks), applied to B host matrices of order m
+ * (row-major, B x m x m).  Ainv receives the inverses; ipiv (B x m, may be NULL) the
+ * LAPACK-order row interchanges (0-based: position c was exchanged with ipiv[c]), equal to
+ * dgetrf's wherever no step spans more than one chunk (m <= 512).  Replaces the LU of the
+ * interface system in mergeX_/mergeS_ (HPSAlgorithm.hpp:906-929) for testing only.
+ * EF_ERR_SINGULAR when a pivot is zero. */
+int ef_debug_invert(ef_ctx ctx, int m, int B, const double* A, double* Ainv, int* ipiv);
+
 /* Stage timers measured with HIP events on the mesh's stream, in milliseconds, of the
  * last build, upwards and solve calls: ms[0] build-stage, ms[1] upwards-stage,
  * ms[2] solve-stage (the timer names of HPSAlgorithm.hpp:125, 183, 295). */

@@ -147,6 +147,23 @@ inline emu_u2 __builtin_amdgcn_permlane32_swap(unsigned old, unsigned src, bool,
     const unsigned from_old = (unsigned)__shfl((double)old, l ^ 32);
     return {{l >= 32 ? from_src : old, l < 32 ? from_old : src}};
 }
+// v_mov_b32 with a DPP source (the controls the product uses): quad_perm, row_shl/row_shr 1..15,
+// row_bcast:15 (lane 15 of the previous row) and row_bcast:31 (lane 31, rows 2 and 3).  Lanes
+// in rows or banks masked off keep `old`; lanes without a source lane get 0 with bound_ctrl, else `old`.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
+    const int l = (int)(threadIdx.x % 64), i = l & 15, r = l >> 4;
+    int s = -1;
+    if (ctrl >= 0 && ctrl <= 0xff) s = (l & ~3) | ((ctrl >> (2 * (l & 3))) & 3);
+    else if (ctrl >= 0x101 && ctrl <= 0x10f) { if (i + (ctrl - 0x100) < 16) s = l + (ctrl - 0x100); }
+    else if (ctrl >= 0x111 && ctrl <= 0x11f) { if (i - (ctrl - 0x110) >= 0) s = l - (ctrl - 0x110); }
+    else if (ctrl == 0x142) { if (r >= 1) s = 16 * r - 1; }
+    else if (ctrl == 0x143) { if (r >= 2) s = 31; }
+    else { fprintf(stderr, "emulator: DPP control 0x%x not modelled\n", ctrl); abort(); }
+    const int v = (int)(long long)__shfl((double)src, s < 0 ? l : s);  // every lane takes part
+    if (!((row_mask >> r) & 1) || !((bank_mask >> (i >> 2)) & 1)) return old;
+    if (s < 0) return bound_ctrl ? 0 : old;
+    return v;
+}
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 // a scheduling barrier on the GPU (lanes of a wave run in lockstep); here the lanes meet
 #define __builtin_amdgcn_wave_barrier() emu_wave_sync()

@@ -90,3 +90,12 @@ def test_emu_graph_replay(emu_ctx):
     capture is open) replay bitwise like the eager launches."""
     leaves = omesh.refine_by_function(lambda x, y: (x < 0.2) & (y < 0.2), 4, 4, 0, 1, 0, 1, 0, 3)
     gp.graph_replay_case(emu_ctx, leaves, 4, (0, 1, 0, 1), k=2)
+
+
+@pytest.mark.parametrize("m,B,integer", [(16, 3, False), (64, 2, True), (96, 1, False)])
+def test_emu_batched_inverse_pivots(emu_ctx, m, B, integer):
+    gp.inverse_case(emu_ctx, m, B, seed=m + B, integer=integer)
+
+
+def test_emu_batched_inverse_pivot_ties(emu_ctx):
+    gp.inverse_case(emu_ctx, 64, 3, seed=5, sparse=True)

@@ -315,3 +315,55 @@ def test_graph_replay_matches_eager(ctx):
     say("graph replay, adaptive N=8 mesh with coarsening")
     leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 8, 8, -10, 10, -10, 10, 0, 5)
     graph_replay_case(ctx, leaves, 8, (-10, 10, -10, 10))
+
+
+def inverse_case(ctx, m, B, seed, integer=False, check_pivots=True, sparse=False):
+    """ef_debug_invert (the merges' batched Gauss-Jordan inversion) on random matrices: the
+    inverse against numpy's, and the row interchanges against LAPACK dgetrf's (scipy
+    lu_factor), index for index, when no step spans more than one 512-row chunk.  Integer
+    matrices make ties in |pivot| common: the first maximum of the interchanged column wins,
+    as in idamax.  Sparse integer matrices keep ties alive after elimination, including ties
+    with rows that an earlier interchange of the same step moved."""
+    import scipy.linalg as sla
+    from ellipticforest_amd import _lib
+    rng = np.random.default_rng(seed)
+    if sparse:
+        A = np.empty((B, m, m))
+        for b in range(B):
+            while True:
+                A[b] = rng.integers(-3, 4, (m, m)) * (rng.random((m, m)) < 0.15)
+                if np.linalg.matrix_rank(A[b]) == m:
+                    break
+    else:
+        A = (rng.integers(-3, 4, (B, m, m)).astype(np.float64) if integer else rng.standard_normal((B, m, m)))
+    Ainv = np.empty_like(A)
+    ipiv = np.empty((B, m), np.int32)
+    _lib.check(ctx.L.ef_debug_invert(ctx.h, m, B, A.ctypes.data_as(_lib._dp), Ainv.ctypes.data_as(_lib._dp),
+                                     ipiv.ctypes.data_as(_lib._ip)), ctx.L)
+    for b in range(B):
+        ref = np.linalg.inv(A[b])
+        cond = np.linalg.cond(A[b])
+        assert rel(Ainv[b], ref) < 1e-15 * cond * m, (m, b, rel(Ainv[b], ref), cond)
+        if check_pivots:
+            assert np.array_equal(ipiv[b], sla.lu_factor(A[b])[1]), (m, b)
+
+
+@pytest.mark.parametrize("m,B,integer", [(16, 5, False), (64, 40, False), (64, 7, True), (96, 3, True),
+                                         (200, 2, False), (512, 1, False), (512, 2, True)])
+def test_batched_inverse_pivots(ctx, m, B, integer):
+    say(f"batched inverse m={m} B={B} integer={integer}")
+    inverse_case(ctx, m, B, seed=m + B, integer=integer)
+
+
+@pytest.mark.parametrize("m,B", [(64, 12), (128, 4), (256, 2), (480, 1)])
+def test_batched_inverse_pivot_ties(ctx, m, B):
+    say(f"batched inverse with pivot ties m={m} B={B}")
+    inverse_case(ctx, m, B, seed=7 * m + B, sparse=True)
+
+
+@pytest.mark.parametrize("m,B", [(1024, 1), (1056, 2), (2048, 1)])
+def test_batched_inverse_tournament(ctx, m, B):
+    """Steps over more than one chunk choose pivots by tournament: the inverse is checked, the
+    interchanges are not LAPACK's."""
+    say(f"batched inverse (tournament) m={m} B={B}")
+    inverse_case(ctx, m, B, seed=m, check_pivots=False)
