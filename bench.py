@@ -42,6 +42,7 @@ METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds
 TRAFFIC_PER_LAUNCH = 572.9e6          # leaf solves of this workload, 16 right-hand sides
 TRAFFIC_MAP_D2N = 319.0e6             # particular Neumann data, same leaf set and k
 TRAFFIC_VAR_BETA = 1329.8e6           # variable-beta leaf solves, packed symmetric blocks
+TRAFFIC_GEMM_PER_LAUNCH = 33.31e6     # tree_gemm_kernel in the sweep, per launch: profiles/tree_gemm_kernel_adaptive_poisson_sweep.md
 
 
 def log(msg):
@@ -110,6 +111,14 @@ def sweep_bytes(D, N, k):
     return 8.0 * (64.0 * float(np.sum(ns ** 2)) + k * D.nleaves * 2.0 * N * N)
 
 
+def gemm_bytes(D, k):
+    """Algorithmic HBM bytes of the tree GEMMs of one step: per merge the operators X^-1 (16 n^2),
+    the 16 n^2 nonzeros of H and S (32 n^2) read once, and the vectors of its products
+    (w = X^-1 hDiff: 8 n k, h += H w: 32 n k, u = S g + w: 16 n k), FP64."""
+    ns, _ = tree_counts(D)
+    return 8.0 * float(np.sum(64.0 * ns ** 2 + 56.0 * ns * k))
+
+
 def build_flops(D, N, nclasses):
     """Matrix-core work of buildStage.  dense: the reference's count (dgesv of X against the dense
     S_RHS, dense H S, L21 T L12, leaf DtN by 32 N^4), sum 810.67 n^3 + 48 m^3 + 32 N^4 per leaf
@@ -175,6 +184,11 @@ def gpu_run(leaves, N, k, steps, warmup, ref, nlat):
         D.sync()
         lat.append((time.perf_counter() - t1) * 1e3)
     rel1 = float(np.abs(u1.download().reshape(nl, N * N) - ref["u"]).max() / np.abs(ref["u"]).max())
+    # the sweep's dominant kernel: its GEMM launches alone, between HIP events on the mesh's stream
+    import ctypes
+    gms, glaunch = ctypes.c_double(0.0), ctypes.c_int(0)
+    ef._lib.check(ctx.L.ef_mesh_time_sweep_gemms(D.h, k, max(steps, 5), ctypes.byref(gms), ctypes.byref(glaunch)), ctx.L)
+    gbytes = gemm_bytes(D, k)
     nbytes = sweep_bytes(D, N, k)
     dense, executed = build_flops(D, N, len({lv for lv, _, _ in leaves}))  # FISHPACK leaf classes: one per leaf level
     b_s = builds[-1]["event_s"]
@@ -182,8 +196,65 @@ def gpu_run(leaves, N, k, steps, warmup, ref, nlat):
                 rel_err=max(rel, rel1), upwards_ms=st["upwards-stage"], solve_ms=st["solve-stage"],
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)), nlat=nlat,
                 sweep_bytes=nbytes, sweep_gbs=nbytes / (ms_step / 1e3) / 1e9,
+                gemm_ms=gms.value, gemm_launches=glaunch.value, gemm_bytes=gbytes, gemm_gbs=gbytes / (gms.value / 1e3) / 1e9,
                 build_tflops_dense=dense / b_s / 1e12, build_tflops_executed=executed / b_s / 1e12,
                 dense_flops=dense, executed_flops=executed, nnodes=D.nnodes, M=D.M)
+
+
+def serving_run(leaves, N, nreq, max_batch, seed=11):
+    """The serving workload on the factorised mesh: ``nreq`` requests, each f = K ~ U{1..8}
+    Gaussians and Dirichlet data of 8 Fourier modes (serving.workload_params), evaluated on the
+    device.  saturated: all requests queued at once (capacity); poisson: the same count at
+    Poisson arrivals of a fixed rate (80 % of capacity), latency = arrival -> result complete in
+    HBM; plain: one request at a time through the same path, k = 1, waiting for each.  A sample
+    of results is fetched and compared with a one-request solve (bitwise) and its max |u|."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import serving
+    ctx = ef.Context(0)
+    srv = serving.Server(ctx, max_batch=max_batch)
+    dm = srv.add_mesh("cfg4", leaves, N, BOUNDS)
+    P = serving.workload_params(np.random.default_rng(seed), BOUNDS, nreq)
+    for p in P[:2 * max_batch]:
+        srv.submit_load("cfg4", p, want_u=False)
+    srv.run()
+    srv.completed.clear()
+    t0 = time.perf_counter()
+    for p in P:
+        srv.submit_load("cfg4", p, want_u=False)
+    srv.run()
+    cap = nreq / (time.perf_counter() - t0)
+    sat = srv.stats()
+    srv.completed.clear()
+    rate = 0.8 * cap
+    reqs = serving.drive_poisson(srv, "cfg4", P, rate_hz=rate, want_u_every=max(nreq // 20, 1), seed=seed)
+    poi = srv.stats()
+    sample = [q for q in reqs if q.u is not None]
+    same = True
+    for q in sample[:3]:
+        alone = srv.submit_load("cfg4", q.params)
+        srv.run()
+        same = same and bool(np.array_equal(alone.u, q.u))
+    srv.completed.clear()
+    plain = serving.Server(ctx, max_batch=1, nslots=1, buckets=None)
+    plain.meshes["cfg4"] = srv.meshes["cfg4"]  # same factorised mesh
+    t0 = time.perf_counter()
+    nplain = min(300, nreq)
+    for p in P[:nplain]:
+        plain.submit_load("cfg4", p, want_u=False)
+        plain.run()
+    plain_rate = nplain / (time.perf_counter() - t0)
+    pst = plain.stats()
+    return dict(workload="serving: %d requests on the adaptive Poisson mesh (%d leaves, %dx%d cells), each f = "
+                "K~U{1..8} Gaussians (centres uniform, widths log-uniform [0.2,2], amplitudes N(0,1)), Dirichlet = "
+                "8 Fourier modes; loads evaluated on the device from 64 parameters per request" % (
+                    nreq, len(leaves), N, N),
+                max_batch=max_batch, slots=srv.nslots, bucketed_widths=list(srv.buckets),
+                saturated_solves_per_s=cap, saturated_p50_ms=sat["p50_ms"], saturated_p99_ms=sat["p99_ms"],
+                poisson_rate_hz=rate, solves_per_s=poi["solves_per_s"], p50_ms=poi["p50_ms"], p99_ms=poi["p99_ms"],
+                plain_per_request_solves_per_s=plain_rate, plain_p50_ms=pst["p50_ms"], plain_p99_ms=pst["p99_ms"],
+                served_over_plain=cap / plain_rate, batch_invariant_sample=same, results_fetched=len(sample),
+                note="latency: request arrival to its solution complete in HBM (results stay on the device; a "
+                     "fetched 16 MB field adds its PCIe copy)")
 
 
 def leaf_bounds(leaves, N):
@@ -402,6 +473,25 @@ def traffic_probe(bounds, N, k, copy_doubles=32 << 20, level=6, NV=32):
         f"copy reads and writes {copy_doubles * 8} bytes each")
 
 
+def gemm_traffic_probe(leaves, N, k, copy_doubles=32 << 20):
+    """Counter-pass workload: the step's tree GEMMs of the default mesh at k right-hand sides
+    (3 passes through ef_mesh_time_sweep_gemms, after one warm pass) and 3 device-to-device
+    copies of 256 MiB on the library's copy kernel (the calibration of the byte counters)."""
+    import ctypes
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    ctx = ef.Context(0)
+    D = ef.DeviceMesh(ctx, leaves, N, BOUNDS)
+    D.build()
+    ms, nl = ctypes.c_double(0.0), ctypes.c_int(0)
+    _lib.check(ctx.L.ef_mesh_time_sweep_gemms(D.h, k, 3, ctypes.byref(ms), ctypes.byref(nl)), ctx.L)
+    a, b = ef.DeviceArray(ctx, (copy_doubles,)), ef.DeviceArray(ctx, (copy_doubles,))
+    for _ in range(3):
+        _lib.check(ctx.L.ef_memcpy(ctx.h, b.ptr, a.ptr, copy_doubles * 8, 2), ctx.L)
+    log(f"gemm traffic probe: {nl.value} GEMM launches per pass, {ms.value:.3f} ms per pass, "
+        f"algorithmic {gemm_bytes(D, k) / 1e6:.1f} MB per pass")
+
+
 def copy_bandwidth(doubles=64 << 20, reps=5):
     """Achievable HBM bandwidth of a plain copy on this GPU (the library's copy kernel through
     ef_memcpy, 512 MiB read + 512 MiB written), best of `reps`, in GB/s."""
@@ -458,6 +548,11 @@ def main():
     ap.add_argument("--latency-requests", type=int, default=200)
     ap.add_argument("--patch-nrhs", type=int, default=16)
     ap.add_argument("--leaf-components", action="store_true", help="also time the leaf-patch kernels on their own")
+    ap.add_argument("--gemm-traffic-probe", action="store_true",
+                    help="only the step's tree GEMMs (1 + 3 passes) and 3 copies of 256 MiB on the library's copy "
+                         "kernel (the calibration of HBM byte counters), then exit")
+    ap.add_argument("--serving-requests", type=int, default=10000, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serving-batch", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only 5 leaf-solve, 5 particular-Neumann-data and 5 variable-beta leaf-solve launches and 3 "
@@ -472,6 +567,9 @@ def main():
     log(f"workload: {len(leaves)} leaves, N={args.N}")
     if args.traffic_probe:
         traffic_probe(leaf_bounds(leaves, args.N), args.N, args.patch_nrhs)
+        return
+    if args.gemm_traffic_probe:
+        gemm_traffic_probe(leaves, args.N, args.nrhs)
         return
     threads = os.cpu_count() or 1
     if args.no_cpu_baseline:
@@ -509,12 +607,19 @@ def main():
                    p99_ms=g["p99_ms"], rel_err_vs_fp64_oracle=None if args.no_cpu_baseline else g["rel_err"],
                    wall_ms_per_step=g["wall_ms_per_step"], upwards_ms=g["upwards_ms"], solve_ms=g["solve_ms"],
                    build_calls=g["builds"], latency_requests=g["nlat"])
+        per_launch = g["gemm_bytes"] / max(g["gemm_launches"], 1)
         out["roofline"] = dict(
+            bound="hbm", achieved=g["gemm_gbs"], peak=8000.0, unit="GB/s", frac=g["gemm_gbs"] / 8000.0,
+            traffic=TRAFFIC_GEMM_PER_LAUNCH, kernel="tree_gemm_kernel (the step's tree GEMMs: w = X^-1 hDiff, "
+            "h += H w, u = S g + w; %d launches per step incl. split-K reduces)" % g["gemm_launches"],
+            algorithmic_bytes_per_launch=per_launch, gemm_ms_per_step=g["gemm_ms"],
+            note="achieved = algorithmic bytes of the step's GEMM launches / their HIP-event time (GEMMs alone, "
+                 "ef_mesh_time_sweep_gemms); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / "
+                 "WRITE_SIZE passes scaled by the copy-kernel factors (profiles/)")
+        out["step_roofline"] = dict(
             bound="hbm", achieved=g["sweep_gbs"], peak=8000.0, unit="GB/s", frac=g["sweep_gbs"] / 8000.0,
-            traffic=None, kernel="HPS sweep step (upwards + solve: leaf kernels, tree_gemm_kernel, gather/scatter)",
             algorithmic_bytes_per_step=g["sweep_bytes"],
-            note="achieved = algorithmic operator + vector bytes of one step / HIP-event step time; traffic "
-                 "(HBM bytes by counters) is in profiles/, not measured in this run")
+            note="whole step: operator bytes + leaf loads and solutions over the HIP-event step time")
         out["build_roofline"] = dict(
             bound="mfma", peak=78.6, unit="TFLOP/s", achieved_dense_count=g["build_tflops_dense"],
             achieved_executed=g["build_tflops_executed"], frac_executed=g["build_tflops_executed"] / 78.6,
@@ -523,6 +628,15 @@ def main():
     except Exception as e:  # the device path reports why it did not run
         out["error"] = f"{type(e).__name__}: {e}"
         log(out["error"])
+    if args.serving_requests > 0 and "error" not in out:
+        try:
+            sv = serving_run(leaves, args.N, args.serving_requests, args.serving_batch)
+            log(f"serving: {sv['saturated_solves_per_s']:.0f} solves/s saturated, Poisson {sv['solves_per_s']:.0f}/s "
+                f"p50 {sv['p50_ms']:.2f} ms p99 {sv['p99_ms']:.2f} ms; plain {sv['plain_per_request_solves_per_s']:.0f}/s")
+            out["serving"] = sv
+        except Exception as e:
+            out["serving_error"] = f"{type(e).__name__}: {e}"
+            log(out["serving_error"])
     if args.leaf_components:
         try:
             cpu_patch = dict(check=None) if args.no_cpu_baseline else cpu_patch_baseline(

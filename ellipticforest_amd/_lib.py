@@ -31,7 +31,8 @@ EXPORTS = [
     "ef_mesh_create", "ef_mesh_destroy", "ef_mesh_footprint", "ef_mesh_info", "ef_mesh_nodes",
     "ef_mesh_set_stream", "ef_mesh_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
-    "ef_memcpy", "ef_debug_invert", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
+    "ef_memcpy", "ef_debug_invert", "ef_host_alloc", "ef_host_free", "ef_mesh_copy_async", "ef_mesh_record",
+    "ef_mesh_query", "ef_mesh_wait", "ef_mesh_eval_load", "ef_mesh_time_sweep_gemms", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
     "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_apply_device_noalloc",
     "ef_patches_dtn", "ef_patches_sync",
     "ef_patches_last_ms", "ef_patches_stream", "ef_timer_create", "ef_timer_destroy", "ef_timer_record",
@@ -99,6 +100,14 @@ def load_library(path):
     L.ef_device_free.argtypes = [_vp, _vp]
     L.ef_memcpy.argtypes = [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int]
     L.ef_debug_invert.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
+    L.ef_host_alloc.argtypes = [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
+    L.ef_host_free.argtypes = [_vp, _vp]
+    L.ef_mesh_copy_async.argtypes = [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_int]
+    L.ef_mesh_record.argtypes = [_vp, ctypes.c_int]
+    L.ef_mesh_query.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.ef_mesh_wait.argtypes = [_vp, ctypes.c_int]
+    L.ef_mesh_eval_load.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+    L.ef_mesh_time_sweep_gemms.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
     L.ef_mesh_stage_ms.argtypes = [_vp, _dp]
     L.ef_patches_create.argtypes = [_vp, ctypes.POINTER(PatchBatch), ctypes.POINTER(Coeffs), ctypes.POINTER(_vp)]
     L.ef_patches_destroy.argtypes = [_vp]

@@ -223,6 +223,42 @@ struct SweepPlan {
 
 }  // namespace
 
+constexpr int EF_MAX_SLOTS = 8;
+
+// Serving workload loads (one request = 64 doubles): f = sum of up to 8 Gaussians
+// amp exp(-|p - c|^2 / (2 w^2)) at leaf cell centres (x_i = xl + dx/2 + i dx, cell j + i N),
+// r = sum of 8 modes a cos(kx x + ky y + phi) at the root boundary points.
+__global__ void __launch_bounds__(256) serve_load_kernel(const double* __restrict__ geo, int nl, int N,
+                                                          const double* __restrict__ params, int nrhs,
+                                                          double* __restrict__ f, const double* __restrict__ pts,
+                                                          int m, double* __restrict__ r) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long NN = (long long)N * N, nf = (long long)nrhs * nl * NN;
+    if (e < nf) {
+        const long long rhs = e / (nl * NN), rem = e % (nl * NN);
+        const int leaf = (int)(rem / NN), cell = (int)(rem % NN), i = cell / N, j = cell % N;
+        const double* g = geo + 4LL * leaf;
+        const double x = (g[0] + g[2] / 2) + i * g[2], y = (g[1] + g[3] / 2) + j * g[3];
+        const double* P = params + 64 * rhs;
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const double amp = P[4 * q], dxc = x - P[4 * q + 1], dyc = y - P[4 * q + 2], w = P[4 * q + 3];
+            if (amp != 0.0) acc += amp * exp(-(dxc * dxc + dyc * dyc) / (2.0 * w * w));
+        }
+        f[e] = acc;
+    } else if (e < nf + (long long)nrhs * m) {
+        const long long t = e - nf, rhs = t / m;
+        const int p = (int)(t % m);
+        const double x = pts[p], y = pts[m + p];
+        const double* P = params + 64 * rhs + 32;
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc += P[4 * q] * cos(P[4 * q + 1] * x + P[4 * q + 2] * y + P[4 * q + 3]);
+        r[t] = acc;
+    }
+}
+
 struct ef_mesh_s {
     ef_ctx ctx = nullptr;
     hipStream_t own_stream = nullptr;
@@ -276,6 +312,10 @@ struct ef_mesh_s {
     };
     std::vector<SweepGraph> graphs;
     unsigned long long graph_clock = 0;
+    // serving: slot events, leaf geometry and root boundary points of the load generator
+    hipEvent_t slot_ev[EF_MAX_SLOTS] = {};
+    bool slot_rec[EF_MAX_SLOTS] = {};
+    DevBuf<double> leafgeo, rootpts;
 };
 
 namespace {
@@ -1082,6 +1122,8 @@ int ef_mesh_destroy(ef_mesh mesh) {
     for (auto& pr : mesh->ev)
         for (auto& e : pr)
             if (e) (void)hipEventDestroy(e);
+    for (auto& e : mesh->slot_ev)
+        if (e) (void)hipEventDestroy(e);
     hipStream_t s = mesh->own_stream;
     delete mesh;
     if (s) (void)hipStreamDestroy(s);
@@ -1285,6 +1327,149 @@ int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind) {
         EF_LAUNCH_CHECK();
         EF_HIP(hipStreamSynchronize(nullptr));
     } else if (bytes) EF_HIP(hipMemcpy(dst, src, bytes, k));
+    EF_API_END
+}
+
+int ef_mesh_time_sweep_gemms(ef_mesh mesh, int nrhs, int reps, double* ms_per_pass, int* launches) {
+    EF_API_BEGIN
+    if (!mesh || !ms_per_pass || !launches) throw EfError(EF_ERR_ARG, "null argument");
+    if (!mesh->built) throw EfError(EF_ERR_STATE, "build stage first");
+    if (nrhs < 1 || reps < 1) throw EfError(EF_ERR_ARG, "nrhs and reps must be >= 1");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    ensure_vectors(mesh, nrhs);
+    SweepPlan& sp = sweep_plan(mesh, nrhs);
+    const Plan& P = mesh->P;
+    hipStream_t s = mesh->stream;
+    int count = 0;
+    auto pass = [&] {
+        for (int dd = P.maxdepth; dd >= 0; dd--) {
+            const LevelSweep& L = sp.lv[dd];
+            if (L.cnt == 0) continue;
+            if (!mesh->homogeneous_rhs) {
+                run_gemm(L.gw, s);
+                run_gemm(L.gh, s);
+                count += (L.gw.nwaves > 0) + (L.gw.nelem > 0) + (L.gh.nwaves > 0) + (L.gh.nelem > 0);
+            }
+        }
+        for (int dd = 0; dd <= P.maxdepth; dd++) {
+            const LevelSweep& L = sp.lv[dd];
+            if (L.cnt == 0) continue;
+            run_gemm(L.gu, s);
+            count += (L.gu.nwaves > 0) + (L.gu.nelem > 0);
+        }
+    };
+    pass();  // warm
+    hipEvent_t e0, e1;
+    EF_HIP(hipEventCreate(&e0));
+    EF_HIP(hipEventCreate(&e1));
+    count = 0;
+    EF_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) pass();
+    EF_HIP(hipEventRecord(e1, s));
+    EF_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    EF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_pass = ms / reps;
+    *launches = count / reps;
+    EF_API_END
+}
+
+// ----- serving data path -----------------------------------------------------------
+int ef_host_alloc(ef_ctx ctx, size_t bytes, void** ptr) {
+    EF_API_BEGIN
+    if (!ctx || !ptr) throw EfError(EF_ERR_ARG, "null argument");
+    EF_HIP(hipSetDevice(ctx->device));
+    EF_HIP(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    EF_API_END
+}
+
+int ef_host_free(ef_ctx ctx, void* ptr) {
+    EF_API_BEGIN
+    if (!ctx) throw EfError(EF_ERR_ARG, "null context");
+    if (ptr) EF_HIP(hipHostFree(ptr));
+    EF_API_END
+}
+
+int ef_mesh_copy_async(ef_mesh mesh, void* dst, const void* src, size_t bytes, int kind) {
+    EF_API_BEGIN
+    if (!mesh || (bytes && (!dst || !src))) throw EfError(EF_ERR_ARG, "null argument");
+    if (kind < 0 || kind > 2) throw EfError(EF_ERR_ARG, "copy kind must be 0, 1 or 2");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                          : hipMemcpyDeviceToDevice;
+    if (bytes) EF_HIP(hipMemcpyAsync(dst, src, bytes, k, mesh->stream));
+    EF_API_END
+}
+
+int ef_mesh_record(ef_mesh mesh, int slot) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    if (slot < 0 || slot >= EF_MAX_SLOTS) throw EfError(EF_ERR_ARG, "slot out of range");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    if (!mesh->slot_ev[slot]) EF_HIP(hipEventCreateWithFlags(&mesh->slot_ev[slot], hipEventDisableTiming));
+    EF_HIP(hipEventRecord(mesh->slot_ev[slot], mesh->stream));
+    mesh->slot_rec[slot] = true;
+    EF_API_END
+}
+
+int ef_mesh_query(ef_mesh mesh, int slot, int* done) {
+    EF_API_BEGIN
+    if (!mesh || !done) throw EfError(EF_ERR_ARG, "null argument");
+    if (slot < 0 || slot >= EF_MAX_SLOTS) throw EfError(EF_ERR_ARG, "slot out of range");
+    *done = 1;
+    if (!mesh->slot_rec[slot]) return EF_OK;
+    const hipError_t e = hipEventQuery(mesh->slot_ev[slot]);
+    if (e == hipErrorNotReady) { (void)hipGetLastError(); *done = 0; }
+    else if (e != hipSuccess) throw EfError(EF_ERR_DEVICE, std::string("hipEventQuery: ") + hipGetErrorString(e));
+    EF_API_END
+}
+
+int ef_mesh_wait(ef_mesh mesh, int slot) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    if (slot < 0 || slot >= EF_MAX_SLOTS) throw EfError(EF_ERR_ARG, "slot out of range");
+    if (mesh->slot_rec[slot]) EF_HIP(hipEventSynchronize(mesh->slot_ev[slot]));
+    EF_API_END
+}
+
+int ef_mesh_eval_load(ef_mesh mesh, const double* params_dev, int nrhs, double* f_dev, double* r_dev) {
+    EF_API_BEGIN
+    if (!mesh || !params_dev || !f_dev || !r_dev) throw EfError(EF_ERR_ARG, "null argument");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    const Plan& P = mesh->P;
+    const int nl = (int)P.leaves.size(), N = P.N, m = 4 * P.nodes[0].s;
+    if (!mesh->leafgeo.p) {
+        std::vector<double> geo(4 * (size_t)nl), pts(2 * (size_t)m);
+        for (int li = 0; li < nl; li++) {
+            const NodeP& nd = P.nodes[P.leaves[li]];
+            geo[4 * li] = nd.xl;
+            geo[4 * li + 1] = nd.yl;
+            geo[4 * li + 2] = (nd.xu - nd.xl) / N;
+            geo[4 * li + 3] = (nd.yu - nd.yl) / N;
+        }
+        // root boundary points, sides W, E, S, N (HPSAlgorithm.hpp:375-400)
+        const NodeP& root = P.nodes[0];
+        const int M = root.s;
+        const double dx = (root.xu - root.xl) / M, dy = (root.yu - root.yl) / M;
+        for (int i = 0; i < M; i++) {
+            const double ys = (root.yl + dy / 2) + i * dy, xs = (root.xl + dx / 2) + i * dx;
+            pts[i] = root.xl;         pts[m + i] = ys;
+            pts[M + i] = root.xu;     pts[m + M + i] = ys;
+            pts[2 * M + i] = xs;      pts[m + 2 * M + i] = root.yl;
+            pts[3 * M + i] = xs;      pts[m + 3 * M + i] = root.yu;
+        }
+        mesh->leafgeo.upload(geo, mesh->stream);
+        mesh->rootpts.upload(pts, mesh->stream);
+    }
+    const long long total = (long long)nrhs * nl * N * N + (long long)nrhs * m;
+    check_range(params_dev, (size_t)nrhs * 64 * 8, "load parameters");
+    check_range(f_dev, (size_t)nrhs * nl * N * N * 8, "load f");
+    check_range(r_dev, (size_t)nrhs * m * 8, "load r");
+    EF_LAUNCH("serve_load_kernel", serve_load_kernel, dim3(blocks_for(total)), dim3(256), mesh->stream,
+              (const double*)mesh->leafgeo.p, nl, N, params_dev, nrhs, f_dev, (const double*)mesh->rootpts.p, m, r_dev);
     EF_API_END
 }
 

@@ -1,25 +1,31 @@
 """Serving loop over factorised meshes (continuous batching of solve requests).
 
 * Meshes are built once and stay resident; a new mesh is admitted when its device
-  footprint (``ef_mesh_footprint``) fits the HBM budget, evicting the least recently
-  used idle meshes if needed.
-* A request carries the load f ([nleaves][N*N]) and the Dirichlet data r ([4M]) for one
-  resident mesh.  The loop takes the mesh with the oldest waiting request, packs up to
-  ``max_batch`` of that mesh's waiting requests as right-hand-side columns into one of
-  two staging slots, and runs upwards + solve on the device for the whole batch.  While
-  one slot runs on the device the host packs the next batch into the other slot, so
-  requests that arrive while others are on the GPU join the next launch.
-* Each right-hand side is reduced in an order that does not depend on the batch width,
-  so a request's result is the same alone and in a batch.
-* ``stats()`` reports solves/s and p50/p99 latency (arrival to result on the host).
-
-Device memory comes from a buffer provider: ``DeviceBuffers`` allocates HBM and moves
-bytes through the library's own entry points (``ef_device_alloc``, ``ef_memcpy``), so the
-process holds a single HIP runtime.
+  footprint (``ef_mesh_footprint`` plus the serving slots) fits the HBM budget, evicting the
+  least recently used idle meshes if needed.
+* A request names a resident mesh and carries either the 64 parameters of a serving-workload
+  load (``submit_load``: 8 Gaussians for f, 8 Fourier modes for the Dirichlet data, evaluated
+  on the device by ``ef_mesh_eval_load``) or explicit host arrays f ([nleaves][N*N]) and r
+  ([4M]) (``submit``).
+* The loop takes the mesh with the oldest waiting request and packs up to ``max_batch`` of
+  that mesh's waiting requests of one kind as right-hand-side columns into a free slot.  A
+  slot owns page-locked host buffers and device buffers; its inputs go up with
+  ``ef_mesh_copy_async`` on the mesh's stream, then the load evaluation, upwards and solve
+  stages and the copies of requested results are queued behind them and an event is
+  recorded.  The host returns at once to admit and pack the next batch while the device
+  works (up to ``nslots`` batches in flight); a batch completes when its event has fired.
+  Requests that arrive while others are on the GPU join the next batch.
+* With ``buckets`` the batch width is rounded up to the next bucket (padding columns are
+  zero loads), so the sweep replays one of a few captured graphs; without it the width is
+  exact and the stages are launched one by one.
+* Every right-hand side is reduced in an order that does not depend on the batch width, so
+  a request's result is the same alone and in a batch.
+* ``stats()`` reports solves/s and p50/p99 latency (arrival to completion).
 """
 from __future__ import annotations
 
 import collections
+import ctypes
 import threading
 import time
 from dataclasses import dataclass, field
@@ -28,61 +34,125 @@ import numpy as np
 
 from . import _lib
 
+NPARAM = 64  # doubles per serving-workload request
 
-class DeviceBuffers:
-    """HBM buffers of the context's GPU (``_lib.DeviceArray``)."""
 
-    def __init__(self, ctx):
+class PinnedArray:
+    """Page-locked host memory (``ef_host_alloc``) viewed as a float64 numpy array."""
+
+    def __init__(self, ctx, shape):
         self.ctx = ctx
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        n = int(np.prod(self.shape))
+        p = _lib._vp()
+        _lib.check(ctx.L.ef_host_alloc(ctx.h, max(n, 1) * 8, ctypes.byref(p)), ctx.L)
+        self.ptr = p.value
+        self.a = np.ctypeslib.as_array((ctypes.c_double * max(n, 1)).from_address(self.ptr))[:n].reshape(self.shape)
 
-    def empty(self, shape):
-        return _lib.DeviceArray(self.ctx, shape)
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.ctx.L.ef_host_free(self.ctx.h, self.ptr)
+            self.ptr = None
 
-    def ptr(self, buf):
-        return buf.ptr
 
-    def upload(self, buf, host, offset=0):
-        buf.upload(np.ascontiguousarray(host).ravel(), offset)
+def workload_params(rng, bounds, count):
+    """Parameters of ``count`` serving-workload requests on the domain ``bounds``: K ~ U{1..8}
+    Gaussians with centres uniform in the domain, widths log-uniform in [0.2, 2] and
+    amplitudes N(0, 1) (amp 0 marks an unused Gaussian), and 8 Dirichlet modes a cos(kx x +
+    ky y + phi) with a ~ N(0, 1), integer kx, ky in [-4, 4] scaled to the domain and phi ~
+    U[0, 2 pi)."""
+    xl, xu, yl, yu = bounds
+    P = np.zeros((count, NPARAM))
+    for i in range(count):
+        K = int(rng.integers(1, 9))
+        for g in range(K):
+            P[i, 4 * g:4 * g + 4] = (rng.standard_normal(), rng.uniform(xl, xu), rng.uniform(yl, yu),
+                                     np.exp(rng.uniform(np.log(0.2), np.log(2.0))))
+        for q in range(8):
+            P[i, 32 + 4 * q:36 + 4 * q] = (rng.standard_normal(), rng.integers(-4, 5) * np.pi / (xu - xl),
+                                           rng.integers(-4, 5) * np.pi / (yu - yl), rng.uniform(0, 2 * np.pi))
+    return P
 
-    def download(self, buf, count):
-        return buf.download(count)
+
+def workload_load(params, leaf_bounds, N, root_points):
+    """Host evaluation of a request's load (the oracle of ``ef_mesh_eval_load``): f per leaf at
+    cell centres, r at the root boundary points."""
+    f = np.zeros((len(leaf_bounds), N * N))
+    for k, (xl, xu, yl, yu) in enumerate(leaf_bounds):
+        dx, dy = (xu - xl) / N, (yu - yl) / N
+        X, Y = np.meshgrid((xl + dx / 2) + np.arange(N) * dx, (yl + dy / 2) + np.arange(N) * dy, indexing="ij")
+        X, Y = X.ravel(), Y.ravel()
+        for g in range(8):
+            a, cx, cy, w = params[4 * g:4 * g + 4]
+            if a != 0.0:
+                f[k] += a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2.0 * w * w))
+    px, py = root_points
+    r = np.zeros(len(px))
+    for q in range(8):
+        a, kx, ky, phi = params[32 + 4 * q:36 + 4 * q]
+        r += a * np.cos(kx * px + ky * py + phi)
+    return f, r
 
 
 @dataclass
 class Request:
     mesh: str
-    f: np.ndarray
-    r: np.ndarray
+    params: np.ndarray | None = None
+    f: np.ndarray | None = None
+    r: np.ndarray | None = None
+    want_u: bool = True
     rid: int = 0
     t_arrival: float = field(default_factory=time.perf_counter)
     t_done: float = 0.0
     u: np.ndarray | None = None
     done: threading.Event = field(default_factory=threading.Event)
 
+    @property
+    def kind(self):
+        return "load" if self.params is not None else "host"
+
+
+class _Slot:
+    def __init__(self, ctx, nl, NN, M4, max_batch):
+        self.params = PinnedArray(ctx, (max_batch, NPARAM))
+        self.dparams = _lib.DeviceArray(ctx, (max_batch, NPARAM))
+        self.f = _lib.DeviceArray(ctx, (max_batch, nl, NN))
+        self.r = _lib.DeviceArray(ctx, (max_batch, M4))
+        self.u = _lib.DeviceArray(ctx, (max_batch, nl, NN))
+        self.hu = PinnedArray(ctx, (max_batch, nl, NN))
+        self.hf = None  # page-locked staging of host-array requests, made on first use
+        self.hr = None
+        self.batch = None
+
 
 class _Resident:
-    def __init__(self, dm, footprint, max_batch, buffers):
+    def __init__(self, ctx, dm, footprint, max_batch, nslots):
         self.dm = dm
         self.footprint = footprint
         self.last_used = time.perf_counter()
         nl, NN, M4 = dm.nleaves, dm.N * dm.N, 4 * dm.M
-        # two staging slots: the device works on one while the host packs the other
-        self.slots = [dict(f=buffers.empty((max_batch, nl, NN)), r=buffers.empty((max_batch, M4)),
-                           u=buffers.empty((max_batch, nl, NN))) for _ in range(2)]
-        self.a = buffers.empty((M4,))
-        buffers.upload(self.a, np.ones(M4))
+        self.slots = [_Slot(ctx, nl, NN, M4, max_batch) for _ in range(nslots)]
+        self.next_slot = 0
+        self.a = _lib.DeviceArray(ctx, (M4,))
+        self.a.upload(np.ones(M4))
 
 
 class Server:
-    def __init__(self, ctx, buffers, hbm_budget_bytes=200 << 30, max_batch=32):
+    def __init__(self, ctx, hbm_budget_bytes=200 << 30, max_batch=32, nslots=3, buckets=(1, 2, 4, 8, 16, 32)):
+        if not 1 <= nslots <= 8:
+            raise ValueError("nslots must be in 1..8")
         self.ctx = ctx
-        self.buffers = buffers
         self.budget = hbm_budget_bytes
         self.max_batch = max_batch
+        self.nslots = nslots
+        self.buckets = sorted(b for b in buckets if b <= max_batch) if buckets else None
+        if self.buckets and self.buckets[-1] != max_batch:
+            self.buckets.append(max_batch)
         self.meshes = collections.OrderedDict()
         self.pending = collections.deque()
         self.lock = threading.Condition()
         self.completed = []
+        self.inflight = collections.deque()
         self._next_id = 0
         self._stop = False
 
@@ -90,14 +160,18 @@ class Server:
     def used_bytes(self):
         return sum(m.footprint for m in self.meshes.values())
 
+    def slot_bytes(self, nleaves, N, M):
+        per = 2 * nleaves * N * N + 4 * M + NPARAM  # f, u, r, parameters (device)
+        return self.nslots * self.max_batch * per * 8
+
     def add_mesh(self, key, leaves, N, bounds, **build_args):
         """Build and keep a factorised mesh; evict idle meshes (LRU) to fit the budget."""
-        fp = _lib.DeviceMesh.footprint(leaves, N, bounds, max_rhs=self.max_batch)
-        fp += 3 * 2 * self.max_batch * len(leaves) * N * N * 8  # staging slots
+        probe = _lib.DeviceMesh.footprint(leaves, N, bounds, max_rhs=self.max_batch)
+        fp = probe + self.slot_bytes(len(leaves), N, 0)
         if fp > self.budget:
             raise MemoryError(f"mesh {key} needs {fp} bytes, budget is {self.budget}")
         with self.lock:
-            busy = {r.mesh for r in self.pending}
+            busy = {r.mesh for r in self.pending} | {res_key for res_key, _ in self._inflight_keys()}
             while self.used_bytes() + fp > self.budget:
                 victim = next((k for k in self.meshes if k not in busy), None)
                 if victim is None:
@@ -106,84 +180,134 @@ class Server:
         dm = _lib.DeviceMesh(self.ctx, leaves, N, bounds)
         dm.build(**build_args)
         with self.lock:
-            self.meshes[key] = _Resident(dm, fp, self.max_batch, self.buffers)
+            self.meshes[key] = _Resident(self.ctx, dm, fp, self.max_batch, self.nslots)
         return dm
 
+    def _inflight_keys(self):
+        return [(k, s) for k, s, _ in self.inflight]
+
     # ----- requests -------------------------------------------------------------------
-    def submit(self, mesh, f, r):
-        req = Request(mesh=mesh, f=np.ascontiguousarray(f, dtype=np.float64),
-                      r=np.ascontiguousarray(r, dtype=np.float64))
+    def _enqueue(self, req):
         with self.lock:
-            if mesh not in self.meshes:
-                raise KeyError(f"mesh {mesh} is not resident")
+            if req.mesh not in self.meshes:
+                raise KeyError(f"mesh {req.mesh} is not resident")
             req.rid = self._next_id
             self._next_id += 1
             self.pending.append(req)
             self.lock.notify()
         return req
 
+    def submit_load(self, mesh, params, want_u=True):
+        p = np.ascontiguousarray(params, dtype=np.float64).ravel()
+        if p.size != NPARAM:
+            raise ValueError(f"a load request has {NPARAM} parameters")
+        return self._enqueue(Request(mesh=mesh, params=p, want_u=want_u))
+
+    def submit(self, mesh, f, r, want_u=True):
+        return self._enqueue(Request(mesh=mesh, f=np.ascontiguousarray(f, dtype=np.float64),
+                                     r=np.ascontiguousarray(r, dtype=np.float64), want_u=want_u))
+
     def _take_batch(self):
         with self.lock:
             if not self.pending:
-                return None, []
-            key = self.pending[0].mesh
+                return None, None, []
+            key, kind = self.pending[0].mesh, self.pending[0].kind
             batch, rest = [], collections.deque()
             while self.pending:
                 q = self.pending.popleft()
-                if q.mesh == key and len(batch) < self.max_batch:
+                if q.mesh == key and q.kind == kind and len(batch) < self.max_batch:
                     batch.append(q)
                 else:
                     rest.append(q)
             self.pending = rest
             self.meshes.move_to_end(key)
-            return self.meshes[key], batch
+            return key, self.meshes[key], batch
 
-    def _pack(self, res, slot, batch):
-        B = self.buffers
+    # ----- device work ----------------------------------------------------------------
+    def _width(self, n):
+        if not self.buckets:
+            return n
+        return next(b for b in self.buckets if b >= n)
+
+    def _launch(self, key, res, si, batch):
+        dm, L, s = res.dm, self.ctx.L, res.slots[si]
+        n, k = len(batch), self._width(len(batch))
+        nl, NN, M4 = dm.nleaves, dm.N * dm.N, 4 * dm.M
+        if batch[0].kind == "load":
+            s.params.a[:n] = np.stack([q.params for q in batch])
+            s.params.a[n:k] = 0.0  # padding columns: zero load, zero boundary data
+            _lib.check(L.ef_mesh_copy_async(dm.h, s.dparams.ptr, s.params.ptr, k * NPARAM * 8, 0), L)
+            _lib.check(L.ef_mesh_eval_load(dm.h, s.dparams.ptr, k, s.f.ptr, s.r.ptr), L)
+        else:
+            if s.hf is None:
+                s.hf = PinnedArray(self.ctx, (self.max_batch, nl, NN))
+                s.hr = PinnedArray(self.ctx, (self.max_batch, M4))
+            for j, q in enumerate(batch):
+                s.hf.a[j] = q.f.reshape(nl, NN)
+                s.hr.a[j] = q.r
+            s.hf.a[n:k] = 0.0
+            s.hr.a[n:k] = 0.0
+            _lib.check(L.ef_mesh_copy_async(dm.h, s.f.ptr, s.hf.ptr, k * nl * NN * 8, 0), L)
+            _lib.check(L.ef_mesh_copy_async(dm.h, s.r.ptr, s.hr.ptr, k * M4 * 8, 0), L)
+        dm.upwards_device(s.f.ptr, k)
+        dm.solve_device(s.f.ptr, s.r.ptr, res.a.ptr, None, s.u.ptr, k)
+        per = nl * NN * 8
         for j, q in enumerate(batch):
-            B.upload(slot["f"], q.f, offset=j * q.f.size)
-            B.upload(slot["r"], q.r, offset=j * q.r.size)
+            if q.want_u:
+                _lib.check(L.ef_mesh_copy_async(dm.h, s.hu.ptr + j * per, s.u.ptr + j * per, per, 1), L)
+        _lib.check(L.ef_mesh_record(dm.h, si), L)
+        s.batch = batch
+        self.inflight.append((key, si, res))
 
-    def _launch(self, res, slot, k):
-        dm, B = res.dm, self.buffers
-        dm.upwards_device(B.ptr(slot["f"]), k)
-        dm.solve_device(B.ptr(slot["f"]), B.ptr(slot["r"]), B.ptr(res.a), None, B.ptr(slot["u"]), k)
+    def _done(self, res, si):
+        flag = ctypes.c_int(0)
+        _lib.check(self.ctx.L.ef_mesh_query(res.dm.h, si, ctypes.byref(flag)), self.ctx.L)
+        return flag.value != 0
 
-    def _finish(self, res, slot, batch):
-        res.dm.sync()
-        per = res.dm.nleaves * res.dm.N * res.dm.N
-        u = self.buffers.download(slot["u"], len(batch) * per).reshape(len(batch), res.dm.nleaves, -1)
+    def _complete(self, block):
+        """Finish the oldest in-flight batch (waiting for it when ``block``)."""
+        key, si, res = self.inflight[0]
+        if not block and not self._done(res, si):
+            return False
+        if block:
+            _lib.check(self.ctx.L.ef_mesh_wait(res.dm.h, si), self.ctx.L)
+        self.inflight.popleft()
+        s = res.slots[si]
         now = time.perf_counter()
-        for j, q in enumerate(batch):
-            q.u = u[j].copy()
+        for j, q in enumerate(s.batch):
+            if q.want_u:
+                q.u = s.hu.a[j].copy()
             q.t_done = now
             q.done.set()
-        self.completed.extend(batch)
+        self.completed.extend(s.batch)
+        s.batch = None
         res.last_used = now
+        return True
 
-    def run(self, stop_when_idle=True, poll_s=1e-4):
-        """Continuous batching: launch batch i, pack batch i+1 while i runs, then finish i."""
-        inflight = None
-        slot_id = 0
+    def run(self, stop_when_idle=True, poll_s=5e-5):
+        """Continuous batching: admit and queue batches while earlier ones run on the device."""
         while not self._stop:
-            res, batch = self._take_batch()
+            while self.inflight and self._complete(block=False):
+                pass
+            key, res, batch = self._take_batch()
             if batch:
-                slot = res.slots[slot_id]
-                self._pack(res, slot, batch)
-                if inflight is not None:
-                    self._finish(*inflight)
-                self._launch(res, slot, len(batch))
-                inflight = (res, slot, batch)
-                slot_id ^= 1
+                si = res.next_slot
+                res.next_slot = (si + 1) % self.nslots
+                while any(k == key and s == si for k, s, _ in self.inflight):
+                    self._complete(block=True)  # the slot's previous batch must be finished
+                self._launch(key, res, si, batch)
                 continue
-            if inflight is not None:
-                self._finish(*inflight)
-                inflight = None
+            if self.inflight:
+                if not self._complete(block=False):
+                    with self.lock:
+                        if not self.pending:
+                            self.lock.wait(timeout=poll_s)
                 continue
             if stop_when_idle:
                 break
             with self.lock:
-                self.lock.wait(timeout=poll_s)
+                if not self.pending:
+                    self.lock.wait(timeout=poll_s)
 
     def stop(self):
         self._stop = True
@@ -198,3 +322,32 @@ class Server:
         t1 = max(q.t_done for q in self.completed)
         return dict(requests=len(lat), solves_per_s=len(lat) / max(t1 - t0, 1e-12),
                     p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)))
+
+
+def drive_poisson(server, mesh, params, rate_hz, want_u_every=0, seed=0):
+    """Submit ``params`` (one request per row) at Poisson arrival times of rate ``rate_hz``
+    from a feeder thread while ``server.run`` serves them; returns the requests."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_hz, len(params))
+    reqs = [None] * len(params)
+
+    def feed():
+        t = time.perf_counter()
+        for i, p in enumerate(params):
+            t += gaps[i]
+            while True:
+                d = t - time.perf_counter()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 1e-3) if d > 2e-4 else 0)
+            reqs[i] = server.submit_load(mesh, p, want_u=bool(want_u_every) and i % want_u_every == 0)
+        server.stop()
+
+    th = threading.Thread(target=feed, daemon=True)
+    server._stop = False
+    th.start()
+    server.run(stop_when_idle=False)
+    th.join()
+    server._stop = False
+    server.run(stop_when_idle=True)  # drain
+    return reqs

@@ -144,6 +144,31 @@ int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_device_free(ef_ctx ctx, void* ptr);
 int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
 
+/* Serving data path (no counterpart in the reference: the request loop keeps factorised
+ * meshes resident and feeds them batches of right-hand sides).
+ * ef_host_alloc / ef_host_free: page-locked host memory, so ef_mesh_copy_async really is
+ * asynchronous.  ef_mesh_copy_async: hipMemcpyAsync on the mesh's stream (kind as ef_memcpy),
+ * ordered with the mesh's stages.  ef_mesh_record / ef_mesh_query / ef_mesh_wait: an event
+ * per slot (0..7) recorded on the mesh's stream, polled, or waited for.
+ * ef_mesh_eval_load: the serving workload's loads for nrhs requests from 64 parameters each
+ * (params_dev, device): f_dev[rhs][leaf][N*N] = sum of 8 Gaussians (amp, cx, cy, w; amp 0
+ * unused) amp exp(-|p - c|^2 / (2 w^2)) at the leaf cell centres, r_dev[rhs][4M] = sum of 8
+ * modes (a, kx, ky, phi) a cos(kx x + ky y + phi) at the root boundary points; queued on the
+ * mesh's stream. */
+int ef_host_alloc(ef_ctx ctx, size_t bytes, void** ptr);
+int ef_host_free(ef_ctx ctx, void* ptr);
+int ef_mesh_copy_async(ef_mesh mesh, void* dst, const void* src, size_t bytes, int kind);
+int ef_mesh_record(ef_mesh mesh, int slot);
+int ef_mesh_query(ef_mesh mesh, int slot, int* done);
+int ef_mesh_wait(ef_mesh mesh, int slot);
+int ef_mesh_eval_load(ef_mesh mesh, const double* params_dev, int nrhs, double* f_dev, double* r_dev);
+
+/* Diagnostic: the tree GEMMs of one upwards + solve step with nrhs right-hand sides alone
+ * (w = X^-1 hDiff and h = H w per level up, u = S g per level down: mergeW_/mergeH_/applyS_,
+ * HPSAlgorithm.hpp:1059-1150, 1165-1230), reps passes between two HIP events on the mesh's
+ * stream over whatever the vector arena holds; ms per pass and kernel launches per pass. */
+int ef_mesh_time_sweep_gemms(ef_mesh mesh, int nrhs, int reps, double* ms_per_pass, int* launches);
+
 /* Diagnostic: the batched inversion every merge uses (blocked Gauss-Jordan with partial
  * pivoting, tournament pivoting over 512-row chunks), applied to B host matrices of order m
  * (row-major, B x m x m).  Ainv receives the inverses; ipiv (B x m, may be NULL) the

@@ -114,6 +114,9 @@ inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new double(0.0); return 0
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return 0; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) { *e = emu_now_ms(); return 0; }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return 0; }
+enum { hipErrorNotReady = 600, hipEventDisableTiming = 2 };
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { *e = new double(0.0); return 0; }
+inline hipError_t hipEventQuery(hipEvent_t) { return 0; }  // emulated work is complete on return
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) { *ms = (float)(*b - *a); return 0; }
 
 typedef double emu_d4 __attribute__((ext_vector_type(4)));

@@ -1,8 +1,10 @@
-"""Serving loop (ellipticforest_amd/serving.py) driven through the CPU-emulation build of
-the library: buffers come from the library's own ef_device_alloc / ef_memcpy (host memory
-in the emulation), so the device-pointer entry points run unchanged.  Checks: a request's result is the same alone and in a batch (bitwise: every
-right-hand side is reduced in a batch-independent order), results match the oracle, and
-meshes are admitted / evicted by footprint."""
+"""Serving loop (ellipticforest_amd/serving.py).  The CPU tests drive it through the
+CPU-emulation build of the library (page-locked buffers, asynchronous copies, slot events and
+the load evaluation map to host memory there, so the same code path runs); the GPU tests run
+it on the device.  Checks: a request's result is the same alone and in a batch (bitwise:
+every right-hand side is reduced in a batch-independent order), results match the oracle,
+device-evaluated loads match their host evaluation, meshes are admitted / evicted by
+footprint, and the Poisson-arrival driver serves every request."""
 import numpy as np
 import pytest
 
@@ -18,28 +20,60 @@ def reference(leaves, N, bounds, f, r):
     return O.solve(r, np.ones_like(r), np.zeros_like(r))
 
 
-def test_batch_invariance_and_parity(emu_ctx):
-    leaves, N, bounds = omesh.uniform(1), 8, (0.0, 1.0, 0.0, 1.0)
-    srv = serving.Server(emu_ctx, serving.DeviceBuffers(emu_ctx), max_batch=4)
+def leaf_bounds(dm):
+    return [tuple(dm.node_bounds[n]) for n in dm.leaf_nodes]
+
+
+def host_array_case(ctx, leaves, N, bounds, nreq, max_batch):
+    srv = serving.Server(ctx, max_batch=max_batch)
     dm = srv.add_mesh("m", leaves, N, bounds)
     rng = np.random.default_rng(7)
-    reqs = [(rng.standard_normal((len(leaves), N * N)), rng.standard_normal(4 * dm.M)) for _ in range(6)]
+    reqs = [(rng.standard_normal((len(leaves), N * N)), rng.standard_normal(4 * dm.M)) for _ in range(nreq)]
     batched = [srv.submit("m", f, r) for f, r in reqs]
     srv.run()
     assert all(q.done.is_set() for q in batched)
-    for (f, r), q in zip(reqs, batched):
+    for i in (0, nreq - 1):
+        f, r = reqs[i]
         alone = srv.submit("m", f, r)
         srv.run()
-        assert np.array_equal(alone.u, q.u)
+        assert np.array_equal(alone.u, batched[i].u)
         ref = reference(leaves, N, bounds, f, r)
-        assert np.abs(q.u - ref).max() / np.abs(ref).max() < 1e-10
+        assert np.abs(batched[i].u - ref).max() / np.abs(ref).max() < 1e-10
     st = srv.stats()
-    assert st["requests"] == 12 and st["p99_ms"] >= st["p50_ms"] > 0
+    assert st["requests"] == nreq + 2 and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def load_case(ctx, leaves, N, bounds, nreq, max_batch, buckets=(1, 2, 4, 8, 16, 32)):
+    srv = serving.Server(ctx, max_batch=max_batch, buckets=buckets)
+    dm = srv.add_mesh("m", leaves, N, bounds)
+    P = serving.workload_params(np.random.default_rng(3), bounds, nreq)
+    reqs = [srv.submit_load("m", p) for p in P]
+    srv.run()
+    import ellipticforest_amd as ef
+    side, px, py = oracle.root_face_points(dm.M, bounds)
+    for i in (0, nreq // 2, nreq - 1):
+        f, r = serving.workload_load(P[i], leaf_bounds(dm), N, (px, py))
+        alone = srv.submit_load("m", P[i])
+        srv.run()
+        assert np.array_equal(alone.u, reqs[i].u)
+        ref = reference(leaves, N, bounds, f, r)
+        assert np.abs(reqs[i].u - ref).max() / np.abs(ref).max() < 1e-10
+    return srv, dm, P, reqs
+
+
+def test_batch_invariance_and_parity(emu_ctx):
+    host_array_case(emu_ctx, omesh.uniform(1), 8, (0.0, 1.0, 0.0, 1.0), nreq=6, max_batch=4)
+
+
+def test_load_requests(emu_ctx):
+    """Requests as workload parameters: device-evaluated loads against the host evaluation
+    (through the solution), bucketed batch widths, invariance alone vs in a batch."""
+    load_case(emu_ctx, omesh.uniform(1), 8, (-1.0, 1.0, -1.0, 1.0), nreq=7, max_batch=4, buckets=(1, 2, 4))
 
 
 def test_admission_by_footprint(emu_ctx):
     leaves, N, bounds = omesh.uniform(1), 8, (0.0, 1.0, 0.0, 1.0)
-    srv = serving.Server(emu_ctx, serving.DeviceBuffers(emu_ctx), max_batch=2)
+    srv = serving.Server(emu_ctx, max_batch=2)
     srv.add_mesh("a", leaves, N, bounds)
     one = srv.used_bytes()
     srv.budget = int(one * 1.5)  # room for one mesh only
@@ -50,3 +84,46 @@ def test_admission_by_footprint(emu_ctx):
     srv.budget = one // 2
     with pytest.raises(MemoryError):
         srv.add_mesh("c", leaves, N, bounds)
+
+
+def test_poisson_driver(emu_ctx):
+    leaves, N, bounds = omesh.uniform(1), 8, (-1.0, 1.0, -1.0, 1.0)
+    srv = serving.Server(emu_ctx, max_batch=4, buckets=None)
+    srv.add_mesh("m", leaves, N, bounds)
+    P = serving.workload_params(np.random.default_rng(5), bounds, 12)
+    reqs = serving.drive_poisson(srv, "m", P, rate_hz=2000.0, want_u_every=4)
+    assert all(q is not None and q.done.is_set() for q in reqs)
+    assert sum(q.u is not None for q in reqs) == 3
+    assert srv.stats()["requests"] == 12
+
+
+def test_workload_params_shape():
+    P = serving.workload_params(np.random.default_rng(0), (-10, 10, -10, 10), 50)
+    K = (P[:, 0:32:4] != 0).sum(1)
+    assert P.shape == (50, serving.NPARAM) and K.min() >= 1 and K.max() <= 8
+    w = P[:, 3:32:4][P[:, 0:32:4] != 0]
+    assert w.min() >= 0.2 and w.max() <= 2.0
+
+
+@pytest.mark.gpu
+def test_gpu_serving_host_arrays(ctx):
+    host_array_case(ctx, omesh.refine_by_function(lambda x, y: np.abs(np.sin(x) + np.sin(y)) > 1.2, 8, 8,
+                                                  -10, 10, -10, 10, 0, 4), 8, (-10, 10, -10, 10), nreq=40,
+                    max_batch=32)
+
+
+@pytest.mark.gpu
+def test_gpu_serving_loads_and_poisson(ctx):
+    """Loads evaluated on the device, widths past 16 columns (two-tile GEMMs), bitwise
+    invariance alone vs in a batch, oracle parity, then a Poisson-arrival run."""
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(np.sin(x) + np.sin(y)) > 1.2, 16, 16, -10, 10, -10, 10, 0, 4)
+    srv, dm, P, reqs = load_case(ctx, leaves, 16, (-10.0, 10.0, -10.0, 10.0), nreq=45, max_batch=32)
+    P2 = serving.workload_params(np.random.default_rng(9), (-10.0, 10.0, -10.0, 10.0), 300)
+    srv.completed.clear()
+    out = serving.drive_poisson(srv, "m", P2, rate_hz=5000.0, want_u_every=50)
+    assert all(q.done.is_set() for q in out)
+    st = srv.stats()
+    assert st["requests"] == 300 and st["p99_ms"] >= st["p50_ms"] > 0
+    alone = srv.submit_load("m", P2[100])
+    srv.run()
+    assert np.array_equal(alone.u, out[100].u)
