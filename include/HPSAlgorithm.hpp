@@ -5,11 +5,13 @@
 // every 4-to-1 merge level by level on the GPU, and upwards/solve move the load and the
 // boundary data in and the leaf solutions out.  The per-family virtual steps of the
 // reference (merge4to1, upwards4to1, split1to4, leafSolve) do not exist as host calls
-// in the batched device path; they are kept as no-op hooks so subclasses still compile.
+// in the batched device path; they are kept so subclasses still compile, and calling one
+// throws std::logic_error.
 #ifndef EF_HPS_ALGORITHM_HPP_
 #define EF_HPS_ALGORITHM_HPP_
 
 #include <functional>
+#include <stdexcept>
 #include <map>
 #include <string>
 #include <vector>
@@ -160,15 +162,22 @@ public:
         app.logHead("End HPS Solve Stage");
     }
 
-    // per-family hooks of the reference; the device path batches them by tree level
-    virtual void merge4to1(PatchType&, PatchType&, PatchType&, PatchType&, PatchType&) {}
-    virtual void upwards4to1(PatchType&, PatchType&, PatchType&, PatchType&, PatchType&) {}
-    virtual void split1to4(PatchType&, PatchType&, PatchType&, PatchType&, PatchType&) {}
-    virtual void leafSolve(PatchType&) {}
+    // Per-family hooks of the reference (HPSAlgorithm.hpp:497-596).  The device path merges,
+    // sweeps and solves whole tree levels inside buildStage / upwardsStage / solveStage and never
+    // calls them, so an override would silently not run: calling one throws instead.
+    virtual void merge4to1(PatchType&, PatchType&, PatchType&, PatchType&, PatchType&) { familyHook("merge4to1"); }
+    virtual void upwards4to1(PatchType&, PatchType&, PatchType&, PatchType&, PatchType&) { familyHook("upwards4to1"); }
+    virtual void split1to4(PatchType&, PatchType&, PatchType&, PatchType&, PatchType&) { familyHook("split1to4"); }
+    virtual void leafSolve(PatchType&) { familyHook("leafSolve"); }
 
     ef_mesh deviceMesh() const { return dev_; }
 
 private:
+    [[noreturn]] static void familyHook(const char* name) {
+        throw std::logic_error(std::string("[EllipticForest::HPSAlgorithm::") + name +
+                               "] per-family hooks are not host calls here: the stages run whole tree levels "
+                               "on the GPU, so overriding or calling this hook has no effect on them");
+    }
     void requireBuilt(const char* where) {
         if (!isBuilt) throw std::runtime_error(std::string("[EllipticForest::HPSAlgorithm::") + where +
                                                "] build stage has not run");
