@@ -401,6 +401,87 @@ def var_beta_cells(bounds, N):
     return out
 
 
+def variable_path_run(steps, level=6, N=32, nrhs=4):
+    """The whole path on the variable-coefficient workload ([-1,1]^2, uniform level 6 = 4,096
+    leaves, 32x32 cells, alpha = 1, beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1,
+    manufactured u = sin(3x) cosh(y)), built in FP64 and in mixed precision (FP32 S and T
+    products, FP64 X^-1, FP64 residual + 2 rounds of iterative refinement per solve): buildStage
+    seconds (HIP events, steady state), upwards + solve per step of nrhs right-hand sides, the
+    discretisation error against u, and the mixed result's error against the FP64 one."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    ctx = ef.Context(0)
+    uex = lambda x, y: np.sin(3 * x) * np.cosh(y)
+    beta = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+
+    def fex(x, y):
+        lap = -8.0 * np.sin(3 * x) * np.cosh(y)
+        ux, uy = 3 * np.cos(3 * x) * np.cosh(y), np.sin(3 * x) * np.sinh(y)
+        bx = np.pi * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y)
+        by = -np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)
+        return beta(x, y) * lap + bx * ux + by * uy - uex(x, y)
+
+    B = (-1.0, 1.0, -1.0, 1.0)
+    leaves = ef.refine_by_function(lambda x, y: np.ones_like(x, dtype=bool), N, N, B, level, level)
+    out = dict(workload="variable-coefficient elliptic on [-1,1]^2, uniform level %d (%d leaves), %dx%d cells, "
+               "alpha=1, beta=1+0.5 sin(2 pi x) cos(2 pi y), lambda=-1, manufactured u=sin(3x)cosh(y), %d "
+               "right-hand sides per step" % (level, len(leaves), N, N, nrhs))
+    u64 = None
+    for name, prec in (("fp64", _lib.EF_PRECISION_FP64), ("mixed", _lib.EF_PRECISION_MIXED)):
+        D = ef.DeviceMesh(ctx, leaves, N, B)
+        lb = [tuple(D.node_bounds[n]) for n in D.leaf_nodes]
+        cell = var_beta_cells(np.array(lb), N)
+        builds = []
+        for _ in range(2):
+            D.build(solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell, precision=prec)
+            D.sync()
+            builds.append(D.stage_ms()["build-stage"] / 1e3)
+        centres = []
+        for (xl, xu, yl, yu) in lb:
+            dx, dy = (xu - xl) / N, (yu - yl) / N
+            X, Y = np.meshgrid(xl + dx / 2 + np.arange(N) * dx, yl + dy / 2 + np.arange(N) * dy, indexing="ij")
+            centres.append((X.ravel(), Y.ravel()))
+        f1 = np.stack([fex(X, Y) for X, Y in centres])
+        ue = np.stack([uex(X, Y) for X, Y in centres])
+        M = D.M
+        dxr = 2.0 / M
+        t = -1 + dxr / 2 + np.arange(M) * dxr
+        px = np.concatenate([np.full(M, -1.0), np.full(M, 1.0), t, t])
+        py = np.concatenate([t, t, np.full(M, -1.0), np.full(M, 1.0)])
+        nl = len(leaves)
+        f = ef.DeviceArray(ctx, (nrhs, nl, N * N))
+        r = ef.DeviceArray(ctx, (nrhs, 4 * M))
+        a = ef.DeviceArray(ctx, (4 * M,))
+        u = ef.DeviceArray(ctx, (nrhs, nl, N * N))
+        f.upload(np.broadcast_to(f1, (nrhs,) + f1.shape))
+        r.upload(np.broadcast_to(uex(px, py), (nrhs, 4 * M)))
+        a.upload(np.ones(4 * M))
+        D.upwards_device(f.ptr, nrhs)
+        D.solve_device(f.ptr, r.ptr, a.ptr, None, u.ptr, nrhs)
+        D.sync()
+        timer = ef.Timer(ctx)
+        timer.start(D.stream)
+        for _ in range(steps):
+            D.upwards_device(f.ptr, nrhs)
+            D.solve_device(f.ptr, r.ptr, a.ptr, None, u.ptr, nrhs)
+        timer.stop(D.stream)
+        ms = timer.elapsed_ms() / steps
+        uu = u.download().reshape(nrhs, nl, N * N)[0]
+        res = dict(build_s=builds[-1], build_first_s=builds[0], ms_per_step=ms, solves_per_s=nrhs / (ms / 1e3),
+                   disc_error=float(np.abs(uu - ue).max()))
+        if u64 is None:
+            u64 = uu
+        else:
+            res["rel_err_vs_fp64"] = float(np.abs(uu - u64).max() / np.abs(u64).max())
+            res["refinement_rounds"] = 2
+        out[name] = res
+        log(f"variable path {name}: build {builds[-1] * 1e3:.1f} ms, step {ms:.3f} ms, disc err {res['disc_error']:.3e}"
+            + (f", vs fp64 {res['rel_err_vs_fp64']:.2e}" if "rel_err_vs_fp64" in res else ""))
+        del D
+    out["mixed_build_speedup"] = out["fp64"]["build_s"] / out["mixed"]["build_s"]
+    return out
+
+
 def gpu_variable_run(steps, warmup, level=6, N=32):
     """Leaf solves of the variable-beta leaf set ([-1,1]^2, uniform level 6, N = 32: 4,096 patches,
     five-point stencil with variable beta), one right-hand side per launch."""
@@ -541,7 +622,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nrhs", type=int, default=16)
+    ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--max-level", type=int, default=7)
     ap.add_argument("--N", type=int, default=16)
     ap.add_argument("--cpu-solves", type=int, default=3)
@@ -552,7 +633,8 @@ def main():
                     help="only the step's tree GEMMs (1 + 3 passes) and 3 copies of 256 MiB on the library's copy "
                          "kernel (the calibration of HBM byte counters), then exit")
     ap.add_argument("--serving-requests", type=int, default=10000, help="requests of the serving run (0: skip)")
-    ap.add_argument("--serving-batch", type=int, default=32)
+    ap.add_argument("--no-variable-path", action="store_true", help="skip the variable-coefficient whole path")
+    ap.add_argument("--serving-batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only 5 leaf-solve, 5 particular-Neumann-data and 5 variable-beta leaf-solve launches and 3 "
@@ -637,6 +719,12 @@ def main():
         except Exception as e:
             out["serving_error"] = f"{type(e).__name__}: {e}"
             log(out["serving_error"])
+    if not args.no_variable_path and "error" not in out:
+        try:
+            out["variable_path"] = variable_path_run(max(args.steps // 4, 3))
+        except Exception as e:
+            out["variable_path_error"] = f"{type(e).__name__}: {e}"
+            log(out["variable_path_error"])
     if args.leaf_components:
         try:
             cpu_patch = dict(check=None) if args.no_cpu_baseline else cpu_patch_baseline(

@@ -32,7 +32,8 @@ EXPORTS = [
     "ef_mesh_set_stream", "ef_mesh_stream", "ef_mesh_set_options", "ef_build", "ef_upwards", "ef_upwards_device",
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
     "ef_memcpy", "ef_debug_invert", "ef_host_alloc", "ef_host_free", "ef_mesh_copy_async", "ef_mesh_record",
-    "ef_mesh_query", "ef_mesh_wait", "ef_mesh_eval_load", "ef_mesh_time_sweep_gemms", "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
+    "ef_mesh_query", "ef_mesh_wait", "ef_mesh_eval_load", "ef_mesh_time_sweep_gemms", "ef_mesh_set_refinement",
+    "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
     "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_apply_device_noalloc",
     "ef_patches_dtn", "ef_patches_sync",
     "ef_patches_last_ms", "ef_patches_stream", "ef_timer_create", "ef_timer_destroy", "ef_timer_record",
@@ -108,6 +109,7 @@ def load_library(path):
     L.ef_mesh_wait.argtypes = [_vp, ctypes.c_int]
     L.ef_mesh_eval_load.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     L.ef_mesh_time_sweep_gemms.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
+    L.ef_mesh_set_refinement.argtypes = [_vp, ctypes.c_int]
     L.ef_mesh_stage_ms.argtypes = [_vp, _dp]
     L.ef_patches_create.argtypes = [_vp, ctypes.POINTER(PatchBatch), ctypes.POINTER(Coeffs), ctypes.POINTER(_vp)]
     L.ef_patches_destroy.argtypes = [_vp]
@@ -286,6 +288,10 @@ class DeviceMesh:
         c = Coeffs(solver_type, alpha0, beta0, lambda0, cp)
         self._check(self.L.ef_build(self.h, ctypes.byref(c), precision))
         del keep
+
+    def set_refinement(self, steps):
+        """Iterative-refinement rounds of each solve of a mixed-precision build."""
+        self._check(self.L.ef_mesh_set_refinement(self.h, int(steps)))
 
     def upwards(self, f):
         """f: [nrhs][nleaves][N*N] (or [nleaves][N*N] for one right-hand side)."""

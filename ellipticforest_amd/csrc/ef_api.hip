@@ -259,6 +259,63 @@ __global__ void __launch_bounds__(256) serve_load_kernel(const double* __restric
     }
 }
 
+// FP64 residual of the uniform-mesh five-point system, res = f - A u, for the refinement of a
+// mixed-precision solve.  On a uniform mesh the HPS interface condition (equal and opposite
+// normal derivatives, h_L + h_R = 0) gives g = (u_L + u_R) / 2 at every interface, so the
+// leaf stencils with ghost 2g - u (FiniteVolumeSolver.cpp:100-184) are the global 5-point
+// operator: a cell's neighbour across a leaf side is the neighbouring leaf's edge cell, and on
+// the physical boundary the ghost is 2 g_D - u with g_D = r / a at the root boundary point.
+// Stencil: per cell (variable coefficients, cW cE cS cN cC) or per leaf class.
+struct ResidualArgs {
+    const double* u;
+    const double* f;  // null: zero load
+    double* res;
+    int k, nl, N, M;
+    const int* nbr;  // nl x 4 neighbour leaf (W, E, S, N) or -1
+    const int* pos;  // nl x 2 (ix, iy) on the uniform level
+    const double* r;
+    const double* a;
+    const double* stencil;  // variable: 5 N^2 per leaf, or null
+    const PatchCoef* coef;  // constant: per class
+    const int* cls;
+};
+__global__ void __launch_bounds__(256) mixed_residual_kernel(ResidualArgs p) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    const long long NN = (long long)p.N * p.N, per = (long long)p.nl * NN;
+    if (e >= (long long)p.k * per) return;
+    const long long rhs = e / per, rem = e % per;
+    const int leaf = (int)(rem / NN), I = (int)(rem % NN), N = p.N, i = I / N, j = I % N;
+    double cW, cE, cS, cN, cC;
+    if (p.stencil) {
+        const double* st = p.stencil + (long long)leaf * 5 * NN;
+        cW = st[I]; cE = st[NN + I]; cS = st[2 * NN + I]; cN = st[3 * NN + I]; cC = st[4 * NN + I];
+    } else {
+        const PatchCoef c = p.coef[p.cls[leaf]];
+        cW = cE = c.cx; cS = cN = c.cy; cC = -2.0 * c.cx - 2.0 * c.cy + c.lam;
+    }
+    const double* U = p.u + rhs * per;
+    const double uc = U[(long long)leaf * NN + I];
+    const int ix = p.pos[2 * leaf], iy = p.pos[2 * leaf + 1], M = p.M;
+    const double* R = p.r + rhs * 4LL * M;
+    auto ghost = [&](int side, int t) { return 2.0 * (R[side * M + t] / p.a[side * M + t]) - uc; };
+    double vW, vE, vS, vN;
+    if (i > 0) vW = U[(long long)leaf * NN + I - N];
+    else { const int nb = p.nbr[4 * leaf]; vW = nb >= 0 ? U[(long long)nb * NN + (N - 1) * N + j] : ghost(0, iy * N + j); }
+    if (i < N - 1) vE = U[(long long)leaf * NN + I + N];
+    else { const int nb = p.nbr[4 * leaf + 1]; vE = nb >= 0 ? U[(long long)nb * NN + j] : ghost(1, iy * N + j); }
+    if (j > 0) vS = U[(long long)leaf * NN + I - 1];
+    else { const int nb = p.nbr[4 * leaf + 2]; vS = nb >= 0 ? U[(long long)nb * NN + i * N + N - 1] : ghost(2, ix * N + i); }
+    if (j < N - 1) vN = U[(long long)leaf * NN + I + 1];
+    else { const int nb = p.nbr[4 * leaf + 3]; vN = nb >= 0 ? U[(long long)nb * NN + i * N] : ghost(3, ix * N + i); }
+    const double Au = cC * uc + cW * vW + cE * vE + cS * vS + cN * vN;
+    p.res[e] = (p.f ? p.f[e] : 0.0) - Au;
+}
+
+__global__ void __launch_bounds__(256) axpy_kernel(double* y, const double* x, long long n) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) y[e] += x[e];
+}
+
 struct ef_mesh_s {
     ef_ctx ctx = nullptr;
     hipStream_t own_stream = nullptr;
@@ -312,6 +369,10 @@ struct ef_mesh_s {
     };
     std::vector<SweepGraph> graphs;
     unsigned long long graph_clock = 0;
+    // mixed precision: refinement steps per solve, uniform-mesh neighbour tables, work
+    int refine_steps = 2;
+    DevBuf<int> mix_nbr, mix_pos;
+    DevBuf<double> mix_res, mix_del, mix_zero;
     // serving: slot events, leaf geometry and root boundary points of the load generator
     hipEvent_t slot_ev[EF_MAX_SLOTS] = {};
     bool slot_rec[EF_MAX_SLOTS] = {};
@@ -568,8 +629,9 @@ void plan_build(ef_mesh_s* M) {
             }
         }
         gp.asmd = M->bstore.stage(ad);
-        gp.gS = plan_gemm(pS, n <= 16 ? 1 : 2, M->bstore, st);
-        gp.gT = plan_gemm(pT, 2, M->bstore, st);
+        const bool f32 = M->precision == EF_PRECISION_MIXED;  // mergeS_ / mergeT_ products on FP32 matrix cores
+        gp.gS = plan_gemm(pS, n <= 16 ? 1 : 2, M->bstore, st, f32);
+        gp.gT = plan_gemm(pT, 2, M->bstore, st, f32);
         M->gplans.push_back(std::move(gp));
     }
     M->bstore.flush(st);
@@ -590,8 +652,10 @@ void stage_end(ef_mesh_s* M, int st) {
 void check_tree_kernels() {
     static bool done = false;
     if (done) return;
-    check_kernel_resources((const void*)tree_gemm_kernel<1, 1>, "tree_gemm_kernel<1,1>");
-    check_kernel_resources((const void*)tree_gemm_kernel<2, 2>, "tree_gemm_kernel<2,2>");
+    check_kernel_resources((const void*)tree_gemm_kernel<1, 1, false>, "tree_gemm_kernel<1,1>");
+    check_kernel_resources((const void*)tree_gemm_kernel<2, 2, false>, "tree_gemm_kernel<2,2>");
+    check_kernel_resources((const void*)tree_gemm_kernel<1, 1, true>, "tree_gemm_kernel<1,1,f32>");
+    check_kernel_resources((const void*)tree_gemm_kernel<2, 2, true>, "tree_gemm_kernel<2,2,f32>");
     check_kernel_resources((const void*)tree_gemm_reduce_kernel, "tree_gemm_reduce_kernel");
     check_kernel_resources((const void*)gj_panel_kernel<0, 1, 1, true>, "gj_panel_kernel<0,1,1,fin>");
     check_kernel_resources((const void*)gj_panel_kernel<0, 1, 2, true>, "gj_panel_kernel<0,1,2,fin>");
@@ -631,7 +695,8 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     M->info.ensure(std::max(P.ninternal, 1));
     // the merge descriptors depend only on these addresses and on the leaf DtN each leaf
     // reads: planned again only when one of them moved
-    std::vector<const void*> sig = {M->Tar.p, M->Sar.p, M->Xiar.p, M->wX.p, M->gjd.p, M->gji.p, M->info.p};
+    std::vector<const void*> sig = {M->Tar.p, M->Sar.p, M->Xiar.p, M->wX.p, M->gjd.p, M->gji.p, M->info.p,
+                                    (const void*)(intptr_t)(M->precision + 1)};
     for (size_t li = 0; li < P.leaves.size(); li++) sig.push_back(leaf_T_ptr(M, (int)li));
     if (sig != M->plan_sig) {
         M->plan_sig.clear();
@@ -693,13 +758,42 @@ void do_build(ef_mesh_s* M, const ef_coeffs& c) {
     M->built = true;
 }
 
+// Mixed precision needs the five-point residual of the whole mesh, which this build forms on
+// uniform meshes: every leaf at one level, neighbours found by position.
+void setup_mixed(ef_mesh_s* M) {
+    const Plan& P = M->P;
+    if (M->homogeneous_rhs) throw EfError(EF_ERR_ARG, "mixed precision refines solves with a load (homogeneous-rhs is off)");
+    const int nl = (int)P.leaves.size();
+    const int L = P.nodes[P.leaves[0]].level;
+    for (int li = 0; li < nl; li++)
+        if (P.nodes[P.leaves[li]].level != L)
+            throw EfError(EF_ERR_ARG, "mixed precision needs a uniform mesh (every leaf at one level)");
+    const int side = 1 << L;
+    std::vector<int> at((size_t)side * side, -1), pos(2 * (size_t)nl), nbr(4 * (size_t)nl, -1);
+    for (int li = 0; li < nl; li++) {
+        const NodeP& nd = P.nodes[P.leaves[li]];
+        at[(size_t)nd.ix * side + nd.iy] = li;
+        pos[2 * li] = nd.ix;
+        pos[2 * li + 1] = nd.iy;
+    }
+    for (int li = 0; li < nl; li++) {
+        const int ix = pos[2 * li], iy = pos[2 * li + 1];
+        if (ix > 0) nbr[4 * li] = at[(size_t)(ix - 1) * side + iy];
+        if (ix < side - 1) nbr[4 * li + 1] = at[(size_t)(ix + 1) * side + iy];
+        if (iy > 0) nbr[4 * li + 2] = at[(size_t)ix * side + iy - 1];
+        if (iy < side - 1) nbr[4 * li + 3] = at[(size_t)ix * side + iy + 1];
+    }
+    M->mix_nbr.upload(nbr, M->stream);
+    M->mix_pos.upload(pos, M->stream);
+}
+
 // ----- sweeps ----------------------------------------------------------------
 // The upwards and the Dirichlet solve sweeps are captured once per (k, device pointers) into
 // a HIP graph and replayed with one hipGraphLaunch (EF_GRAPH=0 launches them one by one).
 // Captured work is kernels only: every allocation, descriptor upload and host check happens
 // before capture; graphs are dropped whenever the arenas or descriptor tables they point
 // into are reallocated.
-constexpr size_t MAX_GRAPHS = 16;
+constexpr size_t MAX_GRAPHS = 64;
 
 bool graphs_on() {  // read on every sweep, so a process can compare both ways
     const char* e = std::getenv("EF_GRAPH");
@@ -947,8 +1041,8 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     M->f_last = f_dev;
 }
 
-void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const double* a_dev, const double* b_dev,
-              bool robin, double* u_dev, int k) {
+void do_solve_once(ef_mesh_s* M, const double* f_dev, const double* r_dev, const double* a_dev, const double* b_dev,
+                   bool robin, double* u_dev, int k) {
     if (!M->built) throw EfError(EF_ERR_STATE, "solve stage called before the build stage");
     if (k < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
     if (!M->homogeneous_rhs && M->k_up != k)
@@ -1029,6 +1123,48 @@ void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const doub
         if (info) throw EfError(EF_ERR_SINGULAR, "zero pivot in the root boundary system");
     }
     downward();
+}
+
+// Mixed precision: the solve with the FP32-product operators, then refine_steps rounds of
+// iterative refinement -- the FP64 residual of the five-point system (mixed_residual_kernel),
+// the correction from an upwards + solve pass with that residual as the load and zero
+// Dirichlet data, and u += correction.  The rounds reuse the upward state, so a mixed solve
+// consumes its upwards stage (the next solve needs a new one).
+void do_solve(ef_mesh_s* M, const double* f_dev, const double* r_dev, const double* a_dev, const double* b_dev,
+              bool robin, double* u_dev, int k) {
+    if (M->precision != EF_PRECISION_MIXED) {
+        do_solve_once(M, f_dev, r_dev, a_dev, b_dev, robin, u_dev, k);
+        return;
+    }
+    if (robin) throw EfError(EF_ERR_ARG, "mixed precision refines Dirichlet root conditions only (b = 0)");
+    do_solve_once(M, f_dev, r_dev, a_dev, b_dev, false, u_dev, k);
+    const Plan& P = M->P;
+    const int nl = (int)P.leaves.size(), N = P.N, m = 4 * P.nodes[0].s;
+    const long long cnt = (long long)k * nl * N * N;
+    M->mix_res.ensure(cnt);
+    M->mix_del.ensure(cnt);
+    if (M->mix_zero.n < (size_t)k * m) {
+        M->mix_zero.alloc((size_t)k * m);
+        EF_LAUNCH("fill_kernel", fill_kernel, dim3(blocks_for((long long)k * m)), dim3(256), M->stream, M->mix_zero.p, 0.0,
+                  (long long)k * m);
+    }
+    check_range(u_dev, (size_t)cnt * 8, "solution u");
+    if (f_dev) check_range(f_dev, (size_t)cnt * 8, "load f");
+    ResidualArgs ra{};
+    ra.u = u_dev; ra.f = f_dev; ra.res = M->mix_res.p;
+    ra.k = k; ra.nl = nl; ra.N = N; ra.M = P.nodes[0].s;
+    ra.nbr = M->mix_nbr.p; ra.pos = M->mix_pos.p; ra.r = r_dev; ra.a = a_dev;
+    ra.stencil = M->variable ? M->vstencil.p : nullptr;
+    ra.coef = M->classes.p; ra.cls = M->leaf_cls.p;
+    for (int it = 0; it < M->refine_steps; it++) {
+        EF_LAUNCH("mixed_residual_kernel", mixed_residual_kernel, dim3(blocks_for(cnt)), dim3(256), M->stream, ra);
+        do_upwards(M, M->mix_res.p, k);
+        do_solve_once(M, M->mix_res.p, M->mix_zero.p, a_dev, nullptr, false, M->mix_del.p, k);
+        EF_LAUNCH("axpy_kernel", axpy_kernel, dim3(blocks_for(cnt)), dim3(256), M->stream, u_dev,
+                  (const double*)M->mix_del.p, cnt);
+    }
+    M->k_up = 0;
+    M->f_last = nullptr;
 }
 
 }  // namespace
@@ -1212,8 +1348,9 @@ int ef_build(ef_mesh mesh, const ef_coeffs* coeffs, int precision) {
     if (!mesh || !coeffs) throw EfError(EF_ERR_ARG, "null argument");
     if (coeffs->solver_type != EF_SOLVER_FISHPACK && coeffs->solver_type != EF_SOLVER_FIVE_POINT)
         throw EfError(EF_ERR_ARG, "unknown solver type");
-    if (precision != EF_PRECISION_FP64) throw EfError(EF_ERR_ARG, "only FP64 precision is available in this build");
+    if (precision != EF_PRECISION_FP64 && precision != EF_PRECISION_MIXED) throw EfError(EF_ERR_ARG, "unknown precision");
     EF_HIP(hipSetDevice(mesh->ctx->device));
+    if (precision == EF_PRECISION_MIXED) setup_mixed(mesh);
     mesh->built = false;
     mesh->sweeps.clear();
     mesh->k_up = 0;
@@ -1389,6 +1526,14 @@ int ef_host_free(ef_ctx ctx, void* ptr) {
     EF_API_BEGIN
     if (!ctx) throw EfError(EF_ERR_ARG, "null context");
     if (ptr) EF_HIP(hipHostFree(ptr));
+    EF_API_END
+}
+
+int ef_mesh_set_refinement(ef_mesh mesh, int steps) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    if (steps < 0 || steps > 16) throw EfError(EF_ERR_ARG, "refinement steps must be in 0..16");
+    mesh->refine_steps = steps;
     EF_API_END
 }
 

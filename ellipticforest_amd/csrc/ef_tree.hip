@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -29,8 +30,22 @@ namespace efk {
 typedef double td4 __attribute__((ext_vector_type(4)));
 typedef double td2 __attribute__((ext_vector_type(2)));
 
+typedef float tf4 __attribute__((ext_vector_type(4)));
+
 __device__ inline td4 mfma_f64(double a, double b, td4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// v_mfma_f32_16x16x4_f32: the same operand layout (lane l: A[l%16][l/16], B[l/16][l%16]);
+// the accumulator holds rows 4 (l/16) + v of column l%16 (the f64 instruction: rows 4 v + l/16)
+__device__ inline tf4 mfma_f32(double a, double b, tf4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, (float)b, c, 0, 0, 0);
+}
+template <bool F32>
+using GemmAcc = std::conditional_t<F32, tf4, td4>;
+template <bool F32>
+__device__ inline GemmAcc<F32> mfma_acc(double a, double b, GemmAcc<F32> c) {
+    if constexpr (F32) return mfma_f32(a, b, c);
+    else return mfma_f64(a, b, c);
 }
 
 // lane `src`'s value of x in every lane (src uniform)
@@ -53,10 +68,10 @@ __device__ inline T pick4(const T (&a)[4], int c) {
 // Within each block of 16 k values lane group q takes k = kb + 4q + s at MFMA step s: A is
 // read as two 16-byte vectors per lane and row (a 128-byte row segment per lane group), B as
 // one row segment of 16 consecutive columns per lane group and step.
-template <int RT, int TN>
+template <int RT, int TN, bool F32>
 __device__ inline void gemm_segment(const double* __restrict__ a0p, const double* __restrict__ a1p,
                                     const double* const (&bp)[TN], long long ldb, int K, double alpha,
-                                    td4 (&acc)[2][TN]) {
+                                    GemmAcc<F32> (&acc)[2][TN]) {
     const int q = (int)(threadIdx.x & 63) >> 4;
     // the operands of block kb: A rows (two 16-byte vectors per row) and B columns
     struct Blk {
@@ -94,8 +109,8 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         for (int s = 0; s < 4; s++)
 #pragma unroll
             for (int j = 0; j < TN; j++) {
-                acc[0][j] = mfma_f64(a0[s], o.b[j][s], acc[0][j]);
-                if (RT == 2) acc[1][j] = mfma_f64(a1[s], o.b[j][s], acc[1][j]);
+                acc[0][j] = mfma_acc<F32>(a0[s], o.b[j][s], acc[0][j]);
+                if (RT == 2) acc[1][j] = mfma_acc<F32>(a1[s], o.b[j][s], acc[1][j]);
             }
     };
     const int nfull = K / 16;
@@ -131,13 +146,13 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         for (int s = 0; s < 4; s++)
 #pragma unroll
             for (int j = 0; j < TN; j++) {
-                acc[0][j] = mfma_f64(a0[s], b[j][s], acc[0][j]);
-                if (RT == 2) acc[1][j] = mfma_f64(a1[s], b[j][s], acc[1][j]);
+                acc[0][j] = mfma_acc<F32>(a0[s], b[j][s], acc[0][j]);
+                if (RT == 2) acc[1][j] = mfma_acc<F32>(a1[s], b[j][s], acc[1][j]);
             }
     }
 }
 
-template <int RT, int TN>
+template <int RT, int TN, bool F32>
 __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restrict__ probs,
                                                          const int* __restrict__ start, int nprob) {
     const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -162,11 +177,11 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
     int cj[TN];
 #pragma unroll
     for (int j = 0; j < TN; j++) cj[j] = min(col0 + 16 * j + r, p.N - 1);
-    td4 acc[2][TN];
+    GemmAcc<F32> acc[2][TN];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < TN; j++) acc[i][j] = td4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < TN; j++) acc[i][j] = GemmAcc<F32>{0, 0, 0, 0};
     // k range of this wave: all of both segments, or one chunk of one segment
     int kb0 = 0, ke0 = p.K0, kb1 = 0, ke1 = p.K1;
     if (p.kc > 0) {
@@ -178,14 +193,14 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
         const double* bp[TN];
 #pragma unroll
         for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj[j] * p.bcs;
-        gemm_segment<RT, TN>(p.A0 + (long long)rows[0] * p.lda0 + kb0, p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp,
+        gemm_segment<RT, TN, F32>(p.A0 + (long long)rows[0] * p.lda0 + kb0, p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp,
                          p.ldb0, ke0 - kb0, p.alpha0, acc);
     }
     if (ke1 > kb1) {
         const double* bp[TN];
 #pragma unroll
         for (int j = 0; j < TN; j++) bp[j] = p.B1 + (long long)kb1 * p.ldb1 + (long long)cj[j] * p.bcs;
-        gemm_segment<RT, TN>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
+        gemm_segment<RT, TN, F32>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
                          p.ldb1, ke1 - kb1, p.alpha1, acc);
     }
     // with RT = 1 both accumulator rows hold the same 16 rows; only the first is stored
@@ -195,12 +210,12 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
         for (int j = 0; j < TN; j++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = row0 + 16 * i + 4 * v + q, col = col0 + 16 * j + r;
+                const int row = row0 + 16 * i + (F32 ? 4 * q + v : 4 * v + q), col = col0 + 16 * j + r;
                 if (row < p.M && col < p.N) {
                     if (p.kc > 0) {
-                        p.P[((long long)chunk * p.M + row) * p.N + col] = acc[i][j][v];
+                        p.P[((long long)chunk * p.M + row) * p.N + col] = (double)acc[i][j][v];
                     } else {
-                        double o = acc[i][j][v];
+                        double o = (double)acc[i][j][v];
                         if (p.C) o += p.beta * p.C[(long long)row * p.ldc + (long long)col * p.ccs];
                         p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
                     }
@@ -208,9 +223,10 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
             }
 }
 
-template __global__ void tree_gemm_kernel<1, 1>(const TreeGemm*, const int*, int);
-template __global__ void tree_gemm_kernel<2, 1>(const TreeGemm*, const int*, int);
-template __global__ void tree_gemm_kernel<2, 2>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 1, false>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<2, 2, false>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 1, true>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<2, 2, true>(const TreeGemm*, const int*, int);
 
 // D = sum over chunks of the partials (in chunk order) + beta C, one thread per element of the
 // split problems of a launch.
@@ -869,9 +885,10 @@ __global__ void __launch_bounds__(256) tree_root_robin_kernel(const double* T, c
 // ---------------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------------
-GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& store, hipStream_t s) {
+GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& store, hipStream_t s, bool f32) {
     GemmLaunch g;
     g.tn = tn;
+    g.f32 = f32;
     g.rt = (tn == 1) ? 1 : 2;
     if (probs_in.empty()) return g;
     std::vector<TreeGemm> probs = probs_in;
@@ -935,8 +952,14 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
 void run_gemm(const GemmLaunch& g, hipStream_t s) {
     if (g.nwaves == 0) return;
     const dim3 grid(blocks_for(g.nwaves, 4)), block(256);
-    if (g.tn == 1) EF_LAUNCH("tree_gemm_kernel<1,1>", (tree_gemm_kernel<1, 1>), grid, block, s, g.probs, g.start, g.nprob);
-    else EF_LAUNCH("tree_gemm_kernel<2,2>", (tree_gemm_kernel<2, 2>), grid, block, s, g.probs, g.start, g.nprob);
+    if (g.f32) {
+        if (g.tn == 1)
+            EF_LAUNCH("tree_gemm_kernel<1,1,f32>", (tree_gemm_kernel<1, 1, true>), grid, block, s, g.probs, g.start, g.nprob);
+        else
+            EF_LAUNCH("tree_gemm_kernel<2,2,f32>", (tree_gemm_kernel<2, 2, true>), grid, block, s, g.probs, g.start, g.nprob);
+    } else if (g.tn == 1)
+        EF_LAUNCH("tree_gemm_kernel<1,1>", (tree_gemm_kernel<1, 1, false>), grid, block, s, g.probs, g.start, g.nprob);
+    else EF_LAUNCH("tree_gemm_kernel<2,2>", (tree_gemm_kernel<2, 2, false>), grid, block, s, g.probs, g.start, g.nprob);
     if (g.nelem > 0)
         EF_LAUNCH("tree_gemm_reduce_kernel", tree_gemm_reduce_kernel, dim3(blocks_for(g.nelem)), block, s, g.probs,
                   g.estart, g.nprob);

@@ -70,7 +70,9 @@ struct TreeGemm {
 };
 static_assert(sizeof(TreeGemm) == 144, "TreeGemm layout");
 
-template <int RT, int TN>
+// F32: operands rounded to FP32, v_mfma_f32_16x16x4_f32 with FP32 accumulation, FP64 output
+// (+ beta C in FP64); the mixed-precision build's S and T products.
+template <int RT, int TN, bool F32>
 __global__ void tree_gemm_kernel(const TreeGemm* probs, const int* start, int nprob);
 __global__ void tree_gemm_reduce_kernel(const TreeGemm* probs, const long long* estart, int nprob);
 
@@ -164,12 +166,13 @@ struct GemmLaunch {
     long long* estart = nullptr;  // reduce: first element of each problem (split problems only)
     int nprob = 0, nwaves = 0, rt = 2, tn = 2;
     long long nelem = 0;
+    bool f32 = false;  // FP32 matrix cores (mixed precision)
 };
 // Checks every problem (extents inside their allocations, alignment), allocates the partials of
 // split problems, stages problems and wave offsets in `store` (device memory the caller keeps
 // alive; valid on the device after store.flush) and returns the launch.  tn = 1 (16 x 16 blocks) suits N <= 16, tn = 2 (32 x 32) the rest.
 struct DevStore;
-GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s);
+GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s, bool f32 = false);
 void run_gemm(const GemmLaunch& g, hipStream_t s);
 
 // Inversion of `B` matrices of order m held in wa (B x m x m, destroyed); X^-1 into xi

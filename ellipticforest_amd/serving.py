@@ -20,6 +20,9 @@
   exact and the stages are launched one by one.
 * Every right-hand side is reduced in an order that does not depend on the batch width, so
   a request's result is the same alone and in a batch.
+* Results stay in the slot's device buffer; a request that wants its solution on the host
+  (``want_u``) has it copied into one of the slot's ``fetch_rows`` page-locked rows, so at
+  most that many such requests share a batch.
 * ``stats()`` reports solves/s and p50/p99 latency (arrival to completion).
 """
 from __future__ import annotations
@@ -113,35 +116,37 @@ class Request:
 
 
 class _Slot:
-    def __init__(self, ctx, nl, NN, M4, max_batch):
+    def __init__(self, ctx, nl, NN, M4, max_batch, fetch_rows):
         self.params = PinnedArray(ctx, (max_batch, NPARAM))
         self.dparams = _lib.DeviceArray(ctx, (max_batch, NPARAM))
         self.f = _lib.DeviceArray(ctx, (max_batch, nl, NN))
         self.r = _lib.DeviceArray(ctx, (max_batch, M4))
         self.u = _lib.DeviceArray(ctx, (max_batch, nl, NN))
-        self.hu = PinnedArray(ctx, (max_batch, nl, NN))
+        self.hu = PinnedArray(ctx, (fetch_rows, nl, NN))  # results fetched to the host
         self.hf = None  # page-locked staging of host-array requests, made on first use
         self.hr = None
         self.batch = None
 
 
 class _Resident:
-    def __init__(self, ctx, dm, footprint, max_batch, nslots):
+    def __init__(self, ctx, dm, footprint, max_batch, nslots, fetch_rows):
         self.dm = dm
         self.footprint = footprint
         self.last_used = time.perf_counter()
         nl, NN, M4 = dm.nleaves, dm.N * dm.N, 4 * dm.M
-        self.slots = [_Slot(ctx, nl, NN, M4, max_batch) for _ in range(nslots)]
+        self.slots = [_Slot(ctx, nl, NN, M4, max_batch, fetch_rows) for _ in range(nslots)]
         self.next_slot = 0
         self.a = _lib.DeviceArray(ctx, (M4,))
         self.a.upload(np.ones(M4))
 
 
 class Server:
-    def __init__(self, ctx, hbm_budget_bytes=200 << 30, max_batch=32, nslots=3, buckets=(1, 2, 4, 8, 16, 32)):
+    def __init__(self, ctx, hbm_budget_bytes=200 << 30, max_batch=32, nslots=3, buckets=(1, 2, 4, 8, 16, 32, 64),
+                 fetch_rows=8):
         if not 1 <= nslots <= 8:
             raise ValueError("nslots must be in 1..8")
         self.ctx = ctx
+        self.fetch_rows = max(1, min(fetch_rows, max_batch))  # results a batch can return to the host
         self.budget = hbm_budget_bytes
         self.max_batch = max_batch
         self.nslots = nslots
@@ -180,7 +185,7 @@ class Server:
         dm = _lib.DeviceMesh(self.ctx, leaves, N, bounds)
         dm.build(**build_args)
         with self.lock:
-            self.meshes[key] = _Resident(self.ctx, dm, fp, self.max_batch, self.nslots)
+            self.meshes[key] = _Resident(self.ctx, dm, fp, self.max_batch, self.nslots, self.fetch_rows)
         return dm
 
     def _inflight_keys(self):
@@ -213,10 +218,13 @@ class Server:
                 return None, None, []
             key, kind = self.pending[0].mesh, self.pending[0].kind
             batch, rest = [], collections.deque()
+            fetch = 0
             while self.pending:
                 q = self.pending.popleft()
-                if q.mesh == key and q.kind == kind and len(batch) < self.max_batch:
+                if q.mesh == key and q.kind == kind and len(batch) < self.max_batch and \
+                        (not q.want_u or fetch < self.fetch_rows):
                     batch.append(q)
+                    fetch += q.want_u
                 else:
                     rest.append(q)
             self.pending = rest
@@ -252,9 +260,12 @@ class Server:
         dm.upwards_device(s.f.ptr, k)
         dm.solve_device(s.f.ptr, s.r.ptr, res.a.ptr, None, s.u.ptr, k)
         per = nl * NN * 8
+        s.rows = {}
         for j, q in enumerate(batch):
             if q.want_u:
-                _lib.check(L.ef_mesh_copy_async(dm.h, s.hu.ptr + j * per, s.u.ptr + j * per, per, 1), L)
+                row = len(s.rows)
+                s.rows[j] = row
+                _lib.check(L.ef_mesh_copy_async(dm.h, s.hu.ptr + row * per, s.u.ptr + j * per, per, 1), L)
         _lib.check(L.ef_mesh_record(dm.h, si), L)
         s.batch = batch
         self.inflight.append((key, si, res))
@@ -276,7 +287,7 @@ class Server:
         now = time.perf_counter()
         for j, q in enumerate(s.batch):
             if q.want_u:
-                q.u = s.hu.a[j].copy()
+                q.u = s.hu.a[s.rows[j]].copy()
             q.t_done = now
             q.done.set()
         self.completed.extend(s.batch)

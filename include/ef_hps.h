@@ -46,7 +46,7 @@ extern "C" {
 
 /* Arithmetic of the operators. */
 #define EF_PRECISION_FP64 0
-#define EF_PRECISION_MIXED 1     /* FP32 merge products, FP64 interface solve and refinement */
+#define EF_PRECISION_MIXED 1     /* FP32 merge products, FP64 interface inverse, refined solves */
 
 typedef struct ef_ctx_s* ef_ctx;
 typedef struct ef_mesh_s* ef_mesh;
@@ -143,6 +143,14 @@ int ef_mesh_sync(ef_mesh mesh);
 int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_device_free(ef_ctx ctx, void* ptr);
 int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* Mixed precision (EF_PRECISION_MIXED at ef_build; no counterpart in the reference, which is
+ * FP64 throughout): the S and T products of every merge on FP32 matrix cores (FP32 operands and
+ * accumulation), X^-1 and the leaf operators in FP64; each solve stage then runs `steps` rounds
+ * of iterative refinement: the FP64 residual of the five-point system, a correction solve with
+ * it as the load and zero Dirichlet data, u += correction.  Uniform meshes, Dirichlet root data,
+ * homogeneous-rhs off; a mixed solve consumes its upwards stage.  Default 2 rounds. */
+int ef_mesh_set_refinement(ef_mesh mesh, int steps);
 
 /* Serving data path (no counterpart in the reference: the request loop keeps factorised
  * meshes resident and feeds them batches of right-hand sides).

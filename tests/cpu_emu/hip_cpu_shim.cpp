@@ -152,6 +152,23 @@ emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int) {
     return out;
 }
 
+emu_f4 emu_mfma_f32(float a, float b, emu_f4 c, int, int, int) {
+    const int t = threadIdx.x, w = t / 64 * 64, l = t % 64;
+    const int p = g_par;
+    g_par ^= 1;
+    g_mfma_a[p][t] = a;
+    g_mfma_b[p][t] = b;
+    emu_wave_sync();
+    emu_f4 out = c;
+    for (int v = 0; v < 4; v++) {
+        const int row = 4 * (l / 16) + v, col = l % 16;  // the f32 accumulator layout
+        float s = 0.0f;
+        for (int k = 0; k < 4; k++) s += (float)g_mfma_a[p][w + row + 16 * k] * (float)g_mfma_b[p][w + col + 16 * k];
+        out[v] += s;
+    }
+    return out;
+}
+
 unsigned long long __ballot(int pred) {
     const int t = threadIdx.x, w = t / 64 * 64;
     const int p = g_par;

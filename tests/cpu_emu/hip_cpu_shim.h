@@ -125,6 +125,9 @@ emu_d4 emu_mfma_f64(double a, double b, emu_d4 c, int, int, int);
 double __shfl(double v, int src, int width = 64);
 inline double __shfl_xor(double v, int mask, int width = 64) { return __shfl(v, (int)(threadIdx.x % 64) ^ mask, width); }
 #define __builtin_amdgcn_mfma_f64_16x16x4f64 emu_mfma_f64
+typedef float emu_f4 __attribute__((ext_vector_type(4)));
+emu_f4 emu_mfma_f32(float a, float b, emu_f4 c, int, int, int);
+#define __builtin_amdgcn_mfma_f32_16x16x4f32 emu_mfma_f32
 inline int __double2loint(double x) { unsigned long long u; std::memcpy(&u, &x, 8); return (int)(unsigned)u; }
 inline int __double2hiint(double x) { unsigned long long u; std::memcpy(&u, &x, 8); return (int)(unsigned)(u >> 32); }
 inline double __hiloint2double(int hi, int lo) {

@@ -99,3 +99,9 @@ def test_emu_batched_inverse_pivots(emu_ctx, m, B, integer):
 
 def test_emu_batched_inverse_pivot_ties(emu_ctx):
     gp.inverse_case(emu_ctx, 64, 3, seed=5, sparse=True)
+
+
+def test_emu_mixed_precision(emu_ctx):
+    """FP32 matrix-core products (emulated v_mfma_f32_16x16x4 layout) and the refinement loop."""
+    gp.mixed_case(emu_ctx, 2, 8, variable=True)
+    gp.test_mixed_precision_refuses_adaptive(emu_ctx)

@@ -367,3 +367,65 @@ def test_batched_inverse_tournament(ctx, m, B):
     interchanges are not LAPACK's."""
     say(f"batched inverse (tournament) m={m} B={B}")
     inverse_case(ctx, m, B, seed=m, check_pivots=False)
+
+
+def mixed_case(ctx, level, N, variable):
+    """Mixed precision (FP32 S and T products, FP64 X^-1, FP64 residual + iterative refinement)
+    against the FP64 build of the same mesh: the operators differ at FP32 rounding (neither
+    equal nor wrong), the unrefined solve is off by FP32-level error, and refinement brings the
+    solution to the FP64 one (<= 1e-8 is the target; 1e-10 is asserted after two rounds)."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    leaves, bounds = omesh.uniform(level), (-1.0, 1.0, -1.0, 1.0)
+    uf, ff = (U3, F3) if variable else (U, F)
+
+    def make(prec):
+        D = ef.DeviceMesh(ctx, leaves, N, bounds)
+        if variable:
+            cell = np.concatenate([oracle.sample_coeffs(N, tuple(D.node_bounds[n]), ALPHA, BETA, LAMBDA).ravel()
+                                   for n in D.leaf_nodes])
+            D.build(solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell, precision=prec)
+        else:
+            D.build(precision=prec)
+        return D
+
+    D64, Dm = make(_lib.EF_PRECISION_FP64), make(_lib.EF_PRECISION_MIXED)
+    f = np.stack([ff(*oracle.cell_centres(N, tuple(D64.node_bounds[n]))[:2]) for n in D64.leaf_nodes])
+    side, px, py = oracle.root_face_points(D64.M, bounds)
+    r, a = uf(px, py), np.ones_like(px)
+    D64.upwards(f)
+    u64 = D64.solve(r, a)
+    for w in ("S", "T"):
+        d = rel(Dm.get(0, w), D64.get(0, w))
+        assert 1e-13 < d < 1e-4, (w, d)
+    errs = {}
+    for steps in (0, 1, 2):
+        Dm.set_refinement(steps)
+        Dm.upwards(f)
+        errs[steps] = rel(Dm.solve(r, a), u64)
+    say(f"mixed precision vs FP64: {errs}")
+    assert 1e-12 < errs[0] < 1e-3
+    assert errs[1] < 1e-8 and errs[2] < 1e-10
+    with pytest.raises(Exception):
+        Dm.solve(r, a)  # the refinement consumed the upwards stage
+    return errs
+
+
+def test_mixed_precision_poisson(ctx):
+    say("mixed precision: Poisson, uniform level 3, N=16")
+    mixed_case(ctx, 3, 16, variable=False)
+
+
+def test_mixed_precision_variable_beta(ctx):
+    say("mixed precision: variable beta, uniform level 3, N=16")
+    mixed_case(ctx, 3, 16, variable=True)
+
+
+def test_mixed_precision_refuses_adaptive(ctx):
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import _lib
+    leaves = omesh.refine_by_function(lambda x, y: (x < 0.2) & (y < 0.2), 8, 8, 0, 1, 0, 1, 0, 3)
+    assert len({l[0] for l in leaves}) > 1
+    D = ef.DeviceMesh(ctx, leaves, 8, (0, 1, 0, 1))
+    with pytest.raises(Exception):
+        D.build(precision=_lib.EF_PRECISION_MIXED)
