@@ -154,15 +154,11 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
 
 template <int RT, int TN, bool F32>
 __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restrict__ probs,
-                                                         const int* __restrict__ start, int nprob) {
+                                                         const int* __restrict__ start,
+                                                         const int* __restrict__ wprob, int nwaves) {
     const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    if (w >= start[nprob]) return;  // uniform per wave
-    int lo = 0, hi = nprob - 1;     // last problem whose first wave is <= w
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= w) lo = mid;
-        else hi = mid - 1;
-    }
+    if (w >= nwaves) return;  // uniform per wave
+    const int lo = __builtin_amdgcn_readfirstlane(wprob[w]);  // the wave's problem (a table, not a search)
     const TreeGemm p = probs[lo];
     const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
     constexpr int RW = 16 * RT, CW = 16 * TN;
@@ -223,10 +219,10 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
             }
 }
 
-template __global__ void tree_gemm_kernel<1, 1, false>(const TreeGemm*, const int*, int);
-template __global__ void tree_gemm_kernel<2, 2, false>(const TreeGemm*, const int*, int);
-template __global__ void tree_gemm_kernel<1, 1, true>(const TreeGemm*, const int*, int);
-template __global__ void tree_gemm_kernel<2, 2, true>(const TreeGemm*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 1, false>(const TreeGemm*, const int*, const int*, int);
+template __global__ void tree_gemm_kernel<2, 2, false>(const TreeGemm*, const int*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 1, true>(const TreeGemm*, const int*, const int*, int);
+template __global__ void tree_gemm_kernel<2, 2, true>(const TreeGemm*, const int*, const int*, int);
 
 // D = sum over chunks of the partials (in chunk order) + beta C, one thread per element of the
 // split problems of a launch.
@@ -940,8 +936,12 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
     estart[probs.size()] = elems;
     g.nprob = (int)probs.size();
     g.nwaves = (int)total;
+    std::vector<int> wprob((size_t)total);
+    for (size_t i = 0; i < probs.size(); i++)
+        for (int w = start[i]; w < start[i + 1]; w++) wprob[w] = (int)i;
     g.probs = store.stage(probs);
     g.start = store.stage(start);
+    g.wprob = store.stage(wprob);
     if (any_split) {
         g.estart = store.stage(estart);
         g.nelem = elems;
@@ -954,12 +954,16 @@ void run_gemm(const GemmLaunch& g, hipStream_t s) {
     const dim3 grid(blocks_for(g.nwaves, 4)), block(256);
     if (g.f32) {
         if (g.tn == 1)
-            EF_LAUNCH("tree_gemm_kernel<1,1,f32>", (tree_gemm_kernel<1, 1, true>), grid, block, s, g.probs, g.start, g.nprob);
+            EF_LAUNCH("tree_gemm_kernel<1,1,f32>", (tree_gemm_kernel<1, 1, true>), grid, block, s, g.probs, g.start, g.wprob,
+                      g.nwaves);
         else
-            EF_LAUNCH("tree_gemm_kernel<2,2,f32>", (tree_gemm_kernel<2, 2, true>), grid, block, s, g.probs, g.start, g.nprob);
+            EF_LAUNCH("tree_gemm_kernel<2,2,f32>", (tree_gemm_kernel<2, 2, true>), grid, block, s, g.probs, g.start, g.wprob,
+                      g.nwaves);
     } else if (g.tn == 1)
-        EF_LAUNCH("tree_gemm_kernel<1,1>", (tree_gemm_kernel<1, 1, false>), grid, block, s, g.probs, g.start, g.nprob);
-    else EF_LAUNCH("tree_gemm_kernel<2,2>", (tree_gemm_kernel<2, 2, false>), grid, block, s, g.probs, g.start, g.nprob);
+        EF_LAUNCH("tree_gemm_kernel<1,1>", (tree_gemm_kernel<1, 1, false>), grid, block, s, g.probs, g.start, g.wprob,
+                  g.nwaves);
+    else EF_LAUNCH("tree_gemm_kernel<2,2>", (tree_gemm_kernel<2, 2, false>), grid, block, s, g.probs, g.start, g.wprob,
+                   g.nwaves);
     if (g.nelem > 0)
         EF_LAUNCH("tree_gemm_reduce_kernel", tree_gemm_reduce_kernel, dim3(blocks_for(g.nelem)), block, s, g.probs,
                   g.estart, g.nprob);

@@ -73,7 +73,7 @@ static_assert(sizeof(TreeGemm) == 144, "TreeGemm layout");
 // F32: operands rounded to FP32, v_mfma_f32_16x16x4_f32 with FP32 accumulation, FP64 output
 // (+ beta C in FP64); the mixed-precision build's S and T products.
 template <int RT, int TN, bool F32>
-__global__ void tree_gemm_kernel(const TreeGemm* probs, const int* start, int nprob);
+__global__ void tree_gemm_kernel(const TreeGemm* probs, const int* start, const int* wprob, int nwaves);
 __global__ void tree_gemm_reduce_kernel(const TreeGemm* probs, const long long* estart, int nprob);
 
 // ---------------------------------------------------------------------------------------
@@ -163,6 +163,7 @@ __global__ void tree_root_robin_kernel(const double* T, const double* a, const d
 struct GemmLaunch {
     TreeGemm* probs = nullptr;
     int* start = nullptr;
+    int* wprob = nullptr;         // problem of each wave
     long long* estart = nullptr;  // reduce: first element of each problem (split problems only)
     int nprob = 0, nwaves = 0, rt = 2, tn = 2;
     long long nelem = 0;

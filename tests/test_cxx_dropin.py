@@ -69,12 +69,12 @@ def oracle_errors(leaves, N):
 
 
 def test_dropin_uniform(example):
-    v = run(example, "--uniform=1", "--max-level=2", "--nx=8")
-    assert int(v["leaves"][0]) == 16
+    v = run(example, "--uniform=1", "--max-level=1", "--nx=8")
+    assert int(v["leaves"][0]) == 4
     got = np.array([float(t) for t in v["errors"]])
-    ref = oracle_errors(omesh.uniform(2), 8)
+    ref = oracle_errors(omesh.uniform(1), 8)
     assert np.abs(got - ref).max() <= 1e-10 * ref.max()
-    assert int(v["root_T_rows"][0]) == 4 * 32
+    assert int(v["root_T_rows"][0]) == 4 * 16
     assert float(v["timer build-stage"][0]) > 0
 
 

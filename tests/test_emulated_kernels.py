@@ -15,6 +15,7 @@ def test_emu_uniform_L1_all_operators(emu_ctx):
     assert gp.rel(ug, uo) < gp.TOL_U
 
 
+@pytest.mark.skipif(not __import__("os").environ.get("EF_SLOW_TESTS"), reason="a minute on the emulator; the GPU runs it")
 def test_emu_golden_global_solve(emu_ctx):
     gp.test_uniform_equals_global_solve(emu_ctx, "global_5pt_L2N8_helmholtz")
 
@@ -38,7 +39,7 @@ def test_emu_double_coarsening(emu_ctx):
 
 
 def test_emu_robin_root(emu_ctx):
-    gp.test_robin_root_condition(emu_ctx)
+    gp.test_robin_root_condition(emu_ctx, level=1)
 
 
 def test_emu_homogeneous_rhs(emu_ctx):
@@ -92,16 +93,16 @@ def test_emu_graph_replay(emu_ctx):
     gp.graph_replay_case(emu_ctx, leaves, 4, (0, 1, 0, 1), k=2)
 
 
-@pytest.mark.parametrize("m,B,integer", [(16, 3, False), (64, 2, True), (96, 1, False)])
+@pytest.mark.parametrize("m,B,integer", [(16, 3, False), (48, 1, True), (96, 1, False)])
 def test_emu_batched_inverse_pivots(emu_ctx, m, B, integer):
     gp.inverse_case(emu_ctx, m, B, seed=m + B, integer=integer)
 
 
 def test_emu_batched_inverse_pivot_ties(emu_ctx):
-    gp.inverse_case(emu_ctx, 64, 3, seed=5, sparse=True)
+    gp.inverse_case(emu_ctx, 48, 1, seed=5, sparse=True)
 
 
 def test_emu_mixed_precision(emu_ctx):
     """FP32 matrix-core products (emulated v_mfma_f32_16x16x4 layout) and the refinement loop."""
-    gp.mixed_case(emu_ctx, 2, 8, variable=True)
+    gp.mixed_case(emu_ctx, 1, 8, variable=True)
     gp.test_mixed_precision_refuses_adaptive(emu_ctx)

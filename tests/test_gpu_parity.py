@@ -164,10 +164,10 @@ def test_homogeneous_rhs_option(ctx):
     assert rel(D.solve(r, a), uo) < TOL_U
 
 
-def test_robin_root_condition(ctx):
+def test_robin_root_condition(ctx, level=2):
     import ellipticforest_amd as ef
-    say("root condition a u + b du/dn = r with b != 0, uniform level 2 N=8")
-    leaves = omesh.uniform(2)
+    say(f"root condition a u + b du/dn = r with b != 0, uniform level {level} N=8")
+    leaves = omesh.uniform(level)
     O = oracle.Forest(leaves, 8, (0, 1, 0, 1), solver=oracle.FISHPACK)
     O.build()
     f = O.sample_rhs(F)
