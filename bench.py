@@ -4,7 +4,7 @@ Workload (default): adaptive Poisson on [-10,10]^2, f = -(sin x + sin y), Dirich
 u = sin x + sin y, refineByFunction(|f| > 1.2, min level 0, max level 7), 16x16 cells per
 patch (7,750 leaves, root side 512), FISHPACK semantics, FP64.
 
-A step is one upwards + solve pass of a batch of --nrhs right-hand sides whose loads and
+A step is one upwards + solve pass of a batch of --nrhs (default 64) right-hand sides whose loads and
 boundary data are already resident in HBM (HPSAlgorithm::upwardsStage + solveStage over the
 whole tree); value = solves per second over the K timed steps, queued back to back on the
 mesh's stream between one pair of HIP events.  Reported beside it: buildStage seconds (HIP

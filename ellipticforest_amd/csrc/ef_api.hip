@@ -654,6 +654,8 @@ void check_tree_kernels() {
     if (done) return;
     check_kernel_resources((const void*)tree_gemm_kernel<1, 1, false>, "tree_gemm_kernel<1,1>");
     check_kernel_resources((const void*)tree_gemm_kernel<2, 2, false>, "tree_gemm_kernel<2,2>");
+    check_kernel_resources((const void*)tree_gemm_kernel<1, 2, false>, "tree_gemm_kernel<1,2>");
+    check_kernel_resources((const void*)tree_gemm_kernel<1, 2, true>, "tree_gemm_kernel<1,2,f32>");
     check_kernel_resources((const void*)tree_gemm_kernel<1, 1, true>, "tree_gemm_kernel<1,1,f32>");
     check_kernel_resources((const void*)tree_gemm_kernel<2, 2, true>, "tree_gemm_kernel<2,2,f32>");
     check_kernel_resources((const void*)tree_gemm_reduce_kernel, "tree_gemm_reduce_kernel");

@@ -220,8 +220,10 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
 }
 
 template __global__ void tree_gemm_kernel<1, 1, false>(const TreeGemm*, const int*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 2, false>(const TreeGemm*, const int*, const int*, int);
 template __global__ void tree_gemm_kernel<2, 2, false>(const TreeGemm*, const int*, const int*, int);
 template __global__ void tree_gemm_kernel<1, 1, true>(const TreeGemm*, const int*, const int*, int);
+template __global__ void tree_gemm_kernel<1, 2, true>(const TreeGemm*, const int*, const int*, int);
 template __global__ void tree_gemm_kernel<2, 2, true>(const TreeGemm*, const int*, const int*, int);
 
 // D = sum over chunks of the partials (in chunk order) + beta C, one thread per element of the
@@ -887,6 +889,9 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
     g.f32 = f32;
     g.rt = (tn == 1) ? 1 : 2;
     if (probs_in.empty()) return g;
+    int maxM = 0;
+    for (const TreeGemm& p : probs_in) maxM = std::max(maxM, p.M);
+    if (maxM <= 16) g.rt = 1;  // 16-row tiles: a 32-row tile would compute a clamped copy of the same rows
     std::vector<TreeGemm> probs = probs_in;
     std::vector<int> start(probs.size() + 1, 0);
     std::vector<long long> estart(probs.size() + 1, 0);
@@ -952,18 +957,18 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
 void run_gemm(const GemmLaunch& g, hipStream_t s) {
     if (g.nwaves == 0) return;
     const dim3 grid(blocks_for(g.nwaves, 4)), block(256);
+#define EF_GEMM_LAUNCH(RT_, TN_, F32_, NAME_)                                                          \
+    EF_LAUNCH(NAME_, (tree_gemm_kernel<RT_, TN_, F32_>), grid, block, s, g.probs, g.start, g.wprob, g.nwaves)
     if (g.f32) {
-        if (g.tn == 1)
-            EF_LAUNCH("tree_gemm_kernel<1,1,f32>", (tree_gemm_kernel<1, 1, true>), grid, block, s, g.probs, g.start, g.wprob,
-                      g.nwaves);
-        else
-            EF_LAUNCH("tree_gemm_kernel<2,2,f32>", (tree_gemm_kernel<2, 2, true>), grid, block, s, g.probs, g.start, g.wprob,
-                      g.nwaves);
-    } else if (g.tn == 1)
-        EF_LAUNCH("tree_gemm_kernel<1,1>", (tree_gemm_kernel<1, 1, false>), grid, block, s, g.probs, g.start, g.wprob,
-                  g.nwaves);
-    else EF_LAUNCH("tree_gemm_kernel<2,2>", (tree_gemm_kernel<2, 2, false>), grid, block, s, g.probs, g.start, g.wprob,
-                   g.nwaves);
+        if (g.tn == 1) EF_GEMM_LAUNCH(1, 1, true, "tree_gemm_kernel<1,1,f32>");
+        else if (g.rt == 1) EF_GEMM_LAUNCH(1, 2, true, "tree_gemm_kernel<1,2,f32>");
+        else EF_GEMM_LAUNCH(2, 2, true, "tree_gemm_kernel<2,2,f32>");
+    } else {
+        if (g.tn == 1) EF_GEMM_LAUNCH(1, 1, false, "tree_gemm_kernel<1,1>");
+        else if (g.rt == 1) EF_GEMM_LAUNCH(1, 2, false, "tree_gemm_kernel<1,2>");
+        else EF_GEMM_LAUNCH(2, 2, false, "tree_gemm_kernel<2,2>");
+    }
+#undef EF_GEMM_LAUNCH
     if (g.nelem > 0)
         EF_LAUNCH("tree_gemm_reduce_kernel", tree_gemm_reduce_kernel, dim3(blocks_for(g.nelem)), block, s, g.probs,
                   g.estart, g.nprob);

@@ -106,3 +106,8 @@ def test_emu_mixed_precision(emu_ctx):
     """FP32 matrix-core products (emulated v_mfma_f32_16x16x4 layout) and the refinement loop."""
     gp.mixed_case(emu_ctx, 1, 8, variable=True)
     gp.test_mixed_precision_refuses_adaptive(emu_ctx)
+
+
+def test_emu_multi_rhs_wide_batch(emu_ctx):
+    """20 right-hand sides: the two-column-tile GEMM variants, bitwise against one at a time."""
+    gp.test_multi_rhs_batch_equals_single(emu_ctx, k=20, maxl=2, cols=(0, 19))

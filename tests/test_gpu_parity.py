@@ -128,23 +128,27 @@ def test_adaptive_linear_exactness(ctx):
         assert np.abs(u[k] - lin(X, Y)).max() < 1e-10
 
 
-def test_multi_rhs_batch_equals_single(ctx):
+def test_multi_rhs_batch_equals_single(ctx, k=5, maxl=4, cols=None):
     import ellipticforest_amd as ef
-    say("multi-RHS: batch of 5 vs one at a time, adaptive levels 0..4 N=8")
-    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 8, 8, -10, 10, -10, 10, 0, 4)
+    say(f"multi-RHS: batch of {k} vs one at a time, adaptive levels 0..{maxl} N=8")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, 8, 8, -10, 10, -10, 10, 0, maxl)
     D = ef.DeviceMesh(ctx, leaves, 8, (-10, 10, -10, 10))
     D.build()
     rng = np.random.default_rng(0)
-    k = 5
     f = rng.standard_normal((k, len(leaves), 64))
     r = rng.standard_normal((k, 4 * D.M))
     a = np.ones(4 * D.M)
     D.upwards(f)
     ub = D.solve(r, a)
-    for j in range(k):
+    for j in (range(k) if cols is None else cols):
         D.upwards(f[j])
         us = D.solve(r[j], a)
-        assert rel(ub[j], us) < 1e-13
+        assert np.array_equal(ub[j], us), j  # every column reduced in a batch-independent order
+
+
+def test_multi_rhs_wide_batch(ctx):
+    """More than 16 columns: two-column-tile GEMMs (and 16-row tiles where M <= 16)."""
+    test_multi_rhs_batch_equals_single(ctx, k=40, maxl=4, cols=(0, 17, 39))
 
 
 def test_homogeneous_rhs_option(ctx):
