@@ -117,6 +117,7 @@ long long patch_default_blocks() {
 // 16*NB, and the MFMA lane tables: QA[x][y][s] = Q[16x + r][16y + 4s + q],
 // QB[x][y][s] = Q[16x + 4s + q][16y + r], lane = 16q + r.
 void patch_fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>& QB, std::vector<double>& mu) {
+    if (NB < 1 || NB > 2 || N < 1 || N > 16 * NB) throw EfError(EF_ERR_INTERNAL, "patch side outside the padded basis");
     const int Np = 16 * NB;
     std::vector<double> Q((size_t)Np * Np, 0.0);
     for (int i = 0; i < N; i++)
@@ -144,6 +145,8 @@ void patch_fd_tables(int N, int NB, std::vector<double>& QA, std::vector<double>
 
 void patch_fd_dinv(int N, int NB, const std::vector<double>& mu, const std::vector<PatchCoef>& coef,
                    std::vector<double>& out) {
+    if (NB < 1 || NB > 2 || N < 1 || N > 16 * NB || mu.size() < (size_t)N)
+        throw EfError(EF_ERR_INTERNAL, "patch side outside the padded basis");
     const int Np = 16 * NB;
     out.assign(coef.size() * (size_t)Np * Np, 0.0);
     for (size_t c = 0; c < coef.size(); c++)
@@ -153,6 +156,7 @@ void patch_fd_dinv(int N, int NB, const std::vector<double>& mu, const std::vect
 }
 
 void launch_patch_fd(const PatchFDArgs& a, int NB, long long max_blocks, hipStream_t s) {
+    if (NB < 1 || NB > 2 || a.N < 1 || a.N > 16 * NB) throw EfError(EF_ERR_INTERNAL, "patch side outside the padded basis");
     const long long waves =
         (a.op == PATCH_OP_DTN) ? (long long)a.npatches * 4 * a.N : (long long)a.npatches * a.nrhs;
     if (waves == 0) return;

@@ -72,3 +72,13 @@ def test_footprint_rejects_broken_leaf_lists():
 def test_no_device_fails_loudly():
     with pytest.raises(ef.EfError):
         ef.Context(0)
+
+
+def test_mesh_rejects_leaf_patches_above_32():
+    """The mesh path's leaf kernels pad patches to 16 or 32 cells: N = 48 or 64 is refused at
+    planning (single grids of any size run on ef_patches_*)."""
+    from ellipticforest_amd import _lib
+    for N in (48, 64):
+        with pytest.raises(_lib.EfError):
+            _lib.DeviceMesh.footprint([(1, 0, 0), (1, 1, 0), (1, 0, 1), (1, 1, 1)], N, (0, 1, 0, 1))
+    _lib.DeviceMesh.footprint([(1, 0, 0), (1, 1, 0), (1, 0, 1), (1, 1, 1)], 32, (0, 1, 0, 1))
