@@ -208,6 +208,7 @@ struct GroupPlan {  // device descriptors of one merge group
 
 struct LevelSweep {  // device descriptors of one depth of the sweeps, k right-hand sides
     int cnt = 0, nmax = 0;
+    int up_cnt = 0, up_nmax = 0;  // merges whose hDiff and boundary h are gathered (a coarsened child)
     const UpDesc* up = nullptr;
     GemmLaunch gw, gh;
     const DownDesc* down = nullptr;
@@ -890,22 +891,37 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
             const int n = nd.n;
             UpDesc u{};
             u.n = n;
+            bool plain = (n % 4) == 0;  // no child coarsened: hDiff and tau's boundary h are read in place
             for (int c = 0; c < 4; c++) {
                 const NodeP& ch = P.nodes[nd.child[c]];
                 u.tag[c] = ch.tag;
                 u.clen[c] = 4 * ch.s;
                 u.ch_off[c] = ch.h_off;
                 vcheck(ch.h_off, 4LL * ch.s, "child h");
+                plain = plain && ch.tag == 0 && ch.s == n;
             }
             u.hd_off = nd.hd_off; u.w_off = nd.w_off; u.h_off = nd.h_off;
             vcheck(nd.hd_off, 4LL * n, "hDiff"); vcheck(nd.w_off, 4LL * n, "w"); vcheck(nd.h_off, 8LL * n, "h");
-            ud.push_back(u);
+            if (!plain) {  // coarsenUpwards_ of a tagged child: gathered (with L21) into hDiff and tau's h
+                ud.push_back(u);
+                L.up_nmax = std::max(L.up_nmax, n);
+            }
             L.nmax = std::max(L.nmax, n);
             const double* Xi = M->Xiar.p + nd.Xi_off;
             // w = X^-1 hDiff   (mergeW_ with the stored inverse instead of a re-factorisation)
             TreeGemm p{};
             p.A0 = Xi; p.lda0 = 4 * n; p.K0 = 4 * n; p.alpha0 = 1.0;
-            p.B0 = V + nd.hd_off; p.ldb0 = 1; p.bcs = (int)vrows;
+            if (plain) {
+                // hDiff block r = h of child c2(r) on side s2(r) minus h of child c1(r) on side s1(r)
+                p.B0 = V; p.ldb0 = 1; p.bcs = (int)vrows;
+                p.dn = n;
+                for (int r = 0; r < 4; r++) {
+                    p.d2[r] = u.ch_off[if_c2(r)] + (long long)if_s2(r) * n;
+                    p.d1[r] = u.ch_off[if_c1(r)] + (long long)if_s1(r) * n;
+                }
+            } else {
+                p.B0 = V + nd.hd_off; p.ldb0 = 1; p.bcs = (int)vrows;
+            }
             p.D = V + nd.w_off; p.ldd = 1; p.dcs = (int)vrows;
             p.M = 4 * n; p.N = k;
             p.kc = split_kc(p.K0, 0);
@@ -922,7 +938,9 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 h.B1 = V + nd.w_off + (long long)r1 * n; h.ldb1 = 1; h.alpha1 = 1.0;
                 h.bcs = (int)vrows;
                 h.D = V + nd.h_off + (long long)q * n; h.ldd = 1; h.dcs = (int)vrows;
-                h.C = h.D; h.ldc = 1; h.ccs = (int)vrows; h.beta = 1.0;
+                // the boundary part of tau's h: the child's own h on side sq (gathered into D when coarsened)
+                h.C = plain ? V + u.ch_off[c] + (long long)sq * n : h.D;
+                h.ldc = 1; h.ccs = (int)vrows; h.beta = 1.0;
                 h.M = n; h.N = k;
                 h.kc = split_kc(h.K0, h.K1);
                 ph.push_back(h);
@@ -961,7 +979,8 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 us[stp - 1].push_back(v);
             }
         }
-        L.cnt = (int)ud.size();
+        L.cnt = (int)dn.size();
+        L.up_cnt = (int)ud.size();
         check_range(V, (size_t)vrows * k * 8, "vector arena");
         L.up = sp->store.stage(ud);
         L.down = sp->store.stage(dn);
@@ -1032,8 +1051,10 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
             for (int dd = P.maxdepth; dd >= 0; dd--) {
                 const LevelSweep& L = sp.lv[dd];
                 if (L.cnt == 0) continue;
-                EF_LAUNCH("tree_up_gather_kernel", tree_up_gather_kernel, dim3(blocks_for(12LL * L.nmax * k), L.cnt),
-                          dim3(256), s, L.up, M->vec.p, k, P.vec_rows);
+                if (L.up_cnt > 0)
+                    EF_LAUNCH("tree_up_gather_kernel", tree_up_gather_kernel,
+                              dim3(blocks_for(12LL * L.up_nmax * k), L.up_cnt), dim3(256), s, L.up, M->vec.p, k,
+                              P.vec_rows);
                 run_gemm(L.gw, s);
                 run_gemm(L.gh, s);
             }

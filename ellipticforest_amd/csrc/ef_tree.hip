@@ -68,11 +68,23 @@ __device__ inline T pick4(const T (&a)[4], int c) {
 // Within each block of 16 k values lane group q takes k = kb + 4q + s at MFMA step s: A is
 // read as two 16-byte vectors per lane and row (a 128-byte row segment per lane group), B as
 // one row segment of 16 consecutive columns per lane group and step.
-template <int RT, int TN, bool F32>
+// hDiff mode of segment 0: row offsets of the two children's segments of each interface block
+struct DiffRows {
+    int dn, kbase;  // block size; absolute row of the segment's first k
+    long long d2[4], d1[4];
+};
+
+template <int RT, int TN, bool F32, bool DIFF>
 __device__ inline void gemm_segment(const double* __restrict__ a0p, const double* __restrict__ a1p,
                                     const double* const (&bp)[TN], long long ldb, int K, double alpha,
-                                    GemmAcc<F32> (&acc)[2][TN]) {
+                                    GemmAcc<F32> (&acc)[2][TN], const DiffRows& dr) {
     const int q = (int)(threadIdx.x & 63) >> 4;
+    // hDiff rows: absolute row ka lies in interface block r = ka / dn; both children's offsets
+    auto drow = [&](int ka, long long& o2, long long& o1) {
+        const int r = ka / dr.dn, i = ka - r * dr.dn;
+        o2 = pick4(dr.d2, r) + i;
+        o1 = pick4(dr.d1, r) + i;
+    };
     // the operands of block kb: A rows (two 16-byte vectors per row) and B columns
     struct Blk {
         td2 x0, x1, y0, y1;
@@ -86,7 +98,17 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
             o.y0 = *(const td2*)(a1p + k);
             o.y1 = *(const td2*)(a1p + k + 2);
         }
-        if (ldb == 1) {  // B contiguous along k: two 16-byte vectors per column
+        if (DIFF) {  // four consecutive rows lie in one interface block (dn % 4 == 0)
+            long long o2, o1;
+            drow(dr.kbase + k, o2, o1);
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                const td2 u0 = *(const td2*)(bp[j] + o2), u1 = *(const td2*)(bp[j] + o2 + 2);
+                const td2 v0 = *(const td2*)(bp[j] + o1), v1 = *(const td2*)(bp[j] + o1 + 2);
+                o.b[j][0] = u0[0] - v0[0]; o.b[j][1] = u0[1] - v0[1];
+                o.b[j][2] = u1[0] - v1[0]; o.b[j][3] = u1[1] - v1[1];
+            }
+        } else if (ldb == 1) {  // B contiguous along k: two 16-byte vectors per column
 #pragma unroll
             for (int j = 0; j < TN; j++) {
                 const td2 u0 = *(const td2*)(bp[j] + k), u1 = *(const td2*)(bp[j] + k + 2);
@@ -139,8 +161,15 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
                 const double y = a1p[kk];
                 a1[s] = ok ? alpha * y : 0.0;
             }
+            if (DIFF) {
+                long long o2, o1;
+                drow(dr.kbase + kk, o2, o1);
 #pragma unroll
-            for (int j = 0; j < TN; j++) b[j][s] = bp[j][(long long)kk * ldb];
+                for (int j = 0; j < TN; j++) b[j][s] = bp[j][o2] - bp[j][o1];
+            } else {
+#pragma unroll
+                for (int j = 0; j < TN; j++) b[j][s] = bp[j][(long long)kk * ldb];
+            }
         }
 #pragma unroll
         for (int s = 0; s < 4; s++)
@@ -185,19 +214,33 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
         if (kbeg < p.K0) { kb0 = kbeg; ke0 = min(kend, p.K0); kb1 = ke1 = 0; }
         else { kb0 = ke0 = 0; kb1 = kbeg - p.K0; ke1 = min(kend - p.K0, p.K1); }
     }
+    DiffRows dr;
+    dr.dn = p.dn;
+    dr.kbase = kb0;
+#pragma unroll
+    for (int r4 = 0; r4 < 4; r4++) { dr.d2[r4] = p.d2[r4]; dr.d1[r4] = p.d1[r4]; }
     if (ke0 > kb0) {
         const double* bp[TN];
+        if (p.dn > 0) {  // hDiff mode: column bases, rows resolved per interface block
 #pragma unroll
-        for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj[j] * p.bcs;
-        gemm_segment<RT, TN, F32>(p.A0 + (long long)rows[0] * p.lda0 + kb0, p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp,
-                         p.ldb0, ke0 - kb0, p.alpha0, acc);
+            for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)cj[j] * p.bcs;
+            gemm_segment<RT, TN, F32, true>(p.A0 + (long long)rows[0] * p.lda0 + kb0,
+                                            p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp, 1, ke0 - kb0, p.alpha0,
+                                            acc, dr);
+        } else {
+#pragma unroll
+            for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj[j] * p.bcs;
+            gemm_segment<RT, TN, F32, false>(p.A0 + (long long)rows[0] * p.lda0 + kb0,
+                                             p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp, p.ldb0, ke0 - kb0,
+                                             p.alpha0, acc, dr);
+        }
     }
     if (ke1 > kb1) {
         const double* bp[TN];
 #pragma unroll
         for (int j = 0; j < TN; j++) bp[j] = p.B1 + (long long)kb1 * p.ldb1 + (long long)cj[j] * p.bcs;
-        gemm_segment<RT, TN, F32>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
-                         p.ldb1, ke1 - kb1, p.alpha1, acc);
+        gemm_segment<RT, TN, F32, false>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
+                                         p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
     }
     // with RT = 1 both accumulator rows hold the same 16 rows; only the first is stored
 #pragma unroll
@@ -914,9 +957,23 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
         const double* As[2] = {p.A0, p.A1};
         const double* Bs[2] = {p.B0, p.B1};
         const int lda[2] = {p.lda0, p.lda1}, ldb[2] = {p.ldb0, p.ldb1}, K[2] = {p.K0, p.K1};
+        if (p.dn > 0) {  // hDiff mode: every block's two child segments inside B0's allocation
+            if (p.dn % 4 != 0 || p.ldb0 != 1 || p.K0 != 4 * p.dn || p.K1 > 0)
+                throw EfError(EF_ERR_INTERNAL, "hDiff GEMM needs four blocks of a multiple of 4 rows, contiguous");
+            for (int r4 = 0; r4 < 4; r4++)
+                for (const long long o : {p.d2[r4], p.d1[r4]}) {
+                    if (o < 0 || (o & 1)) throw EfError(EF_ERR_INTERNAL, "hDiff GEMM row offset negative or odd");
+                    check_strided(p.B0 + o, p.dn, p.N, 1, p.bcs, "GEMM hDiff block");
+                }
+        }
         for (int sg = 0; sg < 2; sg++) {
             if (K[sg] <= 0) continue;
             check_matrix(As[sg], p.M, K[sg], lda[sg], "GEMM A");
+            if (sg == 0 && p.dn > 0) {
+                if (((uintptr_t)As[0] & 15) != 0 || (lda[0] & 1) != 0 || ((uintptr_t)p.B0 & 15) != 0 || (p.bcs & 1) != 0)
+                    throw EfError(EF_ERR_INTERNAL, "hDiff GEMM operands not 16-byte aligned");
+                continue;
+            }
             check_strided(Bs[sg], K[sg], p.N, ldb[sg], p.bcs, "GEMM B");
             if (((uintptr_t)As[sg] & 15) != 0 || (lda[sg] & 1) != 0)
                 throw EfError(EF_ERR_INTERNAL, "GEMM A operand not 16-byte aligned with an even leading dimension");

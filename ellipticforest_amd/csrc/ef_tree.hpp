@@ -67,8 +67,13 @@ struct TreeGemm {
     double alpha0, alpha1, beta;
     int nks, pad;                        // split-K chunk count (set by plan_gemm)
     double* P;                           // partials, nks x M x N (split problems)
+    // hDiff mode (dn > 0, segment 0 only, ldb0 = 1): row kk of B0 is B0[d2[r] + i] - B0[d1[r] + i]
+    // with r = kk / dn, i = kk % dn (dn a multiple of 4): the four interface blocks of mergeW_'s
+    // hDiff read straight from the children's h, each block the difference of two children
+    int dn, dpad;
+    long long d2[4], d1[4];
 };
-static_assert(sizeof(TreeGemm) == 144, "TreeGemm layout");
+static_assert(sizeof(TreeGemm) == 216, "TreeGemm layout");
 
 // F32: operands rounded to FP32, v_mfma_f32_16x16x4_f32 with FP32 accumulation, FP64 output
 // (+ beta C in FP64); the mixed-precision build's S and T products.
