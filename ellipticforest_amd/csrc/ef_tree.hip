@@ -214,6 +214,22 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
         if (kbeg < p.K0) { kb0 = kbeg; ke0 = min(kend, p.K0); kb1 = ke1 = 0; }
         else { kb0 = ke0 = 0; kb1 = kbeg - p.K0; ke1 = min(kend - p.K0, p.K1); }
     }
+    // C (when the product adds to it and is not split) loaded before the products, so its
+    // latency overlaps them; rows and columns clamped as the stores are guarded
+    double cv[RT][TN][4];
+    const bool has_c = p.C != nullptr && p.kc == 0;
+    if (has_c) {
+#pragma unroll
+        for (int i = 0; i < RT; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int row = min(row0 + 16 * i + (F32 ? 4 * q + v : 4 * v + q), p.M - 1);
+                    const int col = min(col0 + 16 * j + r, p.N - 1);
+                    cv[i][j][v] = p.C[(long long)row * p.ldc + (long long)col * p.ccs];
+                }
+    }
     DiffRows dr;
     dr.dn = p.dn;
     dr.kbase = kb0;
@@ -255,7 +271,7 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
                         p.P[((long long)chunk * p.M + row) * p.N + col] = (double)acc[i][j][v];
                     } else {
                         double o = (double)acc[i][j][v];
-                        if (p.C) o += p.beta * p.C[(long long)row * p.ldc + (long long)col * p.ccs];
+                        if (has_c) o += p.beta * cv[i][j][v];
                         p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
                     }
                 }
@@ -675,6 +691,17 @@ __global__ void __launch_bounds__(256) gj_update_kernel(GJArgs a) {
     }
     const int ra = min(row0 + r, m - 1), rb = min(row0 + 16 + r, m - 1);
     const int c0 = min(col0 + r, m - 1), c1 = min(col0 + 16 + r, m - 1);
+    // the block's current values, loaded before the products so their latency overlaps them
+    double cv[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = min(row0 + 16 * i + 4 * v + q, m - 1);
+            const double* crow = Wa + (long long)src[row] * m;
+            cv[i][0][v] = crow[c0];
+            cv[i][1][v] = crow[c1];
+        }
     td4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -709,8 +736,7 @@ __global__ void __launch_bounds__(256) gj_update_kernel(GJArgs a) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = row0 + 16 * i + 4 * v + q, col = col0 + 16 * j + r;
-                if (row < m && col < m)
-                    Wb[(long long)row * m + col] = acc[i][j][v] + Wa[(long long)src[row] * m + col];
+                if (row < m && col < m) Wb[(long long)row * m + col] = acc[i][j][v] + cv[i][j][v];
             }
 }
 
