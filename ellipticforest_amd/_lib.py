@@ -33,6 +33,7 @@ EXPORTS = [
     "ef_solve", "ef_solve_device", "ef_get_operator", "ef_mesh_sync", "ef_device_alloc", "ef_device_free",
     "ef_memcpy", "ef_debug_invert", "ef_host_alloc", "ef_host_free", "ef_mesh_copy_async", "ef_mesh_record",
     "ef_mesh_query", "ef_mesh_wait", "ef_mesh_eval_load", "ef_mesh_time_sweep_gemms", "ef_mesh_set_refinement",
+    "ef_mesh_set_autotune", "ef_mesh_sweep_tiles",
     "ef_mesh_stage_ms", "ef_patches_create", "ef_patches_destroy", "ef_patches_set_stream",
     "ef_patches_info", "ef_patches_apply", "ef_patches_apply_device", "ef_patches_apply_device_noalloc",
     "ef_patches_dtn", "ef_patches_sync",
@@ -110,6 +111,8 @@ def load_library(path):
     L.ef_mesh_eval_load.argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     L.ef_mesh_time_sweep_gemms.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip]
     L.ef_mesh_set_refinement.argtypes = [_vp, ctypes.c_int]
+    L.ef_mesh_set_autotune.argtypes = [_vp, ctypes.c_int]
+    L.ef_mesh_sweep_tiles.argtypes = [_vp, ctypes.c_int, _ip, _ip]
     L.ef_mesh_stage_ms.argtypes = [_vp, _dp]
     L.ef_patches_create.argtypes = [_vp, ctypes.POINTER(PatchBatch), ctypes.POINTER(Coeffs), ctypes.POINTER(_vp)]
     L.ef_patches_destroy.argtypes = [_vp]
@@ -288,6 +291,18 @@ class DeviceMesh:
         c = Coeffs(solver_type, alpha0, beta0, lambda0, cp)
         self._check(self.L.ef_build(self.h, ctypes.byref(c), precision))
         del keep
+
+    def set_autotune(self, mode):
+        """Sweep GEMM tiles: 1/True timed (default), 0/False fixed rule, 11/12/22 one forced tile."""
+        self._check(self.L.ef_mesh_set_autotune(self.h, int(mode)))
+
+    def sweep_tiles(self, k):
+        """Chosen tiles per depth (root first): [[w, h, u], ...] as rows x 10 + columns (units of 16)."""
+        n = ctypes.c_int(0)
+        self._check(self.L.ef_mesh_sweep_tiles(self.h, int(k), ctypes.cast(None, _ip), ctypes.byref(n)))
+        out = np.zeros(3 * n.value, np.int32)
+        self._check(self.L.ef_mesh_sweep_tiles(self.h, int(k), out.ctypes.data_as(_ip), ctypes.byref(n)))
+        return out.reshape(-1, 3)
 
     def set_refinement(self, steps):
         """Iterative-refinement rounds of each solve of a mixed-precision build."""

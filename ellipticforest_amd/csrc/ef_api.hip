@@ -208,6 +208,7 @@ struct GroupPlan {  // device descriptors of one merge group
 
 struct LevelSweep {  // device descriptors of one depth of the sweeps, k right-hand sides
     int cnt = 0, nmax = 0;
+    int tile[3] = {0, 0, 0};  // row tile x 10 + column tile (16-element units) of the w, h and u products
     int up_cnt = 0, up_nmax = 0;  // merges whose hDiff and boundary h are gathered (a coarsened child)
     const UpDesc* up = nullptr;
     GemmLaunch gw, gh;
@@ -370,6 +371,8 @@ struct ef_mesh_s {
     };
     std::vector<SweepGraph> graphs;
     unsigned long long graph_clock = 0;
+    int autotune = 1;  // sweep GEMM tiles: 1 timed at the first sweep of a width, 0 fixed rule, 11/12/22 forced
+    std::map<int, int> tuned;  // (depth, product, 16-column tiles of k) -> chosen tile
     // mixed precision: refinement steps per solve, uniform-mesh neighbour tables, work
     int refine_steps = 2;
     DevBuf<int> mix_nbr, mix_pos;
@@ -856,6 +859,52 @@ void ensure_vectors(ef_mesh_s* M, int k) {
     EF_HIP(hipMemsetAsync(M->vec.p, 0, (size_t)M->P.vec_rows * k * sizeof(double), M->stream));
     M->kcap = k;
     M->sweeps.clear();  // descriptors point into the arena
+    M->k_up = 0;        // and the upward state it held is gone
+}
+
+// Tile autotuner: at the first sweep of a width k, each level's w, h and u products are planned
+// with 16x16, 16x32 and 32x32 blocks per wave, timed (HIP events on the mesh's stream, best of 3
+// passes after one warm pass, over whatever the vector arena holds) and the fastest kept.  The
+// choice cannot change a result: every output element sums its k range in the same order under
+// any tile shape.  EF_AUTOTUNE=0 (or ef_mesh_set_autotune) keeps the fixed rule.
+bool autotune_on() {
+    const char* e = std::getenv("EF_AUTOTUNE");
+    return !(e && std::strcmp(e, "0") == 0);
+}
+
+GemmLaunch tune_gemm(ef_mesh_s* M, const std::vector<TreeGemm>& probs, DevStore& store, int& tile) {
+    hipStream_t s = M->stream;
+    const int cand[3][2] = {{1, 1}, {1, 2}, {2, 2}};  // (rt, tn)
+    GemmLaunch best;
+    float best_ms = 0.0f;
+    tile = 0;
+    if (probs.empty()) return plan_gemm(probs, 1, store, s);
+    hipEvent_t e0, e1;
+    EF_HIP(hipEventCreate(&e0));
+    EF_HIP(hipEventCreate(&e1));
+    for (const auto& c : cand) {
+        GemmLaunch g = plan_gemm(probs, c[1], store, s, false, c[0]);
+        store.flush(s);
+        run_gemm(g, s);  // warm
+        float ms = 0.0f;
+        for (int rep = 0; rep < 3; rep++) {
+            EF_HIP(hipEventRecord(e0, s));
+            run_gemm(g, s);
+            EF_HIP(hipEventRecord(e1, s));
+            EF_HIP(hipEventSynchronize(e1));
+            float t = 0.0f;
+            EF_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms = (rep == 0) ? t : std::min(ms, t);
+        }
+        if (tile == 0 || ms < best_ms) {
+            best = g;
+            best_ms = ms;
+            tile = 10 * g.rt + g.tn;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
 }
 
 SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
@@ -984,9 +1033,34 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
         check_range(V, (size_t)vrows * k * 8, "vector arena");
         L.up = sp->store.stage(ud);
         L.down = sp->store.stage(dn);
-        L.gw = plan_gemm(pw, tn, sp->store, st);
-        L.gh = plan_gemm(ph, tn, sp->store, st);
-        L.gu = plan_gemm(pu, tn, sp->store, st);
+        if (M->autotune == 1 && autotune_on()) {
+            // widths with the same number of 16-column tiles share a choice: a tile's cost does
+            // not depend on how many of its columns are live
+            const std::vector<TreeGemm>* pr[3] = {&pw, &ph, &pu};
+            GemmLaunch* gl[3] = {&L.gw, &L.gh, &L.gu};
+            for (int t3 = 0; t3 < 3; t3++) {
+                const int key = (dd * 3 + t3) * 4096 + (k + 15) / 16;
+                auto hit = M->tuned.find(key);
+                if (hit != M->tuned.end()) {
+                    L.tile[t3] = hit->second;
+                    const int tl = hit->second ? hit->second : 11;  // 0: no product at this depth
+                    *gl[t3] = plan_gemm(*pr[t3], tl % 10, sp->store, st, false, tl / 10);
+                } else {
+                    M->k_up = 0;  // the timed passes overwrite the arena's w, h and interface rows
+                    *gl[t3] = tune_gemm(M, *pr[t3], sp->store, L.tile[t3]);
+                    M->tuned[key] = L.tile[t3];
+                }
+            }
+        } else {
+            const bool forced = M->autotune > 1;
+            const int ftn = forced ? M->autotune % 10 : tn, frt = forced ? M->autotune / 10 : 0;
+            L.gw = plan_gemm(pw, ftn, sp->store, st, false, frt);
+            L.gh = plan_gemm(ph, ftn, sp->store, st, false, frt);
+            L.gu = plan_gemm(pu, ftn, sp->store, st, false, frt);
+            L.tile[0] = pw.empty() ? 0 : 10 * L.gw.rt + L.gw.tn;
+            L.tile[1] = ph.empty() ? 0 : 10 * L.gh.rt + L.gh.tn;
+            L.tile[2] = pu.empty() ? 0 : 10 * L.gu.rt + L.gu.tn;
+        }
         for (auto& v : us) {
             int nfmax = 0;
             for (auto& x : v) nfmax = std::max(nfmax, x.nfine);
@@ -1068,12 +1142,12 @@ void do_solve_once(ef_mesh_s* M, const double* f_dev, const double* r_dev, const
                    bool robin, double* u_dev, int k) {
     if (!M->built) throw EfError(EF_ERR_STATE, "solve stage called before the build stage");
     if (k < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
-    if (!M->homogeneous_rhs && M->k_up != k)
-        throw EfError(EF_ERR_STATE, "solve stage needs a preceding upwards stage with the same number of right-hand sides");
     Plan& P = M->P;
     hipStream_t s = M->stream;
     ensure_vectors(M, k);
-    SweepPlan& sp = sweep_plan(M, k);
+    SweepPlan& sp = sweep_plan(M, k);  // (a new, tuned plan has overwritten any upward state)
+    if (!M->homogeneous_rhs && M->k_up != k)
+        throw EfError(EF_ERR_STATE, "solve stage needs a preceding upwards stage with the same number of right-hand sides");
     if (sp.store.pending()) throw EfError(EF_ERR_INTERNAL, "sweep descriptors staged but not uploaded");
     const NodeP& root = P.nodes[0];
     const int m = 4 * root.s;
@@ -1361,7 +1435,11 @@ int ef_mesh_set_options(ef_mesh mesh, int cache_operators, int homogeneous_rhs) 
     if ((cache_operators != 0) != mesh->cache_operators && mesh->built)
         throw EfError(EF_ERR_STATE, "cache-operators must be set before the build stage");
     mesh->cache_operators = cache_operators != 0;
-    if ((homogeneous_rhs != 0) != mesh->homogeneous_rhs) mesh->sweeps.clear();
+    if ((homogeneous_rhs != 0) != mesh->homogeneous_rhs) {
+        clear_graphs(mesh);  // captured sweeps read the descriptors of the plans dropped here
+        mesh->sweeps.clear();
+        mesh->k_up = 0;
+    }
     mesh->homogeneous_rhs = homogeneous_rhs != 0;
     EF_API_END
 }
@@ -1549,6 +1627,34 @@ int ef_host_free(ef_ctx ctx, void* ptr) {
     EF_API_BEGIN
     if (!ctx) throw EfError(EF_ERR_ARG, "null context");
     if (ptr) EF_HIP(hipHostFree(ptr));
+    EF_API_END
+}
+
+int ef_mesh_set_autotune(ef_mesh mesh, int mode) {
+    EF_API_BEGIN
+    if (!mesh) throw EfError(EF_ERR_ARG, "null mesh");
+    if (mode != 0 && mode != 1 && mode != 11 && mode != 12 && mode != 22)
+        throw EfError(EF_ERR_ARG, "autotune mode must be 0, 1, 11, 12 or 22");
+    mesh->autotune = mode;
+    mesh->tuned.clear();
+    mesh->sweeps.clear();  // re-planned (and re-tuned) at the next sweep of each width,
+    mesh->k_up = 0;        // so the next solve needs a new upwards stage
+    clear_graphs(mesh);
+    EF_API_END
+}
+
+int ef_mesh_sweep_tiles(ef_mesh mesh, int nrhs, int* tiles, int* levels) {
+    EF_API_BEGIN
+    if (!mesh || !levels) throw EfError(EF_ERR_ARG, "null argument");
+    if (!mesh->built) throw EfError(EF_ERR_STATE, "build stage first");
+    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    EF_HIP(hipSetDevice(mesh->ctx->device));
+    ensure_vectors(mesh, nrhs);
+    SweepPlan& sp = sweep_plan(mesh, nrhs);
+    *levels = (int)sp.lv.size();
+    if (tiles)
+        for (size_t d = 0; d < sp.lv.size(); d++)
+            for (int t3 = 0; t3 < 3; t3++) tiles[3 * d + t3] = sp.lv[d].tile[t3];
     EF_API_END
 }
 

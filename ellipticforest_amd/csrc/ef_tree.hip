@@ -952,7 +952,7 @@ __global__ void __launch_bounds__(256) tree_root_robin_kernel(const double* T, c
 // ---------------------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------------------
-GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& store, hipStream_t s, bool f32) {
+GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& store, hipStream_t s, bool f32, int rt) {
     GemmLaunch g;
     g.tn = tn;
     g.f32 = f32;
@@ -961,6 +961,7 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
     int maxM = 0;
     for (const TreeGemm& p : probs_in) maxM = std::max(maxM, p.M);
     if (maxM <= 16) g.rt = 1;  // 16-row tiles: a 32-row tile would compute a clamped copy of the same rows
+    if (rt == 1 || (rt == 2 && tn == 2)) g.rt = rt;  // an explicit choice (the tile autotuner)
     std::vector<TreeGemm> probs = probs_in;
     std::vector<int> start(probs.size() + 1, 0);
     std::vector<long long> estart(probs.size() + 1, 0);

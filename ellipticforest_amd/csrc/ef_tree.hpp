@@ -178,7 +178,11 @@ struct GemmLaunch {
 // split problems, stages problems and wave offsets in `store` (device memory the caller keeps
 // alive; valid on the device after store.flush) and returns the launch.  tn = 1 (16 x 16 blocks) suits N <= 16, tn = 2 (32 x 32) the rest.
 struct DevStore;
-GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s, bool f32 = false);
+// rt = 0 picks the row tile from tn (16 rows for tn = 1 or problems of <= 16 rows, else 32);
+// rt = 1 or 2 forces it (2 only with tn = 2).  The tile shape never changes a result bit: every
+// output element sums its k range in the same order.
+GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs, int tn, DevStore& store, hipStream_t s, bool f32 = false,
+                     int rt = 0);
 void run_gemm(const GemmLaunch& g, hipStream_t s);
 
 // Inversion of `B` matrices of order m held in wa (B x m x m, destroyed); X^-1 into xi

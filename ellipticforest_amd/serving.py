@@ -184,6 +184,10 @@ class Server:
                 del self.meshes[victim]
         dm = _lib.DeviceMesh(self.ctx, leaves, N, bounds)
         dm.build(**build_args)
+        # plan (and time the GEMM tiles of) every batch width before requests arrive: the widest
+        # first, so the vector arena is sized once
+        for w in sorted(self.buckets or range(1, self.max_batch + 1), reverse=True):
+            dm.sweep_tiles(w)
         with self.lock:
             self.meshes[key] = _Resident(self.ctx, dm, fp, self.max_batch, self.nslots, self.fetch_rows)
         return dm

@@ -144,6 +144,20 @@ int ef_device_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_device_free(ef_ctx ctx, void* ptr);
 int ef_memcpy(ef_ctx ctx, void* dst, const void* src, size_t bytes, int kind);
 
+/* Tile autotuner of the sweeps (no counterpart in the reference): at the first upwards or solve
+ * stage of each number of right-hand sides, every level's GEMMs are timed with 16x16, 16x32 and
+ * 32x32 blocks per wave and the fastest is kept (results do not depend on the choice).  The
+ * timed passes overwrite the upward state, so the solve stage after a newly tuned width needs its
+ * own upwards stage (as it does anyway).  mode 1 (default) tunes; 0 (or EF_AUTOTUNE=0) keeps the
+ * fixed rule (16x16 up to 16 right-hand sides, else 32x32, 16x32 for operators of <= 16 rows);
+ * 11, 12 or 22 forces one tile (rows x 10 + columns, in units of 16) everywhere.  Setting it drops
+ * the sweep plans and the upward state.
+ * ef_mesh_sweep_tiles: the tiles for nrhs right-hand sides, per depth (root first) the w, h and u
+ * products as rows x 10 + columns (0: no product at that depth); tiles holds 3 x *levels ints and
+ * may be NULL to query *levels. */
+int ef_mesh_set_autotune(ef_mesh mesh, int mode);
+int ef_mesh_sweep_tiles(ef_mesh mesh, int nrhs, int* tiles, int* levels);
+
 /* Mixed precision (EF_PRECISION_MIXED at ef_build; no counterpart in the reference, which is
  * FP64 throughout): the S and T products of every merge on FP32 matrix cores (FP32 operands and
  * accumulation), X^-1 and the leaf operators in FP64; each solve stage then runs `steps` rounds

@@ -31,6 +31,14 @@ def emulator_library():
     return _lib.load_library(build_emu.LIB)
 
 
+@pytest.fixture(autouse=True)
+def _emulated_fixed_tiles(request, monkeypatch):
+    """On the emulator, timing the sweep tile shapes measures nothing and costs minutes: keep the
+    fixed rule there, except in the tests of the tile choice itself."""
+    if ("emu_ctx" in request.fixturenames or EMULATOR) and "sweep_tiles" not in request.node.name:
+        monkeypatch.setenv("EF_AUTOTUNE", "0")
+
+
 @pytest.fixture(scope="session")
 def emu_ctx():
     import ellipticforest_amd as ef

@@ -111,3 +111,8 @@ def test_emu_mixed_precision(emu_ctx):
 def test_emu_multi_rhs_wide_batch(emu_ctx):
     """20 right-hand sides: the two-column-tile GEMM variants, bitwise against one at a time."""
     gp.test_multi_rhs_batch_equals_single(emu_ctx, k=20, maxl=2, cols=(0, 19))
+
+
+def test_emu_sweep_tiles(emu_ctx):
+    """Every sweep tile shape (forced and timed) gives the fixed rule's bits."""
+    gp.tile_case(emu_ctx, 17, 1, 8)

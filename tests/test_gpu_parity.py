@@ -151,6 +151,47 @@ def test_multi_rhs_wide_batch(ctx):
     test_multi_rhs_batch_equals_single(ctx, k=40, maxl=4, cols=(0, 17, 39))
 
 
+def tile_case(ctx, k, maxl, N):
+    """Sweep GEMM tile shapes: the fixed rule, each forced tile (16x16, 16x32, 32x32) and the
+    timed choice give the same bits (every output element sums its k range in one order)."""
+    import ellipticforest_amd as ef
+    say(f"sweep tiles: fixed rule vs forced 11/12/22 vs timed, k={k} levels 0..{maxl} N={N}")
+    leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, N, N, -10, 10, -10, 10, 0, maxl)
+    D = ef.DeviceMesh(ctx, leaves, N, (-10, 10, -10, 10))
+    D.build()
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((k, len(leaves), N * N))
+    r = rng.standard_normal((k, 4 * D.M))
+    a = np.ones(4 * D.M)
+    out = {}
+    for mode in (0, 11, 12, 22, 1):
+        D.set_autotune(mode)
+        tiles = D.sweep_tiles(k)
+        assert tiles.shape == (D.maxdepth + 1, 3)
+        used = set(tiles.ravel().tolist()) - {0}
+        assert used <= {11, 12, 22} and used, tiles
+        if mode > 1:
+            assert used == {mode}
+        if mode == 0:
+            assert used <= ({11} if k <= 16 else {12, 22})
+        D.upwards(f)
+        out[mode] = D.solve(r, a)
+    for mode in (11, 12, 22, 1):
+        assert np.array_equal(out[mode], out[0]), mode
+    # a tuned width's solve needs its own upwards stage: the timed passes overwrite the state
+    from ellipticforest_amd._lib import EfError
+    D.upwards(f)
+    D.sweep_tiles(k + 1)
+    with pytest.raises(EfError, match="preceding upwards"):
+        D.solve(r, a)
+    return out[0]
+
+
+def test_sweep_tiles_bitwise(ctx):
+    tile_case(ctx, 40, 5, 8)
+    tile_case(ctx, 7, 4, 16)
+
+
 def test_homogeneous_rhs_option(ctx):
     import ellipticforest_amd as ef
     say("homogeneous-rhs option: leaves solve with f = 0, w skipped")
