@@ -50,6 +50,7 @@ struct NodeP {
     long long S_off = -1, Xi_off = -1;
     // vector arena row offsets
     long long h_off = -1, g_off = -1, gc_off = -1, gt_off = -1, hd_off = -1, w_off = -1, ui_off = -1;
+    long long gsrc[4] = {-1, -1, -1, -1};  // where side s of g (s rows) is read in the downward sweep
 };
 
 struct Group {  // merges at one depth with one interface size, contiguous in the arenas
@@ -72,6 +73,7 @@ struct Plan {
     long long work_X = 0;                      // max over groups of B m^2
     long long gj_d = 0, gj_i = 0;              // GJ work (doubles, ints) of the largest group
     int max_n = 0;
+    bool side_reads = false;                   // sides of uncoarsened nodes read in place (gsrc)
 };
 
 int build_node(Plan& P, int parent, int level, int ix, int iy, double xl, double xu, double yl, double yu,
@@ -112,6 +114,39 @@ int ilog2(int v) {
 
 // Sizes and coarsening tags (tagPatchesForCoarsening_, HPSAlgorithm.hpp:676-696;
 // mergePatch_ :1004-1009 makes the parent 2n cells wide), merge groups, arena offsets.
+// Where the downward sweep reads each node's boundary data g, side by side.  The root's g and
+// the g of a coarsened node (interpolated by the uncoarsen steps) are written to the node's own
+// rows.  Any other node's side s is a segment of its parent's side or interface potential
+// (split1to4 copies exactly that segment, HPSAlgorithm.hpp:562-577), so it is read there and
+// never copied -- when every block of the sweeps' gathered GEMM operands is a multiple of 4
+// rows at even offsets (the MFMA operand reads); otherwise every node's g is written out.
+void plan_sides(Plan& P) {
+    bool ok = true;
+    for (const NodeP& nd : P.nodes)
+        if (!nd.leaf && (nd.s % 4 != 0 || nd.n % 2 != 0)) ok = false;
+    P.side_reads = ok;
+    auto own = [](NodeP& nd) {
+        for (int sd = 0; sd < 4; sd++) nd.gsrc[sd] = nd.g_off + (long long)sd * nd.s;
+    };
+    own(P.nodes[0]);
+    for (int dd = 0; dd <= P.maxdepth; dd++)
+        for (int id : P.internal_at[dd]) {
+            const NodeP& nd = P.nodes[id];
+            for (int c = 0; c < 4; c++) {
+                NodeP& ch = P.nodes[nd.child[c]];
+                if (!ok || ch.tag > 0) {
+                    own(ch);
+                    continue;
+                }
+                for (int sd = 0; sd < 4; sd++) {
+                    const int q = child_src_q(c, sd);
+                    ch.gsrc[sd] = q >= 0 ? nd.gsrc[q >> 1] + (long long)(q & 1) * nd.n
+                                         : nd.ui_off + (long long)child_src_r(c, sd) * nd.n;
+                }
+            }
+        }
+}
+
 void plan_tree(Plan& P, const ef_mesh_desc& d) {
     if (d.N < 4 || (d.N & 1)) throw EfError(EF_ERR_ARG, "N must be even and >= 4");
     if (d.N > 32)
@@ -194,6 +229,7 @@ void plan_tree(Plan& P, const ef_mesh_desc& d) {
         }
     }
     P.T_elems = T; P.S_elems = S; P.Xi_elems = Xi; P.vec_rows = rows;
+    plan_sides(P);
 }
 
 // ---------------------------------------------------------------------------
@@ -212,7 +248,8 @@ struct LevelSweep {  // device descriptors of one depth of the sweeps, k right-h
     int up_cnt = 0, up_nmax = 0;  // merges whose hDiff and boundary h are gathered (a coarsened child)
     const UpDesc* up = nullptr;
     GemmLaunch gw, gh;
-    const DownDesc* down = nullptr;
+    const SideCopy* copies = nullptr;  // children's g written out (coarsened children, or all)
+    int copy_cnt = 0;
     GemmLaunch gu;
     std::vector<const VecStepDesc*> unc;
     std::vector<int> unc_cnt, unc_nf;
@@ -465,7 +502,7 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
             L.cls = -1;
             L.node = P.leaves[li];
             L.h_off = nd.h_off;
-            L.g_off = nd.g_off;
+            for (int sd = 0; sd < 4; sd++) L.gs[sd] = nd.gsrc[sd];
             L.coef = M->vcoef.p + li * 6 * NN;
             L.stencil = M->vstencil.p + li * 5 * NN;
             L.Sinv = M->vSinv.p + li * (size_t)N * NN;
@@ -513,9 +550,9 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     M->leafT.ensure((size_t)ncls * 16 * NN);
     M->classes.upload(M->classes_h, M->stream);
     M->leaf_cls.upload(M->leaf_class, M->stream);
-    std::vector<long long> goff(nl), hoff(nl);
+    std::vector<long long> goff(4 * nl), hoff(nl);
     for (size_t li = 0; li < nl; li++) {
-        goff[li] = P.nodes[P.leaves[li]].g_off;
+        for (int sd = 0; sd < 4; sd++) goff[4 * li + sd] = P.nodes[P.leaves[li]].gsrc[sd];
         hoff[li] = P.nodes[P.leaves[li]].h_off;
     }
     M->leaf_goff.upload(goff, M->stream);
@@ -677,7 +714,7 @@ void check_tree_kernels() {
     check_kernel_resources((const void*)tree_coarsen_kernel, "tree_coarsen_kernel");
     check_kernel_resources((const void*)tree_up_gather_kernel, "tree_up_gather_kernel");
     check_kernel_resources((const void*)tree_uncoarsen_kernel, "tree_uncoarsen_kernel");
-    check_kernel_resources((const void*)tree_down_scatter_kernel, "tree_down_scatter_kernel");
+    check_kernel_resources((const void*)tree_side_copy_kernel, "tree_side_copy_kernel");
     check_kernel_resources((const void*)tree_root_dirichlet_kernel, "tree_root_dirichlet_kernel");
     check_kernel_resources((const void*)tree_root_robin_kernel, "tree_root_robin_kernel");
     done = true;
@@ -933,7 +970,8 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
         LevelSweep& L = sp->lv[dd];
         std::vector<UpDesc> ud;
         std::vector<TreeGemm> pw, ph, pu;
-        std::vector<DownDesc> dn;
+        std::vector<SideCopy> cp;
+        int dn_cnt = 0;
         std::vector<std::vector<VecStepDesc>> us;
         for (int id : P.internal_at[dd]) {
             const NodeP& nd = P.nodes[id];
@@ -964,6 +1002,7 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 // hDiff block r = h of child c2(r) on side s2(r) minus h of child c1(r) on side s1(r)
                 p.B0 = V; p.ldb0 = 1; p.bcs = (int)vrows;
                 p.dn = n;
+                p.dsub = 1;
                 for (int r = 0; r < 4; r++) {
                     p.d2[r] = u.ch_off[if_c2(r)] + (long long)if_s2(r) * n;
                     p.d1[r] = u.ch_off[if_c1(r)] + (long long)if_s1(r) * n;
@@ -994,20 +1033,42 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 h.kc = split_kc(h.K0, h.K1);
                 ph.push_back(h);
             }
-            // downward: u_int = S g + w   (applyS_, HPSAlgorithm.hpp:1194-1230)
-            DownDesc dsc{};
-            dsc.n = n; dsc.tag = nd.tag;
-            dsc.gc_off = nd.gc_off; dsc.g_off = nd.g_off; dsc.w_off = nd.w_off; dsc.ui_off = nd.ui_off;
-            for (int c = 0; c < 4; c++) {
-                dsc.cg_off[c] = P.nodes[nd.child[c]].gc_off;
-                vcheck(dsc.cg_off[c], 4LL * n, "child g");
+            // downward: u_int = S g + w   (applyS_, HPSAlgorithm.hpp:1194-1230), g read side by
+            // side where it lies; then the children's g written out where the sweep needs a copy
+            // (split1to4: coarsened children, or every child without side reads)
+            vcheck(nd.ui_off, 4LL * n, "interface potential");
+            bool own_g = true;
+            for (int sd = 0; sd < 4; sd++) {
+                vcheck(nd.gsrc[sd], 2LL * n, "side of g");
+                own_g = own_g && nd.gsrc[sd] == nd.g_off + 2LL * n * sd;
             }
-            vcheck(nd.g_off, 8LL * n, "g"); vcheck(nd.ui_off, 4LL * n, "interface potential");
-            dsc.S = M->Sar.p + nd.S_off;
-            dn.push_back(dsc);
+            dn_cnt++;
+            for (int c = 0; c < 4; c++) {
+                const NodeP& ch = P.nodes[nd.child[c]];
+                if (P.side_reads && ch.tag == 0) continue;
+                SideCopy sc{};
+                sc.len = n;
+                sc.dst = ch.gc_off;
+                vcheck(sc.dst, 4LL * n, "child g");
+                for (int sd = 0; sd < 4; sd++) {
+                    const int q = child_src_q(c, sd);
+                    sc.src[sd] = q >= 0 ? nd.gsrc[q >> 1] + (long long)(q & 1) * n
+                                        : nd.ui_off + (long long)child_src_r(c, sd) * n;
+                    vcheck(sc.src[sd], n, "side source");
+                }
+                cp.push_back(sc);
+            }
             TreeGemm pd{};
-            pd.A0 = dsc.S; pd.lda0 = 8 * n; pd.K0 = 8 * n; pd.alpha0 = 1.0;
-            pd.B0 = V + nd.g_off; pd.ldb0 = 1; pd.bcs = (int)vrows;
+            pd.A0 = M->Sar.p + nd.S_off; pd.lda0 = 8 * n; pd.K0 = 8 * n; pd.alpha0 = 1.0;
+            if (own_g) {
+                pd.B0 = V + nd.g_off;
+            } else {  // block-gather of the four sides (2n rows each)
+                pd.B0 = V;
+                pd.dn = 2 * n;
+                pd.dsub = 0;
+                for (int sd = 0; sd < 4; sd++) pd.d2[sd] = nd.gsrc[sd];
+            }
+            pd.ldb0 = 1; pd.bcs = (int)vrows;
             pd.D = V + nd.ui_off; pd.ldd = 1; pd.dcs = (int)vrows;
             if (!M->homogeneous_rhs) { pd.C = V + nd.w_off; pd.ldc = 1; pd.ccs = (int)vrows; pd.beta = 1.0; }
             pd.M = 4 * n; pd.N = k;
@@ -1028,11 +1089,12 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 us[stp - 1].push_back(v);
             }
         }
-        L.cnt = (int)dn.size();
+        L.cnt = dn_cnt;
+        L.copy_cnt = (int)cp.size();
         L.up_cnt = (int)ud.size();
         check_range(V, (size_t)vrows * k * 8, "vector arena");
         L.up = sp->store.stage(ud);
-        L.down = sp->store.stage(dn);
+        L.copies = sp->store.stage(cp);
         if (M->autotune == 1 && autotune_on()) {
             // widths with the same number of 16-column tiles share a choice: a tile's cost does
             // not depend on how many of its columns are live
@@ -1167,8 +1229,9 @@ void do_solve_once(ef_mesh_s* M, const double* f_dev, const double* r_dev, const
                           dim3(blocks_for(4LL * L.unc_nf[stp] * k), L.unc_cnt[stp]), dim3(256), s, L.unc[stp], V, k,
                           P.vec_rows);
             run_gemm(L.gu, s);
-            EF_LAUNCH("tree_down_scatter_kernel", tree_down_scatter_kernel, dim3(blocks_for(16LL * L.nmax * k), L.cnt),
-                      dim3(256), s, L.down, V, k, P.vec_rows);
+            if (L.copy_cnt > 0)
+                EF_LAUNCH("tree_side_copy_kernel", tree_side_copy_kernel, dim3(blocks_for(4LL * L.nmax * k), L.copy_cnt),
+                          dim3(256), s, L.copies, V, k, P.vec_rows);
         }
         launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, f_zero ? 1 : 0, u_dev, k);
     };
@@ -1871,7 +1934,17 @@ int ef_get_operator(ef_mesh mesh, int node, int which, int rhs, double* out, int
             }
             break;
         }
-        case 5: copy_vec(nd.g_off, 4 * nd.s); break;
+        case 5: {  // g, side by side from where the downward sweep reads it
+            const int len = nd.s;
+            double* o = out;
+            for (int sd = 0; sd < 4; sd++) {
+                copy_vec(nd.gsrc[sd], len);
+                if (out) out += len;
+            }
+            out = o;
+            *rows = 4 * len;
+            break;
+        }
         case 7: {  // h, coarsened in place by coarsenUpwards_ for tagged nodes
             const int len = 4 * nd.s;
             if (!out) { *rows = len >> nd.tag; *cols = 1; break; }

@@ -14,7 +14,11 @@
 //  * multi-RHS vectors (h, g, w, u_int) live in per-batch arenas of R rows and k columns,
 //    right-hand side major: element (row, rhs) is at rhs * R + row, so one right-hand side's
 //    node vector is contiguous;
-//  * leaf f / u for k right-hand sides: [rhs][leaf][cell].
+//  * leaf f / u for k right-hand sides: [rhs][leaf][cell];
+//  * a node's boundary data g is read side by side: side s (WESN) of a node of side length m
+//    is m consecutive rows at an arena offset of its own, which is either the node's g (when
+//    the sweep writes it there) or the segment of an ancestor's interface potential or of the
+//    root's g that the side lies on.
 #pragma once
 #include <cstddef>
 #include <cstdint>
@@ -37,6 +41,11 @@ __host__ __device__ inline EF_GLOBAL T* gmem(T* p) { return (EF_GLOBAL T*)p; }
 
 enum Side { W = 0, E = 1, S = 2, N = 3 };
 
+template <class T>
+__host__ __device__ inline T pick4(const T (&a)[4], int c) {  // a[c] without indexing a private array
+    return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
+}
+
 // coarsen_ step (HPSAlgorithm.hpp:707-741): T (4m x 4m) -> L21 T L12 (2m x 2m) per side block.
 struct CoarsenDesc {
     const double* src;  // 4m x 4m
@@ -49,7 +58,7 @@ struct LeafDesc {
     int cls;        // unused by the block-tridiagonal kernels (-1)
     int node;       // node id
     long long h_off; // row offset of h in the vector arena (4N rows)
-    long long g_off; // row offset of g
+    long long gs[4]; // row offsets of the four sides of g (N rows each)
     const double* coef;  // variable coefficients: 6 N*N arrays (alpha, bW, bE, bS, bN, lambda) or null
     double* Sinv;        // variable coefficients: N blocks of N x N, inverses of the block-LU pivots
     double* stencil;     // variable coefficients: 5 N*N arrays cW, cE, cS, cN, cC
@@ -72,16 +81,13 @@ struct UpDesc {
     const double* H;        // unused by the gather
 };
 
-// Downward family step (split1to4, HPSAlgorithm.hpp:562-577) for one merge.
-struct DownDesc {
-    int n;
-    int tag;                // tau's own n_coarsens (uncoarsen_)
-    long long gc_off;       // tau g as delivered by its parent (coarse, 4*(s>>tag) rows)
-    long long g_off;        // tau g at full size (8n rows)
-    long long w_off;        // w (4n)
-    long long ui_off;       // interior potential work (4n)
-    long long cg_off[4];    // children's coarse-g offsets (each 4n)
-    const double* S;        // 4n x 8n
+// Boundary data of a child written out by the downward sweep (split1to4, HPSAlgorithm.hpp:
+// 562-577): the four sides, len rows each, copied from their sources (segments of the parent's
+// boundary data or interface potential) to dst + s * len.
+struct SideCopy {
+    long long src[4];
+    long long dst;
+    int len, pad;
 };
 
 // Batched patch solves by fast diagonalisation (constant coefficients per patch).
@@ -104,7 +110,8 @@ struct PatchFDArgs {
     const double* g;        // boundary data, or null (zero); see g_off
     double* out;            // u [nrhs][npatches][N*N] | h (see h_off) | T [npatches][4N][4N]
     // Vector addressing.  Null offsets: g and h are [nrhs][npatches][4N].  Otherwise the
-    // multi-RHS arena layout: element t of patch p, rhs c at c * vk + off[p] + t.
+    // multi-RHS arena layout: element t of h of patch p, rhs c at c * vk + h_off[p] + t, and
+    // element i of side s of g at c * vk + g_off[4 p + s] + i.
     const long long* g_off;
     const long long* h_off;
     int vk;
@@ -146,15 +153,13 @@ struct VecStepDesc {
 // The planner fills these on the host and the kernels read them on the device: both
 // compiler passes must agree on every size and offset (x86-64 and amdgcn are both LP64).
 static_assert(sizeof(CoarsenDesc) == 24, "CoarsenDesc");
-static_assert(sizeof(LeafDesc) == 88, "LeafDesc");
+static_assert(sizeof(LeafDesc) == 112, "LeafDesc");
 static_assert(offsetof(LeafDesc, h_off) == 8, "LeafDesc.h_off");
-static_assert(offsetof(LeafDesc, coef) == 24, "LeafDesc.coef");
+static_assert(offsetof(LeafDesc, coef) == 48, "LeafDesc.coef");
 static_assert(sizeof(UpDesc) == 112, "UpDesc");
 static_assert(offsetof(UpDesc, ch_off) == 40, "UpDesc.ch_off");
 static_assert(offsetof(UpDesc, Xi) == 96, "UpDesc.Xi");
-static_assert(sizeof(DownDesc) == 80, "DownDesc");
-static_assert(offsetof(DownDesc, gc_off) == 8, "DownDesc.gc_off");
-static_assert(offsetof(DownDesc, S) == 72, "DownDesc.S");
+static_assert(sizeof(SideCopy) == 48, "SideCopy");
 static_assert(sizeof(VecStepDesc) == 24, "VecStepDesc");
 static_assert(sizeof(PatchCoef) == 40, "PatchCoef");
 static_assert(sizeof(PatchFDArgs) == 112, "PatchFDArgs");

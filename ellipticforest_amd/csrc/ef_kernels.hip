@@ -355,16 +355,21 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
     const int tid = threadIdx.x;
     const int NN = Nn * Nn;
     const double* fl = f_zero ? nullptr : f + ((long long)rhs * nleaves + leaf) * NN;
-    const double* g = (mode != 0) ? garena + (long long)rhs * vcs + L.g_off : nullptr;
+    // side sd of g: Nn rows at the leaf's own offset for that side
+    const double* g = (mode != 0) ? garena + (long long)rhs * vcs : nullptr;
+    const double* gW = g ? g + L.gs[W] : nullptr;
+    const double* gE = g ? g + L.gs[E] : nullptr;
+    const double* gS = g ? g + L.gs[S] : nullptr;
+    const double* gN = g ? g + L.gs[N] : nullptr;
     for (int e = tid; e < NN; e += 256) {
         const int i = e / Nn, j = e % Nn;
         double v = fl ? fl[e] : 0.0;
         if (g && (i == 0 || j == 0 || i == Nn - 1 || j == Nn - 1)) {
             double add = 0.0;
-            if (i == 0) add += L.stencil[e] * g[0 * Nn + j];
-            if (i == Nn - 1) add += L.stencil[NN + e] * g[1 * Nn + j];
-            if (j == 0) add += L.stencil[2 * NN + e] * g[2 * Nn + i];
-            if (j == Nn - 1) add += L.stencil[3 * NN + e] * g[3 * Nn + i];
+            if (i == 0) add += L.stencil[e] * gW[j];
+            if (i == Nn - 1) add += L.stencil[NN + e] * gE[j];
+            if (j == 0) add += L.stencil[2 * NN + e] * gS[i];
+            if (j == Nn - 1) add += L.stencil[3 * NN + e] * gN[i];
             v += -2.0 * add;
         }
         b[e] = v;
@@ -382,7 +387,7 @@ __global__ void __launch_bounds__(256) bt_solve_kernel(const LeafDesc* leaves, i
             const int sd = t / Nn;
             const double dd = (sd < 2) ? L.dx : L.dy;
             const double sg = (sd == W || sd == S) ? 2.0 / dd : -2.0 / dd;
-            h[t] = sg * (b[i * Nn + j] - (g ? g[t] : 0.0));
+            h[t] = sg * (b[i * Nn + j] - (g ? g[pick4(L.gs, sd) + (t - sd * Nn)] : 0.0));
         }
     }
 }
@@ -427,7 +432,8 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
     const int j = lane % NT, h = lane / NT;
     const LeafDesc L = leaves[leaf];
     const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
-    const auto* g = (mode != 0) ? gmem(garena) + (long long)rhs * vcs + L.g_off : nullptr;
+    // side sd of g: NT rows at the leaf's own offset for that side
+    const auto* g = (mode != 0) ? gmem(garena) + (long long)rhs * vcs : nullptr;
     const auto* cWv = gmem(L.stencil);
     const auto* cEv = cWv + NN;
     const auto* cSv = cWv + 2 * NN;
@@ -437,10 +443,10 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
         double b = fl ? fl[I] : 0.0;
         if (g) {
             double add = 0.0;
-            if (i == 0) add += cWv[I] * g[0 * NT + j];
-            if (i == NT - 1) add += cEv[I] * g[1 * NT + j];
-            if (j == 0) add += cSv[I] * g[2 * NT + i];
-            if (j == NT - 1) add += cNv[I] * g[3 * NT + i];
+            if (i == 0) add += cWv[I] * g[L.gs[W] + j];
+            if (i == NT - 1) add += cEv[I] * g[L.gs[E] + j];
+            if (j == 0) add += cSv[I] * g[L.gs[S] + i];
+            if (j == NT - 1) add += cNv[I] * g[L.gs[N] + i];
             b += -2.0 * add;
         }
         return b;
@@ -507,7 +513,10 @@ __global__ void __launch_bounds__(256) bt_solve_wave_kernel(const LeafDesc* leav
     // h_S = (2/dy)(u_S - g_S), h_N = -(2/dy)(u_N - g_N)
     auto* hv = gmem(harena) + (long long)rhs * vcs + L.h_off;
     const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
-    auto gv = [&](int t) { return g ? g[t] : 0.0; };
+    auto gv = [&](int t) {
+        const int sd = t / NT;
+        return g ? g[pick4(L.gs, sd) + (t - sd * NT)] : 0.0;
+    };
     for (int t = lane; t < 4 * NT; t += 64) {
         const int sd = t / NT, m = t % NT;
         double v;
@@ -780,7 +789,8 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
     const int j = lane % NT, h = lane / NT;
     const LeafDesc L = leaves[leaf];
     const auto* fl = f_zero ? nullptr : gmem(f) + ((long long)rhs * nleaves + leaf) * NN;
-    const auto* g = (mode != 0) ? gmem(garena) + (long long)rhs * vcs + L.g_off : nullptr;
+    // side sd of g: NT rows at the leaf's own offset for that side
+    const auto* g = (mode != 0) ? gmem(garena) + (long long)rhs * vcs : nullptr;
     auto* zl = gmem(zbuf) + ((long long)rhs * nleaves + leaf) * NN;
     const auto* cWv = gmem(L.stencil);
     const auto* cEv = cWv + NN;
@@ -792,10 +802,10 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
         double b = fl ? fl[I] : 0.0;
         if (g) {
             double add = 0.0;
-            if (i == 0) add += cWv[I] * g[0 * NT + j];
-            if (i == NT - 1) add += cEv[I] * g[1 * NT + j];
-            if (j == 0) add += cSv[I] * g[2 * NT + i];
-            if (j == NT - 1) add += cNv[I] * g[3 * NT + i];
+            if (i == 0) add += cWv[I] * g[L.gs[W] + j];
+            if (i == NT - 1) add += cEv[I] * g[L.gs[E] + j];
+            if (j == 0) add += cSv[I] * g[L.gs[S] + i];
+            if (j == NT - 1) add += cNv[I] * g[L.gs[N] + i];
             b += -2.0 * add;
         }
         return b;
@@ -896,7 +906,10 @@ bt_solve_sym_kernel(const LeafDesc* leaves, int nleaves, int k, int mode, const 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     auto* hv = gmem(harena) + (long long)rhs * vcs + L.h_off;
     const double sx = 2.0 / L.dx, sy = 2.0 / L.dy;
-    auto gv = [&](int t) { return g ? g[t] : 0.0; };
+    auto gv = [&](int t) {
+        const int sd = t / NT;
+        return g ? g[pick4(L.gs, sd) + (t - sd * NT)] : 0.0;
+    };
     for (int t = lane; t < 4 * NT; t += 64) {
         const int sd = t / NT, mm = t % NT;
         double v;

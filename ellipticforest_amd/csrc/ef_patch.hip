@@ -456,7 +456,7 @@ int ef_patches_create(ef_ctx ctx, const ef_patch_batch* b, const ef_coeffs* c, e
             L.cls = -1;
             L.node = (int)i;
             L.h_off = (long long)i * 4 * N;
-            L.g_off = (long long)i * 4 * N;
+            for (int sd = 0; sd < 4; sd++) L.gs[sd] = (long long)i * 4 * N + (long long)sd * N;
             L.coef = p->vcoef.p + i * 6 * NN;
             L.stencil = p->vstencil.p + i * 5 * NN;
             L.Sinv = p->vSinv.p + i * N * NN;

@@ -110,12 +110,24 @@ __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int
             for (int m = 0; m < NB; m++) in.G[m] = 0.0;
             return;
         }
+        // 4N = 64 NB entries, side t / N at offset so[side] from gb (the arena's per-side row
+        // offsets of the patch, or consecutive sides); only the offsets are chosen in a branch
         const double* gb;
-        long long gs;
-        if (ix.g_off) { gb = p.g + (long long)rhs * p.vk + ix.g_off[patch]; gs = 1; }
-        else { gb = p.g + slot * 4 * Nn; gs = 1; }
+        long long so[4];
+        if (ix.g_off) {
+            gb = p.g + (long long)rhs * p.vk;
 #pragma unroll
-        for (int m = 0; m < NB; m++) in.G[m] = gmem(gb)[(long long)(64 * m + lane) * gs];  // 4N = 64 NB entries
+            for (int sd = 0; sd < 4; sd++) so[sd] = ix.g_off[4LL * patch + sd];
+        } else {
+            gb = p.g + slot * 4 * Nn;
+#pragma unroll
+            for (int sd = 0; sd < 4; sd++) so[sd] = (long long)sd * Nn;
+        }
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            const int t = 64 * m + lane, sd = t / Nn;
+            in.G[m] = gmem(gb)[pick4(so, sd) + (t - sd * Nn)];
+        }
         return;
     }
     const double* f = p.f ? p.f + slot * Nn * Nn : nullptr;
@@ -139,16 +151,27 @@ __device__ inline void patch_fetch(const PatchFDArgs& p, const PatchIdx& ix, int
                 in.F[bi][bk][s] = v;
             }
     if (OP == PATCH_OP_DTN) return;
-    const double* g = nullptr;
-    long long gs = 1;
     if (p.g) {
-        if (ix.g_off) g = p.g + (long long)rhs * p.vk + ix.g_off[patch];
-        else g = p.g + slot * 4 * Nn;
-    }
+        const double* gb;
+        long long so[4];
+        if (ix.g_off) {
+            gb = p.g + (long long)rhs * p.vk;
 #pragma unroll
-    for (int m = 0; m < NB; m++) {
-        const int t = 64 * m + lane;
-        in.G[m] = (g && t < 4 * Nn) ? g[(long long)t * gs] : 0.0;
+            for (int sd = 0; sd < 4; sd++) so[sd] = ix.g_off[4LL * patch + sd];
+        } else {
+            gb = p.g + slot * 4 * Nn;
+#pragma unroll
+            for (int sd = 0; sd < 4; sd++) so[sd] = (long long)sd * Nn;
+        }
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+            const int t = 64 * m + lane, sd = min(t / Nn, 3);
+            const double x = gb[pick4(so, sd) + min(t - sd * Nn, Nn - 1)];
+            in.G[m] = t < 4 * Nn ? x : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NB; m++) in.G[m] = 0.0;
     }
 }
 

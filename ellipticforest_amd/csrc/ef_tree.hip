@@ -55,10 +55,7 @@ __device__ inline double readlane_f64(double x, int src) {
     return __hiloint2double(hi, lo);
 }
 
-template <class T>
-__device__ inline T pick4(const T (&a)[4], int c) {
-    return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
-}
+
 
 // ---------------------------------------------------------------------------------------
 // Batched GEMM
@@ -68,16 +65,19 @@ __device__ inline T pick4(const T (&a)[4], int c) {
 // Within each block of 16 k values lane group q takes k = kb + 4q + s at MFMA step s: A is
 // read as two 16-byte vectors per lane and row (a 128-byte row segment per lane group), B as
 // one row segment of 16 consecutive columns per lane group and step.
-// hDiff mode of segment 0: row offsets of the two children's segments of each interface block
+// block-gather mode of segment 0: row offsets of each block's segment (hDiff: the two children's
+// segments, subtracted; g: the side's segment alone)
+// (MODE 1 subtracts the d1 segment, MODE 2 reads the d2 segment alone)
 struct DiffRows {
     int dn, kbase;  // block size; absolute row of the segment's first k
     long long d2[4], d1[4];
 };
 
-template <int RT, int TN, bool F32, bool DIFF>
+template <int RT, int TN, bool F32, int MODE>
 __device__ inline void gemm_segment(const double* __restrict__ a0p, const double* __restrict__ a1p,
                                     const double* const (&bp)[TN], long long ldb, int K, double alpha,
                                     GemmAcc<F32> (&acc)[2][TN], const DiffRows& dr) {
+    constexpr bool DIFF = MODE != 0;
     const int q = (int)(threadIdx.x & 63) >> 4;
     // hDiff rows: absolute row ka lies in interface block r = ka / dn; both children's offsets
     auto drow = [&](int ka, long long& o2, long long& o1) {
@@ -98,15 +98,23 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
             o.y0 = *(const td2*)(a1p + k);
             o.y1 = *(const td2*)(a1p + k + 2);
         }
-        if (DIFF) {  // four consecutive rows lie in one interface block (dn % 4 == 0)
+        if (DIFF) {  // four consecutive rows lie in one block (dn % 4 == 0)
             long long o2, o1;
             drow(dr.kbase + k, o2, o1);
+            if (MODE == 1) {
 #pragma unroll
-            for (int j = 0; j < TN; j++) {
-                const td2 u0 = *(const td2*)(bp[j] + o2), u1 = *(const td2*)(bp[j] + o2 + 2);
-                const td2 v0 = *(const td2*)(bp[j] + o1), v1 = *(const td2*)(bp[j] + o1 + 2);
-                o.b[j][0] = u0[0] - v0[0]; o.b[j][1] = u0[1] - v0[1];
-                o.b[j][2] = u1[0] - v1[0]; o.b[j][3] = u1[1] - v1[1];
+                for (int j = 0; j < TN; j++) {
+                    const td2 u0 = *(const td2*)(bp[j] + o2), u1 = *(const td2*)(bp[j] + o2 + 2);
+                    const td2 v0 = *(const td2*)(bp[j] + o1), v1 = *(const td2*)(bp[j] + o1 + 2);
+                    o.b[j][0] = u0[0] - v0[0]; o.b[j][1] = u0[1] - v0[1];
+                    o.b[j][2] = u1[0] - v1[0]; o.b[j][3] = u1[1] - v1[1];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    const td2 u0 = *(const td2*)(bp[j] + o2), u1 = *(const td2*)(bp[j] + o2 + 2);
+                    o.b[j][0] = u0[0]; o.b[j][1] = u0[1]; o.b[j][2] = u1[0]; o.b[j][3] = u1[1];
+                }
             }
         } else if (ldb == 1) {  // B contiguous along k: two 16-byte vectors per column
 #pragma unroll
@@ -165,7 +173,7 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
                 long long o2, o1;
                 drow(dr.kbase + kk, o2, o1);
 #pragma unroll
-                for (int j = 0; j < TN; j++) b[j][s] = bp[j][o2] - bp[j][o1];
+                for (int j = 0; j < TN; j++) b[j][s] = MODE == 1 ? bp[j][o2] - bp[j][o1] : bp[j][o2];
             } else {
 #pragma unroll
                 for (int j = 0; j < TN; j++) b[j][s] = bp[j][(long long)kk * ldb];
@@ -240,23 +248,24 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
         if (p.dn > 0) {  // hDiff mode: column bases, rows resolved per interface block
 #pragma unroll
             for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)cj[j] * p.bcs;
-            gemm_segment<RT, TN, F32, true>(p.A0 + (long long)rows[0] * p.lda0 + kb0,
-                                            p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp, 1, ke0 - kb0, p.alpha0,
-                                            acc, dr);
+            const double* a0 = p.A0 + (long long)rows[0] * p.lda0 + kb0;
+            const double* a1 = p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0;
+            if (p.dsub) gemm_segment<RT, TN, F32, 1>(a0, a1, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
+            else gemm_segment<RT, TN, F32, 2>(a0, a1, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
         } else {
 #pragma unroll
             for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj[j] * p.bcs;
-            gemm_segment<RT, TN, F32, false>(p.A0 + (long long)rows[0] * p.lda0 + kb0,
-                                             p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp, p.ldb0, ke0 - kb0,
-                                             p.alpha0, acc, dr);
+            gemm_segment<RT, TN, F32, 0>(p.A0 + (long long)rows[0] * p.lda0 + kb0,
+                                         p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp, p.ldb0, ke0 - kb0,
+                                         p.alpha0, acc, dr);
         }
     }
     if (ke1 > kb1) {
         const double* bp[TN];
 #pragma unroll
         for (int j = 0; j < TN; j++) bp[j] = p.B1 + (long long)kb1 * p.ldb1 + (long long)cj[j] * p.bcs;
-        gemm_segment<RT, TN, F32, false>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
-                                         p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
+        gemm_segment<RT, TN, F32, 0>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
+                                     p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
     }
     // with RT = 1 both accumulator rows hold the same 16 rows; only the first is stored
 #pragma unroll
@@ -889,37 +898,19 @@ __global__ void __launch_bounds__(256) tree_uncoarsen_kernel(const VecStepDesc* 
     Vr[d.dst_off + row] = s;
 }
 
-// children's g (applyS_, HPSAlgorithm.hpp:1194-1230): alpha = [aW, ab, aS, ag], beta = [ab, bE,
-// bS, bo], gamma = [gW, go, ag, gN], omega = [go, oE, bo, oN].  A child's side lies either on
-// tau's boundary (block q of tau's g) or on an interface (block r of the interface potential).
-__device__ inline int child_src_q(int c, int s) {
-    // side s of child c on tau's boundary: the WESN block q of (c, s), or -1
-    const bool west = (c & 1) == 0, south = (c >> 1) == 0;
-    if (s == W && west) return c == 0 ? 0 : 1;
-    if (s == E && !west) return c == 1 ? 2 : 3;
-    if (s == S && south) return c == 0 ? 4 : 5;
-    if (s == N && !south) return c == 2 ? 6 : 7;
-    return -1;
-}
-__device__ inline int child_src_r(int c, int s) {
-    for (int r = 0; r < 4; r++)
-        if (side_on_if(c, r) == s) return r;
-    return 0;
-}
-
-__global__ void __launch_bounds__(256) tree_down_scatter_kernel(const DownDesc* __restrict__ descs, double* V, int k,
-                                                                 long long vrows) {
-    const DownDesc d = descs[blockIdx.y];
-    const int n = d.n;
+// A child's boundary data written out (for a coarsened child, whose g the uncoarsen steps
+// interpolate; or for every child where the sweeps cannot read sides in place): side s of
+// descriptor blockIdx.y from src[s], one thread per (rhs, row).
+__global__ void __launch_bounds__(256) tree_side_copy_kernel(const SideCopy* __restrict__ descs, double* V, int k,
+                                                              long long vrows) {
+    const SideCopy d = descs[blockIdx.y];
+    const int len = d.len;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 16LL * n * k) return;
-    const int rhs = (int)(e / (16 * n)), row = (int)(e % (16 * n));
-    const int c = row / (4 * n), rr = row % (4 * n);
-    const int s = rr / n, i = rr % n;
-    const int q = child_src_q(c, s);
+    if (e >= 4LL * len * k) return;
+    const int rhs = (int)(e / (4 * len)), row = (int)(e % (4 * len));
+    const int s = row / len, i = row - s * len;
     double* Vr = V + (long long)rhs * vrows;
-    const double v = (q >= 0) ? Vr[d.g_off + (long long)q * n + i] : Vr[d.ui_off + (long long)child_src_r(c, s) * n + i];
-    Vr[pick4(d.cg_off, c) + rr] = v;
+    Vr[d.dst + row] = Vr[pick4(d.src, s) + i];
 }
 
 // g = r / a: the LU solve of diag(a) in solveStage (HPSAlgorithm.hpp:402-420); r is [rhs][m],
@@ -984,13 +975,14 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
         const double* As[2] = {p.A0, p.A1};
         const double* Bs[2] = {p.B0, p.B1};
         const int lda[2] = {p.lda0, p.lda1}, ldb[2] = {p.ldb0, p.ldb1}, K[2] = {p.K0, p.K1};
-        if (p.dn > 0) {  // hDiff mode: every block's two child segments inside B0's allocation
-            if (p.dn % 4 != 0 || p.ldb0 != 1 || p.K0 != 4 * p.dn || p.K1 > 0)
-                throw EfError(EF_ERR_INTERNAL, "hDiff GEMM needs four blocks of a multiple of 4 rows, contiguous");
+        if (p.dn > 0) {  // block-gather mode: every block's segments inside B0's allocation
+            if (p.dn % 4 != 0 || p.ldb0 != 1 || p.K0 != 4 * p.dn || p.K1 > 0 || (p.dsub != 0 && p.dsub != 1))
+                throw EfError(EF_ERR_INTERNAL, "block-gather GEMM needs four blocks of a multiple of 4 rows, contiguous");
             for (int r4 = 0; r4 < 4; r4++)
-                for (const long long o : {p.d2[r4], p.d1[r4]}) {
-                    if (o < 0 || (o & 1)) throw EfError(EF_ERR_INTERNAL, "hDiff GEMM row offset negative or odd");
-                    check_strided(p.B0 + o, p.dn, p.N, 1, p.bcs, "GEMM hDiff block");
+                for (int h2 = 0; h2 < 1 + p.dsub; h2++) {
+                    const long long o = h2 ? p.d1[r4] : p.d2[r4];
+                    if (o < 0 || (o & 1)) throw EfError(EF_ERR_INTERNAL, "block-gather GEMM row offset negative or odd");
+                    check_strided(p.B0 + o, p.dn, p.N, 1, p.bcs, "GEMM gathered block");
                 }
         }
         for (int sg = 0; sg < 2; sg++) {
@@ -998,7 +990,7 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
             check_matrix(As[sg], p.M, K[sg], lda[sg], "GEMM A");
             if (sg == 0 && p.dn > 0) {
                 if (((uintptr_t)As[0] & 15) != 0 || (lda[0] & 1) != 0 || ((uintptr_t)p.B0 & 15) != 0 || (p.bcs & 1) != 0)
-                    throw EfError(EF_ERR_INTERNAL, "hDiff GEMM operands not 16-byte aligned");
+                    throw EfError(EF_ERR_INTERNAL, "block-gather GEMM operands not 16-byte aligned");
                 continue;
             }
             check_strided(Bs[sg], K[sg], p.N, ldb[sg], p.bcs, "GEMM B");

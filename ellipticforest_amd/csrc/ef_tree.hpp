@@ -41,6 +41,24 @@ __host__ __device__ inline int child_if(int c, int slot) {
     // alpha: {0, 2}, beta: {1, 2}, gamma: {0, 3}, omega: {1, 3}
     return slot == 0 ? (c & 1) : 2 + (c >> 1);
 }
+// children's g (applyS_, HPSAlgorithm.hpp:1194-1230): alpha = [aW, ab, aS, ag], beta = [ab, bE,
+// bS, bo], gamma = [gW, go, ag, gN], omega = [go, oE, bo, oN].  A child's side lies either on
+// tau's boundary (block q of tau's g: half q & 1 of tau's side q >> 1) or on an interface
+// (block r of the interface potential).
+__host__ __device__ inline int child_src_q(int c, int s) {
+    // side s of child c on tau's boundary: the WESN block q of (c, s), or -1
+    const bool west = (c & 1) == 0, south = (c >> 1) == 0;
+    if (s == W && west) return c == 0 ? 0 : 1;
+    if (s == E && !west) return c == 1 ? 2 : 3;
+    if (s == S && south) return c == 0 ? 4 : 5;
+    if (s == N && !south) return c == 2 ? 6 : 7;
+    return -1;
+}
+__host__ __device__ inline int child_src_r(int c, int s) {
+    for (int r = 0; r < 4; r++)
+        if (side_on_if(c, r) == s) return r;
+    return 0;
+}
 
 // ---------------------------------------------------------------------------------------
 // Batched two-segment GEMM on v_mfma_f64_16x16x4_f64, row-major operands:
@@ -67,10 +85,12 @@ struct TreeGemm {
     double alpha0, alpha1, beta;
     int nks, pad;                        // split-K chunk count (set by plan_gemm)
     double* P;                           // partials, nks x M x N (split problems)
-    // hDiff mode (dn > 0, segment 0 only, ldb0 = 1): row kk of B0 is B0[d2[r] + i] - B0[d1[r] + i]
-    // with r = kk / dn, i = kk % dn (dn a multiple of 4): the four interface blocks of mergeW_'s
-    // hDiff read straight from the children's h, each block the difference of two children
-    int dn, dpad;
+    // Block-gather mode (dn > 0, segment 0 only, ldb0 = 1): row kk of B0 is B0[d2[r] + i] -
+    // B0[d1[r] + i] with r = kk / dn, i = kk % dn (dn a multiple of 4): the four interface
+    // blocks of mergeW_'s hDiff read straight from the children's h, each block the difference
+    // of two children.  dsub = 0: row kk is B0[d2[r] + i] alone (d1 unused): the four sides of
+    // a node's g read where they lie.
+    int dn, dsub;
     long long d2[4], d1[4];
 };
 static_assert(sizeof(TreeGemm) == 216, "TreeGemm layout");
@@ -154,7 +174,7 @@ __global__ void tree_up_gather_kernel(const UpDesc* d, double* V, int k, long lo
 // uncoarsen_ step: g <- blkdiag4(L12(nfine)) g
 __global__ void tree_uncoarsen_kernel(const VecStepDesc* d, double* V, int k, long long vrows);
 // applyS_ scatter: children's g from tau's g (WESN) and the interface potential
-__global__ void tree_down_scatter_kernel(const DownDesc* d, double* V, int k, long long vrows);
+__global__ void tree_side_copy_kernel(const SideCopy* d, double* V, int k, long long vrows);
 // root Dirichlet data g = r / a (solveStage with b = 0)
 __global__ void tree_root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k, long long vrows);
 // root Robin system A = diag(a) + diag(b) T, rhs = r - b h

@@ -39,10 +39,17 @@ def gpu_examples(tmp_path_factory):
     return dict(zip(efbuild.EXAMPLES, exes)), os.path.dirname(_lib.LIB_PATH)
 
 
+def emu_env(libdir):
+    env = dict(os.environ, LD_LIBRARY_PATH=libdir)
+    if "emu_lib" in libdir:  # timing tile shapes on the emulator measures nothing
+        env["EF_AUTOTUNE"] = "0"
+    return env
+
+
 def run(example, *args, name="elliptic_single"):
     exes, libdir = example
     exe = exes[name]
-    env = dict(os.environ, LD_LIBRARY_PATH=libdir)
+    env = emu_env(libdir)
     out = subprocess.run([exe] + list(args), capture_output=True, text=True, env=env, timeout=900)
     assert out.returncode == 0, out.stderr
     vals = {}
@@ -165,7 +172,7 @@ def check_fv_solver_output(out, max_n=32, five_point=False):
 
 def run_raw(example, name, *args):
     exes, libdir = example
-    env = dict(os.environ, LD_LIBRARY_PATH=libdir)
+    env = emu_env(libdir)
     out = subprocess.run([exes[name]] + list(args), capture_output=True, text=True, env=env, timeout=900)
     assert out.returncode == 0, out.stderr
     return out.stdout
