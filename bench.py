@@ -42,7 +42,7 @@ METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds
 TRAFFIC_PER_LAUNCH = 572.9e6          # leaf solves of this workload, 16 right-hand sides
 TRAFFIC_MAP_D2N = 319.0e6             # particular Neumann data, same leaf set and k
 TRAFFIC_VAR_BETA = 1329.8e6           # variable-beta leaf solves, packed symmetric blocks
-TRAFFIC_GEMM_PER_LAUNCH = 76.37e6     # tree_gemm_kernel in the sweep at 64 columns, per launch: profiles/tree_gemm_kernel_adaptive_poisson_sweep.md
+TRAFFIC_GEMM_PER_LAUNCH = 84.35e6     # tree_gemm_kernel in the sweep at 64 columns, per launch: profiles/tree_gemm_kernel_adaptive_poisson_sweep.md
 
 
 def log(msg):
@@ -111,12 +111,22 @@ def sweep_bytes(D, N, k):
     return 8.0 * (64.0 * float(np.sum(ns ** 2)) + k * D.nleaves * 2.0 * N * N)
 
 
+def plain_merges(D):
+    """Per merge (in node order): True when no child is coarsened, so its w product reads hDiff as
+    differences of the children's h in place (8 n k reads) instead of a gathered hDiff (4 n k)."""
+    info = D.node_info
+    return np.array([all(info[info[i, 7 + c], 5] == 0 for c in range(4))
+                     for i in range(D.nnodes) if not info[i, 3]], bool)
+
+
 def gemm_bytes(D, k):
     """Algorithmic HBM bytes of the tree GEMMs of one step: per merge the operators X^-1 (16 n^2),
     the 16 n^2 nonzeros of H and S (32 n^2) read once, and the vectors of its products
-    (w = X^-1 hDiff: 8 n k, h += H w: 32 n k, u = S g + w: 16 n k), FP64."""
+    (w = X^-1 hDiff: 12 n k with hDiff formed from the children's h in place, 8 n k from a gathered
+    hDiff; h += H w: 32 n k; u = S g + w: 16 n k, g read side by side where it lies), FP64."""
     ns, _ = tree_counts(D)
-    return 8.0 * float(np.sum(64.0 * ns ** 2 + 56.0 * ns * k))
+    vec = np.where(plain_merges(D), 60.0, 56.0)
+    return 8.0 * float(np.sum(64.0 * ns ** 2 + vec * ns * k))
 
 
 def build_flops(D, N, nclasses):
