@@ -699,7 +699,7 @@ void check_tree_kernels() {
     check_kernel_resources((const void*)tree_gemm_kernel<1, 2, true>, "tree_gemm_kernel<1,2,f32>");
     check_kernel_resources((const void*)tree_gemm_kernel<1, 1, true>, "tree_gemm_kernel<1,1,f32>");
     check_kernel_resources((const void*)tree_gemm_kernel<2, 2, true>, "tree_gemm_kernel<2,2,f32>");
-    check_kernel_resources((const void*)tree_gemm_reduce_kernel, "tree_gemm_reduce_kernel");
+    check_kernel_resources((const void*)tree_gemm_split_kernel, "tree_gemm_split_kernel");
     check_kernel_resources((const void*)gj_panel_kernel<0, 1, 1, true>, "gj_panel_kernel<0,1,1,fin>");
     check_kernel_resources((const void*)gj_panel_kernel<0, 1, 2, true>, "gj_panel_kernel<0,1,2,fin>");
     check_kernel_resources((const void*)gj_panel_kernel<0, 2, 2, true>, "gj_panel_kernel<0,2,2,fin>");
@@ -921,6 +921,11 @@ GemmLaunch tune_gemm(ef_mesh_s* M, const std::vector<TreeGemm>& probs, DevStore&
     EF_HIP(hipEventCreate(&e1));
     for (const auto& c : cand) {
         GemmLaunch g = plan_gemm(probs, c[1], store, s, false, c[0]);
+        if (g.kw > 1) {  // split-K launches have one shape (16 x 16 tiles, a wave per chunk)
+            tile = 11;
+            best = g;
+            break;
+        }
         store.flush(s);
         run_gemm(g, s);  // warm
         float ms = 0.0f;
@@ -1649,14 +1654,14 @@ int ef_mesh_time_sweep_gemms(ef_mesh mesh, int nrhs, int reps, double* ms_per_pa
             if (!mesh->homogeneous_rhs) {
                 run_gemm(L.gw, s);
                 run_gemm(L.gh, s);
-                count += (L.gw.nwaves > 0) + (L.gw.nelem > 0) + (L.gh.nwaves > 0) + (L.gh.nelem > 0);
+                count += (L.gw.nwaves > 0) + (L.gh.nwaves > 0);
             }
         }
         for (int dd = 0; dd <= P.maxdepth; dd++) {
             const LevelSweep& L = sp.lv[dd];
             if (L.cnt == 0) continue;
             run_gemm(L.gu, s);
-            count += (L.gu.nwaves > 0) + (L.gu.nelem > 0);
+            count += (L.gu.nwaves > 0);
         }
     };
     pass();  // warm

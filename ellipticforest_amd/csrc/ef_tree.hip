@@ -213,9 +213,7 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
     const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
     constexpr int RW = 16 * RT, CW = 16 * TN;
     const int tm = (p.M + RW - 1) / RW, tn = (p.N + CW - 1) / CW;
-    int t = w - start[lo];
-    const int chunk = t / (tm * tn);
-    t -= chunk * tm * tn;
+    const int t = w - start[lo];
     const int row0 = (t / tn) * RW, col0 = (t % tn) * CW;
     int rows[RT];
 #pragma unroll
@@ -228,17 +226,13 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < TN; j++) acc[i][j] = GemmAcc<F32>{0, 0, 0, 0};
-    // k range of this wave: all of both segments, or one chunk of one segment
-    int kb0 = 0, ke0 = p.K0, kb1 = 0, ke1 = p.K1;
-    if (p.kc > 0) {
-        const int kbeg = chunk * p.kc, kend = kbeg + p.kc;
-        if (kbeg < p.K0) { kb0 = kbeg; ke0 = min(kend, p.K0); kb1 = ke1 = 0; }
-        else { kb0 = ke0 = 0; kb1 = kbeg - p.K0; ke1 = min(kend - p.K0, p.K1); }
-    }
-    // C (when the product adds to it and is not split) loaded before the products, so its
-    // latency overlaps them; rows and columns clamped as the stores are guarded
+    // this wave sums both segments over their whole k range (split-K launches run on
+    // tree_gemm_split_kernel)
+    const int kb0 = 0, ke0 = p.K0, kb1 = 0, ke1 = p.K1;
+    // C (when the product adds to it) loaded before the products, so its latency overlaps
+    // them; rows and columns clamped as the stores are guarded
     double cv[RT][TN][4];
-    const bool has_c = p.C != nullptr && p.kc == 0;
+    const bool has_c = p.C != nullptr;
     if (has_c) {
 #pragma unroll
         for (int i = 0; i < RT; i++)
@@ -289,13 +283,9 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
             for (int v = 0; v < 4; v++) {
                 const int row = row0 + 16 * i + (F32 ? 4 * q + v : 4 * v + q), col = col0 + 16 * j + r;
                 if (row < p.M && col < p.N) {
-                    if (p.kc > 0) {
-                        p.P[((long long)chunk * p.M + row) * p.N + col] = (double)acc[i][j][v];
-                    } else {
-                        double o = (double)acc[i][j][v];
-                        if (has_c) o += p.beta * cv[i][j][v];
-                        p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
-                    }
+                    double o = (double)acc[i][j][v];
+                    if (has_c) o += p.beta * cv[i][j][v];
+                    p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
                 }
             }
 }
@@ -307,26 +297,82 @@ template __global__ void tree_gemm_kernel<1, 1, true>(const TreeGemm*, const int
 template __global__ void tree_gemm_kernel<1, 2, true>(const TreeGemm*, const int*, const int*, int);
 template __global__ void tree_gemm_kernel<2, 2, true>(const TreeGemm*, const int*, const int*, int);
 
-// D = sum over chunks of the partials (in chunk order) + beta C, one thread per element of the
-// split problems of a launch.
-__global__ void __launch_bounds__(256) tree_gemm_reduce_kernel(const TreeGemm* __restrict__ probs,
-                                                                const long long* __restrict__ estart, int nprob) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= estart[nprob]) return;
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (estart[mid] <= e) lo = mid;
-        else hi = mid - 1;
-    }
+// Split-K inside the workgroup: workgroup g computes the 16 x 16 output tile g - start[p] of
+// problem p = gprob[g]; its wave w sums chunk w, k in [w kc, (w + 1) kc), of the concatenated k
+// range of the two segments (a chunk may span both), and wave 0 adds the chunks' partials in
+// chunk order, then beta C.  The chunks depend on the problem only, never on N, so a column's
+// result does not depend on the batch; nothing goes through HBM between the chunks and no second
+// launch sums them.  blockDim = 64 x (the launch's largest chunk count, <= 16).
+__global__ void __launch_bounds__(1024) tree_gemm_split_kernel(const TreeGemm* __restrict__ probs,
+                                                                const int* __restrict__ start,
+                                                                const int* __restrict__ gprob, int ngroups) {
+    __shared__ double part[16][4][64];  // [chunk][accumulator slot][lane], 32 KiB
+    const int g = (int)blockIdx.x;
+    if (g >= ngroups) return;  // uniform per workgroup
+    const int lo = __builtin_amdgcn_readfirstlane(gprob[g]);
     const TreeGemm p = probs[lo];
-    const long long i = e - estart[lo];
-    const int row = (int)(i / p.N), col = (int)(i % p.N);
-    const long long mn = (long long)p.M * p.N;
-    double o = 0.0;
-    for (int c = 0; c < p.nks; c++) o += p.P[c * mn + i];
-    if (p.C) o += p.beta * p.C[(long long)row * p.ldc + (long long)col * p.ccs];
-    p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+    const int q = lane >> 4, r = lane & 15;
+    const int tn = (p.N + 15) / 16;
+    const int t = g - start[lo];
+    const int row0 = (t / tn) * 16, col0 = (t % tn) * 16;
+    const int rowc = min(row0 + r, p.M - 1), cj = min(col0 + r, p.N - 1);
+    GemmAcc<false> acc[2][1];
+    acc[0][0] = GemmAcc<false>{0, 0, 0, 0};
+    acc[1][0] = GemmAcc<false>{0, 0, 0, 0};
+    const int K = p.K0 + p.K1, kbeg = wave * p.kc, kend = min(kbeg + p.kc, K);
+    // wave 0 loads C before its products (it alone writes D)
+    double cv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (wave == 0 && p.C) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = min(row0 + 4 * v + q, p.M - 1);
+            cv[v] = p.C[(long long)row * p.ldc + (long long)cj * p.ccs];
+        }
+    }
+    if (kbeg < K) {  // uniform per wave
+        const int kb0 = min(kbeg, p.K0), ke0 = min(kend, p.K0);
+        const int kb1 = max(kbeg, p.K0) - p.K0, ke1 = max(kend, p.K0) - p.K0;
+        DiffRows dr;
+        dr.dn = p.dn;
+        dr.kbase = kb0;
+#pragma unroll
+        for (int r4 = 0; r4 < 4; r4++) { dr.d2[r4] = p.d2[r4]; dr.d1[r4] = p.d1[r4]; }
+        if (ke0 > kb0) {
+            const double* bp[1];
+            const double* a0 = p.A0 + (long long)rowc * p.lda0 + kb0;
+            if (p.dn > 0) {
+                bp[0] = p.B0 + (long long)cj * p.bcs;
+                if (p.dsub) gemm_segment<1, 1, false, 1>(a0, a0, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
+                else gemm_segment<1, 1, false, 2>(a0, a0, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
+            } else {
+                bp[0] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj * p.bcs;
+                gemm_segment<1, 1, false, 0>(a0, a0, bp, p.ldb0, ke0 - kb0, p.alpha0, acc, dr);
+            }
+        }
+        if (ke1 > kb1) {
+            const double* bp[1] = {p.B1 + (long long)kb1 * p.ldb1 + (long long)cj * p.bcs};
+            const double* a1 = p.A1 + (long long)rowc * p.lda1 + kb1;
+            gemm_segment<1, 1, false, 0>(a1, a1, bp, p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
+        }
+    }
+    if (wave > 0) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) part[wave][v][lane] = acc[0][0][v];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int row = row0 + 4 * v + q, col = col0 + r;
+        if (row < p.M && col < p.N) {
+            double o = 0.0;
+            o += acc[0][0][v];
+            for (int c = 1; c < p.nks; c++) o += part[c][v][lane];
+            if (p.C) o += p.beta * cv[v];
+            p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -967,15 +1013,28 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
     if (maxM <= 16) g.rt = 1;  // 16-row tiles: a 32-row tile would compute a clamped copy of the same rows
     if (rt == 1 || (rt == 2 && tn == 2)) g.rt = rt;  // an explicit choice (the tile autotuner)
     std::vector<TreeGemm> probs = probs_in;
+    // Split-K: when a problem asks for k chunks (kc > 0), the whole launch runs on
+    // tree_gemm_split_kernel with one wave per chunk of a 16 x 16 tile; problems that asked for
+    // none are cut into chunks of the same count (rounded to 16).  The chunks depend on the
+    // launch's problems only, never on N.
+    int kw = 1;
+    for (const TreeGemm& p : probs)
+        if (p.M > 0 && p.N > 0 && p.kc > 0) {
+            if (p.kc % 16 != 0) throw EfError(EF_ERR_INTERNAL, "split-K chunk must be a multiple of 16");
+            kw = std::max(kw, (p.K0 + p.K1 + p.kc - 1) / p.kc);
+        }
+    if (kw > 16) throw EfError(EF_ERR_INTERNAL, "split-K launch with more than 16 chunks per tile");
+    if (kw > 1 && f32) throw EfError(EF_ERR_INTERNAL, "split-K launch on the FP32 matrix cores");
+    if (kw > 1) {
+        g.kw = kw;
+        g.rt = g.tn = 1;
+    }
     std::vector<int> start(probs.size() + 1, 0);
-    std::vector<long long> estart(probs.size() + 1, 0);
-    const int rw = 16 * g.rt, cw = 16 * tn;
-    long long total = 0, elems = 0;
-    bool any_split = false;
+    const int rw = 16 * g.rt, cw = 16 * g.tn;
+    long long total = 0;
     for (size_t i = 0; i < probs.size(); i++) {
         TreeGemm& p = probs[i];
         start[i] = (int)total;
-        estart[i] = elems;
         p.nks = 1;
         p.P = nullptr;
         if (p.bcs == 0) p.bcs = 1;
@@ -1012,39 +1071,38 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
             if (ldb[sg] == 1 && (((uintptr_t)Bs[sg] & 15) != 0 || (p.bcs & 1) != 0))
                 throw EfError(EF_ERR_INTERNAL, "GEMM B operand contiguous along k but not 16-byte aligned");
         }
-        if (p.kc > 0) {
-            if (p.kc % 16 != 0 || (p.K1 > 0 && p.K0 % p.kc != 0))
-                throw EfError(EF_ERR_INTERNAL, "split-K chunk must be a multiple of 16 dividing the first segment");
-            p.nks = (p.K0 + p.K1 + p.kc - 1) / p.kc;
-            if (p.nks <= 1) { p.kc = 0; p.nks = 1; }
+        if (g.kw > 1) {
+            const int Kt = p.K0 + p.K1;
+            if (p.kc == 0) p.kc = std::max(16, ((Kt + g.kw - 1) / g.kw + 15) / 16 * 16);
+            p.nks = (Kt + p.kc - 1) / p.kc;
+            if (p.nks > g.kw) throw EfError(EF_ERR_INTERNAL, "split-K chunks exceed the launch's waves per tile");
+            // a chunk that starts in segment 1 starts at an even k there (16-byte operand reads)
+            if (p.K1 > 0 && (p.K0 & 1)) throw EfError(EF_ERR_INTERNAL, "split-K over two segments needs an even K0");
+        } else {
+            p.kc = 0;
         }
-        if (p.kc > 0) {
-            p.P = store.alloc_doubles((size_t)p.nks * p.M * p.N);
-            elems += (long long)p.M * p.N;
-            any_split = true;
-        }
-        total += (long long)((p.M + rw - 1) / rw) * ((p.N + cw - 1) / cw) * p.nks;
+        total += (long long)((p.M + rw - 1) / rw) * ((p.N + cw - 1) / cw);
         if (total > 0x7fffffffLL) throw EfError(EF_ERR_INTERNAL, "GEMM launch too large");
     }
     start[probs.size()] = (int)total;
-    estart[probs.size()] = elems;
     g.nprob = (int)probs.size();
-    g.nwaves = (int)total;
+    g.nwaves = (int)total;  // waves (tile kernel) or workgroups (split kernel)
     std::vector<int> wprob((size_t)total);
     for (size_t i = 0; i < probs.size(); i++)
         for (int w = start[i]; w < start[i + 1]; w++) wprob[w] = (int)i;
     g.probs = store.stage(probs);
     g.start = store.stage(start);
     g.wprob = store.stage(wprob);
-    if (any_split) {
-        g.estart = store.stage(estart);
-        g.nelem = elems;
-    }
     return g;
 }
 
 void run_gemm(const GemmLaunch& g, hipStream_t s) {
     if (g.nwaves == 0) return;
+    if (g.kw > 1) {
+        EF_LAUNCH("tree_gemm_split_kernel", tree_gemm_split_kernel, dim3(g.nwaves), dim3(64 * g.kw), s, g.probs,
+                  g.start, g.wprob, g.nwaves);
+        return;
+    }
     const dim3 grid(blocks_for(g.nwaves, 4)), block(256);
 #define EF_GEMM_LAUNCH(RT_, TN_, F32_, NAME_)                                                          \
     EF_LAUNCH(NAME_, (tree_gemm_kernel<RT_, TN_, F32_>), grid, block, s, g.probs, g.start, g.wprob, g.nwaves)
@@ -1058,9 +1116,6 @@ void run_gemm(const GemmLaunch& g, hipStream_t s) {
         else EF_GEMM_LAUNCH(2, 2, false, "tree_gemm_kernel<2,2>");
     }
 #undef EF_GEMM_LAUNCH
-    if (g.nelem > 0)
-        EF_LAUNCH("tree_gemm_reduce_kernel", tree_gemm_reduce_kernel, dim3(blocks_for(g.nelem)), block, s, g.probs,
-                  g.estart, g.nprob);
 }
 
 size_t gj_work_doubles(int m, int B) {

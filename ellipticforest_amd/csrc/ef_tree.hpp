@@ -67,10 +67,10 @@ __host__ __device__ inline int child_src_r(int c, int s) {
 // A0 is M x K0 (lda0), B0 is K0 x N (ldb0), ...  C may alias D (read, then written, by the
 // same lane).  One wave computes a (16 RT) x (16 TN) block; the blocks of all problems of a
 // launch form one flat list (problem p owns waves start[p] .. start[p+1]-1).
-// Split-K (kc > 0): the concatenated k range of the two segments is cut into nks chunks of kc
-// (a multiple of 16 that divides K0 when K1 > 0); each block of each chunk writes its partial
-// product to P[chunk][M][N], and tree_gemm_reduce_kernel sums the chunks in order (fixed order,
-// so a column's result does not depend on the other columns of the batch).
+// Split-K (kc > 0): the concatenated k range of the two segments is cut into nks chunks of kc (a
+// multiple of 16); a launch with such a problem runs on tree_gemm_split_kernel, one workgroup per
+// 16 x 16 tile and one wave per chunk, the chunks' partials summed in chunk order through LDS
+// (fixed order, so a column's result does not depend on the other columns of the batch).
 // ---------------------------------------------------------------------------------------
 struct TreeGemm {
     const double* A0;
@@ -84,7 +84,7 @@ struct TreeGemm {
     int bcs, ccs, dcs, kc;               // column strides (0 means 1); split-K chunk length (0: none)
     double alpha0, alpha1, beta;
     int nks, pad;                        // split-K chunk count (set by plan_gemm)
-    double* P;                           // partials, nks x M x N (split problems)
+    double* P;                           // unused (kept for the layout)
     // Block-gather mode (dn > 0, segment 0 only, ldb0 = 1): row kk of B0 is B0[d2[r] + i] -
     // B0[d1[r] + i] with r = kk / dn, i = kk % dn (dn a multiple of 4): the four interface
     // blocks of mergeW_'s hDiff read straight from the children's h, each block the difference
@@ -99,7 +99,7 @@ static_assert(sizeof(TreeGemm) == 216, "TreeGemm layout");
 // (+ beta C in FP64); the mixed-precision build's S and T products.
 template <int RT, int TN, bool F32>
 __global__ void tree_gemm_kernel(const TreeGemm* probs, const int* start, const int* wprob, int nwaves);
-__global__ void tree_gemm_reduce_kernel(const TreeGemm* probs, const long long* estart, int nprob);
+__global__ void tree_gemm_split_kernel(const TreeGemm* probs, const int* start, const int* gprob, int ngroups);
 
 // ---------------------------------------------------------------------------------------
 // In-place blocked Gauss-Jordan inversion with partial pivoting of a batch of B equally
@@ -188,10 +188,9 @@ __global__ void tree_root_robin_kernel(const double* T, const double* a, const d
 struct GemmLaunch {
     TreeGemm* probs = nullptr;
     int* start = nullptr;
-    int* wprob = nullptr;         // problem of each wave
-    long long* estart = nullptr;  // reduce: first element of each problem (split problems only)
+    int* wprob = nullptr;         // problem of each wave (of each workgroup for split-K launches)
     int nprob = 0, nwaves = 0, rt = 2, tn = 2;
-    long long nelem = 0;
+    int kw = 0;                   // > 1: split-K launch, waves per 16 x 16 tile
     bool f32 = false;  // FP32 matrix cores (mixed precision)
 };
 // Checks every problem (extents inside their allocations, alignment), allocates the partials of
