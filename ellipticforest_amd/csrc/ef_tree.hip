@@ -29,6 +29,11 @@ namespace efk {
 
 typedef double td4 __attribute__((ext_vector_type(4)));
 typedef double td2 __attribute__((ext_vector_type(2)));
+// loads through the global address space: the descriptors' pointers are generic, and a generic
+// (FLAT) load also counts against the LDS counter, so the compiler waits for every load in
+// flight before each use -- which defeats the GEMM's software pipeline
+__device__ inline td2 gld2(const double* p) { return *(const EF_GLOBAL td2*)gmem(p); }
+__device__ inline double gld(const double* p) { return *gmem(p); }
 
 typedef float tf4 __attribute__((ext_vector_type(4)));
 
@@ -67,7 +72,8 @@ __device__ inline double readlane_f64(double x, int src) {
 // one row segment of 16 consecutive columns per lane group and step.
 // block-gather mode of segment 0: row offsets of each block's segment (hDiff: the two children's
 // segments, subtracted; g: the side's segment alone)
-// (MODE 1 subtracts the d1 segment, MODE 2 reads the d2 segment alone)
+// (MODE 1 subtracts the d1 segment, MODE 2 reads the d2 segment alone; MODE 0: B contiguous
+// along k, MODE 3: B strided along k)
 struct DiffRows {
     int dn, kbase;  // block size; absolute row of the segment's first k
     long long d2[4], d1[4];
@@ -77,7 +83,7 @@ template <int RT, int TN, bool F32, int MODE>
 __device__ inline void gemm_segment(const double* __restrict__ a0p, const double* __restrict__ a1p,
                                     const double* const (&bp)[TN], long long ldb, int K, double alpha,
                                     GemmAcc<F32> (&acc)[2][TN], const DiffRows& dr) {
-    constexpr bool DIFF = MODE != 0;
+    constexpr bool DIFF = MODE == 1 || MODE == 2;
     const int q = (int)(threadIdx.x & 63) >> 4;
     // hDiff rows: absolute row ka lies in interface block r = ka / dn; both children's offsets
     auto drow = [&](int ka, long long& o2, long long& o1) {
@@ -92,11 +98,11 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
     };
     auto load = [&](int kb, Blk& o) {
         const int k = kb + 4 * q;
-        o.x0 = *(const td2*)(a0p + k);
-        o.x1 = *(const td2*)(a0p + k + 2);
+        o.x0 = gld2(a0p + k);
+        o.x1 = gld2(a0p + k + 2);
         if (RT == 2) {
-            o.y0 = *(const td2*)(a1p + k);
-            o.y1 = *(const td2*)(a1p + k + 2);
+            o.y0 = gld2(a1p + k);
+            o.y1 = gld2(a1p + k + 2);
         }
         if (DIFF) {  // four consecutive rows lie in one block (dn % 4 == 0)
             long long o2, o1;
@@ -104,29 +110,29 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
             if (MODE == 1) {
 #pragma unroll
                 for (int j = 0; j < TN; j++) {
-                    const td2 u0 = *(const td2*)(bp[j] + o2), u1 = *(const td2*)(bp[j] + o2 + 2);
-                    const td2 v0 = *(const td2*)(bp[j] + o1), v1 = *(const td2*)(bp[j] + o1 + 2);
+                    const td2 u0 = gld2(bp[j] + o2), u1 = gld2(bp[j] + o2 + 2);
+                    const td2 v0 = gld2(bp[j] + o1), v1 = gld2(bp[j] + o1 + 2);
                     o.b[j][0] = u0[0] - v0[0]; o.b[j][1] = u0[1] - v0[1];
                     o.b[j][2] = u1[0] - v1[0]; o.b[j][3] = u1[1] - v1[1];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < TN; j++) {
-                    const td2 u0 = *(const td2*)(bp[j] + o2), u1 = *(const td2*)(bp[j] + o2 + 2);
+                    const td2 u0 = gld2(bp[j] + o2), u1 = gld2(bp[j] + o2 + 2);
                     o.b[j][0] = u0[0]; o.b[j][1] = u0[1]; o.b[j][2] = u1[0]; o.b[j][3] = u1[1];
                 }
             }
-        } else if (ldb == 1) {  // B contiguous along k: two 16-byte vectors per column
+        } else if (MODE == 0) {  // B contiguous along k: two 16-byte vectors per column
 #pragma unroll
             for (int j = 0; j < TN; j++) {
-                const td2 u0 = *(const td2*)(bp[j] + k), u1 = *(const td2*)(bp[j] + k + 2);
+                const td2 u0 = gld2(bp[j] + k), u1 = gld2(bp[j] + k + 2);
                 o.b[j][0] = u0[0]; o.b[j][1] = u0[1]; o.b[j][2] = u1[0]; o.b[j][3] = u1[1];
             }
         } else {
 #pragma unroll
             for (int j = 0; j < TN; j++)
 #pragma unroll
-                for (int s = 0; s < 4; s++) o.b[j][s] = bp[j][(long long)(k + s) * ldb];
+                for (int s = 0; s < 4; s++) o.b[j][s] = gld(bp[j] + (long long)(k + s) * ldb);
         }
     };
     auto mma = [&](const Blk& o) {
@@ -176,20 +182,20 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         for (int s = 0; s < 4; s++) {
             const bool ok = k + s < K;
             const int kk = ok ? k + s : K - 1;
-            const double x = a0p[kk];
+            const double x = gld(a0p + kk);
             a0[s] = ok ? alpha * x : 0.0;
             if (RT == 2) {
-                const double y = a1p[kk];
+                const double y = gld(a1p + kk);
                 a1[s] = ok ? alpha * y : 0.0;
             }
             if (DIFF) {
                 long long o2, o1;
                 drow(dr.kbase + kk, o2, o1);
 #pragma unroll
-                for (int j = 0; j < TN; j++) b[j][s] = MODE == 1 ? bp[j][o2] - bp[j][o1] : bp[j][o2];
+                for (int j = 0; j < TN; j++) b[j][s] = MODE == 1 ? gld(bp[j] + o2) - gld(bp[j] + o1) : gld(bp[j] + o2);
             } else {
 #pragma unroll
-                for (int j = 0; j < TN; j++) b[j][s] = bp[j][(long long)kk * ldb];
+                for (int j = 0; j < TN; j++) b[j][s] = gld(bp[j] + (long long)kk * ldb);
             }
         }
 #pragma unroll
@@ -203,7 +209,8 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
 }
 
 template <int RT, int TN, bool F32>
-__global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restrict__ probs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8)))
+tree_gemm_kernel(const TreeGemm* __restrict__ probs,
                                                          const int* __restrict__ start,
                                                          const int* __restrict__ wprob, int nwaves) {
     const int w = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -242,7 +249,7 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
                 for (int v = 0; v < 4; v++) {
                     const int row = min(row0 + 16 * i + (F32 ? 4 * q + v : 4 * v + q), p.M - 1);
                     const int col = min(col0 + 16 * j + r, p.N - 1);
-                    cv[i][j][v] = p.C[(long long)row * p.ldc + (long long)col * p.ccs];
+                    cv[i][j][v] = gld(p.C + (long long)row * p.ldc + (long long)col * p.ccs);
                 }
     }
     DiffRows dr;
@@ -262,17 +269,20 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
         } else {
 #pragma unroll
             for (int j = 0; j < TN; j++) bp[j] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj[j] * p.bcs;
-            gemm_segment<RT, TN, F32, 0>(p.A0 + (long long)rows[0] * p.lda0 + kb0,
-                                         p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0, bp, p.ldb0, ke0 - kb0,
-                                         p.alpha0, acc, dr);
+            const double* a0 = p.A0 + (long long)rows[0] * p.lda0 + kb0;
+            const double* a1 = p.A0 + (long long)rows[RT - 1] * p.lda0 + kb0;
+            if (p.ldb0 == 1) gemm_segment<RT, TN, F32, 0>(a0, a1, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
+            else gemm_segment<RT, TN, F32, 3>(a0, a1, bp, p.ldb0, ke0 - kb0, p.alpha0, acc, dr);
         }
     }
     if (ke1 > kb1) {
         const double* bp[TN];
 #pragma unroll
         for (int j = 0; j < TN; j++) bp[j] = p.B1 + (long long)kb1 * p.ldb1 + (long long)cj[j] * p.bcs;
-        gemm_segment<RT, TN, F32, 0>(p.A1 + (long long)rows[0] * p.lda1 + kb1, p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1, bp,
-                                     p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
+        const double* a0 = p.A1 + (long long)rows[0] * p.lda1 + kb1;
+        const double* a1 = p.A1 + (long long)rows[RT - 1] * p.lda1 + kb1;
+        if (p.ldb1 == 1) gemm_segment<RT, TN, F32, 0>(a0, a1, bp, 1, ke1 - kb1, p.alpha1, acc, dr);
+        else gemm_segment<RT, TN, F32, 3>(a0, a1, bp, p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
     }
     // with RT = 1 both accumulator rows hold the same 16 rows; only the first is stored
 #pragma unroll
@@ -285,7 +295,7 @@ __global__ void __launch_bounds__(256) tree_gemm_kernel(const TreeGemm* __restri
                 if (row < p.M && col < p.N) {
                     double o = (double)acc[i][j][v];
                     if (has_c) o += p.beta * cv[i][j][v];
-                    p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
+                    *gmem(p.D + (long long)row * p.ldd + (long long)col * p.dcs) = o;
                 }
             }
 }
@@ -327,7 +337,7 @@ __global__ void __launch_bounds__(1024) tree_gemm_split_kernel(const TreeGemm* _
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = min(row0 + 4 * v + q, p.M - 1);
-            cv[v] = p.C[(long long)row * p.ldc + (long long)cj * p.ccs];
+            cv[v] = gld(p.C + (long long)row * p.ldc + (long long)cj * p.ccs);
         }
     }
     if (kbeg < K) {  // uniform per wave
@@ -347,13 +357,15 @@ __global__ void __launch_bounds__(1024) tree_gemm_split_kernel(const TreeGemm* _
                 else gemm_segment<1, 1, false, 2>(a0, a0, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
             } else {
                 bp[0] = p.B0 + (long long)kb0 * p.ldb0 + (long long)cj * p.bcs;
-                gemm_segment<1, 1, false, 0>(a0, a0, bp, p.ldb0, ke0 - kb0, p.alpha0, acc, dr);
+                if (p.ldb0 == 1) gemm_segment<1, 1, false, 0>(a0, a0, bp, 1, ke0 - kb0, p.alpha0, acc, dr);
+                else gemm_segment<1, 1, false, 3>(a0, a0, bp, p.ldb0, ke0 - kb0, p.alpha0, acc, dr);
             }
         }
         if (ke1 > kb1) {
             const double* bp[1] = {p.B1 + (long long)kb1 * p.ldb1 + (long long)cj * p.bcs};
             const double* a1 = p.A1 + (long long)rowc * p.lda1 + kb1;
-            gemm_segment<1, 1, false, 0>(a1, a1, bp, p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
+            if (p.ldb1 == 1) gemm_segment<1, 1, false, 0>(a1, a1, bp, 1, ke1 - kb1, p.alpha1, acc, dr);
+            else gemm_segment<1, 1, false, 3>(a1, a1, bp, p.ldb1, ke1 - kb1, p.alpha1, acc, dr);
         }
     }
     if (wave > 0) {
@@ -370,7 +382,7 @@ __global__ void __launch_bounds__(1024) tree_gemm_split_kernel(const TreeGemm* _
             o += acc[0][0][v];
             for (int c = 1; c < p.nks; c++) o += part[c][v][lane];
             if (p.C) o += p.beta * cv[v];
-            p.D[(long long)row * p.ldd + (long long)col * p.dcs] = o;
+            *gmem(p.D + (long long)row * p.ldd + (long long)col * p.dcs) = o;
         }
     }
 }
