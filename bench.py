@@ -397,17 +397,79 @@ def gpu_particular_run(bounds, N, k, steps, warmup):
     return dict(ms=ms, value=P * k / (ms / 1e3), bytes=nbytes, gbs=nbytes / (ms / 1e3) / 1e9)
 
 
-def var_beta_cells(bounds, N):
-    """alpha, beta at the four faces and lambda per cell (FiniteVolumeSolver.cpp:64-78) for the
-    variable-beta coefficients alpha = 1, beta = 1 + 0.5 sin(2 pi x) cos(2 pi y), lambda = -1."""
-    beta = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+def var_beta_cells(bounds, N, beta=None, lam=-1.0):
+    """alpha, beta at the four faces and lambda per cell (FiniteVolumeSolver.cpp:64-78) for
+    alpha = 1 and the given beta (default 1 + 0.5 sin(2 pi x) cos(2 pi y)) and constant lambda."""
+    if beta is None:
+        beta = lambda x, y: 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
     out = np.empty((len(bounds), 6, N * N))
     for p, (xl, xu, yl, yu) in enumerate(bounds):
         dx, dy = (xu - xl) / N, (yu - yl) / N
         X, Y = np.meshgrid(xl + (np.arange(N) + 0.5) * dx, yl + (np.arange(N) + 0.5) * dy, indexing="ij")
         X, Y = X.ravel(), Y.ravel()
         out[p] = [np.ones_like(X), beta(X - dx / 2, Y), beta(X + dx / 2, Y), beta(X, Y - dy / 2),
-                  beta(X, Y + dy / 2), -np.ones_like(X)]
+                  beta(X, Y + dy / 2), np.full_like(X, lam)]
+    return out
+
+
+def multi_mesh_run(nmeshes, nreq, max_batch=32, tick=64, seed=1, budget_frac=0.6):
+    """Multi-mesh serving with admission by HBM footprint: ``nmeshes`` meshes drawn with seed 1
+    -- max level ~ U{4..8}, N ~ {8, 16, 32}, alpha = 1, beta = 1 + 0.5 sin(a x + b y) with a, b ~
+    U[0, 4], lambda ~ U[-2, 0], on [-10,10]^2 refined where |sin x + sin y| > 1.2, five-point
+    leaves, FP64 -- registered with one server whose budget holds ``budget_frac`` of their
+    footprints.  ``nreq`` cfg4-style load requests (Gaussians + Fourier Dirichlet modes) arrive
+    in ticks of ``tick`` requests for one mesh each, the mesh drawn with probability 1/(i + 1)
+    (normalised); a request for a mesh that is not resident builds it (evicting least recently
+    used idle meshes).  Latency: arrival to result in HBM, builds included."""
+    import ellipticforest_amd as ef
+    from ellipticforest_amd import serving, _lib
+    ctx = ef.Context(0)
+    rng = np.random.default_rng(seed)
+    srv = serving.Server(ctx, max_batch=max_batch, buckets=(1, 2, 4, 8, 16, 32))
+    specs = []
+    for i in range(nmeshes):
+        maxl, N = int(rng.integers(4, 9)), int(rng.choice([8, 16, 32]))
+        a, b, lam = float(rng.uniform(0, 4)), float(rng.uniform(0, 4)), float(rng.uniform(-2, 0))
+        leaves = ef.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, N, N, BOUNDS, 0, maxl)
+        fp = _lib.DeviceMesh.footprint(leaves, N, BOUNDS, max_rhs=max_batch, variable=True) + \
+            srv.slot_bytes(len(leaves), N, 0)
+        probe = ef.DeviceMesh(ctx, leaves, N, BOUNDS)
+        lb = np.array([tuple(probe.node_bounds[n]) for n in probe.leaf_nodes])
+        del probe
+        beta = (lambda aa, bb: (lambda x, y: 1.0 + 0.5 * np.sin(aa * x + bb * y)))(a, b)
+        cell = var_beta_cells(lb, N, beta, lam)
+        srv.register_mesh(f"m{i}", leaves, N, BOUNDS, solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+        specs.append(dict(key=f"m{i}", max_level=maxl, N=N, leaves=len(leaves), a=a, b=b, lam=lam,
+                          footprint_gb=fp / 1e9))
+        log(f"multi-mesh: registered m{i}: max level {maxl}, N {N}, {len(leaves)} leaves, {fp / 1e9:.2f} GB")
+    fps = [sp["footprint_gb"] * 1e9 for sp in specs]
+    srv.budget = int(max(budget_frac * sum(fps), 1.05 * max(fps)))
+    P = serving.workload_params(rng, BOUNDS, nreq)
+    pop = 1.0 / np.arange(1, nmeshes + 1)
+    ticks = rng.choice(nmeshes, size=(nreq + tick - 1) // tick, p=pop / pop.sum())
+    t0 = time.perf_counter()
+    for t, s0 in enumerate(range(0, nreq, tick)):
+        for i in range(s0, min(nreq, s0 + tick)):
+            srv.submit_load(specs[ticks[t]]["key"], P[i], want_u=False)
+        srv.run()
+        if t % 8 == 7:
+            log(f"multi-mesh: tick {t + 1}/{len(ticks)}, {len(srv.admissions)} builds, {srv.evictions} evictions, "
+                f"{time.perf_counter() - t0:.1f} s")
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    bt = np.array([t for _, t in srv.admissions])
+    out = dict(workload="multi-mesh serving: %d meshes (seed %d: max level U{4..8}, N in {8,16,32}, beta = 1 + 0.5 "
+               "sin(a x + b y), a, b ~ U[0,4], lambda ~ U[-2,0], alpha = 1, five-point, FP64) on [-10,10]^2 "
+               "refined where |sin x + sin y| > 1.2; %d load requests in ticks of %d for one mesh each, drawn with "
+               "probability proportional to 1/(i+1); "
+               "HBM budget %.1f GB = %.0f%% of the meshes' footprints" % (nmeshes, seed, nreq, tick,
+                                                                         srv.budget / 1e9, 100 * budget_frac),
+               meshes=specs, solves_per_s=nreq / wall, p50_ms=st["p50_ms"], p99_ms=st["p99_ms"],
+               admissions=len(bt), evictions=srv.evictions, build_s_mean=float(bt.mean()),
+               build_s_max=float(bt.max()), build_s_total=float(bt.sum()),
+               note="precision FP64: mixed-precision builds refine on uniform meshes only")
+    log(f"multi-mesh: {out['solves_per_s']:.0f} solves/s, p50 {out['p50_ms']:.1f} ms p99 {out['p99_ms']:.1f} ms, "
+        f"{len(bt)} builds ({bt.sum():.2f} s), {srv.evictions} evictions")
     return out
 
 
@@ -644,6 +706,8 @@ def main():
                          "kernel (the calibration of HBM byte counters), then exit")
     ap.add_argument("--serving-requests", type=int, default=10000, help="requests of the serving run (0: skip)")
     ap.add_argument("--no-variable-path", action="store_true", help="skip the variable-coefficient whole path")
+    ap.add_argument("--multi-mesh-requests", type=int, default=1536, help="requests of the multi-mesh run (0: skip)")
+    ap.add_argument("--multi-mesh-count", type=int, default=6)
     ap.add_argument("--serving-batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true", help="skip the oracle timings (profiling runs)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -703,7 +767,7 @@ def main():
         out["roofline"] = dict(
             bound="hbm", achieved=g["gemm_gbs"], peak=8000.0, unit="GB/s", frac=g["gemm_gbs"] / 8000.0,
             traffic=TRAFFIC_GEMM_PER_LAUNCH, kernel="tree_gemm_kernel (the step's tree GEMMs: w = X^-1 hDiff, "
-            "h += H w, u = S g + w; %d launches per step incl. split-K reduces)" % g["gemm_launches"],
+            "h += H w, u = S g + w; %d launches per step, split-K chunks summed inside the workgroup)" % g["gemm_launches"],
             algorithmic_bytes_per_launch=per_launch, gemm_ms_per_step=g["gemm_ms"],
             note="achieved = algorithmic bytes of the step's GEMM launches / their HIP-event time (GEMMs alone, "
                  "ef_mesh_time_sweep_gemms); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE / "
@@ -729,6 +793,12 @@ def main():
         except Exception as e:
             out["serving_error"] = f"{type(e).__name__}: {e}"
             log(out["serving_error"])
+    if args.multi_mesh_requests > 0 and "error" not in out:
+        try:
+            out["multi_mesh"] = multi_mesh_run(args.multi_mesh_count, args.multi_mesh_requests)
+        except Exception as e:
+            out["multi_mesh_error"] = f"{type(e).__name__}: {e}"
+            log(out["multi_mesh_error"])
     if not args.no_variable_path and "error" not in out:
         try:
             out["variable_path"] = variable_path_run(max(args.steps // 4, 3))

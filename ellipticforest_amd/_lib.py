@@ -24,6 +24,7 @@ EF_SOLVER_FIVE_POINT = 0
 EF_SOLVER_FISHPACK = 1
 EF_PRECISION_FP64 = 0
 EF_PRECISION_MIXED = 1
+EF_FOOTPRINT_VARIABLE = 0x100
 
 # Every symbol include/ef_hps.h declares.
 EXPORTS = [
@@ -355,13 +356,14 @@ class DeviceMesh:
         return out if which in ("T", "S", "X", "H") else out.ravel()
 
     @staticmethod
-    def footprint(leaves, N, bounds, precision=EF_PRECISION_FP64, max_rhs=1):
+    def footprint(leaves, N, bounds, precision=EF_PRECISION_FP64, max_rhs=1, variable=False):
         lv = np.array([l[0] for l in leaves], np.int32)
         ix = np.array([l[1] for l in leaves], np.int32)
         iy = np.array([l[2] for l in leaves], np.int32)
         d = MeshDesc(len(leaves), iptr(lv), iptr(ix), iptr(iy), N, *[float(b) for b in bounds])
         out = ctypes.c_size_t()
-        check(lib().ef_mesh_footprint(ctypes.byref(d), precision, max_rhs, ctypes.byref(out)))
+        flags = precision | (EF_FOOTPRINT_VARIABLE if variable else 0)
+        check(lib().ef_mesh_footprint(ctypes.byref(d), flags, max_rhs, ctypes.byref(out)))
         return int(out.value)
 
 

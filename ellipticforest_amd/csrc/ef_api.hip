@@ -1184,7 +1184,15 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
     SweepPlan& sp = sweep_plan(M, k);
     if (sp.store.pending()) throw EfError(EF_ERR_INTERNAL, "sweep descriptors staged but not uploaded");
     check_range(f_dev, (size_t)k * P.leaves.size() * P.N * P.N * 8, "upwards load f");
-    if (M->variable && M->packed) M->vz.ensure((size_t)P.leaves.size() * k * P.N * P.N);
+    if (M->variable && M->packed) {
+        // the packed leaves' solve work grows with k; sweeps captured against the old buffer
+        // (at a smaller k, while the vector arena was already large enough) must not replay
+        const size_t need = (size_t)P.leaves.size() * k * P.N * P.N;
+        if (M->vz.n < need) {
+            clear_graphs(M);
+            M->vz.ensure(need);
+        }
+    }
     run_sweep(M, 0, k, {f_dev, nullptr, nullptr, nullptr}, M->homogeneous_rhs ? 1 : 0, [&] {
         // leaf particular Neumann data (particularNeumannData, FiniteVolumeSolver.cpp:457-462)
         launch_leaf(M, 0, f_dev, 0, nullptr, k);
@@ -1436,13 +1444,15 @@ int ef_mesh_footprint(const ef_mesh_desc* desc, int precision, int max_rhs, size
     if (!desc || !bytes) throw EfError(EF_ERR_ARG, "null argument");
     Plan P;
     plan_tree(P, *desc);
-    const long long N = P.N;
+    const long long N = P.N, nl = (long long)P.leaves.size(), k = std::max(max_rhs, 1);
+    const bool variable = (precision & EF_FOOTPRINT_VARIABLE) != 0;
     long long elems = P.T_elems + P.S_elems + P.Xi_elems + P.work_X + P.gj_d;
     elems += (P.gj_i + P.ninternal + 1) / 2;                     // int work
-    elems += (long long)P.leaves.size() * 16 * N * N;           // leaf DtN upper bound (one per leaf)
-    elems += P.vec_rows * std::max(max_rhs, 1);
-    elems += 2LL * std::max(max_rhs, 1) * (long long)P.leaves.size() * N * N;  // f and u staging
-    (void)precision;
+    elems += nl * 16 * N * N;                                   // leaf DtN upper bound (one per leaf)
+    elems += P.vec_rows * k;
+    elems += 2LL * k * nl * N * N;                              // f and u staging
+    if (variable)  // samples (6 N^2), stencil (5 N^2), pivot inverses (N^3), packed halves, solve work
+        elems += nl * (11 * N * N + N * N * N + N * bt_packed_entries((int)N)) + nl * k * N * N;
     *bytes = (size_t)elems * sizeof(double);
     EF_API_END
 }

@@ -2,7 +2,9 @@
 
 * Meshes are built once and stay resident; a new mesh is admitted when its device
   footprint (``ef_mesh_footprint`` plus the serving slots) fits the HBM budget, evicting the
-  least recently used idle meshes if needed.
+  least recently used idle meshes if needed.  A mesh registered with ``register_mesh`` is
+  admitted (built) when a request names it while it is not resident; if the meshes holding
+  the budget are busy, the loop first drains the batches in flight.
 * A request names a resident mesh and carries either the 64 parameters of a serving-workload
   load (``submit_load``: 8 Gaussians for f, 8 Fourier modes for the Dirichlet data, evaluated
   on the device by ``ef_mesh_eval_load``) or explicit host arrays f ([nleaves][N*N]) and r
@@ -154,6 +156,9 @@ class Server:
         if self.buckets and self.buckets[-1] != max_batch:
             self.buckets.append(max_batch)
         self.meshes = collections.OrderedDict()
+        self.recipes = {}
+        self.admissions = []  # (key, build seconds) of every mesh built
+        self.evictions = 0
         self.pending = collections.deque()
         self.lock = threading.Condition()
         self.completed = []
@@ -171,7 +176,8 @@ class Server:
 
     def add_mesh(self, key, leaves, N, bounds, **build_args):
         """Build and keep a factorised mesh; evict idle meshes (LRU) to fit the budget."""
-        probe = _lib.DeviceMesh.footprint(leaves, N, bounds, max_rhs=self.max_batch)
+        probe = _lib.DeviceMesh.footprint(leaves, N, bounds, max_rhs=self.max_batch,
+                                          variable=build_args.get("cell") is not None)
         fp = probe + self.slot_bytes(len(leaves), N, 0)
         if fp > self.budget:
             raise MemoryError(f"mesh {key} needs {fp} bytes, budget is {self.budget}")
@@ -182,8 +188,12 @@ class Server:
                 if victim is None:
                     raise MemoryError("no idle mesh to evict")
                 del self.meshes[victim]
+                self.evictions += 1
+        t0 = time.perf_counter()
         dm = _lib.DeviceMesh(self.ctx, leaves, N, bounds)
         dm.build(**build_args)
+        dm.sync()
+        self.admissions.append((key, time.perf_counter() - t0))
         # plan (and time the GEMM tiles of) every batch width before requests arrive: the widest
         # first, so the vector arena is sized once
         for w in sorted(self.buckets or range(1, self.max_batch + 1), reverse=True):
@@ -192,11 +202,27 @@ class Server:
             self.meshes[key] = _Resident(self.ctx, dm, fp, self.max_batch, self.nslots, self.fetch_rows)
         return dm
 
+    def register_mesh(self, key, leaves, N, bounds, **build_args):
+        """Keep how to build a mesh; it is admitted when a request names it and it is not resident."""
+        self.recipes[key] = (leaves, N, bounds, build_args)
+
+    def _admit(self, key):
+        """Build a registered mesh for a request that names it (the submitting thread drives the
+        loop meanwhile: call it from the thread that runs ``run``)."""
+        leaves, N, bounds, build_args = self.recipes[key]
+        try:
+            self.add_mesh(key, leaves, N, bounds, **build_args)
+        except MemoryError:
+            self.run()  # the meshes holding the budget were busy: finish their batches, then evict
+            self.add_mesh(key, leaves, N, bounds, **build_args)
+
     def _inflight_keys(self):
         return [(k, s) for k, s, _ in self.inflight]
 
     # ----- requests -------------------------------------------------------------------
     def _enqueue(self, req):
+        if req.mesh not in self.meshes and req.mesh in self.recipes:
+            self._admit(req.mesh)
         with self.lock:
             if req.mesh not in self.meshes:
                 raise KeyError(f"mesh {req.mesh} is not resident")

@@ -47,6 +47,9 @@ extern "C" {
 /* Arithmetic of the operators. */
 #define EF_PRECISION_FP64 0
 #define EF_PRECISION_MIXED 1     /* FP32 merge products, FP64 interface inverse, refined solves */
+/* ef_mesh_footprint only: or'ed into precision, counts the per-leaf storage of variable
+ * coefficients (per-cell alpha, beta, lambda: samples, stencil, block-LU pivots, leaf DtN) */
+#define EF_FOOTPRINT_VARIABLE 0x100
 
 typedef struct ef_ctx_s* ef_ctx;
 typedef struct ef_mesh_s* ef_mesh;
@@ -82,7 +85,8 @@ int ef_refine_by_function(ef_refine_fn fn, void* user, int nx, int ny, double xl
 
 int ef_mesh_create(ef_ctx ctx, const ef_mesh_desc* desc, ef_mesh* out);
 int ef_mesh_destroy(ef_mesh mesh);
-/* Device bytes the mesh needs for operators plus vectors of max_rhs right-hand sides. */
+/* Device bytes the mesh needs for operators plus vectors of max_rhs right-hand sides
+ * (precision | EF_FOOTPRINT_VARIABLE for variable-coefficient leaves). */
 int ef_mesh_footprint(const ef_mesh_desc* desc, int precision, int max_rhs, size_t* bytes);
 /* info[0..7] = nnodes, nleaves, root side M, max depth, N, internal nodes, max interface n, 0 */
 int ef_mesh_info(ef_mesh mesh, int* info);

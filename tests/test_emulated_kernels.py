@@ -116,3 +116,13 @@ def test_emu_multi_rhs_wide_batch(emu_ctx):
 def test_emu_sweep_tiles(emu_ctx):
     """Every sweep tile shape (forced and timed) gives the fixed rule's bits."""
     gp.tile_case(emu_ctx, 17, 1, 8)
+
+
+def test_emu_variable_adaptive(emu_ctx):
+    """Variable beta with coarsened children (five-point leaves of several sizes)."""
+    gp.test_variable_coefficients_adaptive(emu_ctx, maxl=2, N=8)
+
+
+def test_emu_variable_graph_widths(emu_ctx):
+    """A captured variable-coefficient sweep is not replayed after its work buffer grew."""
+    gp.variable_graph_widths_case(emu_ctx)

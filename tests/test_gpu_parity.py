@@ -283,6 +283,16 @@ def test_variable_coefficients_uniform(ctx):
     assert rel(ug, uo) < TOL_U
 
 
+def test_variable_coefficients_adaptive(ctx, maxl=4, N=8, check_ops=True):
+    """Variable beta on an adaptive mesh (coarsened children, leaves of several sizes), every
+    operator and the solution against the oracle."""
+    say(f"variable beta, lambda=-1, adaptive levels 0..{maxl}, N={N}, [-1,1]^2")
+    leaves = omesh.refine_by_function(lambda x, y: (x < -0.4) & (y < 0.1), N, N, -1, 1, -1, 1, 0, maxl)
+    assert len({lv for lv, _, _ in leaves}) > 1
+    uo, ug, err = variable_both(ctx, leaves, N, check_ops=check_ops)
+    assert rel(ug, uo) < TOL_U
+
+
 @pytest.mark.parametrize("name", ["global_5pt_varbeta_L2N8", "global_5pt_varbeta_L3N16"])
 def test_variable_equals_global_solve(ctx, name):
     """The variable-beta operator on a uniform mesh against the independent global 5-point
@@ -355,6 +365,58 @@ def graph_replay_case(ctx, leaves, N, bounds, k=3):
     D.sync()
     assert np.array_equal(sweep(0), eager[0])
     return eager
+
+
+def variable_graph_widths_case(ctx, level=1, N=8):
+    """Captured sweeps of a variable-coefficient mesh whose vector arena was sized up front
+    (as the serving loop plans every bucket width): widths 2, 4, then 2 again, each against the
+    same calls launched kernel by kernel.  The solve work buffer of the packed five-point leaves
+    grows with the width; a sweep captured before it grew must not be replayed."""
+    from ellipticforest_amd import _lib
+    say(f"variable-coefficient graph replay across widths, uniform level {level} N={N}")
+    leaves = omesh.uniform(level)
+    bounds = (-1.0, 1.0, -1.0, 1.0)
+    D = _lib.DeviceMesh(ctx, leaves, N, bounds)
+    cell = np.concatenate([oracle.sample_coeffs(N, tuple(D.node_bounds[n]), ALPHA, BETA, LAMBDA).ravel()
+                           for n in D.leaf_nodes])
+    D.build(solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    D.sweep_tiles(8)  # sizes the vector arena for 8 right-hand sides
+    nl, m = len(leaves), 4 * D.M
+    rng = np.random.default_rng(5)
+    a = np.ones(m)
+    ad = _lib.DeviceArray(ctx, a.shape)
+    ad.upload(a)
+
+    bufs = {}  # one set of device buffers per width, reused (as a serving slot reuses its own)
+
+    def sweep(k, seed):
+        r2 = np.random.default_rng(seed)
+        f, r = r2.standard_normal((k, nl, N * N)), r2.standard_normal((k, m))
+        if k not in bufs:
+            bufs[k] = (_lib.DeviceArray(ctx, f.shape), _lib.DeviceArray(ctx, r.shape), _lib.DeviceArray(ctx, f.shape))
+        fd, rd, ud = bufs[k]
+        fd.upload(f)
+        rd.upload(r)
+        D.upwards_device(fd.ptr, k)
+        D.solve_device(fd.ptr, rd.ptr, ad.ptr, None, ud.ptr, k)
+        D.sync()
+        return ud.download().reshape(f.shape)
+
+    os.environ["EF_GRAPH"] = "0"
+    try:
+        e2, e4 = sweep(2, 1), sweep(4, 2)
+    finally:
+        del os.environ["EF_GRAPH"]
+    del D  # a fresh mesh: its first width-2 sweep is captured while the work buffer fits 2
+    D = _lib.DeviceMesh(ctx, leaves, N, bounds)
+    D.build(solver_type=_lib.EF_SOLVER_FIVE_POINT, cell=cell)
+    D.sweep_tiles(8)
+    for k, seed, ref in ((2, 1, e2), (4, 2, e4), (2, 1, e2), (4, 2, e4)):
+        assert np.array_equal(sweep(k, seed), ref), k
+
+
+def test_variable_graph_widths(ctx):
+    variable_graph_widths_case(ctx)
 
 
 def test_graph_replay_matches_eager(ctx):

@@ -86,6 +86,43 @@ def test_admission_by_footprint(emu_ctx):
         srv.add_mesh("c", leaves, N, bounds)
 
 
+def test_admission_on_demand(emu_ctx):
+    """Registered meshes (different N, variable coefficients) are built when a request names
+    them; with room for two, a third evicts the least recently used idle one, and a request for
+    an evicted mesh builds it again.  Results match a fresh single-mesh server bitwise."""
+    import ellipticforest_amd as ef
+    bounds = (-1.0, 1.0, -1.0, 1.0)
+    specs = {"a": (omesh.uniform(1), 8), "b": (omesh.uniform(1), 4), "c": (omesh.uniform(1), 8)}
+
+    def cells(leaves, N, lam):
+        dm = ef.DeviceMesh(emu_ctx, leaves, N, bounds)
+        beta = lambda x, y: 1.0 + 0.5 * np.sin(x + 2 * y)
+        return np.concatenate([oracle.sample_coeffs(N, tuple(dm.node_bounds[n]), lambda x, y: np.ones_like(x), beta,
+                                                    lambda x, y: lam + 0 * x).ravel() for n in dm.leaf_nodes])
+
+    srv = serving.Server(emu_ctx, max_batch=2, buckets=None)
+    args = {}
+    for i, (key, (leaves, N)) in enumerate(specs.items()):
+        args[key] = dict(solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=cells(leaves, N, -0.5 * i))
+        srv.register_mesh(key, leaves, N, bounds, **args[key])
+    fp = {k: ef._lib.DeviceMesh.footprint(l, N, bounds, max_rhs=2, variable=True) + srv.slot_bytes(len(l), N, 0)
+          for k, (l, N) in specs.items()}
+    srv.budget = fp["a"] + fp["c"] + fp["b"] // 2  # two meshes fit, not three
+    P = serving.workload_params(np.random.default_rng(4), bounds, 4)
+    reqs = []
+    for key, p in zip(("a", "b", "c", "a"), P):
+        reqs.append(srv.submit_load(key, p))
+        srv.run()
+    assert [k for k, _ in srv.admissions] == ["a", "b", "c", "a"] and srv.evictions >= 2
+    assert len(srv.meshes) <= 2
+    for key, p, q in zip(("a", "b", "c", "a"), P, reqs):
+        one = serving.Server(emu_ctx, max_batch=2, buckets=None)
+        one.add_mesh(key, specs[key][0], specs[key][1], bounds, **args[key])
+        alone = one.submit_load(key, p)
+        one.run()
+        assert np.array_equal(alone.u, q.u)
+
+
 def test_poisson_driver(emu_ctx):
     leaves, N, bounds = omesh.uniform(1), 8, (-1.0, 1.0, -1.0, 1.0)
     srv = serving.Server(emu_ctx, max_batch=4, buckets=None)
