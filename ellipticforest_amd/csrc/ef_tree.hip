@@ -1030,10 +1030,13 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
     // none are cut into chunks of the same count (rounded to 16).  The chunks depend on the
     // launch's problems only, never on N.
     int kw = 1;
-    for (const TreeGemm& p : probs)
+    for (TreeGemm& p : probs)
         if (p.M > 0 && p.N > 0 && p.kc > 0) {
             if (p.kc % 16 != 0) throw EfError(EF_ERR_INTERNAL, "split-K chunk must be a multiple of 16");
-            kw = std::max(kw, (p.K0 + p.K1 + p.kc - 1) / p.kc);
+            const int K = p.K0 + p.K1;
+            // at most 16 chunks (waves per workgroup): longer ranges get longer chunks
+            if ((K + p.kc - 1) / p.kc > 16) p.kc = ((K + 15) / 16 + 15) / 16 * 16;
+            kw = std::max(kw, (K + p.kc - 1) / p.kc);
         }
     if (kw > 16) throw EfError(EF_ERR_INTERNAL, "split-K launch with more than 16 chunks per tile");
     if (kw > 1 && f32) throw EfError(EF_ERR_INTERNAL, "split-K launch on the FP32 matrix cores");
