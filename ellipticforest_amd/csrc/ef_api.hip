@@ -10,12 +10,14 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -378,6 +380,8 @@ struct ef_mesh_s {
     bool dominant = false;                 // variable, every leaf diagonally dominant: no pivoting
     bool packed = false;                   // variable and symmetric: packed inverse pivot blocks
     DevBuf<double> vcoef, vstencil, vSinv, vSinvP, vwork, vz;
+    double* coef_pin = nullptr;  // page-locked staging of the per-cell coefficients (doubles: coef_pin_n)
+    size_t coef_pin_n = 0;
     // operator arenas
     DevBuf<double> Tar, Sar, Xiar;
     // build work: X of a group, Gauss-Jordan work, zero-pivot flags of every merge
@@ -476,6 +480,25 @@ bool coefficients_vary(const Plan& P, const ef_coeffs& c) {
     return false;
 }
 
+// fn(lo, hi) over [0, n) on up to 16 host threads (the calling thread takes the first chunk)
+template <class F>
+void parallel_chunks(size_t n, F&& fn) {
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), std::max<size_t>(1, n / 16));
+    if (nt <= 1) {
+        fn(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t t = 1; t < nt; t++) {
+        const size_t lo = t * per, hi = std::min(n, lo + per);
+        if (lo < hi) th.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+    }
+    fn(size_t(0), std::min(n, per));
+    for (auto& x : th) x.join();
+}
+
 void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     Plan& P = M->P;
     const int N = P.N;
@@ -486,12 +509,32 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
     M->variable = coefficients_vary(P, c);
     if (M->variable) {
         if (N > BT_MAX_N) throw EfError(EF_ERR_ARG, "variable coefficients need N <= " + std::to_string(BT_MAX_N));
-        M->vcoef.ensure(nl * 6 * NN);
-        efk::copy_h2d(M->vcoef.p, c.cell, sizeof(double) * nl * 6 * NN, M->stream);
+        // the samples go up in one asynchronous copy from page-locked memory, filled by host
+        // threads; the symmetry and dominance scans run on the host meanwhile (the stream is
+        // synchronised before the buffer is refilled, so no earlier copy still reads it)
+        const size_t ncoef = nl * 6 * NN;
+        M->vcoef.ensure(ncoef);
+        if (M->coef_pin_n < ncoef) {
+            if (M->coef_pin) EF_HIP(hipHostFree(M->coef_pin));
+            M->coef_pin = nullptr;
+            M->coef_pin_n = 0;
+            EF_HIP(hipHostMalloc((void**)&M->coef_pin, ncoef * sizeof(double), hipHostMallocDefault));
+            M->coef_pin_n = ncoef;
+        }
+        EF_HIP(hipStreamSynchronize(M->stream));  // no copy of an earlier build still reads the buffer
+        parallel_chunks(nl, [&](size_t lo, size_t hi) {
+            std::memcpy(M->coef_pin + lo * 6 * NN, c.cell + lo * 6 * NN, (hi - lo) * 6 * NN * sizeof(double));
+        });
+        EF_HIP(hipMemcpyAsync(M->vcoef.p, M->coef_pin, ncoef * sizeof(double), hipMemcpyHostToDevice, M->stream));
         M->vstencil.ensure(nl * 5 * NN);
         M->vSinv.ensure(nl * (size_t)N * NN);
-        M->packed = (N == 8 || N == 16 || N == 32);
-        for (size_t li = 0; li < nl && M->packed; li++) M->packed = efk::cells_symmetric(c.cell + li * 6 * NN, N);
+        std::atomic<bool> sym(N == 8 || N == 16 || N == 32);
+        if (sym)
+            parallel_chunks(nl, [&](size_t lo, size_t hi) {
+                for (size_t li = lo; li < hi && sym.load(std::memory_order_relaxed); li++)
+                    if (!efk::cells_symmetric(c.cell + li * 6 * NN, N)) sym = false;
+            });
+        M->packed = sym;
         const long long NP = bt_packed_entries(N);
         if (M->packed) M->vSinvP.ensure(nl * (size_t)N * NP);
         M->leafT.ensure(nl * 16 * NN);
@@ -512,9 +555,12 @@ void setup_leaf_classes(ef_mesh_s* M, const ef_coeffs& c) {
             L.dx = (nd.xu - nd.xl) / N;
             L.dy = (nd.yu - nd.yl) / N;
         }
-        M->dominant = true;
-        for (size_t li = 0; li < nl && M->dominant; li++)
-            M->dominant = efk::cells_dominant(c.cell + li * 6 * NN, N, ld[li].dx, ld[li].dy);
+        std::atomic<bool> dom(true);
+        parallel_chunks(nl, [&](size_t lo, size_t hi) {
+            for (size_t li = lo; li < hi && dom.load(std::memory_order_relaxed); li++)
+                if (!efk::cells_dominant(c.cell + li * 6 * NN, N, ld[li].dx, ld[li].dy)) dom = false;
+        });
+        M->dominant = dom;
         M->leafdesc.upload(ld, M->stream);
         return;
     }
@@ -1434,6 +1480,7 @@ int ef_mesh_destroy(ef_mesh mesh) {
     for (auto& e : mesh->slot_ev)
         if (e) (void)hipEventDestroy(e);
     hipStream_t s = mesh->own_stream;
+    if (mesh->coef_pin) (void)hipHostFree(mesh->coef_pin);
     delete mesh;
     if (s) (void)hipStreamDestroy(s);
     EF_API_END
