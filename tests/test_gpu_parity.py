@@ -193,10 +193,10 @@ def test_sweep_tiles_bitwise(ctx):
     tile_case(ctx, 7, 4, 16)
 
 
-def test_homogeneous_rhs_option(ctx):
+def test_homogeneous_rhs_option(ctx, level=2):
     import ellipticforest_amd as ef
     say("homogeneous-rhs option: leaves solve with f = 0, w skipped")
-    leaves = omesh.uniform(2)
+    leaves = omesh.uniform(level)
     O = oracle.Forest(leaves, 8, (0, 1, 0, 1), solver=oracle.FISHPACK, homogeneous_rhs=True)
     O.build()
     f = O.sample_rhs(F)
