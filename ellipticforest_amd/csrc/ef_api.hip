@@ -250,8 +250,10 @@ struct LevelSweep {  // device descriptors of one depth of the sweeps, k right-h
     int up_cnt = 0, up_nmax = 0;  // merges whose hDiff and boundary h are gathered (a coarsened child)
     const UpDesc* up = nullptr;
     GemmLaunch gw, gh;
-    const SideCopy* copies = nullptr;  // children's g written out (coarsened children, or all)
+    const SideCopy* copies = nullptr;  // children's g written out (without side reads)
     int copy_cnt = 0;
+    const SideCopy* interps = nullptr;  // coarsened children's first uncoarsen step, from the sources
+    int interp_cnt = 0;
     GemmLaunch gu;
     std::vector<const VecStepDesc*> unc;
     std::vector<int> unc_cnt, unc_nf;
@@ -761,6 +763,7 @@ void check_tree_kernels() {
     check_kernel_resources((const void*)tree_up_gather_kernel, "tree_up_gather_kernel");
     check_kernel_resources((const void*)tree_uncoarsen_kernel, "tree_uncoarsen_kernel");
     check_kernel_resources((const void*)tree_side_copy_kernel, "tree_side_copy_kernel");
+    check_kernel_resources((const void*)tree_side_interp_kernel, "tree_side_interp_kernel");
     check_kernel_resources((const void*)tree_root_dirichlet_kernel, "tree_root_dirichlet_kernel");
     check_kernel_resources((const void*)tree_root_robin_kernel, "tree_root_robin_kernel");
     done = true;
@@ -1021,7 +1024,7 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
         LevelSweep& L = sp->lv[dd];
         std::vector<UpDesc> ud;
         std::vector<TreeGemm> pw, ph, pu;
-        std::vector<SideCopy> cp;
+        std::vector<SideCopy> cp, ip;
         int dn_cnt = 0;
         std::vector<std::vector<VecStepDesc>> us;
         for (int id : P.internal_at[dd]) {
@@ -1099,15 +1102,23 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
                 if (P.side_reads && ch.tag == 0) continue;
                 SideCopy sc{};
                 sc.len = n;
-                sc.dst = ch.gc_off;
-                vcheck(sc.dst, 4LL * n, "child g");
                 for (int sd = 0; sd < 4; sd++) {
                     const int q = child_src_q(c, sd);
                     sc.src[sd] = q >= 0 ? nd.gsrc[q >> 1] + (long long)(q & 1) * n
                                         : nd.ui_off + (long long)child_src_r(c, sd) * n;
                     vcheck(sc.src[sd], n, "side source");
                 }
-                cp.push_back(sc);
+                if (P.side_reads) {
+                    // a coarsened child: its first uncoarsen_ step interpolates straight from the
+                    // sources into that step's destination (g, or gt when more steps follow)
+                    sc.dst = ((ch.tag - 1) % 2 == 0) ? ch.g_off : ch.gt_off;
+                    vcheck(sc.dst, 8LL * n, "first uncoarsen step of a child");
+                    ip.push_back(sc);
+                } else {
+                    sc.dst = ch.gc_off;
+                    vcheck(sc.dst, 4LL * n, "child g");
+                    cp.push_back(sc);
+                }
             }
             TreeGemm pd{};
             pd.A0 = M->Sar.p + nd.S_off; pd.lda0 = 8 * n; pd.K0 = 8 * n; pd.alpha0 = 1.0;
@@ -1127,6 +1138,7 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
             pu.push_back(pd);
             // uncoarsen_ steps: coarse gc -> ... -> g, alternating through gt
             for (int stp = 1; stp <= nd.tag; stp++) {
+                if (stp == 1 && P.side_reads) continue;  // done from the sources at the parent's depth
                 if ((int)us.size() < stp) us.resize(stp);
                 VecStepDesc v;
                 const int nfine = nd.s >> (nd.tag - stp);
@@ -1142,10 +1154,12 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
         }
         L.cnt = dn_cnt;
         L.copy_cnt = (int)cp.size();
+        L.interp_cnt = (int)ip.size();
         L.up_cnt = (int)ud.size();
         check_range(V, (size_t)vrows * k * 8, "vector arena");
         L.up = sp->store.stage(ud);
         L.copies = sp->store.stage(cp);
+        L.interps = sp->store.stage(ip);
         if (M->autotune == 1 && autotune_on()) {
             // widths with the same number of 16-column tiles share a choice: a tile's cost does
             // not depend on how many of its columns are live
@@ -1175,6 +1189,7 @@ SweepPlan& sweep_plan(ef_mesh_s* M, int k) {
             L.tile[2] = pu.empty() ? 0 : 10 * L.gu.rt + L.gu.tn;
         }
         for (auto& v : us) {
+            if (v.empty()) continue;
             int nfmax = 0;
             for (auto& x : v) nfmax = std::max(nfmax, x.nfine);
             L.unc.push_back(sp->store.stage(v));
@@ -1291,6 +1306,9 @@ void do_solve_once(ef_mesh_s* M, const double* f_dev, const double* r_dev, const
             if (L.copy_cnt > 0)
                 EF_LAUNCH("tree_side_copy_kernel", tree_side_copy_kernel, dim3(blocks_for(4LL * L.nmax * k), L.copy_cnt),
                           dim3(256), s, L.copies, V, k, P.vec_rows);
+            if (L.interp_cnt > 0)
+                EF_LAUNCH("tree_side_interp_kernel", tree_side_interp_kernel,
+                          dim3(blocks_for(8LL * L.nmax * k), L.interp_cnt), dim3(256), s, L.interps, V, k, P.vec_rows);
         }
         launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, f_zero ? 1 : 0, u_dev, k);
     };

@@ -951,6 +951,19 @@ __global__ void __launch_bounds__(256) tree_up_gather_kernel(const UpDesc* __res
     }
 }
 
+// value i of a side prolonged from nc coarse values src to nf = 2 nc (uncoarsen_'s L12:
+// quadratic end weights, 3/4-1/4 in between)
+__device__ inline double prolong(const double* src, int nc, int nf, int i) {
+    if (i == 0) return 1.40625 * src[0] + (-0.5625) * src[1] + 0.15625 * src[2];
+    if (i == nf - 1) return 0.15625 * src[nc - 3] + (-0.5625) * src[nc - 2] + 1.40625 * src[nc - 1];
+    if (i & 1) {
+        const int j = (i - 1) / 2;
+        return 0.75 * src[j] + 0.25 * src[j + 1];
+    }
+    const int j = (i - 2) / 2;
+    return 0.25 * src[j] + 0.75 * src[j + 1];
+}
+
 __global__ void __launch_bounds__(256) tree_uncoarsen_kernel(const VecStepDesc* __restrict__ descs, double* V, int k,
                                                               long long vrows) {
     const VecStepDesc d = descs[blockIdx.y];
@@ -960,18 +973,12 @@ __global__ void __launch_bounds__(256) tree_uncoarsen_kernel(const VecStepDesc* 
     const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
     const int side = row / nf, i = row % nf;
     double* Vr = V + (long long)rhs * vrows;
-    const double* src = Vr + d.src_off + (long long)side * nc;
-    double s;
-    if (i == 0) s = 1.40625 * src[0] + (-0.5625) * src[1] + 0.15625 * src[2];
-    else if (i == nf - 1) s = 0.15625 * src[nc - 3] + (-0.5625) * src[nc - 2] + 1.40625 * src[nc - 1];
-    else if (i & 1) { const int j = (i - 1) / 2; s = 0.75 * src[j] + 0.25 * src[j + 1]; }
-    else { const int j = (i - 2) / 2; s = 0.25 * src[j] + 0.75 * src[j + 1]; }
-    Vr[d.dst_off + row] = s;
+    Vr[d.dst_off + row] = prolong(Vr + d.src_off + (long long)side * nc, nc, nf, i);
 }
 
-// A child's boundary data written out (for a coarsened child, whose g the uncoarsen steps
-// interpolate; or for every child where the sweeps cannot read sides in place): side s of
-// descriptor blockIdx.y from src[s], one thread per (rhs, row).
+// A child's boundary data written out, where the sweeps cannot read sides in place (sizes that
+// do not allow the 4-row operand blocks): side s of descriptor blockIdx.y from src[s], one thread
+// per (rhs, row).
 __global__ void __launch_bounds__(256) tree_side_copy_kernel(const SideCopy* __restrict__ descs, double* V, int k,
                                                               long long vrows) {
     const SideCopy d = descs[blockIdx.y];
@@ -982,6 +989,21 @@ __global__ void __launch_bounds__(256) tree_side_copy_kernel(const SideCopy* __r
     const int s = row / len, i = row - s * len;
     double* Vr = V + (long long)rhs * vrows;
     Vr[d.dst + row] = Vr[pick4(d.src, s) + i];
+}
+
+// The first uncoarsen_ step of a coarsened child straight from its sides' sources: the coarse
+// side s (len values at src[s]) prolonged to 2 len values at dst + s 2 len, with the weights of
+// tree_uncoarsen_kernel (HPSAlgorithm.hpp:1165-1190).
+__global__ void __launch_bounds__(256) tree_side_interp_kernel(const SideCopy* __restrict__ descs, double* V, int k,
+                                                                long long vrows) {
+    const SideCopy d = descs[blockIdx.y];
+    const int nc = d.len, nf = 2 * nc;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= 4LL * nf * k) return;
+    const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
+    const int side = row / nf, i = row % nf;
+    double* Vr = V + (long long)rhs * vrows;
+    Vr[d.dst + row] = prolong(Vr + pick4(d.src, side), nc, nf, i);
 }
 
 // g = r / a: the LU solve of diag(a) in solveStage (HPSAlgorithm.hpp:402-420); r is [rhs][m],

@@ -175,6 +175,7 @@ __global__ void tree_up_gather_kernel(const UpDesc* d, double* V, int k, long lo
 __global__ void tree_uncoarsen_kernel(const VecStepDesc* d, double* V, int k, long long vrows);
 // applyS_ scatter: children's g from tau's g (WESN) and the interface potential
 __global__ void tree_side_copy_kernel(const SideCopy* d, double* V, int k, long long vrows);
+__global__ void tree_side_interp_kernel(const SideCopy* d, double* V, int k, long long vrows);
 // root Dirichlet data g = r / a (solveStage with b = 0)
 __global__ void tree_root_dirichlet_kernel(const double* r, const double* a, double* g, int m, int k, long long vrows);
 // root Robin system A = diag(a) + diag(b) T, rhs = r - b h
