@@ -1073,7 +1073,6 @@ GemmLaunch plan_gemm(const std::vector<TreeGemm>& probs_in, int tn, DevStore& st
         TreeGemm& p = probs[i];
         start[i] = (int)total;
         p.nks = 1;
-        p.P = nullptr;
         if (p.bcs == 0) p.bcs = 1;
         if (p.ccs == 0) p.ccs = 1;
         if (p.dcs == 0) p.dcs = 1;

@@ -84,7 +84,6 @@ struct TreeGemm {
     int bcs, ccs, dcs, kc;               // column strides (0 means 1); split-K chunk length (0: none)
     double alpha0, alpha1, beta;
     int nks, pad;                        // split-K chunk count (set by plan_gemm)
-    double* P;                           // unused (kept for the layout)
     // Block-gather mode (dn > 0, segment 0 only, ldb0 = 1): row kk of B0 is B0[d2[r] + i] -
     // B0[d1[r] + i] with r = kk / dn, i = kk % dn (dn a multiple of 4): the four interface
     // blocks of mergeW_'s hDiff read straight from the children's h, each block the difference
@@ -93,7 +92,7 @@ struct TreeGemm {
     int dn, dsub;
     long long d2[4], d1[4];
 };
-static_assert(sizeof(TreeGemm) == 216, "TreeGemm layout");
+static_assert(sizeof(TreeGemm) == 208, "TreeGemm layout");
 
 // F32: operands rounded to FP32, v_mfma_f32_16x16x4_f32 with FP32 accumulation, FP64 output
 // (+ beta C in FP64); the mixed-precision build's S and T products.
@@ -194,8 +193,8 @@ struct GemmLaunch {
     int kw = 0;                   // > 1: split-K launch, waves per 16 x 16 tile
     bool f32 = false;  // FP32 matrix cores (mixed precision)
 };
-// Checks every problem (extents inside their allocations, alignment), allocates the partials of
-// split problems, stages problems and wave offsets in `store` (device memory the caller keeps
+// Checks every problem (extents inside their allocations, alignment), sets the split-K chunks of
+// a split launch, stages problems and wave offsets in `store` (device memory the caller keeps
 // alive; valid on the device after store.flush) and returns the launch.  tn = 1 (16 x 16 blocks) suits N <= 16, tn = 2 (32 x 32) the rest.
 struct DevStore;
 // rt = 0 picks the row tile from tn (16 rows for tn = 1 or problems of <= 16 rows, else 32);
