@@ -164,3 +164,55 @@ def test_gpu_serving_loads_and_poisson(ctx):
     alone = srv.submit_load("m", P2[100])
     srv.run()
     assert np.array_equal(alone.u, out[100].u)
+
+
+@pytest.mark.gpu
+def test_gpu_multi_mesh_admission(ctx):
+    """Three adaptive variable-coefficient meshes (N = 8, 16, 8) registered with a budget for
+    two: requests build meshes on demand and evict idle ones; every result equals the same
+    request on a fresh single-mesh server, bitwise, and one matches the oracle."""
+    import ellipticforest_amd as ef
+    bounds = (-10.0, 10.0, -10.0, 10.0)
+    F = lambda x, y: -(np.sin(x) + np.sin(y))
+    specs = {}
+    for key, (N, maxl, a, lam) in {"a": (8, 4, 1.0, -0.5), "b": (16, 3, 2.5, -1.5), "c": (8, 5, 0.5, 0.0)}.items():
+        leaves = omesh.refine_by_function(lambda x, y: np.abs(F(x, y)) > 1.2, N, N, *bounds, 0, maxl)
+        dm = ef.DeviceMesh(ctx, leaves, N, bounds)
+        beta = (lambda aa: (lambda x, y: 1.0 + 0.5 * np.sin(aa * x + 0.7 * y)))(a)
+        cell = np.concatenate([oracle.sample_coeffs(N, tuple(dm.node_bounds[n]), lambda x, y: np.ones_like(x), beta,
+                                                    (lambda l: (lambda x, y: l + 0 * x))(lam)).ravel()
+                               for n in dm.leaf_nodes])
+        specs[key] = (leaves, N, dict(solver_type=ef._lib.EF_SOLVER_FIVE_POINT, cell=cell), beta, lam)
+    srv = serving.Server(ctx, max_batch=8, buckets=(1, 2, 4, 8))
+    fp = {}
+    for key, (leaves, N, args, _, _) in specs.items():
+        srv.register_mesh(key, leaves, N, bounds, **args)
+        fp[key] = ef._lib.DeviceMesh.footprint(leaves, N, bounds, max_rhs=8, variable=True) + \
+            srv.slot_bytes(len(leaves), N, 0)
+    srv.budget = sorted(fp.values())[-1] + sorted(fp.values())[-2] + 1
+    P = serving.workload_params(np.random.default_rng(8), bounds, 12)
+    order = ["a", "b", "c", "a", "c", "b"] * 2
+    reqs = []
+    for i in range(0, 12, 3):  # ticks of three requests
+        for key, p in zip(order[i:i + 3], P[i:i + 3]):
+            reqs.append((key, p, srv.submit_load(key, p)))
+        srv.run()
+    assert srv.evictions >= 1 and len(srv.admissions) >= 4
+    for key, p, q in reqs[:4]:
+        leaves, N, args, _, _ = specs[key]
+        one = serving.Server(ctx, max_batch=8, buckets=(1, 2, 4, 8))
+        one.add_mesh(key, leaves, N, bounds, **args)
+        alone = one.submit_load(key, p)
+        one.run()
+        assert np.array_equal(alone.u, q.u), key
+    key, p, q = reqs[1]
+    leaves, N, args, beta, lam = specs[key]
+    O = oracle.Forest(leaves, N, bounds, solver=oracle.FIVE_POINT,
+                      coeff_fns=(lambda x, y: np.ones_like(x), beta, lambda x, y: lam + 0 * x))
+    O.build()
+    dm = srv.meshes[key].dm if key in srv.meshes else ef.DeviceMesh(ctx, leaves, N, bounds)
+    side, px, py = oracle.root_face_points(dm.M, bounds)
+    f, r = serving.workload_load(p, [tuple(dm.node_bounds[n]) for n in dm.leaf_nodes], N, (px, py))
+    O.upwards(f)
+    ref = O.solve(r, np.ones_like(r), np.zeros_like(r))
+    assert np.abs(q.u - ref).max() / np.abs(ref).max() < 1e-10
