@@ -92,9 +92,12 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         o1 = pick4(dr.d1, r) + i;
     };
     // the operands of block kb: A rows (two 16-byte vectors per row) and B columns
+    // MODE 1 keeps both children's rows and subtracts them in the products: a subtraction at
+    // load time would wait for the loads as soon as they are issued
     struct Blk {
         td2 x0, x1, y0, y1;
         double b[TN][4];
+        double c[MODE == 1 ? TN : 1][4];
     };
     auto load = [&](int kb, Blk& o) {
         const int k = kb + 4 * q;
@@ -112,8 +115,8 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
                 for (int j = 0; j < TN; j++) {
                     const td2 u0 = gld2(bp[j] + o2), u1 = gld2(bp[j] + o2 + 2);
                     const td2 v0 = gld2(bp[j] + o1), v1 = gld2(bp[j] + o1 + 2);
-                    o.b[j][0] = u0[0] - v0[0]; o.b[j][1] = u0[1] - v0[1];
-                    o.b[j][2] = u1[0] - v1[0]; o.b[j][3] = u1[1] - v1[1];
+                    o.b[j][0] = u0[0]; o.b[j][1] = u0[1]; o.b[j][2] = u1[0]; o.b[j][3] = u1[1];
+                    o.c[j][0] = v0[0]; o.c[j][1] = v0[1]; o.c[j][2] = v1[0]; o.c[j][3] = v1[1];
                 }
             } else {
 #pragma unroll
@@ -145,8 +148,9 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         for (int s = 0; s < 4; s++)
 #pragma unroll
             for (int j = 0; j < TN; j++) {
-                acc[0][j] = mfma_acc<F32>(a0[s], o.b[j][s], acc[0][j]);
-                if (RT == 2) acc[1][j] = mfma_acc<F32>(a1[s], o.b[j][s], acc[1][j]);
+                const double bv = MODE == 1 ? o.b[j][s] - o.c[MODE == 1 ? j : 0][s] : o.b[j][s];
+                acc[0][j] = mfma_acc<F32>(a0[s], bv, acc[0][j]);
+                if (RT == 2) acc[1][j] = mfma_acc<F32>(a1[s], bv, acc[1][j]);
             }
     };
     const int nfull = K / 16;
@@ -157,22 +161,36 @@ __device__ inline void gemm_segment(const double* __restrict__ a0p, const double
         // frees -- no buffer is copied to another (a copy waits for its loads).  Measured at 64
         // right-hand sides, adaptive Poisson sweep GEMMs: PD = 2 0.871 ms, 3 0.969 ms, 4 0.996 ms
         // (more registers, fewer waves per SIMD); a cur/next pair with a copy 0.948 ms.
+        // The steady-state loop issues its loads unconditionally, so the loop head sees one
+        // order of outstanding loads and waits for the older block only (a conditional load
+        // there makes the wait counts cover both blocks).
         constexpr int PD = 2;
         Blk buf[PD];
+        if (nfull >= PD) {
 #pragma unroll
-        for (int j = 0; j < PD; j++)
-            if (j < nfull) load(16 * j, buf[j]);
-        int i = 0;
-        for (; i + PD <= nfull; i += PD) {
+            for (int j = 0; j < PD; j++) load(16 * j, buf[j]);
+            int i = 0;
+            for (; i + 2 * PD <= nfull; i += PD) {
+#pragma unroll
+                for (int j = 0; j < PD; j++) {
+                    mma(buf[j]);
+                    load(16 * (i + j + PD), buf[j]);
+                }
+            }
+            // PD <= nfull - i < 2 PD: one round with the last loads, then the blocks left
 #pragma unroll
             for (int j = 0; j < PD; j++) {
                 mma(buf[j]);
                 if (i + j + PD < nfull) load(16 * (i + j + PD), buf[j]);
             }
-        }
+            i += PD;
 #pragma unroll
-        for (int j = 0; j < PD - 1; j++)
-            if (i + j < nfull) mma(buf[j]);
+            for (int j = 0; j < PD - 1; j++)
+                if (i + j < nfull) mma(buf[j]);
+        } else {
+            load(0, buf[0]);
+            mma(buf[0]);
+        }
         kb = 16 * nfull;
     }
     if (kb < K) {  // partial last block: k >= K contributes zero (addresses clamped to K - 1)
