@@ -1263,7 +1263,7 @@ void do_upwards(ef_mesh_s* M, const double* f_dev, int k) {
                 if (L.cnt == 0) continue;
                 if (L.up_cnt > 0)
                     EF_LAUNCH("tree_up_gather_kernel", tree_up_gather_kernel,
-                              dim3(blocks_for(12LL * L.up_nmax * k), L.up_cnt), dim3(256), s, L.up, M->vec.p, k,
+                              dim3(vec_blocks(12LL * L.up_nmax * k, L.up_cnt), L.up_cnt), dim3(256), s, L.up, M->vec.p, k,
                               P.vec_rows);
                 run_gemm(L.gw, s);
                 run_gemm(L.gh, s);
@@ -1300,15 +1300,15 @@ void do_solve_once(ef_mesh_s* M, const double* f_dev, const double* r_dev, const
             if (L.cnt == 0) continue;
             for (size_t stp = 0; stp < L.unc.size(); stp++)
                 EF_LAUNCH("tree_uncoarsen_kernel", tree_uncoarsen_kernel,
-                          dim3(blocks_for(4LL * L.unc_nf[stp] * k), L.unc_cnt[stp]), dim3(256), s, L.unc[stp], V, k,
+                          dim3(vec_blocks(4LL * L.unc_nf[stp] * k, L.unc_cnt[stp]), L.unc_cnt[stp]), dim3(256), s, L.unc[stp], V, k,
                           P.vec_rows);
             run_gemm(L.gu, s);
             if (L.copy_cnt > 0)
-                EF_LAUNCH("tree_side_copy_kernel", tree_side_copy_kernel, dim3(blocks_for(4LL * L.nmax * k), L.copy_cnt),
+                EF_LAUNCH("tree_side_copy_kernel", tree_side_copy_kernel, dim3(vec_blocks(4LL * L.nmax * k, L.copy_cnt), L.copy_cnt),
                           dim3(256), s, L.copies, V, k, P.vec_rows);
             if (L.interp_cnt > 0)
                 EF_LAUNCH("tree_side_interp_kernel", tree_side_interp_kernel,
-                          dim3(blocks_for(8LL * L.nmax * k), L.interp_cnt), dim3(256), s, L.interps, V, k, P.vec_rows);
+                          dim3(vec_blocks(8LL * L.nmax * k, L.interp_cnt), L.interp_cnt), dim3(256), s, L.interps, V, k, P.vec_rows);
         }
         launch_leaf(M, 1, f_dev ? f_dev : (const double*)V, f_zero ? 1 : 0, u_dev, k);
     };

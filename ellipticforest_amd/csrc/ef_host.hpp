@@ -78,6 +78,16 @@ namespace efk {
 
 inline unsigned blocks_for(long long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// gridDim.x of a descriptor-per-row vector launch (cnt descriptors of at most n elements each,
+// the kernel striding over its elements): about 2048 workgroups in all, never more than n needs.
+// One workgroup per 256 elements of the longest descriptor had dispatched mostly empty
+// workgroups on levels with hundreds of short descriptors.
+inline unsigned vec_blocks(long long n, int cnt) {
+    const long long cap = cnt > 0 ? (2048 + cnt - 1) / cnt : 1;
+    const long long b = blocks_for(n);
+    return (unsigned)(b < cap ? b : (cap < 1 ? 1 : cap));
+}
+
 // Grid and block of a launch against the limits of a gfx950 dispatch; the name is printed
 // first in EF_SYNC_CHECK mode.
 inline void pre_launch(const char* name, dim3 grid, dim3 block) {

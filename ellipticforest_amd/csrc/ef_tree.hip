@@ -949,23 +949,28 @@ __device__ inline double coarse_h(const double* Vr, long long off, int clen, int
     return s * (4.0 / (double)w);
 }
 
+// The vector kernels below (up gather, uncoarsen, side copy, side interpolation) take one
+// descriptor per blockIdx.y and stride over its (rhs, row) elements with gridDim.x workgroups:
+// the host sizes gridDim.x to the launch's total work, not to its longest descriptor, so short
+// descriptors do not leave thousands of empty workgroups to dispatch.
 __global__ void __launch_bounds__(256) tree_up_gather_kernel(const UpDesc* __restrict__ descs, double* V, int k,
                                                               long long vrows) {
     const UpDesc d = descs[blockIdx.y];
     const int n = d.n;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 12LL * n * k) return;
-    const int rhs = (int)(e / (12 * n)), row = (int)(e % (12 * n));
-    double* Vr = V + (long long)rhs * vrows;
-    if (row < 4 * n) {  // hDiff of mergeW_ (HPSAlgorithm.hpp:1059-1087)
-        const int r = row / n, i = row % n;
-        const int c1 = if_c1(r), c2 = if_c2(r);
-        const double h2 = coarse_h(Vr, pick4(d.ch_off, c2), pick4(d.clen, c2), pick4(d.tag, c2), if_s2(r), i);
-        const double h1 = coarse_h(Vr, pick4(d.ch_off, c1), pick4(d.clen, c1), pick4(d.tag, c1), if_s1(r), i);
-        Vr[d.hd_off + row] = h2 - h1;
-    } else {  // boundary part of mergeH_ (:1098-1117), WESN order (:1128-1150)
-        const int rr = row - 4 * n, q = rr / n, i = rr % n, c = q_child(q);
-        Vr[d.h_off + rr] = coarse_h(Vr, pick4(d.ch_off, c), pick4(d.clen, c), pick4(d.tag, c), q_side(q), i);
+    const long long total = 12LL * n * k;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int rhs = (int)(e / (12 * n)), row = (int)(e % (12 * n));
+        double* Vr = V + (long long)rhs * vrows;
+        if (row < 4 * n) {  // hDiff of mergeW_ (HPSAlgorithm.hpp:1059-1087)
+            const int r = row / n, i = row % n;
+            const int c1 = if_c1(r), c2 = if_c2(r);
+            const double h2 = coarse_h(Vr, pick4(d.ch_off, c2), pick4(d.clen, c2), pick4(d.tag, c2), if_s2(r), i);
+            const double h1 = coarse_h(Vr, pick4(d.ch_off, c1), pick4(d.clen, c1), pick4(d.tag, c1), if_s1(r), i);
+            Vr[d.hd_off + row] = h2 - h1;
+        } else {  // boundary part of mergeH_ (:1098-1117), WESN order (:1128-1150)
+            const int rr = row - 4 * n, q = rr / n, i = rr % n, c = q_child(q);
+            Vr[d.h_off + rr] = coarse_h(Vr, pick4(d.ch_off, c), pick4(d.clen, c), pick4(d.tag, c), q_side(q), i);
+        }
     }
 }
 
@@ -986,27 +991,29 @@ __global__ void __launch_bounds__(256) tree_uncoarsen_kernel(const VecStepDesc* 
                                                               long long vrows) {
     const VecStepDesc d = descs[blockIdx.y];
     const int nf = d.nfine, nc = nf / 2;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 4LL * nf * k) return;
-    const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
-    const int side = row / nf, i = row % nf;
-    double* Vr = V + (long long)rhs * vrows;
-    Vr[d.dst_off + row] = prolong(Vr + d.src_off + (long long)side * nc, nc, nf, i);
+    const long long total = 4LL * nf * k;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
+        const int side = row / nf, i = row % nf;
+        double* Vr = V + (long long)rhs * vrows;
+        Vr[d.dst_off + row] = prolong(Vr + d.src_off + (long long)side * nc, nc, nf, i);
+    }
 }
 
 // A child's boundary data written out, where the sweeps cannot read sides in place (sizes that
-// do not allow the 4-row operand blocks): side s of descriptor blockIdx.y from src[s], one thread
-// per (rhs, row).
+// do not allow the 4-row operand blocks): side s of descriptor blockIdx.y from src[s], each
+// thread striding over (rhs, row) elements.
 __global__ void __launch_bounds__(256) tree_side_copy_kernel(const SideCopy* __restrict__ descs, double* V, int k,
                                                               long long vrows) {
     const SideCopy d = descs[blockIdx.y];
     const int len = d.len;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 4LL * len * k) return;
-    const int rhs = (int)(e / (4 * len)), row = (int)(e % (4 * len));
-    const int s = row / len, i = row - s * len;
-    double* Vr = V + (long long)rhs * vrows;
-    Vr[d.dst + row] = Vr[pick4(d.src, s) + i];
+    const long long total = 4LL * len * k;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int rhs = (int)(e / (4 * len)), row = (int)(e % (4 * len));
+        const int s = row / len, i = row - s * len;
+        double* Vr = V + (long long)rhs * vrows;
+        Vr[d.dst + row] = Vr[pick4(d.src, s) + i];
+    }
 }
 
 // The first uncoarsen_ step of a coarsened child straight from its sides' sources: the coarse
@@ -1016,12 +1023,13 @@ __global__ void __launch_bounds__(256) tree_side_interp_kernel(const SideCopy* _
                                                                 long long vrows) {
     const SideCopy d = descs[blockIdx.y];
     const int nc = d.len, nf = 2 * nc;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= 4LL * nf * k) return;
-    const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
-    const int side = row / nf, i = row % nf;
-    double* Vr = V + (long long)rhs * vrows;
-    Vr[d.dst + row] = prolong(Vr + pick4(d.src, side), nc, nf, i);
+    const long long total = 4LL * nf * k;
+    for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+        const int rhs = (int)(e / (4 * nf)), row = (int)(e % (4 * nf));
+        const int side = row / nf, i = row % nf;
+        double* Vr = V + (long long)rhs * vrows;
+        Vr[d.dst + row] = prolong(Vr + pick4(d.src, side), nc, nf, i);
+    }
 }
 
 // g = r / a: the LU solve of diag(a) in solveStage (HPSAlgorithm.hpp:402-420); r is [rhs][m],
