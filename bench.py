@@ -42,7 +42,7 @@ METRIC = "solves/s + p50/p99 latency at 1e-8 rel err vs FP64; buildStage seconds
 TRAFFIC_PER_LAUNCH = 572.9e6          # leaf solves of this workload, 16 right-hand sides
 TRAFFIC_MAP_D2N = 319.0e6             # particular Neumann data, same leaf set and k
 TRAFFIC_VAR_BETA = 1329.8e6           # variable-beta leaf solves, packed symmetric blocks
-TRAFFIC_GEMM_PER_LAUNCH = 130.03e6    # tree_gemm_kernel in the sweep at 64 columns, per launch: profiles/tree_gemm_kernel_adaptive_poisson_sweep.md
+TRAFFIC_GEMM_PER_LAUNCH = 135.36e6    # tree_gemm_kernel in the sweep at 64 columns, per launch: profiles/tree_gemm_kernel_adaptive_poisson_sweep.md
 
 
 def log(msg):
