@@ -271,27 +271,41 @@ constexpr int EF_MAX_SLOTS = 8;
 // Serving workload loads (one request = 64 doubles): f = sum of up to 8 Gaussians
 // amp exp(-|p - c|^2 / (2 w^2)) at leaf cell centres (x_i = xl + dx/2 + i dx, cell j + i N),
 // r = sum of 8 modes a cos(kx x + ky y + phi) at the root boundary points.
+// Workgroup b < nrhs * nl evaluates item b = (rhs, leaf): each Gaussian separates,
+// exp(-dx^2 / (2 w^2)) exp(-dy^2 / (2 w^2)), so the 16 N one-dimensional factors go to LDS and
+// each cell sums 8 products (16 N exponentials per patch instead of 8 N^2; the value differs
+// from the exponential of the sum by rounding only).  Workgroups past the items evaluate r.
 __global__ void __launch_bounds__(256) serve_load_kernel(const double* __restrict__ geo, int nl, int N,
                                                           const double* __restrict__ params, int nrhs,
                                                           double* __restrict__ f, const double* __restrict__ pts,
                                                           int m, double* __restrict__ r) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    const long long NN = (long long)N * N, nf = (long long)nrhs * nl * NN;
-    if (e < nf) {
-        const long long rhs = e / (nl * NN), rem = e % (nl * NN);
-        const int leaf = (int)(rem / NN), cell = (int)(rem % NN), i = cell / N, j = cell % N;
+    __shared__ double fac[2][8][32];  // [axis][Gaussian][cell index along the axis], N <= 32
+    const long long nitems = (long long)nrhs * nl, b = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (b < nitems) {  // uniform per workgroup
+        const long long rhs = b / nl;
+        const int leaf = (int)(b % nl), NN = N * N;
         const double* g = geo + 4LL * leaf;
-        const double x = (g[0] + g[2] / 2) + i * g[2], y = (g[1] + g[3] / 2) + j * g[3];
         const double* P = params + 64 * rhs;
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const double amp = P[4 * q], dxc = x - P[4 * q + 1], dyc = y - P[4 * q + 2], w = P[4 * q + 3];
-            if (amp != 0.0) acc += amp * exp(-(dxc * dxc + dyc * dyc) / (2.0 * w * w));
+        for (int t = tid; t < 16 * N; t += 256) {
+            const int axis = t / (8 * N), rem = t % (8 * N), q = rem / N, i = rem % N;
+            const double amp = P[4 * q], c = P[4 * q + 1 + axis], w = P[4 * q + 3];
+            const double x = (g[axis] + g[2 + axis] / 2) + i * g[2 + axis], d = x - c;
+            fac[axis][q][i] = (amp != 0.0) ? exp(-(d * d) / (2.0 * w * w)) : 0.0;
         }
-        f[e] = acc;
-    } else if (e < nf + (long long)nrhs * m) {
-        const long long t = e - nf, rhs = t / m;
+        __syncthreads();
+        double* fo = f + b * NN;
+        for (int cell = tid; cell < NN; cell += 256) {
+            const int i = cell / N, j = cell % N;
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc += P[4 * q] * (fac[0][q][i] * fac[1][q][j]);
+            fo[cell] = acc;
+        }
+    } else {
+        const long long t = (b - nitems) * 256 + tid;
+        if (t >= (long long)nrhs * m) return;
+        const long long rhs = t / m;
         const int p = (int)(t % m);
         const double x = pts[p], y = pts[m + p];
         const double* P = params + 64 * rhs + 32;
@@ -1881,11 +1895,13 @@ int ef_mesh_eval_load(ef_mesh mesh, const double* params_dev, int nrhs, double* 
         mesh->leafgeo.upload(geo, mesh->stream);
         mesh->rootpts.upload(pts, mesh->stream);
     }
-    const long long total = (long long)nrhs * nl * N * N + (long long)nrhs * m;
+    if (N > 32) throw EfError(EF_ERR_ARG, "load evaluation supports N <= 32");
+    const long long nblocks = (long long)nrhs * nl + blocks_for((long long)nrhs * m);
+    if (nblocks > 0x7fffffffLL) throw EfError(EF_ERR_ARG, "too many load items for one launch");
     check_range(params_dev, (size_t)nrhs * 64 * 8, "load parameters");
     check_range(f_dev, (size_t)nrhs * nl * N * N * 8, "load f");
     check_range(r_dev, (size_t)nrhs * m * 8, "load r");
-    EF_LAUNCH("serve_load_kernel", serve_load_kernel, dim3(blocks_for(total)), dim3(256), mesh->stream,
+    EF_LAUNCH("serve_load_kernel", serve_load_kernel, dim3((unsigned)nblocks), dim3(256), mesh->stream,
               (const double*)mesh->leafgeo.p, nl, N, params_dev, nrhs, f_dev, (const double*)mesh->rootpts.p, m, r_dev);
     EF_API_END
 }

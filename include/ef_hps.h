@@ -180,7 +180,8 @@ int ef_mesh_set_refinement(ef_mesh mesh, int steps);
  * (params_dev, device): f_dev[rhs][leaf][N*N] = sum of 8 Gaussians (amp, cx, cy, w; amp 0
  * unused) amp exp(-|p - c|^2 / (2 w^2)) at the leaf cell centres, r_dev[rhs][4M] = sum of 8
  * modes (a, kx, ky, phi) a cos(kx x + ky y + phi) at the root boundary points; queued on the
- * mesh's stream. */
+ * mesh's stream.  Each Gaussian is evaluated as the product of its x and y factors (equal to
+ * the exponential of the sum up to rounding). */
 int ef_host_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_host_free(ef_ctx ctx, void* ptr);
 int ef_mesh_copy_async(ef_mesh mesh, void* dst, const void* src, size_t bytes, int kind);
