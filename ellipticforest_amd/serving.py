@@ -99,6 +99,39 @@ def workload_load(params, leaf_bounds, N, root_points):
     return f, r
 
 
+_DONE = threading.Condition()  # shared by every request's completion flag
+
+
+class Done:
+    """A request's completion: ``set`` / ``is_set`` / ``wait`` like threading.Event, on one
+    condition shared by all requests (an Event per request cost more than the rest of a
+    submission)."""
+    __slots__ = ("_flag",)
+
+    def __init__(self):
+        self._flag = False
+
+    def set(self):
+        with _DONE:
+            self._flag = True
+            _DONE.notify_all()
+
+    def is_set(self):
+        return self._flag
+
+    def wait(self, timeout=None):
+        with _DONE:
+            return _DONE.wait_for(lambda: self._flag, timeout)
+
+
+def _set_done(reqs):
+    """Mark a finished batch's requests complete under one acquisition of the condition."""
+    with _DONE:
+        for q in reqs:
+            q.done._flag = True
+        _DONE.notify_all()
+
+
 @dataclass
 class Request:
     mesh: str
@@ -110,7 +143,7 @@ class Request:
     t_arrival: float = field(default_factory=time.perf_counter)
     t_done: float = 0.0
     u: np.ndarray | None = None
-    done: threading.Event = field(default_factory=threading.Event)
+    done: Done = field(default_factory=Done)
 
     @property
     def kind(self):
@@ -247,17 +280,18 @@ class Server:
             if not self.pending:
                 return None, None, []
             key, kind = self.pending[0].mesh, self.pending[0].kind
-            batch, rest = [], collections.deque()
+            batch, rest = [], []
             fetch = 0
-            while self.pending:
+            # scan only until the batch is full: the skipped requests go back in front, in order
+            # (a scan of the whole queue per batch made a deep queue cost O(queue) per batch)
+            while self.pending and len(batch) < self.max_batch:
                 q = self.pending.popleft()
-                if q.mesh == key and q.kind == kind and len(batch) < self.max_batch and \
-                        (not q.want_u or fetch < self.fetch_rows):
+                if q.mesh == key and q.kind == kind and (not q.want_u or fetch < self.fetch_rows):
                     batch.append(q)
                     fetch += q.want_u
                 else:
                     rest.append(q)
-            self.pending = rest
+            self.pending.extendleft(reversed(rest))
             self.meshes.move_to_end(key)
             return key, self.meshes[key], batch
 
@@ -319,7 +353,7 @@ class Server:
             if q.want_u:
                 q.u = s.hu.a[s.rows[j]].copy()
             q.t_done = now
-            q.done.set()
+        _set_done(s.batch)
         self.completed.extend(s.batch)
         s.batch = None
         res.last_used = now
