@@ -270,41 +270,58 @@ constexpr int EF_MAX_SLOTS = 8;
 
 // Serving workload loads (one request = 64 doubles): f = sum of up to 8 Gaussians
 // amp exp(-|p - c|^2 / (2 w^2)) at leaf cell centres (x_i = xl + dx/2 + i dx, cell j + i N),
-// r = sum of 8 modes a cos(kx x + ky y + phi) at the root boundary points.
-// Workgroup b < nrhs * nl evaluates item b = (rhs, leaf): each Gaussian separates,
-// exp(-dx^2 / (2 w^2)) exp(-dy^2 / (2 w^2)), so the 16 N one-dimensional factors go to LDS and
-// each cell sums 8 products (16 N exponentials per patch instead of 8 N^2; the value differs
-// from the exponential of the sum by rounding only).  Workgroups past the items evaluate r.
-__global__ void __launch_bounds__(256) serve_load_kernel(const double* __restrict__ geo, int nl, int N,
-                                                          const double* __restrict__ params, int nrhs,
-                                                          double* __restrict__ f, const double* __restrict__ pts,
-                                                          int m, double* __restrict__ r) {
-    __shared__ double fac[2][8][32];  // [axis][Gaussian][cell index along the axis], N <= 32
-    const long long nitems = (long long)nrhs * nl, b = blockIdx.x;
-    const int tid = threadIdx.x;
-    if (b < nitems) {  // uniform per workgroup
-        const long long rhs = b / nl;
-        const int leaf = (int)(b % nl), NN = N * N;
-        const double* g = geo + 4LL * leaf;
-        const double* P = params + 64 * rhs;
+// r = sum of 8 modes a cos(kx x + ky y + phi) at the root boundary points.  Each Gaussian
+// separates, exp(-dx^2 / (2 w^2)) exp(-dy^2 / (2 w^2)) (equal to the exponential of the sum up to
+// rounding), and leaves of one column of the mesh share their x centres (of one row, their y
+// centres): serve_factor_kernel evaluates the one-dimensional factors once per distinct centre
+// and request (fac[rhs][q][v], v over the nxv distinct x centres, then the nyv y centres), and
+// serve_cells_kernel forms each cell from 8 products of them.
+__global__ void __launch_bounds__(256) serve_factor_kernel(const double* __restrict__ xy, int nxv, int nyv,
+                                                            const double* __restrict__ params,
+                                                            double* __restrict__ fac) {
+    const int rhs = blockIdx.y, nv = nxv + nyv;
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= 8LL * nv) return;
+    const int q = (int)(e / nv), v = (int)(e % nv), axis = v >= nxv ? 1 : 0;
+    const double* P = params + 64LL * rhs;
+    const double amp = P[4 * q], c = P[4 * q + 1 + axis], w = P[4 * q + 3], d = xy[v] - c;
+    fac[(long long)rhs * 8 * nv + e] = (amp != 0.0) ? exp(-(d * d) / (2.0 * w * w)) : 0.0;
+}
+
+// f[rhs][leaf][cell]: blockIdx.y = rhs, workgroups stride over the leaves; a leaf's 16 N factors
+// (its x and y centre columns, offsets col[2 leaf], col[2 leaf + 1]) are staged in LDS and each
+// cell sums 8 products of them.
+__global__ void __launch_bounds__(256) serve_cells_kernel(const int* __restrict__ col, int nl, int N, int nxv,
+                                                           int nyv, const double* __restrict__ params,
+                                                           const double* __restrict__ fac, double* __restrict__ f) {
+    __shared__ double sf[2][8][32];  // [axis][Gaussian][cell index along the axis], N <= 32
+    const int rhs = blockIdx.y, NN = N * N, nv = nxv + nyv, tid = threadIdx.x;
+    const double* P = params + 64LL * rhs;
+    const double* F = fac + (long long)rhs * 8 * nv;
+    for (int leaf = blockIdx.x; leaf < nl; leaf += gridDim.x) {
         for (int t = tid; t < 16 * N; t += 256) {
             const int axis = t / (8 * N), rem = t % (8 * N), q = rem / N, i = rem % N;
-            const double amp = P[4 * q], c = P[4 * q + 1 + axis], w = P[4 * q + 3];
-            const double x = (g[axis] + g[2 + axis] / 2) + i * g[2 + axis], d = x - c;
-            fac[axis][q][i] = (amp != 0.0) ? exp(-(d * d) / (2.0 * w * w)) : 0.0;
+            const int v = axis ? nxv + col[2 * leaf + 1] + i : col[2 * leaf] + i;
+            sf[axis][q][i] = F[(long long)q * nv + v];
         }
         __syncthreads();
-        double* fo = f + b * NN;
+        double* fo = f + ((long long)rhs * nl + leaf) * NN;
         for (int cell = tid; cell < NN; cell += 256) {
             const int i = cell / N, j = cell % N;
             double acc = 0.0;
 #pragma unroll
-            for (int q = 0; q < 8; q++) acc += P[4 * q] * (fac[0][q][i] * fac[1][q][j]);
+            for (int q = 0; q < 8; q++) acc += P[4 * q] * (sf[0][q][i] * sf[1][q][j]);
             fo[cell] = acc;
         }
-    } else {
-        const long long t = (b - nitems) * 256 + tid;
-        if (t >= (long long)nrhs * m) return;
+        __syncthreads();  // the next leaf's factors overwrite sf
+    }
+}
+
+__global__ void __launch_bounds__(256) serve_bdata_kernel(const double* __restrict__ params, int nrhs,
+                                                           const double* __restrict__ pts, int m,
+                                                           double* __restrict__ r) {
+    const long long total = (long long)nrhs * m;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < total; t += (long long)gridDim.x * 256) {
         const long long rhs = t / m;
         const int p = (int)(t % m);
         const double x = pts[p], y = pts[m + p];
@@ -437,7 +454,9 @@ struct ef_mesh_s {
     // serving: slot events, leaf geometry and root boundary points of the load generator
     hipEvent_t slot_ev[EF_MAX_SLOTS] = {};
     bool slot_rec[EF_MAX_SLOTS] = {};
-    DevBuf<double> leafgeo, rootpts;
+    DevBuf<double> loadxy, rootpts, loadfac;  // distinct cell-centre x then y values; factor work
+    DevBuf<int> loadcol;                      // per leaf: offsets of its x and y centres in loadxy
+    int load_nxv = 0, load_nyv = 0;
 };
 
 namespace {
@@ -1868,41 +1887,57 @@ int ef_mesh_wait(ef_mesh mesh, int slot) {
 int ef_mesh_eval_load(ef_mesh mesh, const double* params_dev, int nrhs, double* f_dev, double* r_dev) {
     EF_API_BEGIN
     if (!mesh || !params_dev || !f_dev || !r_dev) throw EfError(EF_ERR_ARG, "null argument");
-    if (nrhs < 1) throw EfError(EF_ERR_ARG, "nrhs must be >= 1");
+    if (nrhs < 1 || nrhs > 65535) throw EfError(EF_ERR_ARG, "nrhs must be in 1..65535");
     EF_HIP(hipSetDevice(mesh->ctx->device));
     const Plan& P = mesh->P;
     const int nl = (int)P.leaves.size(), N = P.N, m = 4 * P.nodes[0].s;
-    if (!mesh->leafgeo.p) {
-        std::vector<double> geo(4 * (size_t)nl), pts(2 * (size_t)m);
+    if (!mesh->loadxy.p) {
+        // distinct (xl, dx) and (yl, dy) of the leaves: their N cell centres are computed once
+        std::map<std::pair<double, double>, int> xcols, ycols;
+        std::vector<int> col(2 * (size_t)nl);
+        std::vector<double> xs, ys, pts(2 * (size_t)m);
         for (int li = 0; li < nl; li++) {
             const NodeP& nd = P.nodes[P.leaves[li]];
-            geo[4 * li] = nd.xl;
-            geo[4 * li + 1] = nd.yl;
-            geo[4 * li + 2] = (nd.xu - nd.xl) / N;
-            geo[4 * li + 3] = (nd.yu - nd.yl) / N;
+            const double dx = (nd.xu - nd.xl) / N, dy = (nd.yu - nd.yl) / N;
+            auto ix = xcols.emplace(std::make_pair(nd.xl, dx), (int)xs.size());
+            if (ix.second)
+                for (int i = 0; i < N; i++) xs.push_back((nd.xl + dx / 2) + i * dx);
+            auto iy = ycols.emplace(std::make_pair(nd.yl, dy), (int)ys.size());
+            if (iy.second)
+                for (int j = 0; j < N; j++) ys.push_back((nd.yl + dy / 2) + j * dy);
+            col[2 * li] = ix.first->second;
+            col[2 * li + 1] = iy.first->second;
         }
         // root boundary points, sides W, E, S, N (HPSAlgorithm.hpp:375-400)
         const NodeP& root = P.nodes[0];
         const int M = root.s;
         const double dx = (root.xu - root.xl) / M, dy = (root.yu - root.yl) / M;
         for (int i = 0; i < M; i++) {
-            const double ys = (root.yl + dy / 2) + i * dy, xs = (root.xl + dx / 2) + i * dx;
-            pts[i] = root.xl;         pts[m + i] = ys;
-            pts[M + i] = root.xu;     pts[m + M + i] = ys;
-            pts[2 * M + i] = xs;      pts[m + 2 * M + i] = root.yl;
-            pts[3 * M + i] = xs;      pts[m + 3 * M + i] = root.yu;
+            const double ys0 = (root.yl + dy / 2) + i * dy, xs0 = (root.xl + dx / 2) + i * dx;
+            pts[i] = root.xl;         pts[m + i] = ys0;
+            pts[M + i] = root.xu;     pts[m + M + i] = ys0;
+            pts[2 * M + i] = xs0;     pts[m + 2 * M + i] = root.yl;
+            pts[3 * M + i] = xs0;     pts[m + 3 * M + i] = root.yu;
         }
-        mesh->leafgeo.upload(geo, mesh->stream);
+        mesh->load_nxv = (int)xs.size();
+        mesh->load_nyv = (int)ys.size();
+        xs.insert(xs.end(), ys.begin(), ys.end());
+        mesh->loadxy.upload(xs, mesh->stream);
+        mesh->loadcol.upload(col, mesh->stream);
         mesh->rootpts.upload(pts, mesh->stream);
     }
-    if (N > 32) throw EfError(EF_ERR_ARG, "load evaluation supports N <= 32");
-    const long long nblocks = (long long)nrhs * nl + blocks_for((long long)nrhs * m);
-    if (nblocks > 0x7fffffffLL) throw EfError(EF_ERR_ARG, "too many load items for one launch");
+    const int nxv = mesh->load_nxv, nyv = mesh->load_nyv;
     check_range(params_dev, (size_t)nrhs * 64 * 8, "load parameters");
     check_range(f_dev, (size_t)nrhs * nl * N * N * 8, "load f");
     check_range(r_dev, (size_t)nrhs * m * 8, "load r");
-    EF_LAUNCH("serve_load_kernel", serve_load_kernel, dim3((unsigned)nblocks), dim3(256), mesh->stream,
-              (const double*)mesh->leafgeo.p, nl, N, params_dev, nrhs, f_dev, (const double*)mesh->rootpts.p, m, r_dev);
+    mesh->loadfac.ensure((size_t)nrhs * 8 * (nxv + nyv));
+    EF_LAUNCH("serve_factor_kernel", serve_factor_kernel, dim3(blocks_for(8LL * (nxv + nyv)), nrhs), dim3(256),
+              mesh->stream, (const double*)mesh->loadxy.p, nxv, nyv, params_dev, mesh->loadfac.p);
+    EF_LAUNCH("serve_cells_kernel", serve_cells_kernel, dim3((unsigned)std::min(nl, 512), nrhs), dim3(256),
+              mesh->stream, (const int*)mesh->loadcol.p, nl, N, nxv, nyv, params_dev, (const double*)mesh->loadfac.p,
+              f_dev);
+    EF_LAUNCH("serve_bdata_kernel", serve_bdata_kernel, dim3((unsigned)std::min<long long>(blocks_for((long long)nrhs * m), 1024)),
+              dim3(256), mesh->stream, params_dev, nrhs, (const double*)mesh->rootpts.p, m, r_dev);
     EF_API_END
 }
 

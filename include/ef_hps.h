@@ -181,7 +181,9 @@ int ef_mesh_set_refinement(ef_mesh mesh, int steps);
  * unused) amp exp(-|p - c|^2 / (2 w^2)) at the leaf cell centres, r_dev[rhs][4M] = sum of 8
  * modes (a, kx, ky, phi) a cos(kx x + ky y + phi) at the root boundary points; queued on the
  * mesh's stream.  Each Gaussian is evaluated as the product of its x and y factors (equal to
- * the exponential of the sum up to rounding). */
+ * the exponential of the sum up to rounding), computed once per distinct cell-centre coordinate
+ * of the mesh and request in a work buffer the mesh keeps (8 (X + Y) doubles per request, X and
+ * Y the distinct centre coordinates). */
 int ef_host_alloc(ef_ctx ctx, size_t bytes, void** ptr);
 int ef_host_free(ef_ctx ctx, void* ptr);
 int ef_mesh_copy_async(ef_mesh mesh, void* dst, const void* src, size_t bytes, int kind);
