@@ -290,30 +290,56 @@ __global__ void __launch_bounds__(256) serve_factor_kernel(const double* __restr
 
 // f[rhs][leaf][cell]: blockIdx.y = rhs, workgroups stride over the leaves; a leaf's 16 N factors
 // (its x and y centre columns, offsets col[2 leaf], col[2 leaf + 1]) are staged in LDS and each
-// cell sums 8 products of them.
+// cell sums 8 products of them.  The next leaf's factors are loaded while the current leaf's
+// cells are formed (two LDS buffers, one barrier per leaf).
 __global__ void __launch_bounds__(256) serve_cells_kernel(const int* __restrict__ col, int nl, int N, int nxv,
                                                            int nyv, const double* __restrict__ params,
                                                            const double* __restrict__ fac, double* __restrict__ f) {
-    __shared__ double sf[2][8][32];  // [axis][Gaussian][cell index along the axis], N <= 32
+    __shared__ double sf[2][2][8][32];  // [buffer][axis][Gaussian][cell index along the axis], N <= 32
     const int rhs = blockIdx.y, NN = N * N, nv = nxv + nyv, tid = threadIdx.x;
     const double* P = params + 64LL * rhs;
     const double* F = fac + (long long)rhs * 8 * nv;
-    for (int leaf = blockIdx.x; leaf < nl; leaf += gridDim.x) {
-        for (int t = tid; t < 16 * N; t += 256) {
-            const int axis = t / (8 * N), rem = t % (8 * N), q = rem / N, i = rem % N;
-            const int v = axis ? nxv + col[2 * leaf + 1] + i : col[2 * leaf] + i;
-            sf[axis][q][i] = F[(long long)q * nv + v];
+    double pre[2];  // 16 N <= 512 factors: at most two per thread
+    auto fetch = [&](int lf) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int t = tid + 256 * u;
+            if (t < 16 * N) {
+                const int axis = t / (8 * N), rem = t % (8 * N), q = rem / N, i = rem % N;
+                const int v = axis ? nxv + col[2 * lf + 1] + i : col[2 * lf] + i;
+                pre[u] = F[(long long)q * nv + v];
+            }
         }
-        __syncthreads();
+    };
+    auto stash = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int t = tid + 256 * u;
+            if (t < 16 * N) {
+                const int axis = t / (8 * N), rem = t % (8 * N), q = rem / N, i = rem % N;
+                sf[b][axis][q][i] = pre[u];
+            }
+        }
+    };
+    int leaf = blockIdx.x, buf = 0;
+    if (leaf >= nl) return;  // uniform per workgroup
+    fetch(leaf);
+    stash(0);
+    __syncthreads();
+    for (; leaf < nl; leaf += gridDim.x) {
+        const int next = leaf + (int)gridDim.x;
+        if (next < nl) fetch(next);  // in flight while this leaf's cells are formed
         double* fo = f + ((long long)rhs * nl + leaf) * NN;
         for (int cell = tid; cell < NN; cell += 256) {
             const int i = cell / N, j = cell % N;
             double acc = 0.0;
 #pragma unroll
-            for (int q = 0; q < 8; q++) acc += P[4 * q] * (sf[0][q][i] * sf[1][q][j]);
+            for (int q = 0; q < 8; q++) acc += P[4 * q] * (sf[buf][0][q][i] * sf[buf][1][q][j]);
             fo[cell] = acc;
         }
-        __syncthreads();  // the next leaf's factors overwrite sf
+        if (next < nl) stash(buf ^ 1);  // the other buffer was last read before the previous barrier
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
